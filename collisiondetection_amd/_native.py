@@ -1,0 +1,74 @@
+"""ctypes binding of libccd_amd.so (the C ABI declared in include/ccd_amd.h).
+
+The library is the only compute path: if it is missing or cannot be loaded the import of the
+functions below raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libccd_amd.so")
+
+CCD_OK = 0
+CCD_E_INVALID = 1
+CCD_E_CAPACITY = 2
+CCD_E_OOM = 3
+CCD_E_HIP = 4
+CCD_E_BUDGET = 5
+CCD_E_NODEVICE = 6
+
+PRIM_VERTEX_FACE = 0
+PRIM_EDGE_EDGE = 1
+PRIM_VERTEX_EDGE = 2
+PRIM_VERTEX_VERTEX = 3
+
+CCD_SORTED = 1
+
+# Every exported symbol of include/ccd_amd.h with its ctypes signature.
+_V = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_INT = ctypes.c_int
+_P64 = ctypes.POINTER(ctypes.c_int64)
+SIGNATURES = {
+    "ccd_create": ([ctypes.POINTER(_V), _INT], _INT),
+    "ccd_destroy": ([_V], _INT),
+    "ccd_set_stream": ([_V, _V], _INT),
+    "ccd_synchronize": ([_V], _INT),
+    "ccd_last_error": ([_V], ctypes.c_char_p),
+    "ccd_device_bytes": ([_V], _I64),
+    "ccd_ctcd_primitives": ([_V, _INT, _V, _V, _D, _I64, _V, _V], _INT),
+    "ccd_ctcd_narrowphase": ([_V, _V, _V, _I64, _V, _V, _I64, _V, _V, _I64, _D, _V, _V, _V, _V, _V, _V], _INT),
+    "ccd_ctcd_narrowphase_history": ([_V, _V, _V, _V, _I64, _I64, _V, _V, _I64, _V, _V, _I64, _D, _V, _V, _V, _V,
+                                      _V, _V, _V, _V], _INT),
+    "ccd_kdop_candidates": ([_V, _V, _I64, _V, _V, _I64, _D, _V, _V, _I64, _P64, _V, _I64, _P64, _INT], _INT),
+    "ccd_kdop_candidates_history": ([_V, _V, _I64, _V, _V, _I64, _I64, _D, _V, _V, _I64, _P64, _V, _I64, _P64,
+                                     _INT], _INT),
+    "ccd_detect": ([_V, _V, _I64, _V, _V, _I64, _D, _D, _V, _V, _V, _I64, _P64, _V, _V, _I64, _P64, _P64, _P64],
+                   _INT),
+}
+
+_lib = None
+
+
+class CCDError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ccd error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libccd_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with collisiondetection_amd._build.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib

@@ -1,0 +1,856 @@
+// Broadphase: the candidate stencil sets of KDOPBroadPhase::findCollisionCandidates
+// (src/KDOPBroadPhase.cpp:34-185), built MI355X-first.
+//
+// The reference builds a top-down median-split tree of 13-axis swept k-DOPs and traverses it against
+// itself.  Its output does not depend on the tree's shape (interior bounds are exact unions of child
+// bounds, KDOPBroadPhase.cpp:85-98, so a leaf pair is visited iff its two leaf bounds overlap), so this
+// file builds a linear BVH instead:
+//   1. leaf bounds: per face, 13 axes x (3 vertices x history entries), min/max of pos.axis -/+ eta in
+//      FP64 exactly as KDOPBroadPhase.cpp:60-70 (bit-identical), plus an FP32 copy rounded outward;
+//   2. 63-bit Morton codes of the leaf centres, radix-sorted (rocPRIM);
+//   3. Karras' binary radix tree over the sorted codes (index tie-break for equal codes);
+//   4. bottom-up refit of FP32 interior bounds with one atomic arrival counter per node;
+//   5. escape ("rope") links: in the Karras layout the node after the subtree of a node covering
+//      leaves [l, r] is internal node r+1 when that node's range starts at r+1, else leaf r+1;
+//   6. stackless traversal, one lane per leaf i, starting at leaf i's escape link, which visits
+//      exactly the subtrees holding leaves after i: every unordered face pair is found once.  The FP32
+//      test culls; the leaf-leaf decision is the reference's FP64 13-axis test (touching overlaps)
+//      followed by the shared-vertex test (Mesh.cpp:13-20).  Pairs are appended with one atomic per
+//      wavefront (ballot + popcount prefix);
+//   7. expansion of each face pair into its 6 vertex-face and 9 edge-edge stencils
+//      (KDOPBroadPhase.cpp:150-184) with owner-computes deduplication: a stencil is written only by
+//      the smallest face pair that generates it, which each lane decides from the vertex->face table
+//      and the leaf bounds, so the unique set comes out with no global sort.
+// Meshes with degenerate faces (a repeated vertex) or duplicate faces break the owner rule's
+// assumptions; for them the raw stencils are emitted and deduplicated by sorting.
+#include <cstring>
+#include <rocprim/device/device_merge_sort.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+
+#include "ccd_core.h"
+#include "ccd_internal.h"
+
+namespace {
+
+constexpr int KAX = 13;
+constexpr int NB = 2 * KAX;   // bounds per node: 13 mins then 13 maxs
+constexpr uint32_t NIL = 0xffffffffu;
+
+struct Axes {
+    double a[KAX][3];
+};
+
+// KDOPBroadPhase.cpp:10-32: the 13 directions, each divided by its norm.
+Axes make_axes()
+{
+    static const double raw[KAX][3] = {{1, 0, 0},  {0, 1, 0},  {0, 0, 1},  {1, 1, 0},  {1, -1, 0},
+                                       {0, 1, 1},  {0, 1, -1}, {1, 0, 1},  {1, 0, -1}, {1, 1, 1},
+                                       {1, 1, -1}, {1, -1, 1}, {1, -1, -1}};
+    Axes ax;
+    for (int i = 0; i < KAX; i++) {
+        ccd::V3 v = ccd::mk(raw[i][0], raw[i][1], raw[i][2]);
+        double n = ccd::norm(v);
+        ax.a[i][0] = v.x / n;
+        ax.a[i][1] = v.y / n;
+        ax.a[i][2] = v.z / n;
+    }
+    return ax;
+}
+
+// FP32 neighbours of a float (f finite or +-inf, not NaN).
+__device__ __forceinline__ float next_down(float f)
+{
+    if (f == 0.0f) return __int_as_float(0x80000001);
+    int b = __float_as_int(f);
+    return __int_as_float(f > 0 ? b - 1 : b + 1);
+}
+__device__ __forceinline__ float next_up(float f)
+{
+    if (f == 0.0f) return __int_as_float(0x00000001);
+    int b = __float_as_int(f);
+    return __int_as_float(f > 0 ? b + 1 : b - 1);
+}
+// Outward rounding of a double bound to float: the float interval always contains the double one.
+__device__ __forceinline__ float f32_down(double d)
+{
+    float f = (float)d;
+    if ((double)f > d) f = next_down(f);
+    return f;
+}
+__device__ __forceinline__ float f32_up(double d)
+{
+    float f = (float)d;
+    if ((double)f < d) f = next_up(f);
+    return f;
+}
+
+inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ------------------------------------------------------------------------------------------------
+// 1. Leaf bounds
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void dop_point(const Axes &ax, double x, double y, double z, double eta, double *mn,
+                                          double *mx)
+{
+    ccd::V3 p = ccd::mk(x, y, z);
+#pragma unroll
+    for (int k = 0; k < KAX; k++) {
+        double s = ccd::dot(p, ccd::mk(ax.a[k][0], ax.a[k][1], ax.a[k][2]));
+        mn[k] = ccd::smin(s - eta, mn[k]);
+        mx[k] = ccd::smax(s + eta, mx[k]);
+    }
+}
+
+// Two-entry histories (qs = entry at t=0, qe = entry at t=1) or CSR histories (hoff/hpos).
+__global__ __launch_bounds__(256) void k_leaf_bounds(const int32_t *__restrict__ faces, int64_t F,
+                                                     const double *__restrict__ qs, const double *__restrict__ qe,
+                                                     const int64_t *__restrict__ hoff,
+                                                     const double *__restrict__ hpos, double eta, Axes ax,
+                                                     double *__restrict__ dop64, double *__restrict__ centre)
+{
+    int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    double mn[KAX], mx[KAX];
+#pragma unroll
+    for (int k = 0; k < KAX; k++) {
+        mn[k] = __builtin_inf();
+        mx[k] = -__builtin_inf();
+    }
+    for (int j = 0; j < 3; j++) {
+        int v = faces[3 * f + j];
+        if (hoff) {
+            for (int64_t e = hoff[v]; e < hoff[v + 1]; e++)
+                dop_point(ax, hpos[3 * e], hpos[3 * e + 1], hpos[3 * e + 2], eta, mn, mx);
+        } else {
+            const double *a = qs + 3 * (int64_t)v;
+            const double *b = qe + 3 * (int64_t)v;
+            dop_point(ax, a[0], a[1], a[2], eta, mn, mx);
+            dop_point(ax, b[0], b[1], b[2], eta, mn, mx);
+        }
+    }
+    double *o = dop64 + NB * f;
+#pragma unroll
+    for (int k = 0; k < KAX; k++) {
+        o[k] = mn[k];
+        o[KAX + k] = mx[k];
+    }
+    centre[3 * f + 0] = 0.5 * (mn[0] + mx[0]);
+    centre[3 * f + 1] = 0.5 * (mn[1] + mx[1]);
+    centre[3 * f + 2] = 0.5 * (mn[2] + mx[2]);
+}
+
+// Scene bounds of the centres: per-block partial min/max, then one block folds the partials.
+__global__ __launch_bounds__(256) void k_bounds_partial(const double *__restrict__ centre, int64_t F,
+                                                        double *__restrict__ partial)
+{
+    __shared__ double s[6][256];
+    double lo[3] = {__builtin_inf(), __builtin_inf(), __builtin_inf()};
+    double hi[3] = {-__builtin_inf(), -__builtin_inf(), -__builtin_inf()};
+    for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < F; f += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double v = centre[3 * f + c];
+            lo[c] = v < lo[c] ? v : lo[c];
+            hi[c] = v > hi[c] ? v : hi[c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        s[c][threadIdx.x] = lo[c];
+        s[3 + c][threadIdx.x] = hi[c];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                double a = s[c][threadIdx.x + w];
+                if (a < s[c][threadIdx.x]) s[c][threadIdx.x] = a;
+                double b = s[3 + c][threadIdx.x + w];
+                if (b > s[3 + c][threadIdx.x]) s[3 + c][threadIdx.x] = b;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) partial[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_bounds_final(double *__restrict__ partial, int nparts,
+                                                      double *__restrict__ out)
+{
+    __shared__ double s[6][256];
+    double v[6] = {__builtin_inf(), __builtin_inf(), __builtin_inf(), -__builtin_inf(), -__builtin_inf(),
+                   -__builtin_inf()};
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double a = partial[6 * p + c];
+            double b = partial[6 * p + 3 + c];
+            v[c] = a < v[c] ? a : v[c];
+            v[3 + c] = b > v[3 + c] ? b : v[3 + c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++) s[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                double a = s[c][threadIdx.x + w];
+                if (a < s[c][threadIdx.x]) s[c][threadIdx.x] = a;
+                double b = s[3 + c][threadIdx.x + w];
+                if (b > s[3 + c][threadIdx.x]) s[3 + c][threadIdx.x] = b;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
+}
+
+// ------------------------------------------------------------------------------------------------
+// 2. Morton codes (21 bits per axis)
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t spread21(uint64_t x)
+{
+    x &= 0x1fffffull;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const double *__restrict__ centre, int64_t F,
+                                                const double *__restrict__ box, uint64_t *__restrict__ keys,
+                                                int32_t *__restrict__ vals)
+{
+    int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    uint64_t code = 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        double lo = box[c], hi = box[3 + c];
+        double ext = hi - lo;
+        double u = ext > 0 ? (centre[3 * f + c] - lo) / ext : 0.0;
+        u = u < 0 ? 0 : (u > 1 ? 1 : u);
+        if (!(u == u)) u = 0;   // NaN positions sort first; bounds still decide every pair
+        uint64_t q = (uint64_t)(u * 2097151.0);
+        code |= spread21(q) << c;
+    }
+    keys[f] = code;
+    vals[f] = (int32_t)f;
+}
+
+// ------------------------------------------------------------------------------------------------
+// 3. Karras binary radix tree.  Internal nodes 0..F-2, leaf j is node F-1+j.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ int delta(const uint64_t *__restrict__ k, int64_t F, int64_t i, int64_t j)
+{
+    if (j < 0 || j >= F) return -1;
+    uint64_t a = k[i], b = k[j];
+    if (a == b) return 64 + __clz((uint32_t)(i ^ j));
+    return __clzll(a ^ b);
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint64_t *__restrict__ keys, int64_t F,
+                                                uint32_t *__restrict__ left, uint32_t *__restrict__ rng_first,
+                                                uint32_t *__restrict__ rng_last, uint32_t *__restrict__ parent)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= F - 1) return;
+    int d = (delta(keys, F, i, i + 1) - delta(keys, F, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = delta(keys, F, i, i - d);
+    int64_t lmax = 2;
+    while (delta(keys, F, i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+        if (delta(keys, F, i, i + (l + t) * d) > dmin) l += t;
+    int64_t j = i + l * d;
+    int dnode = delta(keys, F, i, j);
+    int64_t s = 0;
+    int64_t t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(keys, F, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
+    int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    uint32_t lc = (lo == gamma) ? (uint32_t)(F - 1 + gamma) : (uint32_t)gamma;
+    uint32_t rc = (hi == gamma + 1) ? (uint32_t)(F - 1 + gamma + 1) : (uint32_t)(gamma + 1);
+    left[i] = lc;
+    rng_first[i] = (uint32_t)lo;
+    rng_last[i] = (uint32_t)hi;
+    parent[lc] = (uint32_t)i;
+    parent[rc] = (uint32_t)i;
+    // The right child is found from the left: it follows the left child's subtree.
+}
+
+// Escape link of every node (internal and leaf) from the ranges.
+__global__ __launch_bounds__(256) void k_escape(const uint32_t *__restrict__ rng_first,
+                                                const uint32_t *__restrict__ rng_last, int64_t F,
+                                                uint32_t *__restrict__ escape)
+{
+    int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (n >= 2 * F - 1) return;
+    int64_t r = n < F - 1 ? (int64_t)rng_last[n] : n - (F - 1);
+    uint32_t e;
+    if (r == F - 1)
+        e = NIL;
+    else if (r + 1 <= F - 2 && (int64_t)rng_first[r + 1] == r + 1)
+        e = (uint32_t)(r + 1);
+    else
+        e = (uint32_t)(F - 1 + r + 1);
+    escape[n] = e;
+}
+
+// Node record used by the traversal: 26 FP32 bounds, then the first child (internal) or the sorted
+// leaf index (leaf), then the escape link.  112 bytes = 7 x 16.
+struct __align__(16) NodeRec {
+    float b[NB];
+    uint32_t child;
+    uint32_t esc;
+};
+
+// Leaf records in sorted order, and the FP64 bounds and vertex triples in sorted order.
+__global__ __launch_bounds__(256) void k_leaf_records(const int32_t *__restrict__ order,
+                                                      const double *__restrict__ dop64,
+                                                      const int32_t *__restrict__ faces, int64_t F,
+                                                      const uint32_t *__restrict__ escape,
+                                                      NodeRec *__restrict__ nodes, double *__restrict__ sdop64,
+                                                      int4 *__restrict__ sfaces)
+{
+    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= F) return;
+    int64_t f = order[j];
+    const double *src = dop64 + NB * f;
+    NodeRec r;
+#pragma unroll
+    for (int k = 0; k < KAX; k++) {
+        double lo = src[k], hi = src[KAX + k];
+        r.b[k] = f32_down(lo);
+        r.b[KAX + k] = f32_up(hi);
+        sdop64[NB * j + k] = lo;
+        sdop64[NB * j + KAX + k] = hi;
+    }
+    r.child = (uint32_t)j;
+    r.esc = escape[F - 1 + j];
+    nodes[F - 1 + j] = r;
+    sfaces[j] = make_int4(faces[3 * f], faces[3 * f + 1], faces[3 * f + 2], (int)f);
+}
+
+// 4. Bottom-up refit: a lane per leaf climbs while it is the second child to arrive at a node.
+__global__ __launch_bounds__(256) void k_refit(int64_t F, const uint32_t *__restrict__ parent,
+                                               const uint32_t *__restrict__ left, const uint32_t *__restrict__ escape,
+                                               NodeRec *nodes, unsigned int *arrivals)
+{
+    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= F) return;
+    uint32_t node = parent[F - 1 + j];
+    while (true) {
+        __threadfence();   // release this lane's child bounds before announcing the arrival
+        unsigned int prior = atomicAdd(&arrivals[node], 1u);
+        if (prior == 0) return;   // the sibling subtree is not finished yet; its lane continues
+        __threadfence();          // acquire the sibling's bounds
+        uint32_t lc = left[node];
+        // The right child is the node after the left child's subtree, i.e. its escape link.
+        uint32_t rc = escape[lc];
+        NodeRec a = nodes[lc];
+        NodeRec b = nodes[rc];
+        NodeRec r;
+#pragma unroll
+        for (int k = 0; k < KAX; k++) {
+            r.b[k] = a.b[k] < b.b[k] ? a.b[k] : b.b[k];
+            r.b[KAX + k] = a.b[KAX + k] > b.b[KAX + k] ? a.b[KAX + k] : b.b[KAX + k];
+        }
+        r.child = lc;
+        r.esc = escape[node];
+        nodes[node] = r;
+        if (node == 0) return;
+        node = parent[node];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// 6. Traversal: unordered overlapping non-neighbouring face pairs.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ bool overlap32(const float *q, const float *b)
+{
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < KAX; k++) ok = ok && !(q[KAX + k] < b[k] || b[KAX + k] < q[k]);
+    return ok;
+}
+// KDOPBroadPhase.cpp:132-137: disjoint iff some axis separates them strictly.
+__device__ __forceinline__ bool overlap64(const double *__restrict__ a, const double *__restrict__ b)
+{
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < KAX; k++) ok = ok && !(a[KAX + k] < b[k] || b[KAX + k] < a[k]);
+    return ok;
+}
+__device__ __forceinline__ bool neighbours(int4 a, int4 b)   // Mesh.cpp:13-20
+{
+    return a.x == b.x || a.x == b.y || a.x == b.z || a.y == b.x || a.y == b.y || a.y == b.z || a.z == b.x ||
+           a.z == b.y || a.z == b.z;
+}
+
+__global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__restrict__ nodes,
+                                                  const double *__restrict__ sdop64,
+                                                  const int4 *__restrict__ sfaces, int2 *__restrict__ pairs,
+                                                  int64_t cap, unsigned long long *counter)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    bool active = i < F;
+    float q32[NB];
+    double q64[NB];
+    int4 qf = make_int4(0, 0, 0, 0);
+    uint32_t node = NIL;
+    if (active) {
+        const NodeRec &me = nodes[F - 1 + i];
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            q32[k] = me.b[k];
+            q64[k] = sdop64[NB * i + k];
+        }
+        qf = sfaces[i];
+        node = me.esc;
+    }
+    const int lane = __lane_id();
+    while (true) {
+        bool emit = false;
+        int other = 0;
+        if (node != NIL) {
+            const NodeRec &n = nodes[node];
+            float b[NB];
+#pragma unroll
+            for (int k = 0; k < NB; k++) b[k] = n.b[k];
+            uint32_t child = n.child, esc = n.esc;
+            if (overlap32(q32, b)) {
+                if (node < (uint32_t)(F - 1)) {
+                    node = child;
+                } else {
+                    int4 of = sfaces[child];
+                    if (overlap64(q64, sdop64 + NB * (int64_t)child) && !neighbours(qf, of)) {
+                        emit = true;
+                        other = of.w;
+                    }
+                    node = esc;
+                }
+            } else {
+                node = esc;
+            }
+        }
+        uint64_t m = __ballot(emit);
+        if (m) {
+            int leader = __ffsll((unsigned long long)m) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+            base = __shfl(base, leader);
+            if (emit) {
+                uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
+                int64_t pos = (int64_t)base + __popcll(below);
+                if (pos < cap) {
+                    int a = qf.w, b = other;
+                    pairs[pos] = a < b ? make_int2(a, b) : make_int2(b, a);
+                }
+            }
+        }
+        if (__ballot(node != NIL) == 0) break;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Vertex -> face table (CSR)
+// ------------------------------------------------------------------------------------------------
+__global__ void k_vf_count(const int32_t *__restrict__ faces, int64_t F, int32_t *__restrict__ cnt)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= 3 * F) return;
+    atomicAdd(&cnt[faces[i]], 1);
+}
+__global__ void k_vf_fill(const int32_t *__restrict__ faces, int64_t F, const int32_t *__restrict__ off,
+                          int32_t *__restrict__ cursor, int32_t *__restrict__ list)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= 3 * F) return;
+    int v = faces[i];
+    int pos = atomicAdd(&cursor[v], 1);
+    list[off[v] + pos] = (int32_t)(i / 3);
+}
+
+// Mesh checks for the owner rule: a face with a repeated vertex, or two faces with the same vertex
+// set (found as equal sorted triples among the faces of each triple's smallest vertex).
+__global__ void k_mesh_check(const int32_t *__restrict__ faces, int64_t F, const int32_t *__restrict__ off,
+                             const int32_t *__restrict__ list, unsigned long long *flag)
+{
+    int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    int a = faces[3 * f], b = faces[3 * f + 1], c = faces[3 * f + 2];
+    bool bad = (a == b || b == c || a == c);
+    if (!bad) {
+        int lo = a < b ? (a < c ? a : c) : (b < c ? b : c);
+        int hi = a > b ? (a > c ? a : c) : (b > c ? b : c);
+        int md = a + b + c - lo - hi;
+        for (int k = off[lo]; k < off[lo + 1]; k++) {
+            int g = list[k];
+            if (g == (int)f) continue;
+            int x = faces[3 * g], y = faces[3 * g + 1], z = faces[3 * g + 2];
+            int l2 = x < y ? (x < z ? x : z) : (y < z ? y : z);
+            int h2 = x > y ? (x > z ? x : z) : (y > z ? y : z);
+            int m2 = x + y + z - l2 - h2;
+            if (l2 == lo && m2 == md && h2 == hi) bad = true;
+        }
+    }
+    if (bad) atomicAdd(flag, 1ull);
+}
+
+// ------------------------------------------------------------------------------------------------
+// 7. Stencil expansion with owner-computes deduplication
+// ------------------------------------------------------------------------------------------------
+struct MeshView {
+    const int32_t *faces;
+    const int32_t *vf_off;
+    const int32_t *vf_list;
+    const double *dop64;   // by original face id
+    const uint8_t *fixed;
+};
+
+__device__ __forceinline__ int4 face3(const MeshView &m, int f)
+{
+    return make_int4(m.faces[3 * f], m.faces[3 * f + 1], m.faces[3 * f + 2], f);
+}
+__device__ __forceinline__ bool related(const MeshView &m, int a, int b)   // overlapping, not neighbours
+{
+    return !neighbours(face3(m, a), face3(m, b)) && overlap64(m.dop64 + NB * (int64_t)a, m.dop64 + NB * (int64_t)b);
+}
+__device__ __forceinline__ bool has_vertex(const MeshView &m, int f, int v)
+{
+    return m.faces[3 * f] == v || m.faces[3 * f + 1] == v || m.faces[3 * f + 2] == v;
+}
+__device__ __forceinline__ bool is_fixed(const MeshView &m, int v) { return m.fixed && m.fixed[v]; }
+
+// VF stencil (p, face h) generated by pair {a, h} for every face a containing p with a related to h;
+// the owner is the smallest such a.  `mine` is the a of the current pair.
+__device__ __forceinline__ bool vf_owner(const MeshView &m, int p, int h, int mine)
+{
+    for (int k = m.vf_off[p]; k < m.vf_off[p + 1]; k++) {
+        int a = m.vf_list[k];
+        if (a < mine && related(m, a, h)) return false;
+    }
+    return true;
+}
+
+// EE stencil (edge (u0,u1), edge (w0,w1)) generated by pairs {a, b}, a containing both u's and b both
+// w's, a related to b; the owner is the smallest (min(a,b), max(a,b)).  The current pair is (f < g).
+__device__ __forceinline__ bool ee_owner(const MeshView &m, int u0, int u1, int w0, int w1, int f, int g)
+{
+    for (int ka = m.vf_off[u0]; ka < m.vf_off[u0 + 1]; ka++) {
+        int a = m.vf_list[ka];
+        if (!has_vertex(m, a, u1)) continue;
+        for (int kb = m.vf_off[w0]; kb < m.vf_off[w0 + 1]; kb++) {
+            int b = m.vf_list[kb];
+            if (!has_vertex(m, b, w1)) continue;
+            int lo = a < b ? a : b, hi = a < b ? b : a;
+            if (lo < f || (lo == f && hi < g)) {
+                if (related(m, a, b)) return false;
+            }
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ int4 vf_key(int p, int a, int b, int c)   // Stencils.h:13-57
+{
+    int lo = a < b ? (a < c ? a : c) : (b < c ? b : c);
+    int hi = a > b ? (a > c ? a : c) : (b > c ? b : c);
+    int md = a + b + c - lo - hi;
+    return make_int4(p, lo, md, hi);
+}
+__device__ __forceinline__ int4 ee_key(int a, int b, int c, int d)   // Stencils.h:86-117
+{
+    int p0 = a < b ? a : b, p1 = a < b ? b : a;
+    int q0 = c < d ? c : d, q1 = c < d ? d : c;
+    if (p0 > q0) return make_int4(q0, q1, p0, p1);
+    return make_int4(p0, p1, q0, q1);
+}
+
+// Wave-aggregated append of up to one key per lane.
+__device__ __forceinline__ void append(bool want, int4 key, int4 *out, int64_t cap, unsigned long long *counter)
+{
+    uint64_t m = __ballot(want);
+    if (!m) return;
+    int lane = __lane_id();
+    int leader = __ffsll((unsigned long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    if (want) {
+        uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
+        int64_t pos = (int64_t)base + __popcll(below);
+        if (pos < cap) out[pos] = key;
+    }
+}
+
+template <bool OWNER>
+__global__ __launch_bounds__(256) void k_expand(MeshView m, const int2 *__restrict__ pairs,
+                                                const unsigned long long *__restrict__ npairs_dev, int64_t pairs_cap,
+                                                int4 *__restrict__ vf_out, int64_t vf_cap,
+                                                unsigned long long *vf_counter, int4 *__restrict__ ee_out,
+                                                int64_t ee_cap, unsigned long long *ee_counter)
+{
+    int64_t np = (int64_t)*npairs_dev;
+    if (np > pairs_cap) np = pairs_cap;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < np; base += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = base + threadIdx.x;
+        bool valid = i < np;
+        int2 pr = valid ? pairs[i] : make_int2(0, 0);
+        int f = pr.x, g = pr.y;
+        int4 F0 = valid ? face3(m, f) : make_int4(0, 0, 0, 0);
+        int4 G0 = valid ? face3(m, g) : make_int4(0, 0, 0, 0);
+        int fv[3] = {F0.x, F0.y, F0.z};
+        int gv[3] = {G0.x, G0.y, G0.z};
+        bool ffix = valid && is_fixed(m, fv[0]) && is_fixed(m, fv[1]) && is_fixed(m, fv[2]);
+        bool gfix = valid && is_fixed(m, gv[0]) && is_fixed(m, gv[1]) && is_fixed(m, gv[2]);
+        // 6 vertex-face stencils (KDOPBroadPhase.cpp:157-175)
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+            int p = fv[s];
+            bool want = valid && !(is_fixed(m, p) && gfix);
+            if (OWNER && want) want = vf_owner(m, p, g, f);
+            append(want, vf_key(p, gv[0], gv[1], gv[2]), vf_out, vf_cap, vf_counter);
+            int q = gv[s];
+            want = valid && !(is_fixed(m, q) && ffix);
+            if (OWNER && want) want = vf_owner(m, q, f, g);
+            append(want, vf_key(q, fv[0], fv[1], fv[2]), vf_out, vf_cap, vf_counter);
+        }
+        // 9 edge-edge stencils (KDOPBroadPhase.cpp:176-183)
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+            int u0 = fv[s], u1 = fv[(s + 1) % 3];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                int w0 = gv[r], w1 = gv[(r + 1) % 3];
+                bool want = valid && !(is_fixed(m, u0) && is_fixed(m, u1) && is_fixed(m, w0) && is_fixed(m, w1));
+                if (OWNER && want) want = ee_owner(m, u0, u1, w0, w1, f, g);
+                append(want, ee_key(u0, u1, w0, w1), ee_out, ee_cap, ee_counter);
+            }
+        }
+    }
+}
+
+// Stencil order of the reference's std::set (Stencils.h:59-77 and :119-137): lexicographic.
+struct KeyLess {
+    __host__ __device__ bool operator()(const int4 &a, const int4 &b) const
+    {
+        if (a.x != b.x) return a.x < b.x;
+        if (a.y != b.y) return a.y < b.y;
+        if (a.z != b.z) return a.z < b.z;
+        return a.w < b.w;
+    }
+};
+struct KeyEq {
+    __host__ __device__ bool operator()(const int4 &a, const int4 &b) const
+    {
+        return a.x == b.x && a.y == b.y && a.z == b.z && a.w == b.w;
+    }
+};
+
+__global__ void k_copy_count(const unsigned long long *src, int64_t *dst) { *dst = (int64_t)*src; }
+
+}   // namespace
+
+namespace ccdi {
+
+// Sorts n stencil keys (device, 4*n int32) into the reference's set order.
+int sort_stencils(ccd_ctx *ctx, int32_t *keys, int64_t n)
+{
+    if (n <= 1) return CCD_OK;
+    size_t tmp = 0;
+    int4 *sorted = (int4 *)buffer(ctx, "sort_out", (size_t)n * 16 + 16);
+    if (!sorted) return CCD_E_OOM;
+    CCD_HIP_CHECK(ctx, rocprim::merge_sort((void *)nullptr, tmp, (int4 *)keys, sorted, (size_t)n, KeyLess(),
+                                           ctx->stream));
+    void *t = buffer(ctx, "sort_tmp", tmp + 16);
+    if (!t) return CCD_E_OOM;
+    CCD_HIP_CHECK(ctx, rocprim::merge_sort(t, tmp, (int4 *)keys, sorted, (size_t)n, KeyLess(), ctx->stream));
+    CCD_HIP_CHECK(ctx, hipMemcpyAsync(keys, sorted, (size_t)n * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    return CCD_OK;
+}
+
+// Sort + unique in place; the unique count goes to *count_dev.
+static int sort_unique(ccd_ctx *ctx, int32_t *keys, int64_t n, const char *tag, int64_t *count_dev)
+{
+    int rc = sort_stencils(ctx, keys, n);
+    if (rc) return rc;
+    std::string name = std::string("uniq_out_") + tag;
+    int4 *out = (int4 *)buffer(ctx, name.c_str(), (size_t)n * 16 + 16);
+    if (!out) return CCD_E_OOM;
+    size_t tmp = 0;
+    CCD_HIP_CHECK(ctx, rocprim::unique((void *)nullptr, tmp, (int4 *)keys, out, (unsigned long long *)count_dev,
+                                       (size_t)n, KeyEq(), ctx->stream));
+    void *t = buffer(ctx, "uniq_tmp", tmp + 16);
+    if (!t) return CCD_E_OOM;
+    CCD_HIP_CHECK(ctx, rocprim::unique(t, tmp, (int4 *)keys, out, (unsigned long long *)count_dev, (size_t)n,
+                                       KeyEq(), ctx->stream));
+    CCD_HIP_CHECK(ctx, hipMemcpyAsync(keys, out, (size_t)n * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    return CCD_OK;
+}
+
+int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, const double *qe,
+               const int64_t *hist_off, const double *hist_pos, int64_t V, double eta, const uint8_t *fixed,
+               Candidates &out)
+{
+    out = Candidates();
+    int64_t *cnt = ctx->d_counts;
+    CCD_HIP_CHECK(ctx, hipMemsetAsync(cnt, 0, sizeof(int64_t) * NCOUNTERS, ctx->stream));
+    if (F < 2) {
+        CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts, cnt, sizeof(int64_t) * NCOUNTERS, hipMemcpyDeviceToHost,
+                                          ctx->stream));
+        CCD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        return CCD_OK;
+    }
+    static const Axes axes = make_axes();
+    double *dop64 = (double *)buffer(ctx, "dop64", (size_t)F * NB * 8);
+    double *centre = (double *)buffer(ctx, "centre", (size_t)F * 3 * 8);
+    const int nparts = 512;
+    double *partial = (double *)buffer(ctx, "bpartial", (size_t)nparts * 6 * 8 + 64);
+    uint64_t *keys = (uint64_t *)buffer(ctx, "mkeys", (size_t)F * 8);
+    uint64_t *keys2 = (uint64_t *)buffer(ctx, "mkeys2", (size_t)F * 8);
+    int32_t *vals = (int32_t *)buffer(ctx, "mvals", (size_t)F * 4);
+    int32_t *order = (int32_t *)buffer(ctx, "order", (size_t)F * 4);
+    uint32_t *left = (uint32_t *)buffer(ctx, "left", (size_t)F * 4);
+    uint32_t *rf = (uint32_t *)buffer(ctx, "rfirst", (size_t)F * 4);
+    uint32_t *rl = (uint32_t *)buffer(ctx, "rlast", (size_t)F * 4);
+    uint32_t *parent = (uint32_t *)buffer(ctx, "parent", (size_t)(2 * F) * 4);
+    uint32_t *escape = (uint32_t *)buffer(ctx, "escape", (size_t)(2 * F) * 4);
+    unsigned int *arrivals = (unsigned int *)buffer(ctx, "arrivals", (size_t)F * 4);
+    NodeRec *nodes = (NodeRec *)buffer(ctx, "nodes", (size_t)(2 * F) * sizeof(NodeRec));
+    double *sdop64 = (double *)buffer(ctx, "sdop64", (size_t)F * NB * 8);
+    int4 *sfaces = (int4 *)buffer(ctx, "sfaces", (size_t)F * 16);
+    int32_t *vcnt = (int32_t *)buffer(ctx, "vf_cnt", (size_t)(V + 1) * 4);
+    int32_t *voff = (int32_t *)buffer(ctx, "vf_off", (size_t)(V + 1) * 4);
+    int32_t *vlist = (int32_t *)buffer(ctx, "vf_list", (size_t)(3 * F) * 4);
+    if (!dop64 || !centre || !partial || !keys || !keys2 || !vals || !order || !left || !rf || !rl || !parent ||
+        !escape || !arrivals || !nodes || !sdop64 || !sfaces || !vcnt || !voff || !vlist)
+        return CCD_E_OOM;
+    hipStream_t st = ctx->stream;
+
+    hipLaunchKernelGGL(k_leaf_bounds, dim3(blocks_for(F, 256)), dim3(256), 0, st, faces, F, qs, qe, hist_off,
+                       hist_pos, eta, axes, dop64, centre);
+    CCD_LAUNCH_CHECK(ctx);
+    int nb = (int)std::min<int64_t>(nparts, blocks_for(F, 256));
+    hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(256), 0, st, centre, F, partial);
+    CCD_LAUNCH_CHECK(ctx);
+    double *box = partial + 6 * nparts;
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, st, partial, nb, box);
+    CCD_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_morton, dim3(blocks_for(F, 256)), dim3(256), 0, st, centre, F, box, keys, vals);
+    CCD_LAUNCH_CHECK(ctx);
+    size_t tmp = 0;
+    CCD_HIP_CHECK(ctx, rocprim::radix_sort_pairs((void *)nullptr, tmp, keys, keys2, vals, order, (size_t)F, 0, 63,
+                                                 st));
+    void *stmp = buffer(ctx, "radix_tmp", tmp + 16);
+    if (!stmp) return CCD_E_OOM;
+    CCD_HIP_CHECK(ctx, rocprim::radix_sort_pairs(stmp, tmp, keys, keys2, vals, order, (size_t)F, 0, 63, st));
+
+    hipLaunchKernelGGL(k_karras, dim3(blocks_for(F - 1, 256)), dim3(256), 0, st, keys2, F, left, rf, rl, parent);
+    CCD_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_escape, dim3(blocks_for(2 * F - 1, 256)), dim3(256), 0, st, rf, rl, F, escape);
+    CCD_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_leaf_records, dim3(blocks_for(F, 256)), dim3(256), 0, st, order, dop64, faces, F, escape,
+                       nodes, sdop64, sfaces);
+    CCD_LAUNCH_CHECK(ctx);
+    CCD_HIP_CHECK(ctx, hipMemsetAsync(arrivals, 0, (size_t)F * 4, st));
+    hipLaunchKernelGGL(k_refit, dim3(blocks_for(F, 256)), dim3(256), 0, st, F, parent, left, escape, nodes,
+                       arrivals);
+    CCD_LAUNCH_CHECK(ctx);
+
+    // Vertex -> face table and mesh checks.
+    CCD_HIP_CHECK(ctx, hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, st));
+    hipLaunchKernelGGL(k_vf_count, dim3(blocks_for(3 * F, 256)), dim3(256), 0, st, faces, F, vcnt);
+    CCD_LAUNCH_CHECK(ctx);
+    size_t stmp2 = 0;
+    CCD_HIP_CHECK(ctx, rocprim::exclusive_scan((void *)nullptr, stmp2, vcnt, voff, 0, (size_t)(V + 1),
+                                               rocprim::plus<int32_t>(), st));
+    void *scan_tmp = buffer(ctx, "scan_tmp", stmp2 + 16);
+    if (!scan_tmp) return CCD_E_OOM;
+    CCD_HIP_CHECK(ctx, rocprim::exclusive_scan(scan_tmp, stmp2, vcnt, voff, 0, (size_t)(V + 1),
+                                               rocprim::plus<int32_t>(), st));
+    CCD_HIP_CHECK(ctx, hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, st));
+    hipLaunchKernelGGL(k_vf_fill, dim3(blocks_for(3 * F, 256)), dim3(256), 0, st, faces, F, voff, vcnt, vlist);
+    CCD_LAUNCH_CHECK(ctx);
+    unsigned long long *c = (unsigned long long *)cnt;
+    hipLaunchKernelGGL(k_mesh_check, dim3(blocks_for(F, 256)), dim3(256), 0, st, faces, F, voff, vlist, c + C_BAD);
+    CCD_LAUNCH_CHECK(ctx);
+
+    // Traversal, with one retry if the pair buffer was too small.
+    int64_t pcap = std::max<int64_t>(ctx->bufs.count("pairs") ? (int64_t)(ctx->bufs["pairs"].bytes / 8) : 0,
+                                     16 * F + 1024);
+    int2 *pairs = (int2 *)buffer(ctx, "pairs", (size_t)pcap * 8);
+    if (!pairs) return CCD_E_OOM;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        CCD_HIP_CHECK(ctx, hipMemsetAsync(c + C_PAIRS, 0, 8, st));
+        hipLaunchKernelGGL(k_traverse, dim3(blocks_for(F, 256)), dim3(256), 0, st, F, nodes, sdop64, sfaces, pairs,
+                           pcap, c + C_PAIRS);
+        CCD_LAUNCH_CHECK(ctx);
+        CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts, cnt, sizeof(int64_t) * NCOUNTERS, hipMemcpyDeviceToHost, st));
+        CCD_HIP_CHECK(ctx, hipStreamSynchronize(st));
+        if (ctx->h_counts[C_PAIRS] <= pcap) break;
+        pcap = ctx->h_counts[C_PAIRS] + ctx->h_counts[C_PAIRS] / 8 + 1024;
+        pairs = (int2 *)buffer(ctx, "pairs", (size_t)pcap * 8);
+        if (!pairs) return CCD_E_OOM;
+    }
+    int64_t P = ctx->h_counts[C_PAIRS];
+    bool owner = ctx->h_counts[C_BAD] == 0;
+    out.npairs = P;
+
+    // Expansion, with one retry if a stencil buffer was too small.
+    int64_t vcap = std::max<int64_t>(ctx->bufs.count("vf_keys") ? (int64_t)(ctx->bufs["vf_keys"].bytes / 16) : 0,
+                                     owner ? (P * 7) / 2 + 1024 : 6 * P + 1024);
+    int64_t ecap = std::max<int64_t>(ctx->bufs.count("ee_keys") ? (int64_t)(ctx->bufs["ee_keys"].bytes / 16) : 0,
+                                     owner ? (P * 11) / 2 + 1024 : 9 * P + 1024);
+    MeshView mv{faces, voff, vlist, dop64, fixed};
+    for (int attempt = 0; attempt < 2; attempt++) {
+        int4 *vfk = (int4 *)buffer(ctx, "vf_keys", (size_t)vcap * 16);
+        int4 *eek = (int4 *)buffer(ctx, "ee_keys", (size_t)ecap * 16);
+        if (!vfk || !eek) return CCD_E_OOM;
+        CCD_HIP_CHECK(ctx, hipMemsetAsync(c + C_VF, 0, 16, st));
+        unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(P, 256), 65536));
+        if (owner)
+            hipLaunchKernelGGL(k_expand<true>, dim3(grid), dim3(256), 0, st, mv, pairs, c + C_PAIRS, pcap, vfk, vcap,
+                               c + C_VF, eek, ecap, c + C_EE);
+        else
+            hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(256), 0, st, mv, pairs, c + C_PAIRS, pcap, vfk,
+                               vcap, c + C_VF, eek, ecap, c + C_EE);
+        CCD_LAUNCH_CHECK(ctx);
+        CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts, cnt, sizeof(int64_t) * NCOUNTERS, hipMemcpyDeviceToHost, st));
+        CCD_HIP_CHECK(ctx, hipStreamSynchronize(st));
+        int64_t nv = ctx->h_counts[C_VF], ne = ctx->h_counts[C_EE];
+        out.vf = (int32_t *)vfk;
+        out.ee = (int32_t *)eek;
+        out.vf_cap = vcap;
+        out.ee_cap = ecap;
+        if (nv <= vcap && ne <= ecap) break;
+        vcap = std::max(vcap, nv + nv / 8 + 1024);
+        ecap = std::max(ecap, ne + ne / 8 + 1024);
+    }
+    out.nvf = ctx->h_counts[C_VF];
+    out.nee = ctx->h_counts[C_EE];
+    if (!owner) {
+        int rc = sort_unique(ctx, out.vf, out.nvf, "vf", cnt + C_VF);
+        if (rc) return rc;
+        rc = sort_unique(ctx, out.ee, out.nee, "ee", cnt + C_EE);
+        if (rc) return rc;
+        CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts, cnt, sizeof(int64_t) * NCOUNTERS, hipMemcpyDeviceToHost, st));
+        CCD_HIP_CHECK(ctx, hipStreamSynchronize(st));
+        out.nvf = ctx->h_counts[C_VF];
+        out.nee = ctx->h_counts[C_EE];
+    }
+    return CCD_OK;
+}
+
+}   // namespace ccdi

@@ -1,0 +1,744 @@
+// FP64 core of the swept-primitive CCD engine: the conservative eta-CCD primitive tests of
+// evouga/collisiondetection (src/CTCD.cpp) and the stencil-level narrowphase of src/CTCDNarrowPhase.cpp,
+// written for one GPU lane per stencil.
+//
+// Numerics.  Everything the reference decides with is reproduced operation by operation, so this
+// file must be compiled with -ffp-contract=off (no fused multiply-add may be formed from a*b+c):
+//   * dot products are summed ((x0*y0 + x1*y1) + x2*y2), cross products are
+//     (a1*b2 - a2*b1, a2*b0 - a0*b2, a0*b1 - a1*b0) -- the arithmetic of Eigen's Vector3d;
+//   * every polynomial coefficient follows the reference's expression order (CTCD.cpp:179-257,
+//     :536-570, :616-621), is normalised by division by its largest magnitude and degree-reduced
+//     exactly as CTCD.cpp:113-133 does;
+//   * degree <= 2 polynomials use the reference's closed forms (CTCD.cpp:38-56, :143-158), so their
+//     intervals are bit-identical, NaN corner cases included;
+//   * the sign of each piece between roots is decided at its midpoint with plain Horner on the
+//     normalised coefficients (CTCD.cpp:59-79), and std::max / std::min keep their (a < b) ? b : a
+//     and (b < a) ? b : a argument order.
+// Only the roots of degree 3..6 polynomials come from a different method: the reference calls a
+// Jenkins-Traub solver, this file isolates the real roots in [0,1] by recursion on derivatives
+// (critical points split [0,1] into monotone pieces; each piece with a sign change holds exactly one
+// root, found by bracketed Newton iteration).  The real parts of complex roots, which the reference
+// also uses as split points, never change the union of the sign intervals, so vertex-face and
+// vertex-edge results agree except where a root lies within rounding distance of a decision.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define CCD_HD __host__ __device__ inline
+#define CCD_FHD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define CCD_HD inline
+#define CCD_FHD inline
+#endif
+
+namespace ccd {
+
+// ------------------------------------------------------------------------------------------------
+// Scalars and 3-vectors
+// ------------------------------------------------------------------------------------------------
+CCD_FHD double smax(double a, double b) { return (a < b) ? b : a; }   // std::max(a, b)
+CCD_FHD double smin(double a, double b) { return (b < a) ? b : a; }   // std::min(a, b)
+CCD_FHD double dabs(double a) { return __builtin_fabs(a); }
+CCD_FHD double dsqrt(double a)
+{
+#if defined(__HIPCC__)
+    return __builtin_sqrt(a);
+#else
+    return std::sqrt(a);
+#endif
+}
+CCD_FHD double dfma(double a, double b, double c)
+{
+#if defined(__HIPCC__)
+    return __builtin_fma(a, b, c);
+#else
+    return std::fma(a, b, c);
+#endif
+}
+
+struct V3 {
+    double x, y, z;
+};
+CCD_FHD V3 mk(double x, double y, double z)
+{
+    V3 r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    return r;
+}
+CCD_FHD V3 operator-(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+CCD_FHD V3 operator+(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+CCD_FHD V3 operator*(double s, V3 a) { return mk(s * a.x, s * a.y, s * a.z); }
+CCD_FHD double dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+CCD_FHD V3 cross(V3 a, V3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+CCD_FHD double norm(V3 a) { return dsqrt(dot(a, a)); }
+
+// ------------------------------------------------------------------------------------------------
+// Interval lists (CTCD.h:7-26).  CAP is the most pieces a degree-D polynomial can produce.
+// ------------------------------------------------------------------------------------------------
+template <int CAP>
+struct Ivls {
+    double l[CAP], u[CAP];
+    int n;
+};
+
+// TimeInterval(tl, tu): order, then clamp (CTCD.h:9-14).
+template <int CAP>
+CCD_FHD void push_ti(Ivls<CAP> &L, double tl, double tu)
+{
+    if (tl > tu) {
+        double s = tl;
+        tl = tu;
+        tu = s;
+    }
+    tl = smax(tl, 0.0);
+    tu = smin(tu, 1.0);
+    if (L.n < CAP) {
+        L.l[L.n] = tl;
+        L.u[L.n] = tu;
+        L.n++;
+    }
+}
+
+// Horner in the reference's order (CTCD.cpp:68-73), no fused multiply-add.
+template <int D>
+CCD_FHD double horner(const double *op, int degree, double x)
+{
+    double f = op[0];
+#pragma unroll
+    for (int i = 1; i <= D; i++) {
+        if (i <= degree) {
+            f *= x;
+            f += op[i];
+        }
+    }
+    return f;
+}
+
+// checkInterval (CTCD.cpp:59-79)
+template <int D, int CAP>
+CCD_FHD void check_interval(double t1, double t2, const double *op, int degree, Ivls<CAP> &out, bool pos)
+{
+    t1 = smax(0.0, t1);
+    t2 = smax(0.0, t2);
+    t1 = smin(1.0, t1);
+    t2 = smin(1.0, t2);
+    double tmid = (t2 + t1) / 2;
+    double f = horner<D>(op, degree, tmid);
+    if ((pos && f >= 0) || (!pos && f <= 0)) push_ti(out, t1, t2);
+}
+
+// couldHaveRoots (CTCD.cpp:81-94)
+CCD_FHD bool could_have_roots(const double *op, int degree, bool pos)
+{
+    double result = 0;
+    if ((pos && op[0] > 0) || (!pos && op[0] < 0)) result = op[0];
+    for (int i = 1; i < degree; i++) {
+        result *= 1.0;
+        if ((pos && op[i] > 0) || (!pos && op[i] < 0)) result += op[i];
+    }
+    result *= 1.0;
+    result += op[degree];
+    return !((pos && result < 0) || (!pos && result > 0));
+}
+
+// getQuadRoots (CTCD.cpp:38-56)
+CCD_FHD int quad_roots(double a, double b, double c, double &t0, double &t1)
+{
+    int roots = 0;
+    int sign = 1;
+    if (b < 0) sign = -1;
+    double D = b * b - 4 * a * c;
+    if (D >= 0) {
+        roots = 2;
+        double q = -0.5 * (b + sign * dsqrt(D));
+        t0 = q / a;
+        t1 = c / q;
+        if (t0 > t1) {
+            double s = t0;
+            t0 = t1;
+            t1 = s;
+        }
+    }
+    return roots;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Real-root isolation on [0,1] for degree 1..6 (used for reduced degree >= 3 only).
+// c[0..D] descending, c[0] != 0.  Writes the distinct real roots that lie in [0,1], ascending, and
+// returns their count.  A root exactly at an end (c evaluates to exactly 0 there) is reported as
+// exactly that end.
+// ------------------------------------------------------------------------------------------------
+template <int D>
+CCD_FHD double peval(const double *c, double x)
+{
+    double f = c[0];
+#pragma unroll
+    for (int i = 1; i <= D; i++) f = dfma(f, x, c[i]);
+    return f;
+}
+template <int D>
+CCD_FHD void peval_d(const double *c, double x, double &f, double &df)
+{
+    f = c[0];
+    df = 0.0;
+#pragma unroll
+    for (int i = 1; i <= D; i++) {
+        df = dfma(df, x, f);
+        f = dfma(f, x, c[i]);
+    }
+}
+
+// The single root of c in [a, b] given opposite nonzero signs fa, fb at the ends.
+template <int D>
+CCD_HD double bracketed_root(const double *c, double a, double b, double fa, double fb)
+{
+    double lo = a, hi = b;
+    bool neg_lo = fa < 0;
+    // Start from the secant (regula falsi) point.
+    double x = a + (b - a) * (fa / (fa - fb));
+    if (!(x > lo && x < hi)) x = 0.5 * (lo + hi);
+    for (int it = 0; it < 80; it++) {
+        double f, df;
+        peval_d<D>(c, x, f, df);
+        if (f == 0.0) return x;
+        if ((f < 0) == neg_lo)
+            lo = x;
+        else
+            hi = x;
+        double xn = x - f / df;
+        if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
+        double step = xn - x;
+        if (step < 0) step = -step;
+        x = xn;
+        double ax = x < 0 ? -x : x;
+        if (step <= 2.3e-16 * ax + 1e-18 || hi - lo <= 2.3e-16 * ax + 1e-18 || !(hi > lo)) break;
+    }
+    return x;
+}
+
+template <int D>
+struct RootIsolator {
+    CCD_HD static int roots01(const double *c, double *r)
+    {
+        double d[D];
+#pragma unroll
+        for (int i = 0; i < D; i++) d[i] = c[i] * (double)(D - i);
+        double cr[D - 1 > 0 ? D - 1 : 1];
+        int nc = RootIsolator<D - 1>::roots01(d, cr);
+        int nr = 0;
+        double a = 0.0;
+        double fa = peval<D>(c, 0.0);
+        if (fa == 0.0) r[nr++] = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            if (k <= nc) {
+                double b = (k < nc) ? cr[k] : 1.0;
+                if (b > a) {
+                    double fb = peval<D>(c, b);
+                    if (fb == 0.0) {
+                        if (nr == 0 || r[nr - 1] != b) r[nr++] = b;
+                    } else if (fa != 0.0 && ((fa < 0) != (fb < 0))) {
+                        r[nr++] = bracketed_root<D>(c, a, b, fa, fb);
+                    }
+                    a = b;
+                    fa = fb;
+                }
+            }
+        }
+        return nr;
+    }
+};
+template <>
+struct RootIsolator<1> {
+    CCD_HD static int roots01(const double *c, double *r)
+    {
+        double x = -c[1] / c[0];
+        if (x >= 0.0 && x <= 1.0) {
+            r[0] = x == 0.0 ? 0.0 : x;
+            return 1;
+        }
+        return 0;
+    }
+};
+template <>
+struct RootIsolator<2> {
+    CCD_HD static int roots01(const double *c, double *r)
+    {
+        double a = c[0], b = c[1], cc = c[2];
+        double disc = b * b - 4.0 * a * cc;
+        if (disc < 0) return 0;
+        double sq = dsqrt(disc);
+        double q = -0.5 * (b + (b < 0 ? -sq : sq));
+        double x0, x1;
+        if (q == 0.0) {
+            x0 = x1 = -b / (2.0 * a);
+        } else {
+            x0 = q / a;
+            x1 = cc / q;
+        }
+        if (x0 > x1) {
+            double s = x0;
+            x0 = x1;
+            x1 = s;
+        }
+        int n = 0;
+        if (x0 >= 0.0 && x0 <= 1.0) r[n++] = x0;
+        if (x1 >= 0.0 && x1 <= 1.0 && !(n > 0 && r[n - 1] == x1)) r[n++] = x1;
+        return n;
+    }
+};
+
+// ------------------------------------------------------------------------------------------------
+// findIntervals (CTCD.cpp:98-177) for a degree-N polynomial, op modified in place.
+// ------------------------------------------------------------------------------------------------
+template <int N, int CAP>
+CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos)
+{
+    out.n = 0;
+    double maxval = 0;
+#pragma unroll
+    for (int i = 0; i <= N; i++) maxval = smax(maxval, dabs(op[i]));
+    if (maxval != 0) {
+#pragma unroll
+        for (int i = 0; i <= N; i++) op[i] /= maxval;
+    }
+    int reduced = N;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        if (reduced == N - i && op[i] == 0) reduced--;
+    }
+    if (reduced < N) {
+        int s = N - reduced;
+        // op[i] = op[i + s] for i = 0..reduced, with unrolled static indexing.
+        double tmp[N + 1];
+#pragma unroll
+        for (int i = 0; i <= N; i++) tmp[i] = op[i];
+#pragma unroll
+        for (int i = 0; i <= N; i++) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k <= N; k++)
+                if (k == i + s) v = tmp[k];
+            if (i <= reduced) op[i] = v;
+        }
+    }
+
+    double time[N > 2 ? N : 2];
+    int roots = 0;
+    if (reduced > 2) {
+        if (!could_have_roots(op, reduced, pos)) return;
+        if (reduced == 3) roots = RootIsolator<3>::roots01(op, time);
+        if (N >= 4 && reduced == 4) roots = RootIsolator<(N >= 4 ? 4 : 3)>::roots01(op, time);
+        if (N >= 5 && reduced == 5) roots = RootIsolator<(N >= 5 ? 5 : 3)>::roots01(op, time);
+        if (N >= 6 && reduced == 6) roots = RootIsolator<(N >= 6 ? 6 : 3)>::roots01(op, time);
+        // Pieces between consecutive roots (all in [0,1]) and the ends, as CTCD.cpp:160-176.
+        if (roots == 0) {
+            check_interval<N>(0.0, 1.0, op, reduced, out, pos);
+            return;
+        }
+        check_interval<N>(0.0, time[0], op, reduced, out, pos);
+#pragma unroll
+        for (int i = 0; i + 1 < (N > 2 ? N : 2); i++)
+            if (i + 1 < roots) check_interval<N>(time[i], time[i + 1], op, reduced, out, pos);
+        double last = time[0];
+#pragma unroll
+        for (int i = 1; i < (N > 2 ? N : 2); i++)
+            if (i < roots) last = time[i];
+        check_interval<N>(last, 1.0, op, reduced, out, pos);
+        return;
+    } else if (reduced == 2) {
+        roots = quad_roots(op[0], op[1], op[2], time[0], time[1]);
+        // std::sort of two values: swaps only when time[1] < time[0].
+        if (roots == 2 && time[1] < time[0]) {
+            double s = time[0];
+            time[0] = time[1];
+            time[1] = s;
+        }
+    } else if (reduced == 1) {
+        time[0] = -op[1] / op[0];
+        roots = 1;
+    } else {
+        if ((!pos && op[0] <= 0) || (pos && op[0] >= 0)) push_ti(out, 0.0, 1.0);
+        return;
+    }
+    if (roots > 0) {
+        if (time[0] >= 0) check_interval<N>(0, time[0], op, reduced, out, pos);
+        if (roots == 2) {
+            if (!((time[0] < 0 && time[1] < 0) || (time[0] > 1.0 && time[1] > 1.0)))
+                check_interval<N>(time[0], time[1], op, reduced, out, pos);
+        }
+        double last = roots == 2 ? time[1] : time[0];
+        if (last <= 1.0) check_interval<N>(last, 1.0, op, reduced, out, pos);
+    } else {
+        check_interval<N>(0.0, 1.0, op, reduced, out, pos);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Coefficient builders (CTCD.cpp:179-257)
+// ------------------------------------------------------------------------------------------------
+template <int CAP>
+CCD_HD void barycentric_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, Ivls<CAP> &out)
+{
+    double A = dot(x10, x10);
+    double B = 2 * dot(x10, v10);
+    double C = dot(v10, v10);
+    double D = dot(x20, x10);
+    double E = dot(x20, v10) + dot(v20, x10);
+    double F = dot(v20, v10);
+    double G = dot(x30, x20);
+    double H = dot(x30, v20) + dot(v30, x20);
+    double I = dot(v30, v20);
+    double J = dot(x30, x10);
+    double K = dot(x30, v10) + dot(v30, x10);
+    double L = dot(v30, v10);
+    double op[5];
+    op[0] = F * L - C * I;
+    op[1] = F * K + E * L - C * H - B * I;
+    op[2] = F * J + D * L + E * K - C * G - A * I - B * H;
+    op[3] = D * K + E * J - A * H - B * G;
+    op[4] = D * J - A * G;
+    find_intervals<4>(op, out, true);
+}
+
+template <int CAP>
+CCD_HD void plane_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, Ivls<CAP> &out)
+{
+    double op[4];
+    V3 v23 = cross(v20, v30);
+    V3 x2v3 = cross(x20, v30);
+    V3 v2x3 = cross(v20, x30);
+    V3 x23 = cross(x20, x30);
+    op[0] = dot(v10, v23);
+    op[1] = dot(x10, v23) + dot(v10, x2v3) + dot(v10, v2x3);
+    op[2] = dot(x10, x2v3) + dot(x10, v2x3) + dot(v10, x23);
+    op[3] = dot(x10, x23);
+    find_intervals<3>(op, out, true);
+}
+
+template <int CAP>
+CCD_HD void distance_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, double minD, Ivls<CAP> &out)
+{
+    V3 v23 = cross(v20, v30);
+    V3 x2v3 = cross(x20, v30);
+    V3 v2x3 = cross(v20, x30);
+    V3 x23 = cross(x20, x30);
+    double A = dot(v10, v23);
+    double B = dot(x10, v23) + dot(v10, x2v3) + dot(v10, v2x3);
+    double C = dot(x10, x2v3) + dot(x10, v2x3) + dot(v10, x23);
+    double D = dot(x10, x23);
+    V3 E = x23;
+    V3 F = x2v3 + v2x3;
+    V3 G = v23;
+    double op[7];
+    op[0] = A * A;
+    op[1] = 2 * A * B;
+    op[2] = B * B + 2 * A * C - dot(G, G) * minD;
+    op[3] = 2 * A * D + 2 * B * C - 2 * dot(G, F) * minD;
+    op[4] = 2 * B * D + C * C - (2 * dot(G, E) + dot(F, F)) * minD;
+    op[5] = 2 * C * D - 2 * dot(F, E) * minD;
+    op[6] = D * D - dot(E, E) * minD;
+    find_intervals<6>(op, out, false);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Primitive tests (CTCD.cpp:259-692).  Each returns true on collision and then sets t.
+// ------------------------------------------------------------------------------------------------
+
+// edgeEdgeCTCD (CTCD.cpp:259-411): edges (q0,p0) and (q1,p1).
+CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3 p1e, double eta, double &t)
+{
+    double minD = eta * eta;
+    V3 vp0 = p0e - p0s;
+    V3 vp1 = p1e - p1s;
+    V3 vq0 = q0e - q0s;
+    V3 vq1 = q1e - q1s;
+    Ivls<7> raw;
+    distance_poly(p0s - p1s, p0s - q0s, p1s - q1s, vp0 - vp1, vp0 - vq0, vp1 - vq1, minD, raw);
+
+    // Parallel-edge classification of each raw interval at its midpoint (CTCD.cpp:294-308).
+    Ivls<7> coplane, parallel;
+    coplane.n = 0;
+    parallel.n = 0;
+    for (int i = 0; i < raw.n; i++) {
+        double midt = (raw.u[i] + raw.l[i]) / 2;
+        V3 a = (q0s - p0s) + midt * (vq0 - vp0);
+        V3 b = (q1s - p1s) + midt * (vq1 - vp1);
+        if (norm(cross(a, b)) < 1e-8) {
+            parallel.l[parallel.n] = raw.l[i];
+            parallel.u[parallel.n] = raw.u[i];
+            parallel.n++;
+        } else {
+            coplane.l[coplane.n] = raw.l[i];
+            coplane.u[coplane.n] = raw.u[i];
+            coplane.n++;
+        }
+    }
+    if (coplane.n == 0) return false;
+
+    Ivls<5> a0, a1, b0, b1;
+    V3 x10 = p1s - q1s, v10 = vp1 - vq1;
+    barycentric_poly(x10, p0s - q0s, q0s - q1s, v10, vp0 - vq0, vq0 - vq1, a0);
+    if (a0.n == 0) return false;
+    barycentric_poly(x10, q0s - p0s, p0s - q1s, v10, vq0 - vp0, vp0 - vq1, a1);
+    if (a1.n == 0) return false;
+    x10 = p0s - q0s;
+    v10 = vp0 - vq0;
+    barycentric_poly(x10, p1s - q1s, q1s - q0s, v10, vp1 - vq1, vq1 - vq0, b0);
+    if (b0.n == 0) return false;
+    barycentric_poly(x10, q1s - p1s, p1s - q0s, v10, vq1 - vp1, vp1 - vq0, b1);
+    if (b1.n == 0) return false;
+
+    // All combinations (CTCD.cpp:352-396): a combination counts if its five intervals overlap
+    // pairwise and their intersection touches no parallel interval; t is the smallest lower end.
+    bool col = false;
+    double mint = 1.0;
+    for (int i = 0; i < coplane.n; i++)
+        for (int j = 0; j < a0.n; j++)
+            for (int k = 0; k < a1.n; k++)
+                for (int l = 0; l < b0.n; l++)
+                    for (int m = 0; m < b1.n; m++) {
+                        double L[5] = {coplane.l[i], a0.l[j], a1.l[k], b0.l[l], b1.l[m]};
+                        double U[5] = {coplane.u[i], a0.u[j], a1.u[k], b0.u[l], b1.u[m]};
+                        bool ov = true;
+#pragma unroll
+                        for (int x = 0; x < 5; x++)
+#pragma unroll
+                            for (int y = x + 1; y < 5; y++)
+                                if (L[x] > U[y] || L[y] > U[x]) ov = false;
+                        if (!ov) continue;
+                        double il = 0.0, iu = 1.0;
+#pragma unroll
+                        for (int x = 0; x < 5; x++) {
+                            il = smax(L[x], il);
+                            iu = smin(U[x], iu);
+                        }
+                        bool skip = false;
+                        for (int p = 0; p < parallel.n; p++)
+                            if (!(il > parallel.u[p] || parallel.l[p] > iu)) skip = true;
+                        if (!skip) {
+                            mint = smin(mint, il);
+                            col = true;
+                        }
+                    }
+    if (col) t = mint;
+    return col;
+}
+
+// vertexFaceCTCD (CTCD.cpp:413-508): vertex q0 and face (q1,q2,q3).
+CCD_HD bool vertex_face(V3 q0s, V3 q1s, V3 q2s, V3 q3s, V3 q0e, V3 q1e, V3 q2e, V3 q3e, double eta, double &t)
+{
+    double minD = eta * eta;
+    V3 v0 = q0e - q0s;
+    V3 v1 = q1e - q1s;
+    V3 v2 = q2e - q2s;
+    V3 v3 = q3e - q3s;
+    Ivls<4> e1, e2, e3;
+    plane_poly(q0s - q1s, cross(q3s - q1s, q2s - q1s), q3s - q1s, v0 - v1, cross(v3 - v1, v2 - v1), v3 - v1, e1);
+    if (e1.n == 0) return false;
+    plane_poly(q0s - q2s, cross(q1s - q2s, q3s - q2s), q1s - q2s, v0 - v2, cross(v1 - v2, v3 - v2), v1 - v2, e2);
+    if (e2.n == 0) return false;
+    plane_poly(q0s - q3s, cross(q2s - q3s, q1s - q3s), q2s - q3s, v0 - v3, cross(v2 - v3, v1 - v3), v2 - v3, e3);
+    if (e3.n == 0) return false;
+    Ivls<7> cop;
+    distance_poly(q0s - q1s, q2s - q1s, q3s - q1s, v0 - v1, v2 - v1, v3 - v1, minD, cop);
+    if (cop.n == 0) return false;
+
+    bool col = false;
+    double mint = 1.0;
+    for (int i = 0; i < cop.n; i++)
+        for (int j = 0; j < e1.n; j++)
+            for (int k = 0; k < e2.n; k++)
+                for (int l = 0; l < e3.n; l++) {
+                    double L[4] = {cop.l[i], e1.l[j], e2.l[k], e3.l[l]};
+                    double U[4] = {cop.u[i], e1.u[j], e2.u[k], e3.u[l]};
+                    bool ov = true;
+#pragma unroll
+                    for (int x = 0; x < 4; x++)
+#pragma unroll
+                        for (int y = x + 1; y < 4; y++)
+                            if (L[x] > U[y] || L[y] > U[x]) ov = false;
+                    if (!ov) continue;
+                    double il = 0.0;
+#pragma unroll
+                    for (int x = 0; x < 4; x++) il = smax(L[x], il);
+                    mint = smin(il, mint);
+                    col = true;
+                }
+    if (col) t = mint;
+    return col;
+}
+
+// vertexEdgeCTCD (CTCD.cpp:511-602): vertex q0 and edge (q1,q2).
+CCD_HD bool vertex_edge(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double eta, double &t)
+{
+    double minD = eta * eta;
+    V3 v0 = q0e - q0s;
+    V3 v1 = q1e - q1s;
+    V3 v2 = q2e - q2s;
+    V3 ab = q2s - q1s;
+    V3 ac = q0s - q1s;
+    V3 cb = q2s - q0s;
+    V3 vab = v2 - v1;
+    V3 vac = v0 - v1;
+    V3 vcb = v2 - v0;
+    Ivls<3> e1, e2;
+    Ivls<5> colin;
+    double op[5];
+    op[0] = dot(vab, vac);
+    op[1] = dot(ac, vab) + dot(ab, vac);
+    op[2] = dot(ab, ac);
+    find_intervals<2>(op, e1, true);
+    if (e1.n == 0) return false;
+    op[0] = dot(vab, vcb);
+    op[1] = dot(cb, vab) + dot(ab, vcb);
+    op[2] = dot(ab, cb);
+    find_intervals<2>(op, e2, true);
+    if (e2.n == 0) return false;
+    double A = dot(ab, ab);
+    double B = 2 * dot(ab, vab);
+    double C = dot(vab, vab);
+    double D = dot(ac, ac);
+    double E = 2 * dot(ac, vac);
+    double F = dot(vac, vac);
+    double G = dot(ac, ab);
+    double H = dot(vab, ac) + dot(vac, ab);
+    double I = dot(vab, vac);
+    op[4] = A * D - G * G - minD * A;
+    op[3] = B * D + A * E - 2 * G * H - minD * B;
+    op[2] = B * E + A * F + C * D - H * H - 2 * G * I - minD * C;
+    op[1] = B * F + C * E - 2 * H * I;
+    op[0] = C * F - I * I;
+    find_intervals<4>(op, colin, false);
+    if (colin.n == 0) return false;
+
+    bool col = false;
+    double mint = 1.0;
+    for (int i = 0; i < colin.n; i++)
+        for (int j = 0; j < e1.n; j++)
+            for (int k = 0; k < e2.n; k++) {
+                double L[3] = {colin.l[i], e1.l[j], e2.l[k]};
+                double U[3] = {colin.u[i], e1.u[j], e2.u[k]};
+                bool ov = true;
+#pragma unroll
+                for (int x = 0; x < 3; x++)
+#pragma unroll
+                    for (int y = x + 1; y < 3; y++)
+                        if (L[x] > U[y] || L[y] > U[x]) ov = false;
+                if (!ov) continue;
+                double il = 0.0;
+#pragma unroll
+                for (int x = 0; x < 3; x++) il = smax(L[x], il);
+                mint = smin(il, mint);
+                col = true;
+            }
+    if (col) t = mint;
+    return col;
+}
+
+// vertexVertexCTCD (CTCD.cpp:604-692)
+CCD_HD bool vertex_vertex(V3 q1s, V3 q2s, V3 q1e, V3 q2e, double eta, double &t)
+{
+    double min_d = eta * eta;
+    double t1 = 0, t2 = 0;
+    int roots = 0;
+    V3 v1 = q1e - q1s;
+    V3 v2 = q2e - q2s;
+    double a = dot(v1, v1) + dot(v2, v2) - 2 * dot(v1, v2);
+    double b = 2 * (dot(v1, q1s) - dot(v2, q1s) - dot(v1, q2s) + dot(v2, q2s));
+    double c = dot(q1s, q1s) + dot(q2s, q2s) - 2 * dot(q1s, q2s) - min_d;
+    if (a != 0) {
+        roots = quad_roots(a, b, c, t1, t2);
+    } else if (b != 0) {
+        t1 = -c / b;
+        roots = 1;
+    } else {
+        if (c <= 0) {
+            t = 0;
+            return true;
+        }
+        return false;
+    }
+    double op[3] = {a, b, c};
+    Ivls<1> iv;
+    iv.n = 0;
+    if (roots == 2) {
+        check_interval<2>(0, t1, op, 2, iv, false);
+        if (iv.n) {
+            t = 0;
+            return true;
+        }
+        check_interval<2>(t1, t2, op, 2, iv, false);
+        if (iv.n) {
+            t = t1;
+            return true;
+        }
+        check_interval<2>(t2, 1.0, op, 2, iv, false);
+        if (iv.n) {
+            t = t2;
+            return true;
+        }
+        return false;
+    } else if (roots == 1) {
+        check_interval<2>(0, t1, op, 2, iv, false);
+        if (iv.n) {
+            t = 0;
+            return true;
+        }
+        check_interval<2>(t1, 1.0, op, 2, iv, false);
+        if (iv.n) {
+            t = t1;
+            return true;
+        }
+        return false;
+    }
+    check_interval<2>(0, 1.0, op, 2, iv, false);
+    if (iv.n) {
+        t = 0;
+        return true;
+    }
+    return false;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Stencil narrowphase for one stitched window (CTCDNarrowPhase.cpp:43-69 and :91-132).
+// p[0..3] = start positions, e[0..3] = end positions of the stencil's vertices in key order.
+// Returns the firing sub-test (0 = VF/EE, then VE in reference order, then VV) or -1.
+// ------------------------------------------------------------------------------------------------
+CCD_HD int check_vf_window(const V3 *s, const V3 *e, double eta, double &t)
+{
+    if (vertex_face(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t)) return 0;
+    if (vertex_edge(s[0], s[1], s[2], e[0], e[1], e[2], eta, t)) return 1;
+    if (vertex_edge(s[0], s[2], s[3], e[0], e[2], e[3], eta, t)) return 2;
+    if (vertex_edge(s[0], s[3], s[1], e[0], e[3], e[1], eta, t)) return 3;
+    if (vertex_vertex(s[0], s[1], e[0], e[1], eta, t)) return 4;
+    if (vertex_vertex(s[0], s[2], e[0], e[2], eta, t)) return 5;
+    if (vertex_vertex(s[0], s[3], e[0], e[3], eta, t)) return 6;
+    return -1;
+}
+
+CCD_HD int check_ee_window(const V3 *s, const V3 *e, double eta, double &t)
+{
+    if (edge_edge(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t)) return 0;
+    if (vertex_edge(s[0], s[2], s[3], e[0], e[2], e[3], eta, t)) return 1;
+    if (vertex_edge(s[1], s[2], s[3], e[1], e[2], e[3], eta, t)) return 2;
+    if (vertex_edge(s[2], s[0], s[1], e[2], e[0], e[1], eta, t)) return 3;
+    if (vertex_edge(s[3], s[0], s[1], e[3], e[0], e[1], eta, t)) return 4;
+    if (vertex_vertex(s[0], s[2], e[0], e[2], eta, t)) return 5;
+    if (vertex_vertex(s[0], s[3], e[0], e[3], eta, t)) return 6;
+    if (vertex_vertex(s[1], s[2], e[1], e[2], eta, t)) return 7;
+    if (vertex_vertex(s[1], s[3], e[1], e[3], eta, t)) return 8;
+    return -1;
+}
+
+// The position History::stitchCommonHistory assigns at t = 0 for a two-entry history
+// (History.cpp:127-133 with dt = 1, a = 0, b = 1): (1*start + 0*end) / 1, kept literally so that
+// signed zeros come out as in the reference.
+CCD_FHD double stitch0(double s, double e) { return (1.0 * s + 0.0 * e) / 1.0; }
+
+}   // namespace ccd

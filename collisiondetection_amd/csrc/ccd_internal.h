@@ -1,0 +1,93 @@
+// Internal declarations shared by the HIP translation units of libccd_amd.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <unordered_map>
+
+#include "../../include/ccd_amd.h"
+
+struct ccd_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    struct Buf {
+        void *ptr = nullptr;
+        size_t bytes = 0;
+    };
+    std::unordered_map<std::string, Buf> bufs;   // named grow-only device buffers
+    int64_t device_bytes = 0;
+    int64_t *h_counts = nullptr;                  // pinned host mailbox for device counters
+    int64_t *d_counts = nullptr;                  // device counters (CCD_NCOUNTERS int64)
+};
+
+namespace ccdi {
+
+constexpr int NCOUNTERS = 16;
+
+// Counter slots in ctx->d_counts.
+enum Counter {
+    C_PAIRS = 0,    // overlapping non-neighbouring face pairs found by the traversal
+    C_VF = 1,       // unique vertex-face stencils
+    C_EE = 2,       // unique edge-edge stencils
+    C_VF_HITS = 3,  // colliding vertex-face stencils
+    C_EE_HITS = 4,  // colliding edge-edge stencils
+    C_BAD = 5,      // validation failures
+};
+
+int set_error(ccd_ctx *ctx, int code, const char *fmt, ...);
+// Grow-only named device buffer.  Returns nullptr (and sets the context error) on failure.
+void *buffer(ccd_ctx *ctx, const char *name, size_t bytes);
+
+}   // namespace ccdi
+
+#define CCD_HIP_CHECK(ctx, call)                                                                          \
+    do {                                                                                                  \
+        hipError_t _e = (call);                                                                           \
+        if (_e != hipSuccess)                                                                             \
+            return ccdi::set_error((ctx), CCD_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), \
+                                   __FILE__, __LINE__);                                                   \
+    } while (0)
+
+#define CCD_LAUNCH_CHECK(ctx) CCD_HIP_CHECK(ctx, hipGetLastError())
+
+// ----------------------------------------------------------------------------------------------
+// Narrowphase launchers (narrowphase.hip)
+// ----------------------------------------------------------------------------------------------
+namespace ccdi {
+int launch_primitives(ccd_ctx *ctx, int kind, const double *q, const double *eta, double eta_s, int64_t n,
+                      uint8_t *hit, double *t);
+int launch_narrow2(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
+                   const double *eta, double eta_s, int64_t n, uint8_t *hit, double *t, uint8_t *sub);
+int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *ht, const double *hp,
+                       const int32_t *keys, const double *eta, double eta_s, int64_t n, uint8_t *hit, double *t,
+                       uint8_t *sub, int32_t *win);
+// Narrowphase over a stencil list, compacting the colliding stencils (keys + toi) into out_keys /
+// out_toi with a device counter at counters[slot]; at most cap are written.
+int launch_narrow2_compact(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
+                           const int64_t *n_dev, int64_t n_max, double eta, int32_t *out_keys, double *out_toi,
+                           int64_t cap, int64_t *counter);
+// Validation: counts keys outside [0, nverts) into *bad.
+int launch_validate_keys(ccd_ctx *ctx, const int32_t *keys, int64_t n4, int64_t nverts, int64_t *bad);
+int launch_validate_history(ccd_ctx *ctx, const int64_t *off, const double *ht, int64_t nverts, int64_t nentries,
+                            int64_t *bad);
+
+// ----------------------------------------------------------------------------------------------
+// Broadphase (broadphase.hip).  Produces the unique candidate stencils in device buffers owned by
+// the context; *vf / *ee receive their device addresses and the counts are left in ctx->d_counts.
+// ----------------------------------------------------------------------------------------------
+struct Candidates {
+    int32_t *vf = nullptr;
+    int32_t *ee = nullptr;
+    int64_t nvf = 0, nee = 0, npairs = 0;
+    int64_t vf_cap = 0, ee_cap = 0;
+};
+int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, const double *qe,
+               const int64_t *hist_off, const double *hist_pos, int64_t V, double eta, const uint8_t *fixed,
+               Candidates &out);
+int sort_stencils(ccd_ctx *ctx, int32_t *keys, int64_t n);
+}   // namespace ccdi

@@ -1,0 +1,304 @@
+// Narrowphase kernels: one lane per stencil (or per primitive item), FP64, no fused multiply-add.
+//
+// Layout: positions are read straight from the caller's flat double[3V] arrays (24-byte vertex
+// records); stencil keys are int4 (16 bytes, one dwordx4 load per lane).  Consecutive stencils share
+// vertices (the broadphase emits them face pair by face pair), so the gathers hit L2 after the first
+// touch.  The work per stencil is the reference's fixed sub-test order with early exit
+// (CTCDNarrowPhase.cpp:43-69, :91-132), so lanes diverge; the kernel is FP64-VALU bound.
+#include "ccd_core.h"
+#include "ccd_internal.h"
+
+namespace {
+
+using ccd::V3;
+
+__device__ __forceinline__ V3 load3(const double *__restrict__ p, int v)
+{
+    const double *q = p + 3 * (int64_t)v;
+    return ccd::mk(q[0], q[1], q[2]);
+}
+
+__global__ __launch_bounds__(256) void k_primitives(int kind, const double *__restrict__ q,
+                                                    const double *__restrict__ eta, double eta_s, int64_t n,
+                                                    uint8_t *__restrict__ hit, double *__restrict__ tout)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double e = eta ? eta[i] : eta_s;
+    double t = 0.0;
+    bool r = false;
+    if (kind == CCD_PRIM_VERTEX_FACE || kind == CCD_PRIM_EDGE_EDGE) {
+        const double *p = q + 24 * i;
+        V3 a[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) a[k] = ccd::mk(p[3 * k], p[3 * k + 1], p[3 * k + 2]);
+        if (kind == CCD_PRIM_VERTEX_FACE)
+            r = ccd::vertex_face(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], e, t);
+        else
+            r = ccd::edge_edge(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], e, t);
+    } else if (kind == CCD_PRIM_VERTEX_EDGE) {
+        const double *p = q + 18 * i;
+        V3 a[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) a[k] = ccd::mk(p[3 * k], p[3 * k + 1], p[3 * k + 2]);
+        r = ccd::vertex_edge(a[0], a[1], a[2], a[3], a[4], a[5], e, t);
+    } else {
+        const double *p = q + 12 * i;
+        V3 a[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[k] = ccd::mk(p[3 * k], p[3 * k + 1], p[3 * k + 2]);
+        r = ccd::vertex_vertex(a[0], a[1], a[2], a[3], e, t);
+    }
+    hit[i] = r ? 1 : 0;
+    if (tout) tout[i] = r ? t : 0.0;
+}
+
+// Two-entry histories: window (stitch0(start), end), CTCDNarrowPhase.cpp:24-36 with History.cpp:127-133.
+template <bool VF>
+__device__ __forceinline__ int stencil2(const double *__restrict__ qs, const double *__restrict__ qe, int4 k,
+                                        double eta, double &t)
+{
+    int v[4] = {k.x, k.y, k.z, k.w};
+    V3 s[4], e[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        V3 a = load3(qs, v[j]);
+        V3 b = load3(qe, v[j]);
+        s[j] = ccd::mk(ccd::stitch0(a.x, b.x), ccd::stitch0(a.y, b.y), ccd::stitch0(a.z, b.z));
+        e[j] = b;
+    }
+    return VF ? ccd::check_vf_window(s, e, eta, t) : ccd::check_ee_window(s, e, eta, t);
+}
+
+template <bool VF>
+__global__ __launch_bounds__(256) void k_narrow2(const double *__restrict__ qs, const double *__restrict__ qe,
+                                                 const int4 *__restrict__ keys, const double *__restrict__ eta,
+                                                 double eta_s, int64_t n, uint8_t *__restrict__ hit,
+                                                 double *__restrict__ tout, uint8_t *__restrict__ sub)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double t = 0.0;
+    int r = stencil2<VF>(qs, qe, keys[i], eta ? eta[i] : eta_s, t);
+    hit[i] = r >= 0 ? 1 : 0;
+    if (tout) tout[i] = r >= 0 ? t : 0.0;
+    if (sub) sub[i] = r >= 0 ? (uint8_t)r : (uint8_t)255;
+}
+
+// Narrowphase + compaction of the colliding stencils.  The stencil count is read from the device
+// (n_dev) so the launch needs no host round trip; the grid covers n_max.
+template <bool VF>
+__global__ __launch_bounds__(256) void k_narrow2_compact(const double *__restrict__ qs, const double *__restrict__ qe,
+                                                         const int4 *__restrict__ keys,
+                                                         const int64_t *__restrict__ n_dev, double eta,
+                                                         int4 *__restrict__ out_keys, double *__restrict__ out_toi,
+                                                         int64_t cap, unsigned long long *counter)
+{
+    int64_t n = *n_dev;
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    bool hit = false;
+    double t = 0.0;
+    int4 k = make_int4(0, 0, 0, 0);
+    if (i < n) {
+        k = keys[i];
+        hit = stencil2<VF>(qs, qe, k, eta, t) >= 0;
+    }
+    // Wave-aggregated append: one atomic per wavefront.
+    uint64_t mask = __ballot(hit);
+    if (mask == 0) return;
+    int lane = __lane_id();
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader);
+    if (hit) {
+        uint64_t below = mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+        int64_t pos = (int64_t)base + __popcll(below);
+        if (pos < cap) {
+            out_keys[pos] = k;
+            if (out_toi) out_toi[pos] = t;
+        }
+    }
+}
+
+// Multi-entry histories: stitch on the fly (History.cpp:98-140) and test window by window.
+template <bool VF>
+__global__ __launch_bounds__(256) void k_narrow_hist(const int64_t *__restrict__ off, const double *__restrict__ ht,
+                                                     const double *__restrict__ hp, const int4 *__restrict__ keys,
+                                                     const double *__restrict__ eta, double eta_s, int64_t n,
+                                                     uint8_t *__restrict__ hit, double *__restrict__ tout,
+                                                     uint8_t *__restrict__ sub, int32_t *__restrict__ win)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int4 k = keys[i];
+    int v[4] = {k.x, k.y, k.z, k.w};
+    double e = eta ? eta[i] : eta_s;
+    int64_t its[4], ends[4];
+    int64_t total = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        its[j] = off[v[j]];
+        ends[j] = off[v[j] + 1];
+        total += ends[j] - its[j];
+    }
+    V3 prev[4], cur[4];
+    double curtime = 0.0;
+    int w = -1;
+    int r = -1;
+    double t = 0.0;
+    // Each pass advances curtime strictly (entries at or before it are consumed), so the number of
+    // stitched entries is at most the total entry count; the bound also guards malformed input.
+    for (int64_t guard = 0; guard <= total && curtime <= 1.0; guard++) {
+        double newtime = __builtin_inf();
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            int64_t next = its[j] + 1;
+            while (next != ends[j] && ht[next] <= curtime) {
+                ++next;
+                ++its[j];
+            }
+            V3 oldp = ccd::mk(hp[3 * its[j]], hp[3 * its[j] + 1], hp[3 * its[j] + 2]);
+            if (next == ends[j]) {
+                cur[j] = oldp;
+            } else {
+                V3 newp = ccd::mk(hp[3 * next], hp[3 * next + 1], hp[3 * next + 2]);
+                double dt = ht[next] - ht[its[j]];
+                double a = curtime - ht[its[j]];
+                double b = ht[next] - curtime;
+                cur[j] = ccd::mk((b * oldp.x + a * newp.x) / dt, (b * oldp.y + a * newp.y) / dt,
+                                 (b * oldp.z + a * newp.z) / dt);
+                newtime = ccd::smin(newtime, ht[next]);
+            }
+        }
+        if (w >= 0) {
+            r = VF ? ccd::check_vf_window(prev, cur, e, t) : ccd::check_ee_window(prev, cur, e, t);
+            if (r >= 0) break;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) prev[j] = cur[j];
+        w++;
+        curtime = newtime;
+    }
+    hit[i] = r >= 0 ? 1 : 0;
+    if (tout) tout[i] = r >= 0 ? t : 0.0;
+    if (sub) sub[i] = r >= 0 ? (uint8_t)r : (uint8_t)255;
+    if (win) win[i] = r >= 0 ? w : -1;
+}
+
+__global__ void k_validate_keys(const int32_t *__restrict__ keys, int64_t n4, int64_t nverts,
+                                unsigned long long *bad)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    bool b = false;
+    if (i < n4) {
+        int32_t v = keys[i];
+        b = v < 0 || v >= nverts;
+    }
+    uint64_t m = __ballot(b);
+    if (m && __lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(bad, (unsigned long long)__popcll(m));
+}
+
+// A history is valid when off[] is nondecreasing within [0, nentries], each vertex has at least
+// one entry, its times start at 0, end at 1 and never decrease.
+__global__ void k_validate_history(const int64_t *__restrict__ off, const double *__restrict__ ht, int64_t nverts,
+                                   int64_t nentries, unsigned long long *bad)
+{
+    int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    bool b = false;
+    if (v < nverts) {
+        int64_t a = off[v], z = off[v + 1];
+        if (a < 0 || z > nentries || z <= a) {
+            b = true;
+        } else {
+            if (ht[a] != 0.0 || ht[z - 1] != 1.0) b = true;
+            for (int64_t e = a + 1; e < z && !b; e++)
+                if (!(ht[e] >= ht[e - 1])) b = true;
+        }
+    }
+    uint64_t m = __ballot(b);
+    if (m && __lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(bad, (unsigned long long)__popcll(m));
+}
+
+inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}   // namespace
+
+namespace ccdi {
+
+int launch_primitives(ccd_ctx *ctx, int kind, const double *q, const double *eta, double eta_s, int64_t n,
+                      uint8_t *hit, double *t)
+{
+    if (n == 0) return CCD_OK;
+    hipLaunchKernelGGL(k_primitives, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, kind, q, eta, eta_s, n,
+                       hit, t);
+    CCD_LAUNCH_CHECK(ctx);
+    return CCD_OK;
+}
+
+int launch_narrow2(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
+                   const double *eta, double eta_s, int64_t n, uint8_t *hit, double *t, uint8_t *sub)
+{
+    if (n == 0) return CCD_OK;
+    if (vf)
+        hipLaunchKernelGGL(k_narrow2<true>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
+                           (const int4 *)keys, eta, eta_s, n, hit, t, sub);
+    else
+        hipLaunchKernelGGL(k_narrow2<false>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
+                           (const int4 *)keys, eta, eta_s, n, hit, t, sub);
+    CCD_LAUNCH_CHECK(ctx);
+    return CCD_OK;
+}
+
+int launch_narrow2_compact(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
+                           const int64_t *n_dev, int64_t n_max, double eta, int32_t *out_keys, double *out_toi,
+                           int64_t cap, int64_t *counter)
+{
+    if (n_max == 0) return CCD_OK;
+    if (vf)
+        hipLaunchKernelGGL(k_narrow2_compact<true>, dim3(blocks_for(n_max, 256)), dim3(256), 0, ctx->stream, qs,
+                           qe, (const int4 *)keys, n_dev, eta, (int4 *)out_keys, out_toi, cap,
+                           (unsigned long long *)counter);
+    else
+        hipLaunchKernelGGL(k_narrow2_compact<false>, dim3(blocks_for(n_max, 256)), dim3(256), 0, ctx->stream, qs,
+                           qe, (const int4 *)keys, n_dev, eta, (int4 *)out_keys, out_toi, cap,
+                           (unsigned long long *)counter);
+    CCD_LAUNCH_CHECK(ctx);
+    return CCD_OK;
+}
+
+int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *ht, const double *hp,
+                       const int32_t *keys, const double *eta, double eta_s, int64_t n, uint8_t *hit, double *t,
+                       uint8_t *sub, int32_t *win)
+{
+    if (n == 0) return CCD_OK;
+    if (vf)
+        hipLaunchKernelGGL(k_narrow_hist<true>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, off, ht, hp,
+                           (const int4 *)keys, eta, eta_s, n, hit, t, sub, win);
+    else
+        hipLaunchKernelGGL(k_narrow_hist<false>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, off, ht, hp,
+                           (const int4 *)keys, eta, eta_s, n, hit, t, sub, win);
+    CCD_LAUNCH_CHECK(ctx);
+    return CCD_OK;
+}
+
+int launch_validate_keys(ccd_ctx *ctx, const int32_t *keys, int64_t n4, int64_t nverts, int64_t *bad)
+{
+    if (n4 == 0) return CCD_OK;
+    hipLaunchKernelGGL(k_validate_keys, dim3(blocks_for(n4, 256)), dim3(256), 0, ctx->stream, keys, n4, nverts,
+                       (unsigned long long *)bad);
+    CCD_LAUNCH_CHECK(ctx);
+    return CCD_OK;
+}
+
+int launch_validate_history(ccd_ctx *ctx, const int64_t *off, const double *ht, int64_t nverts, int64_t nentries,
+                            int64_t *bad)
+{
+    if (nverts == 0) return CCD_OK;
+    hipLaunchKernelGGL(k_validate_history, dim3(blocks_for(nverts, 256)), dim3(256), 0, ctx->stream, off, ht,
+                       nverts, nentries, (unsigned long long *)bad);
+    CCD_LAUNCH_CHECK(ctx);
+    return CCD_OK;
+}
+
+}   // namespace ccdi

@@ -1,0 +1,137 @@
+/*
+ * ccd_amd.h -- C ABI of the MI355X (gfx950) swept-primitive continuous collision detection engine.
+ *
+ * The entry points below replace, one for one, the reference (evouga/collisiondetection) interfaces
+ * named in each comment.  Pointers marked "device" are HIP device pointers (HBM-resident data, e.g.
+ * a torch CUDA tensor's data_ptr()); sizes are element counts.  Positions are FP64 flat arrays of 3V
+ * values (the layout of an Eigen::VectorXd of stacked vertices), faces are int32 flat arrays of 3F
+ * values (the column-major storage of an Eigen::Matrix3Xi, one column per triangle), stencil keys are
+ * int32 quadruples in the reference's canonical order (VertexFaceStencil: p, q0 <= q1 <= q2;
+ * EdgeEdgeStencil: p0 <= p1, q0 <= q1, p0 <= q0).
+ *
+ * Every call is asynchronous on the context's stream unless stated otherwise, returns a status code
+ * (CCD_OK = 0) and never throws.  A context is not thread-safe: use one per host thread.
+ */
+#ifndef CCD_AMD_H
+#define CCD_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define CCD_OK 0
+#define CCD_E_INVALID 1  /* bad argument (null pointer, negative size, index out of range, ...) */
+#define CCD_E_CAPACITY 2 /* an output buffer is too small; the required count was written */
+#define CCD_E_OOM 3      /* device allocation failed */
+#define CCD_E_HIP 4      /* a HIP runtime call failed */
+#define CCD_E_BUDGET 5   /* the request does not fit the context's HBM budget */
+#define CCD_E_NODEVICE 6 /* no usable gfx950 device */
+
+/* Primitive kinds for ccd_ctcd_primitives. */
+#define CCD_PRIM_VERTEX_FACE 0   /* CTCD::vertexFaceCTCD   (include/CTCD.h:49-58)  24 doubles per item */
+#define CCD_PRIM_EDGE_EDGE 1     /* CTCD::edgeEdgeCTCD     (include/CTCD.h:36-44)  24 doubles per item */
+#define CCD_PRIM_VERTEX_EDGE 2   /* CTCD::vertexEdgeCTCD   (include/CTCD.h:63-70)  18 doubles per item */
+#define CCD_PRIM_VERTEX_VERTEX 3 /* CTCD::vertexVertexCTCD (include/CTCD.h:75-79)  12 doubles per item */
+
+/* Flags for ccd_kdop_candidates. */
+#define CCD_SORTED 1 /* return stencils in the reference's std::set order (Stencils.h:59-77, :119-137) */
+
+typedef struct ccd_ctx ccd_ctx;
+
+/* Context: owns a HIP stream and a grow-only device arena.  No counterpart in the reference, whose
+ * objects allocate per call (KDOPBroadPhase.cpp:38-40). */
+int ccd_create(ccd_ctx **out, int device);
+int ccd_destroy(ccd_ctx *ctx);
+/* Run subsequent work on an external stream (a hipStream_t); NULL restores the context's own. */
+int ccd_set_stream(ccd_ctx *ctx, void *hip_stream);
+int ccd_synchronize(ccd_ctx *ctx);
+/* Human-readable description of the last error on this context (never NULL). */
+const char *ccd_last_error(ccd_ctx *ctx);
+/* Bytes of device memory the context currently holds. */
+int64_t ccd_device_bytes(ccd_ctx *ctx);
+
+/*
+ * Primitive tests on n independent items -- CTCD::{vertexFace,edgeEdge,vertexEdge,vertexVertex}CTCD
+ * (include/CTCD.h:36-79, src/CTCD.cpp:259-692).  Item i's positions are q[i*S .. i*S+S) (device),
+ * S = 24/24/18/12: the start positions of the primitive's vertices in the reference's argument order,
+ * then their end positions.  eta (device, n values) may be NULL to use eta_scalar for all items.
+ * hit[i] = 1 when the reference function returns true; t[i] is then its time of impact (t is
+ * written as 0 when hit[i] = 0, where the reference leaves t untouched).
+ */
+int ccd_ctcd_primitives(ccd_ctx *ctx, int kind, const double *q, const double *eta, double eta_scalar, int64_t n,
+                        uint8_t *hit, double *t);
+
+/*
+ * Narrowphase -- CTCDNarrowPhase::findCollisions (src/CTCDNarrowPhase.cpp:9-22) with checkVFS / checkEES
+ * (:24-135) for two-entry histories (History(qstart) + finishHistory(qend), History.cpp:8-39).
+ * qstart/qend: device, 3*nverts doubles.  vf_keys/ee_keys: device, 4*n int32 keys.  vf_eta/ee_eta:
+ * device per-stencil eta (the double of the reference's std::pair<Stencil,double>), or NULL to use
+ * eta_scalar.  Outputs (device, n each): hit (1 = the stencil is in the reference's output set), t (the
+ * time of impact of the sub-test that fired, which the reference computes and discards), sub (which
+ * sub-test fired: 0 = VF or EE, then VE in the reference's order, then VV; 255 = none).  Any output
+ * pointer except hit may be NULL.
+ */
+int ccd_ctcd_narrowphase(ccd_ctx *ctx, const double *qstart, const double *qend, int64_t nverts,
+                         const int32_t *vf_keys, const double *vf_eta, int64_t nvf, const int32_t *ee_keys,
+                         const double *ee_eta, int64_t nee, double eta_scalar, uint8_t *vf_hit, double *vf_t,
+                         uint8_t *vf_sub, uint8_t *ee_hit, double *ee_t, uint8_t *ee_sub);
+
+/*
+ * Narrowphase over multi-entry histories -- the same as ccd_ctcd_narrowphase for histories with any
+ * number of entries per vertex (History::addHistory, History.cpp:21-26), stitched per stencil as
+ * History::stitchCommonHistory (History.cpp:98-140) and tested window by window.  The history is CSR
+ * (device): vertex v owns entries hist_off[v] .. hist_off[v+1]-1 with times hist_t[] (the first 0,
+ * the last 1, nondecreasing) and positions hist_pos[3*e ..].  t is local to the window that fired
+ * (window index in win, may be NULL), as the reference's CTCD calls see it.
+ */
+int ccd_ctcd_narrowphase_history(ccd_ctx *ctx, const int64_t *hist_off, const double *hist_t,
+                                 const double *hist_pos, int64_t nverts, int64_t nentries, const int32_t *vf_keys,
+                                 const double *vf_eta, int64_t nvf, const int32_t *ee_keys, const double *ee_eta,
+                                 int64_t nee, double eta_scalar, uint8_t *vf_hit, double *vf_t, uint8_t *vf_sub,
+                                 int32_t *vf_win, uint8_t *ee_hit, double *ee_t, uint8_t *ee_sub, int32_t *ee_win);
+
+/*
+ * Broadphase -- KDOPBroadPhase::findCollisionCandidates (src/KDOPBroadPhase.cpp:34-41, leaf bounds
+ * :43-74, traversal :130-185) for two-entry histories.  faces (device, 3*nfaces int32), qstart/qend
+ * (device, 3*nverts doubles), outer_eta, fixed (device, nverts bytes, nonzero = the vertex is in the
+ * reference's fixedVerts set; may be NULL).  Writes the unique candidate stencils into vf_out / ee_out
+ * (device, 4*cap int32) and their counts into *vf_count / *ee_count (host).  When a capacity is too
+ * small the call returns CCD_E_CAPACITY with the required counts written and the buffers partially
+ * filled.  flags: CCD_SORTED for the reference's set order.  Synchronous (the counts are on the host).
+ */
+int ccd_kdop_candidates(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces, const double *qstart, const double *qend,
+                        int64_t nverts, double outer_eta, const uint8_t *fixed, int32_t *vf_out, int64_t vf_cap,
+                        int64_t *vf_count, int32_t *ee_out, int64_t ee_cap, int64_t *ee_count, int flags);
+
+/*
+ * Broadphase over multi-entry histories: as ccd_kdop_candidates, the leaf bounds covering every history
+ * entry of the face's vertices (KDOPBroadPhase.cpp:60-70).  History layout as in
+ * ccd_ctcd_narrowphase_history.
+ */
+int ccd_kdop_candidates_history(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces, const int64_t *hist_off,
+                                const double *hist_pos, int64_t nverts, int64_t nentries, double outer_eta,
+                                const uint8_t *fixed, int32_t *vf_out, int64_t vf_cap, int64_t *vf_count,
+                                int32_t *ee_out, int64_t ee_cap, int64_t *ee_count, int flags);
+
+/*
+ * Detection: broadphase then narrowphase (the sequence of example/AlecTest.cpp:97-111 and
+ * ActiveLayers::collisionDetection, src/ActiveLayers.cpp:188-215) with the candidate stencils kept on
+ * the device.  Every candidate is tested with eta; the colliding stencils are written to vf_hits /
+ * ee_hits (device, 4*cap int32) with their times of impact in vf_toi / ee_toi (device, cap doubles,
+ * may be NULL).  Counts of candidates and of collisions are written to the host (any may be NULL).
+ * Returns CCD_E_CAPACITY, with the required counts, when a collision buffer is too small.
+ * Synchronous.
+ */
+int ccd_detect(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces, const double *qstart, const double *qend,
+               int64_t nverts, double outer_eta, double eta, const uint8_t *fixed, int32_t *vf_hits, double *vf_toi,
+               int64_t vf_cap, int64_t *vf_hit_count, int32_t *ee_hits, double *ee_toi, int64_t ee_cap,
+               int64_t *ee_hit_count, int64_t *vf_candidates, int64_t *ee_candidates);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CCD_AMD_H */

@@ -1,0 +1,282 @@
+"""TEST INFRASTRUCTURE ONLY -- the CPU oracle for the swept-primitive CCD path.
+
+A ctypes front end for ``oracle/liborc.so`` (``oracle/ccd_oracle.cpp``), the CPU restatement of
+evouga/collisiondetection's KDOPBroadPhase -> CTCDNarrowPhase -> CTCD path.  Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import this package, and only
+as the checker (or the timed CPU baseline).  The product (``collisiondetection_amd``) never imports it.
+
+Parity status of each function (what pins it) is listed in ``tests/test_oracle.py``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liborc.so")
+_REF_RPOLY_PATH = os.path.join(_HERE, "_ref", "librpoly_ref.so")
+REFERENCE_DIR = "/root/reference"
+
+_lib = None
+_ref = None
+
+# Stage counters reported by orc_narrowphase (order of the Stage enum in ccd_oracle.cpp).
+STAGES = ("vf_plane", "vf_dist", "ee_dist", "ee_par", "ee_bary", "ve_quad", "ve_quart", "vv",
+          "quad_root", "isolate")
+
+RECORD_DTYPE = np.dtype([("hit", np.int32), ("subtest", np.int32), ("window", np.int32),
+                         ("pad", np.int32), ("t", np.float64), ("margin", np.float64)])
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_lp = ctypes.POINTER(ctypes.c_int64)
+_up = ctypes.POINTER(ctypes.c_uint8)
+
+
+def build(force: bool = False) -> None:
+    """Compile liborc.so (and the reference rpoly checker when the reference tree is present)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "-s", "liborc.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    if os.path.isdir(REFERENCE_DIR) and (force or not os.path.exists(_REF_RPOLY_PATH)):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "-s", "ref"], check=True,
+                       stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_set_dot_order.argtypes = [ctypes.c_int]
+        L.orc_set_rootfinder.argtypes = [ctypes.c_void_p]
+        L.orc_aberth_roots.argtypes = [_dp, ctypes.c_int, _dp, _dp]
+        L.orc_aberth_roots.restype = ctypes.c_int
+        L.orc_find_roots.argtypes = [_dp, ctypes.c_int, _dp, _dp]
+        L.orc_find_roots.restype = ctypes.c_int
+        L.orc_kdop_axes.argtypes = [_dp]
+        L.orc_get_quad_roots.argtypes = [ctypes.c_double] * 3 + [_dp, _dp]
+        L.orc_get_quad_roots.restype = ctypes.c_int
+        L.orc_find_intervals.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_find_intervals.restype = ctypes.c_int
+        for nm in ("orc_vertex_face_ctcd", "orc_edge_edge_ctcd", "orc_vertex_edge_ctcd",
+                   "orc_vertex_vertex_ctcd"):
+            f = getattr(L, nm)
+            f.argtypes = [_dp, ctypes.c_double, _dp, _dp]
+            f.restype = ctypes.c_int
+        L.orc_leaf_dops.argtypes = [_ip, ctypes.c_int64, _lp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp]
+        L.orc_candidates.argtypes = [_ip, ctypes.c_int64, _lp, _dp, _dp, ctypes.c_double, _up, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int]
+        L.orc_candidates.restype = ctypes.c_void_p
+        L.orc_result_count.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_result_count.restype = ctypes.c_int64
+        L.orc_result_copy.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
+        L.orc_result_free.argtypes = [ctypes.c_void_p]
+        L.orc_narrowphase.argtypes = [_lp, _dp, _dp, _ip, _dp, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, _lp]
+        L.orc_stage_count.restype = ctypes.c_int
+        assert L.orc_stage_count() == len(STAGES)
+        _lib = L
+    return _lib
+
+
+def ref_rpoly_available() -> bool:
+    return os.path.exists(_REF_RPOLY_PATH)
+
+
+def _ref_lib():
+    global _ref
+    if _ref is None:
+        if not ref_rpoly_available():
+            raise FileNotFoundError(_REF_RPOLY_PATH)
+        _ref = ctypes.CDLL(_REF_RPOLY_PATH)
+        _ref.ref_rpoly.argtypes = [_dp, ctypes.c_int, _dp, _dp]
+        _ref.ref_rpoly.restype = ctypes.c_int
+    return _ref
+
+
+def use_rootfinder(which: str) -> None:
+    """Select the oracle's root finder: 'aberth' (built in) or 'rpoly' (the reference's, from _ref)."""
+    if which == "aberth":
+        lib().orc_set_rootfinder(None)
+    elif which == "rpoly":
+        fn = ctypes.cast(_ref_lib().ref_rpoly, ctypes.c_void_p).value
+        lib().orc_set_rootfinder(fn)
+    else:
+        raise ValueError(which)
+
+
+def set_dot_order(order: int) -> None:
+    lib().orc_set_dot_order(int(order))
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _l(a):
+    return a.ctypes.data_as(_lp)
+
+
+# ------------------------------------------------------------------------------------------------
+# Roots and intervals
+# ------------------------------------------------------------------------------------------------
+def aberth_roots(op):
+    op = np.ascontiguousarray(op, dtype=np.float64)
+    n = len(op) - 1
+    zr = np.zeros(8)
+    zi = np.zeros(8)
+    k = lib().orc_aberth_roots(_d(op), n, _d(zr), _d(zi))
+    return zr[:max(k, 0)], zi[:max(k, 0)], k
+
+
+def ref_rpoly(op):
+    op = np.array(op, dtype=np.float64)
+    n = len(op) - 1
+    zr = np.zeros(8)
+    zi = np.zeros(8)
+    k = _ref_lib().ref_rpoly(_d(op), n, _d(zr), _d(zi))
+    return zr[:max(k, 0)], zi[:max(k, 0)], k
+
+
+def get_quad_roots(a, b, c):
+    t0 = ctypes.c_double(0.0)
+    t1 = ctypes.c_double(0.0)
+    k = lib().orc_get_quad_roots(a, b, c, ctypes.byref(t0), ctypes.byref(t1))
+    return k, t0.value, t1.value
+
+
+def find_intervals(op, pos: bool):
+    """CTCD::findIntervals on a copy of op -> (list of (l,u), normalised op, margin)."""
+    op = np.ascontiguousarray(op, dtype=np.float64)
+    n = len(op) - 1
+    out = np.zeros(32)
+    nop = np.zeros(n + 1)
+    m = ctypes.c_double(0.0)
+    k = lib().orc_find_intervals(_d(op), n, int(pos), _d(out), 16, _d(nop), ctypes.byref(m))
+    return [(out[2 * i], out[2 * i + 1]) for i in range(k)], nop, m.value
+
+
+# ------------------------------------------------------------------------------------------------
+# Primitives: q holds the start positions then the end positions, as in the CTCD statics.
+# ------------------------------------------------------------------------------------------------
+def _prim(name, q, eta):
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+    t = ctypes.c_double(-1.0)
+    m = ctypes.c_double(0.0)
+    r = getattr(lib(), name)(_d(q), float(eta), ctypes.byref(t), ctypes.byref(m))
+    return bool(r), (t.value if r else None), m.value
+
+
+def vertex_face_ctcd(q, eta):
+    return _prim("orc_vertex_face_ctcd", q, eta)
+
+
+def edge_edge_ctcd(q, eta):
+    return _prim("orc_edge_edge_ctcd", q, eta)
+
+
+def vertex_edge_ctcd(q, eta):
+    return _prim("orc_vertex_edge_ctcd", q, eta)
+
+
+def vertex_vertex_ctcd(q, eta):
+    return _prim("orc_vertex_vertex_ctcd", q, eta)
+
+
+# ------------------------------------------------------------------------------------------------
+# Histories
+# ------------------------------------------------------------------------------------------------
+class Hist:
+    """CSR history: vertex v owns entries off[v]..off[v+1]-1 (times ht, positions hp[3*])."""
+
+    def __init__(self, off, ht, hp):
+        self.off = np.ascontiguousarray(off, dtype=np.int64)
+        self.ht = np.ascontiguousarray(ht, dtype=np.float64)
+        self.hp = np.ascontiguousarray(hp, dtype=np.float64).reshape(-1)
+
+    @staticmethod
+    def two_entry(qstart, qend):
+        """History(qstart) + finishHistory(qend) (History.cpp:8-39): entries at t=0 and t=1."""
+        qs = np.asarray(qstart, dtype=np.float64).reshape(-1, 3)
+        qe = np.asarray(qend, dtype=np.float64).reshape(-1, 3)
+        V = qs.shape[0]
+        off = np.arange(0, 2 * V + 1, 2, dtype=np.int64)
+        ht = np.tile(np.array([0.0, 1.0]), V)
+        hp = np.stack([qs, qe], axis=1).reshape(-1)
+        return Hist(off, ht, hp)
+
+    @property
+    def nverts(self):
+        return len(self.off) - 1
+
+
+def kdop_axes():
+    out = np.zeros(39)
+    lib().orc_kdop_axes(_d(out))
+    return out.reshape(13, 3)
+
+
+def leaf_dops(faces, hist: Hist, eta, aabb=False):
+    faces = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+    F = len(faces) // 3
+    mn = np.zeros((F, 13))
+    mx = np.zeros((F, 13))
+    lib().orc_leaf_dops(_i(faces), F, _l(hist.off), _d(hist.ht), _d(hist.hp), float(eta), int(aabb), _d(mn), _d(mx))
+    return mn, mx
+
+
+METHODS = {"brute": 0, "tree_set": 1, "tree": 2}
+
+
+def candidates(faces, hist: Hist, eta, fixed=None, method="brute", aabb=False, want_pairs=False):
+    """Candidate stencil sets (sorted, unique) as int32 arrays of shape (n, 4): (vf, ee[, pairs])."""
+    faces = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+    F = len(faces) // 3
+    fx = None
+    if fixed is not None:
+        fx_arr = np.zeros(hist.nverts, dtype=np.uint8)
+        fx_arr[np.asarray(list(fixed), dtype=np.int64)] = 1
+        fx = fx_arr.ctypes.data_as(_up)
+    L = lib()
+    h = L.orc_candidates(_i(faces), F, _l(hist.off), _d(hist.ht), _d(hist.hp), float(eta), fx,
+                         METHODS[method], int(aabb), int(want_pairs))
+    try:
+        out = []
+        for which in (0, 1):
+            n = L.orc_result_count(h, which)
+            a = np.zeros((n, 4), dtype=np.int32)
+            if n:
+                L.orc_result_copy(h, which, _i(a))
+            out.append(a)
+        if want_pairs:
+            n = L.orc_result_count(h, 2)
+            p = np.zeros((n, 2), dtype=np.int32)
+            if n:
+                L.orc_result_copy(h, 2, _i(p))
+            out.append(p)
+    finally:
+        L.orc_result_free(h)
+    return tuple(out)
+
+
+def narrowphase(hist: Hist, keys, eta, is_ee: bool, with_counts=False):
+    """CTCDNarrowPhase per stencil -> structured records (hit, subtest, window, t, margin)."""
+    keys = np.ascontiguousarray(keys, dtype=np.int32).reshape(-1, 4)
+    n = keys.shape[0]
+    eta_arr = np.ascontiguousarray(np.broadcast_to(np.asarray(eta, dtype=np.float64), (n,)))
+    rec = np.zeros(n, dtype=RECORD_DTYPE)
+    counts = np.zeros(len(STAGES), dtype=np.int64)
+    if n:
+        lib().orc_narrowphase(_l(hist.off), _d(hist.ht), _d(hist.hp), _i(keys), _d(eta_arr), n, int(is_ee),
+                              rec.ctypes.data_as(ctypes.c_void_p), _l(counts))
+    if with_counts:
+        return rec, dict(zip(STAGES, counts.tolist()))
+    return rec

@@ -1,0 +1,1375 @@
+// TEST INFRASTRUCTURE ONLY -- the CPU oracle.
+//
+// A CPU restatement of evouga/collisiondetection's swept-primitive CCD path, written from a reading of
+// the reference's behaviour.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load this library, and only as the checker (or the timed CPU baseline); the product never does.
+//
+// Every function names the reference file:line it follows.  Floating-point operations are written in
+// the reference's operation order and must be compiled with -ffp-contract=off (see oracle/Makefile):
+//   * Eigen's Vector3d::dot/squaredNorm are summed ((x0*y0 + x1*y1) + x2*y2), the order Eigen >= 3.3
+//     produces for a 3-vector with SSE2 packets (the reference Makefile builds -O2 with no -march);
+//     orc_set_dot_order(1) switches to x0*y0 + (x1*y1 + x2*y2) (Eigen 3.2's unvectorised unroller).
+//   * Eigen's cross() is (a1*b2 - a2*b1, a2*b0 - a0*b2, a0*b1 - a1*b0).
+//   * std::max(a,b) is (a < b) ? b : a and std::min(a,b) is (b < a) ? b : a, argument order kept.
+//
+// The reference's polynomial root finder (src/rpoly.h, Jenkins-Traub) is not restated.  By default the
+// oracle finds all complex roots with its own Aberth-Ehrlich iteration in long double and, like rpoly,
+// hands back the real parts of all of them.  orc_set_rootfinder() substitutes any other function with
+// rpoly's signature -- the tests use it to plug in the reference's own rpoly built in place under
+// oracle/_ref/ from /root/reference/src/rpoly.h.
+//
+// Besides the reference's results, the oracle reports for every primitive test a decision margin: the
+// smallest distance between two quantities whose comparison decided the result (a root and 0 or 1, two
+// roots, two interval endpoints from different lists).  A result with a margin below 1e-12 is
+// "borderline": a different but equally accurate root finder may flip it.
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <set>
+#include <vector>
+
+namespace {
+
+// ----------------------------------------------------------------------------------------------
+// Globals: dot order and root finder.
+// ----------------------------------------------------------------------------------------------
+int g_dot_order = 0;
+typedef int (*rootfinder_fn)(double *op, int degree, double *zeror, double *zeroi);
+rootfinder_fn g_rootfinder = nullptr;   // nullptr = built-in Aberth
+
+// std::max / std::min with the reference's argument order semantics.
+inline double smax(double a, double b) { return (a < b) ? b : a; }
+inline double smin(double a, double b) { return (b < a) ? b : a; }
+
+struct V3 {
+    double x, y, z;
+};
+inline V3 v3(double x, double y, double z) { return V3{x, y, z}; }
+inline V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator*(double s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
+inline double dot(V3 a, V3 b)
+{
+    if (g_dot_order == 0) return (a.x * b.x + a.y * b.y) + a.z * b.z;
+    return a.x * b.x + (a.y * b.y + a.z * b.z);
+}
+inline V3 cross(V3 a, V3 b)
+{
+    return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline double norm(V3 a) { return std::sqrt(dot(a, a)); }
+
+// ----------------------------------------------------------------------------------------------
+// Work counters (for the roofline's algorithmic flop count) and the decision margin.
+// ----------------------------------------------------------------------------------------------
+enum Stage {
+    ST_VF_PLANE = 0,   // one edge-side cubic of vertexFaceCTCD (CTCD.cpp:433-464)
+    ST_VF_DIST,        // the coplanar-distance sextic of vertexFaceCTCD (CTCD.cpp:466-475)
+    ST_EE_DIST,        // the distance sextic of edgeEdgeCTCD (CTCD.cpp:275-288)
+    ST_EE_PAR,         // one parallel-edge check of a raw coplanar interval (CTCD.cpp:294-308)
+    ST_EE_BARY,        // one barycentric quartic of edgeEdgeCTCD (CTCD.cpp:313-350)
+    ST_VE_QUAD,        // one side quadratic of vertexEdgeCTCD (CTCD.cpp:529-555)
+    ST_VE_QUART,       // the colinear-distance quartic of vertexEdgeCTCD (CTCD.cpp:557-573)
+    ST_VV,             // vertexVertexCTCD's quadratic (CTCD.cpp:604-639)
+    ST_QUAD_ROOT,      // one getQuadRoots call that found roots (CTCD.cpp:38-56)
+    ST_ISOLATE,        // one polynomial of degree > 2 handed to the root finder (CTCD.cpp:135-142)
+    ST_COUNT
+};
+
+struct Ctx {
+    double margin = std::numeric_limits<double>::infinity();
+    bool complex_split = false;   // a complex root's real part split [0,1] in the last find_intervals
+    int64_t counts[ST_COUNT] = {0};
+    void gap(double a, double b)
+    {
+        double d = std::fabs(a - b);
+        if (d < margin) margin = d;
+    }
+    // A comparison between two interval endpoints.  Two endpoints that are both exactly 0 or both
+    // exactly 1 come from clamping (or from an exact zero root) and compare the same under any root
+    // finder, so they do not count.
+    // A comparison counts when one end is a root-finder root and the other any real root; a root
+    // against a clamped end (0 or 1) is counted where the root is found.
+    void endpoint_gap(double a, int ak, double b, int bk)
+    {
+        if ((ak == 2 && (bk == 1 || bk == 2)) || (bk == 2 && ak == 1)) gap(a, b);
+    }
+};
+
+// ----------------------------------------------------------------------------------------------
+// Time intervals: include/CTCD.h:7-26, src/CTCD.cpp:10-36
+// ----------------------------------------------------------------------------------------------
+struct TI {
+    double l, u;
+    // Provenance of each end, for the decision margin only: 0 = clamp or constant, 1 = a root from
+    // a closed form (degree <= 2, identical under any root finder), 2 = a real root found by the
+    // root finder (degree > 2), 3 = the real part of a complex root.
+    int lk = 0, uk = 0;
+};
+inline TI make_ti(double tl, double tu)   // CTCD.h:9-14
+{
+    TI r;
+    r.l = tl;
+    r.u = tu;
+    if (r.l > r.u) std::swap(r.l, r.u);
+    r.l = smax(r.l, 0.0);
+    r.u = smin(r.u, 1.0);
+    return r;
+}
+inline bool ti_overlap(const TI &a, const TI &b) { return !(a.l > b.u || b.l > a.u); }   // CTCD.cpp:10-13
+bool ti_overlap_all(const TI *iv, int n, Ctx *ctx)   // CTCD.cpp:15-25 (pairwise)
+{
+    for (int i = 0; i < n; i++)
+        for (int j = i + 1; j < n; j++) {
+            if (ctx) {
+                ctx->endpoint_gap(iv[i].l, iv[i].lk, iv[j].u, iv[j].uk);
+                ctx->endpoint_gap(iv[j].l, iv[j].lk, iv[i].u, iv[i].uk);
+            }
+            if (!ti_overlap(iv[i], iv[j])) return false;
+        }
+    return true;
+}
+TI ti_intersect(const TI *iv, int n)   // CTCD.cpp:27-36
+{
+    TI isect = make_ti(0.0, 1.0);
+    for (int i = 0; i < n; i++) {
+        isect.l = smax(iv[i].l, isect.l);
+        isect.u = smin(iv[i].u, isect.u);
+    }
+    return isect;
+}
+
+// ----------------------------------------------------------------------------------------------
+// Root finding.
+// ----------------------------------------------------------------------------------------------
+
+// Built-in: Aberth-Ehrlich simultaneous iteration (O. Aberth 1973, L. W. Ehrlich 1967) on the
+// polynomial with descending coefficients op[0..degree], in long double complex arithmetic.  Like
+// rpoly it returns the real parts of all roots and strips exact zeros at the origin first, so a root
+// that is exactly 0 is reported as exactly 0.0.
+int aberth_roots(const double *op, int degree, double *zeror, double *zeroi)
+{
+    typedef long double LD;
+    typedef std::complex<LD> C;
+    if (op[0] == 0.0) return -1;
+    int n = degree;
+    int nz = 0;
+    while (n > 0 && op[n] == 0.0) {
+        zeror[nz] = 0.0;
+        zeroi[nz] = 0.0;
+        nz++;
+        n--;
+    }
+    if (n < 1) return nz;
+    LD a[8];
+    for (int i = 0; i <= n; i++) a[i] = (LD)op[i] / (LD)op[0];
+    if (n == 1) {
+        zeror[nz] = (double)(-a[1]);
+        zeroi[nz] = 0.0;
+        return nz + 1;
+    }
+    // Initial guesses on a circle whose radius is the Fujiwara bound, rotated off the real axis.
+    LD rad = 0;
+    for (int i = 1; i <= n; i++) {
+        LD c = std::fabs(a[i]);
+        if (i == n) c /= 2;
+        LD r = std::pow(c, (LD)1.0 / (LD)i);
+        if (r > rad) rad = r;
+    }
+    rad *= 2;
+    if (!(rad > 0)) rad = 1;
+    C z[8];
+    const LD two_pi = 6.283185307179586476925286766559L;
+    for (int k = 0; k < n; k++) z[k] = std::polar(rad, two_pi * k / n + 0.4L);
+    bool done[8] = {false};
+    LD lastw[8];
+    for (int k = 0; k < n; k++) lastw[k] = std::numeric_limits<LD>::infinity();
+    for (int it = 0; it < 500; it++) {
+        bool all = true;
+        for (int k = 0; k < n; k++) {
+            if (done[k]) continue;
+            C p = a[0], dp = 0;
+            for (int i = 1; i <= n; i++) {
+                dp = dp * z[k] + p;
+                p = p * z[k] + a[i];
+            }
+            if (p == C(0)) {
+                done[k] = true;
+                continue;
+            }
+            C ratio = p / dp;
+            C s = 0;
+            for (int j = 0; j < n; j++)
+                if (j != k) s += C(1) / (z[k] - z[j]);
+            C w = ratio / (C(1) - ratio * s);
+            if (!std::isfinite(std::real(w)) || !std::isfinite(std::imag(w))) {
+                w = ratio;
+                if (!std::isfinite(std::real(w)) || !std::isfinite(std::imag(w))) w = C(1e-9L, 1e-9L);
+            }
+            z[k] -= w;
+            LD aw = std::abs(w), sc = std::max(std::abs(z[k]), (LD)1);
+            // Converged at working precision, or stalled at the rounding-noise floor (a clustered
+            // or multiple root, whose correction stops shrinking).
+            if (aw <= 1e-18L * sc || (it > 30 && aw < 1e-6L * sc && aw >= 0.5L * lastw[k]))
+                done[k] = true;
+            else
+                all = false;
+            lastw[k] = aw;
+        }
+        if (all) break;
+    }
+    // Newton polish of each root on the polynomial itself (damps the linear convergence that
+    // Aberth shows at multiple roots only a little, but sharpens simple roots to long double).
+    for (int k = 0; k < n; k++) {
+        for (int it = 0; it < 3; it++) {
+            C p = a[0], dp = 0;
+            for (int i = 1; i <= n; i++) {
+                dp = dp * z[k] + p;
+                p = p * z[k] + a[i];
+            }
+            if (dp == C(0)) break;
+            C w = p / dp;
+            if (std::abs(w) > 1e-12L * std::max(std::abs(z[k]), (LD)1)) break;
+            z[k] -= w;
+        }
+        zeror[nz + k] = (double)std::real(z[k]);
+        zeroi[nz + k] = (double)std::imag(z[k]);
+    }
+    return nz + n;
+}
+
+int find_roots(double *op, int degree, double *zr, double *zi)
+{
+    if (g_rootfinder) {
+        double tmp[8];
+        for (int i = 0; i <= degree; i++) tmp[i] = op[i];
+        return g_rootfinder(tmp, degree, zr, zi);
+    }
+    return aberth_roots(op, degree, zr, zi);
+}
+
+// ----------------------------------------------------------------------------------------------
+// Sign intervals of a polynomial: src/CTCD.cpp:38-177
+// ----------------------------------------------------------------------------------------------
+
+// CTCD.cpp:38-56
+int get_quad_roots(double a, double b, double c, double &t0, double &t1)
+{
+    int roots = 0;
+    int sign = 1;
+    if (b < 0) sign = -1;
+    double D = b * b - 4 * a * c;
+    if (D >= 0) {
+        roots = 2;
+        double q = -0.5 * (b + sign * std::sqrt(D));
+        t0 = q / a;
+        t1 = c / q;
+        if (t0 > t1) std::swap(t1, t0);
+    }
+    return roots;
+}
+
+// CTCD.cpp:59-79
+void check_interval(double t1, double t2, const double *op, int degree, std::vector<TI> &out, bool pos,
+                    int r1 = 0, int r2 = 0)
+{
+    double c1 = smax(0.0, t1);
+    double c2 = smax(0.0, t2);
+    c1 = smin(1.0, c1);
+    c2 = smin(1.0, c2);
+    if (c1 != t1) r1 = 0;   // a clamped end is no longer a root
+    if (c2 != t2) r2 = 0;
+    t1 = c1;
+    t2 = c2;
+    double tmid = (t2 + t1) / 2;
+    double f = op[0];
+    for (int i = 1; i <= degree; i++) {
+        f *= tmid;
+        f += op[i];
+    }
+    if ((pos && f >= 0) || (!pos && f <= 0)) {
+        TI ti = make_ti(t1, t2);
+        ti.lk = (t1 > t2) ? r2 : r1;
+        ti.uk = (t1 > t2) ? r1 : r2;
+        out.push_back(ti);
+    }
+}
+
+// CTCD.cpp:81-94
+bool could_have_roots(const double *op, int degree, bool pos)
+{
+    double result = 0;
+    if ((pos && op[0] > 0) || (!pos && op[0] < 0)) result = op[0];
+    for (int i = 1; i < degree; i++) {
+        result *= 1.0;
+        if ((pos && op[i] > 0) || (!pos && op[i] < 0)) result += op[i];
+    }
+    result *= 1.0;
+    result += op[degree];
+    return !((pos && result < 0) || (!pos && result > 0));
+}
+
+// CTCD.cpp:98-177.  op is modified in place (normalised and degree-reduced) as in the reference.
+void find_intervals(double *op, int n, std::vector<TI> &out, bool pos, Ctx *ctx)
+{
+    int roots = 0;
+    int reduced = n;
+    if (n > 6) return;
+    double time[8];
+    double zeroi[8];
+    double maxval = 0;
+    for (int i = 0; i <= n; i++) maxval = smax(maxval, std::fabs(op[i]));
+    if (maxval != 0)
+        for (int i = 0; i <= n; i++) op[i] /= maxval;
+    for (int i = 0; i < n; i++) {
+        if (op[i] == 0)
+            reduced--;
+        else
+            break;
+    }
+    if (reduced < n)
+        for (int i = 0; i <= reduced; i++) op[i] = op[i + n - reduced];
+
+    if (reduced > 2) {
+        if (!could_have_roots(op, reduced, pos)) return;
+        if (ctx) ctx->counts[ST_ISOLATE]++;
+        roots = find_roots(op, reduced, time, zeroi);
+    } else if (reduced == 2) {
+        roots = get_quad_roots(op[0], op[1], op[2], time[0], time[1]);
+        if (ctx && roots > 0) ctx->counts[ST_QUAD_ROOT]++;
+    } else if (reduced == 1) {
+        time[0] = -op[1] / op[0];
+        roots = 1;
+    } else {
+        if ((!pos && op[0] <= 0) || (pos && op[0] >= 0)) out.push_back(make_ti(0, 1.0));
+        return;
+    }
+
+    if (roots > 0) {
+        double zr_unsorted[8];
+        for (int i = 0; i < roots; i++) zr_unsorted[i] = time[i];
+        if (reduced <= 2)
+            for (int i = 0; i < roots; i++) zeroi[i] = 0.0;
+        std::sort(time, time + roots);
+        if (ctx && reduced > 2) {
+            // Margin: real roots (imaginary part below 1e-6) that split [0,1], against the interval
+            // ends and against each other.  The real part of a complex pair also splits the
+            // interval in the reference, but the polynomial keeps its sign across it, so the union
+            // of the intervals -- all that vertex-face and vertex-edge tests use -- does not depend
+            // on it.  Edge-edge tests classify each piece as parallel or not, which can depend on
+            // it; that case is flagged in edge_edge_ctcd through ctx->complex_split.
+            double rr[8];
+            int nr = 0;
+            for (int i = 0; i < roots; i++) {
+                bool real = std::fabs(zeroi[i]) < 1e-6;
+                if (real)
+                    rr[nr++] = zr_unsorted[i];
+                else if (zr_unsorted[i] >= 0.0 && zr_unsorted[i] <= 1.0)
+                    ctx->complex_split = true;
+            }
+            std::sort(rr, rr + nr);
+            for (int i = 0; i < nr; i++) {
+                if (rr[i] >= -1e-6 && rr[i] <= 1.0 + 1e-6) {
+                    if (!(rr[i] == 0.0 && op[reduced] == 0.0)) ctx->gap(rr[i], 0.0);
+                    ctx->gap(rr[i], 1.0);
+                    if (i + 1 < nr) ctx->gap(rr[i], rr[i + 1]);
+                }
+            }
+        }
+        int isreal[8];
+        for (int i = 0; i < roots; i++) {
+            isreal[i] = reduced > 2 ? 2 : 1;
+            for (int j = 0; j < roots; j++)
+                if (zr_unsorted[j] == time[i] && !(std::fabs(zeroi[j]) < 1e-6)) isreal[i] = 3;
+        }
+        if (time[0] >= 0) check_interval(0, time[0], op, reduced, out, pos, 0, isreal[0]);
+        for (int i = 0; i < roots - 1; i++) {
+            if (!((time[i] < 0 && time[i + 1] < 0) || (time[i] > 1.0 && time[i + 1] > 1.0)))
+                check_interval(time[i], time[i + 1], op, reduced, out, pos, isreal[i], isreal[i + 1]);
+        }
+        if (time[roots - 1] <= 1.0)
+            check_interval(time[roots - 1], 1.0, op, reduced, out, pos, isreal[roots - 1], 0);
+    } else {
+        check_interval(0.0, 1.0, op, reduced, out, pos);
+    }
+}
+
+// CTCD.cpp:179-213
+void barycentric_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, std::vector<TI> &out, Ctx *ctx)
+{
+    if (ctx) ctx->counts[ST_EE_BARY]++;
+    double A = dot(x10, x10);
+    double B = 2 * dot(x10, v10);
+    double C = dot(v10, v10);
+    double D = dot(x20, x10);
+    double E = dot(x20, v10) + dot(v20, x10);
+    double F = dot(v20, v10);
+    double G = dot(x30, x20);
+    double H = dot(x30, v20) + dot(v30, x20);
+    double I = dot(v30, v20);
+    double J = dot(x30, x10);
+    double K = dot(x30, v10) + dot(v30, x10);
+    double L = dot(v30, v10);
+    double op[5];
+    op[0] = F * L - C * I;
+    op[1] = F * K + E * L - C * H - B * I;
+    op[2] = F * J + D * L + E * K - C * G - A * I - B * H;
+    op[3] = D * K + E * J - A * H - B * G;
+    op[4] = D * J - A * G;
+    find_intervals(op, 4, out, true, ctx);
+}
+
+// CTCD.cpp:215-229
+void plane_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, std::vector<TI> &out, Ctx *ctx)
+{
+    if (ctx) ctx->counts[ST_VF_PLANE]++;
+    double op[4];
+    op[0] = dot(v10, cross(v20, v30));
+    op[1] = dot(x10, cross(v20, v30)) + dot(v10, cross(x20, v30)) + dot(v10, cross(v20, x30));
+    op[2] = dot(x10, cross(x20, v30)) + dot(x10, cross(v20, x30)) + dot(v10, cross(x20, x30));
+    op[3] = dot(x10, cross(x20, x30));
+    find_intervals(op, 3, out, true, ctx);
+}
+
+// CTCD.cpp:231-257
+void distance_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, double minD, std::vector<TI> &out,
+                   Ctx *ctx)
+{
+    double A = dot(v10, cross(v20, v30));
+    double B = dot(x10, cross(v20, v30)) + dot(v10, cross(x20, v30)) + dot(v10, cross(v20, x30));
+    double C = dot(x10, cross(x20, v30)) + dot(x10, cross(v20, x30)) + dot(v10, cross(x20, x30));
+    double D = dot(x10, cross(x20, x30));
+    V3 E = cross(x20, x30);
+    V3 F = cross(x20, v30) + cross(v20, x30);
+    V3 G = cross(v20, v30);
+    double op[7];
+    op[0] = A * A;
+    op[1] = 2 * A * B;
+    op[2] = B * B + 2 * A * C - dot(G, G) * minD;
+    op[3] = 2 * A * D + 2 * B * C - 2 * dot(G, F) * minD;
+    op[4] = 2 * B * D + C * C - (2 * dot(G, E) + dot(F, F)) * minD;
+    op[5] = 2 * C * D - 2 * dot(F, E) * minD;
+    op[6] = D * D - dot(E, E) * minD;
+    find_intervals(op, 6, out, false, ctx);
+}
+
+// ----------------------------------------------------------------------------------------------
+// Primitive tests: src/CTCD.cpp:259-692
+// ----------------------------------------------------------------------------------------------
+
+// CTCD.cpp:259-411.  Edges (q0,p0) and (q1,p1).
+bool edge_edge_ctcd(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3 p1e, double eta, double &t,
+                    Ctx *ctx)
+{
+    double minD = eta * eta;
+    std::vector<TI> rawcoplane, a0, a1, b0, b1;
+    V3 x10 = p0s - p1s;
+    V3 x20 = p0s - q0s;
+    V3 x30 = p1s - q1s;
+    V3 vp0 = p0e - p0s;
+    V3 vp1 = p1e - p1s;
+    V3 vq0 = q0e - q0s;
+    V3 vq1 = q1e - q1s;
+    V3 v10 = vp0 - vp1;
+    V3 v20 = vp0 - vq0;
+    V3 v30 = vp1 - vq1;
+    if (ctx) {
+        ctx->counts[ST_EE_DIST]++;
+        ctx->complex_split = false;
+    }
+    distance_poly(x10, x20, x30, v10, v20, v30, minD, rawcoplane, ctx);
+    bool raw_complex_split = ctx && ctx->complex_split;
+
+    std::vector<TI> coplane, parallel;
+    for (size_t i = 0; i < rawcoplane.size(); i++) {
+        if (ctx) ctx->counts[ST_EE_PAR]++;
+        double midt = (rawcoplane[i].u + rawcoplane[i].l) / 2;
+        x10 = (q0s - p0s) + midt * (vq0 - vp0);
+        x20 = (q1s - p1s) + midt * (vq1 - vp1);
+        double cn = norm(cross(x10, x20));
+        if (ctx) ctx->gap(cn / 1e-8, 1.0);   // relative distance to the absolute 1e-8 threshold
+        if (cn < 1e-8)
+            parallel.push_back(rawcoplane[i]);
+        else
+            coplane.push_back(rawcoplane[i]);
+    }
+    if (coplane.empty()) return false;
+    // Pieces split at a complex root's real part are classified parallel or not one by one, so with
+    // a parallel piece the result can depend on those split points: flag it.
+    if (ctx && raw_complex_split && !parallel.empty()) ctx->margin = 0.0;
+
+    x10 = p1s - q1s;
+    v10 = vp1 - vq1;
+    x20 = p0s - q0s;
+    v20 = vp0 - vq0;
+    x30 = q0s - q1s;
+    v30 = vq0 - vq1;
+    barycentric_poly(x10, x20, x30, v10, v20, v30, a0, ctx);
+    if (a0.empty()) return false;
+
+    x20 = q0s - p0s;
+    v20 = vq0 - vp0;
+    x30 = p0s - q1s;
+    v30 = vp0 - vq1;
+    barycentric_poly(x10, x20, x30, v10, v20, v30, a1, ctx);
+    if (a1.empty()) return false;
+
+    x10 = p0s - q0s;
+    v10 = vp0 - vq0;
+    x20 = p1s - q1s;
+    v20 = vp1 - vq1;
+    x30 = q1s - q0s;
+    v30 = vq1 - vq0;
+    barycentric_poly(x10, x20, x30, v10, v20, v30, b0, ctx);
+    if (b0.empty()) return false;
+
+    x20 = q1s - p1s;
+    v20 = vq1 - vp1;
+    x30 = p1s - q0s;
+    v30 = vp1 - vq0;
+    barycentric_poly(x10, x20, x30, v10, v20, v30, b1, ctx);
+    if (b1.empty()) return false;
+
+    bool col = false;
+    double mint = 1.0;
+    TI iv[5];
+    for (size_t i = 0; i < coplane.size(); i++)
+        for (size_t j = 0; j < a0.size(); j++)
+            for (size_t k = 0; k < a1.size(); k++)
+                for (size_t l = 0; l < b0.size(); l++)
+                    for (size_t m = 0; m < b1.size(); m++) {
+                        iv[0] = coplane[i];
+                        iv[1] = a0[j];
+                        iv[2] = a1[k];
+                        iv[3] = b0[l];
+                        iv[4] = b1[m];
+                        if (ti_overlap_all(iv, 5, ctx)) {
+                            TI isect = ti_intersect(iv, 5);
+                            bool skip = false;
+                            for (size_t p = 0; p < parallel.size(); p++) {
+                                TI pc[2] = {isect, parallel[p]};
+                                if (ti_overlap_all(pc, 2, ctx)) {
+                                    skip = true;
+                                    break;
+                                }
+                            }
+                            if (!skip) {
+                                mint = smin(mint, isect.l);
+                                col = true;
+                            }
+                        }
+                    }
+    if (col) {
+        t = mint;
+        return true;
+    }
+    return false;
+}
+
+// CTCD.cpp:413-508.  Vertex q0 against face (q1,q2,q3).
+bool vertex_face_ctcd(V3 q0s, V3 q1s, V3 q2s, V3 q3s, V3 q0e, V3 q1e, V3 q2e, V3 q3e, double eta, double &t,
+                      Ctx *ctx)
+{
+    double minD = eta * eta;
+    V3 v0 = q0e - q0s;
+    V3 v1 = q1e - q1s;
+    V3 v2 = q2e - q2s;
+    V3 v3_ = q3e - q3s;
+    std::vector<TI> coplane, e1, e2, e3;
+
+    V3 x10 = q0s - q1s;
+    V3 v10 = v0 - v1;
+    V3 x20 = cross(q3s - q1s, q2s - q1s);
+    V3 v20 = cross(v3_ - v1, v2 - v1);
+    V3 x30 = q3s - q1s;
+    V3 v30 = v3_ - v1;
+    plane_poly(x10, x20, x30, v10, v20, v30, e1, ctx);
+    if (e1.empty()) return false;
+
+    x10 = q0s - q2s;
+    v10 = v0 - v2;
+    x20 = cross(q1s - q2s, q3s - q2s);
+    v20 = cross(v1 - v2, v3_ - v2);
+    x30 = q1s - q2s;
+    v30 = v1 - v2;
+    plane_poly(x10, x20, x30, v10, v20, v30, e2, ctx);
+    if (e2.empty()) return false;
+
+    x10 = q0s - q3s;
+    v10 = v0 - v3_;
+    x20 = cross(q2s - q3s, q1s - q3s);
+    v20 = cross(v2 - v3_, v1 - v3_);
+    x30 = q2s - q3s;
+    v30 = v2 - v3_;
+    plane_poly(x10, x20, x30, v10, v20, v30, e3, ctx);
+    if (e3.empty()) return false;
+
+    x10 = q0s - q1s;
+    x20 = q2s - q1s;
+    x30 = q3s - q1s;
+    v10 = v0 - v1;
+    v20 = v2 - v1;
+    v30 = v3_ - v1;
+    if (ctx) ctx->counts[ST_VF_DIST]++;
+    distance_poly(x10, x20, x30, v10, v20, v30, minD, coplane, ctx);
+    if (coplane.empty()) return false;
+
+    bool col = false;
+    double mint = 1.0;
+    TI iv[4];
+    for (size_t i = 0; i < coplane.size(); i++)
+        for (size_t j = 0; j < e1.size(); j++)
+            for (size_t k = 0; k < e2.size(); k++)
+                for (size_t l = 0; l < e3.size(); l++) {
+                    iv[0] = coplane[i];
+                    iv[1] = e1[j];
+                    iv[2] = e2[k];
+                    iv[3] = e3[l];
+                    if (ti_overlap_all(iv, 4, ctx)) {
+                        mint = smin(ti_intersect(iv, 4).l, mint);
+                        col = true;
+                    }
+                }
+    if (col) {
+        t = mint;
+        return true;
+    }
+    return false;
+}
+
+// CTCD.cpp:511-602.  Vertex q0 against edge (q1,q2).
+bool vertex_edge_ctcd(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double eta, double &t, Ctx *ctx)
+{
+    double op[5];
+    double minD = eta * eta;
+    V3 v0 = q0e - q0s;
+    V3 v1 = q1e - q1s;
+    V3 v2 = q2e - q2s;
+    std::vector<TI> colin, e1, e2;
+    V3 ab = q2s - q1s;
+    V3 ac = q0s - q1s;
+    V3 cb = q2s - q0s;
+    V3 vab = v2 - v1;
+    V3 vac = v0 - v1;
+    V3 vcb = v2 - v0;
+
+    if (ctx) ctx->counts[ST_VE_QUAD]++;
+    double c = dot(ab, ac);
+    double b = dot(ac, vab) + dot(ab, vac);
+    double a = dot(vab, vac);
+    op[0] = a;
+    op[1] = b;
+    op[2] = c;
+    find_intervals(op, 2, e1, true, ctx);
+    if (e1.empty()) return false;
+
+    if (ctx) ctx->counts[ST_VE_QUAD]++;
+    c = dot(ab, cb);
+    b = dot(cb, vab) + dot(ab, vcb);
+    a = dot(vab, vcb);
+    op[0] = a;
+    op[1] = b;
+    op[2] = c;
+    find_intervals(op, 2, e2, true, ctx);
+    if (e2.empty()) return false;
+
+    if (ctx) ctx->counts[ST_VE_QUART]++;
+    double A = dot(ab, ab);
+    double B = 2 * dot(ab, vab);
+    double C = dot(vab, vab);
+    double D = dot(ac, ac);
+    double E = 2 * dot(ac, vac);
+    double F = dot(vac, vac);
+    double G = dot(ac, ab);
+    double H = dot(vab, ac) + dot(vac, ab);
+    double I = dot(vab, vac);
+    op[4] = A * D - G * G - minD * A;
+    op[3] = B * D + A * E - 2 * G * H - minD * B;
+    op[2] = B * E + A * F + C * D - H * H - 2 * G * I - minD * C;
+    op[1] = B * F + C * E - 2 * H * I;
+    op[0] = C * F - I * I;
+    find_intervals(op, 4, colin, false, ctx);
+    if (colin.empty()) return false;
+
+    double mint = 1.0;
+    bool col = false;
+    TI iv[3];
+    for (size_t i = 0; i < colin.size(); i++)
+        for (size_t j = 0; j < e1.size(); j++)
+            for (size_t k = 0; k < e2.size(); k++) {
+                iv[0] = colin[i];
+                iv[1] = e1[j];
+                iv[2] = e2[k];
+                if (ti_overlap_all(iv, 3, ctx)) {
+                    mint = smin(ti_intersect(iv, 3).l, mint);
+                    col = true;
+                }
+            }
+    if (col) {
+        t = mint;
+        return true;
+    }
+    return false;
+}
+
+// CTCD.cpp:604-692
+bool vertex_vertex_ctcd(V3 q1s, V3 q2s, V3 q1e, V3 q2e, double eta, double &t, Ctx *ctx)
+{
+    if (ctx) ctx->counts[ST_VV]++;
+    int roots = 0;
+    double min_d = eta * eta;
+    double t1 = 0, t2 = 0;
+    V3 v1 = q1e - q1s;
+    V3 v2 = q2e - q2s;
+    double a = dot(v1, v1) + dot(v2, v2) - 2 * dot(v1, v2);
+    double b = 2 * (dot(v1, q1s) - dot(v2, q1s) - dot(v1, q2s) + dot(v2, q2s));
+    double c = dot(q1s, q1s) + dot(q2s, q2s) - 2 * dot(q1s, q2s) - min_d;
+    if (a != 0) {
+        roots = get_quad_roots(a, b, c, t1, t2);
+        if (ctx && roots > 0) ctx->counts[ST_QUAD_ROOT]++;
+    } else if (b != 0) {
+        t1 = -c / b;
+        roots = 1;
+    } else {
+        if (c <= 0) {
+            t = 0;
+            return true;
+        }
+        return false;
+    }
+    double op[3] = {a, b, c};
+    std::vector<TI> iv;
+    if (roots == 2) {
+        check_interval(0, t1, op, 2, iv, false);
+        if (!iv.empty()) {
+            t = 0;
+            return true;
+        }
+        check_interval(t1, t2, op, 2, iv, false);
+        if (!iv.empty()) {
+            t = t1;
+            return true;
+        }
+        check_interval(t2, 1.0, op, 2, iv, false);
+        if (!iv.empty()) {
+            t = t2;
+            return true;
+        }
+        return false;
+    } else if (roots == 1) {
+        check_interval(0, t1, op, 2, iv, false);
+        if (!iv.empty()) {
+            t = 0;
+            return true;
+        }
+        check_interval(t1, 1.0, op, 2, iv, false);
+        if (!iv.empty()) {
+            t = t1;
+            return true;
+        }
+        return false;
+    }
+    check_interval(0, 1.0, op, 2, iv, false);
+    if (!iv.empty()) {
+        t = 0;
+        return true;
+    }
+    return false;
+}
+
+inline V3 load3(const double *p) { return V3{p[0], p[1], p[2]}; }
+
+// ----------------------------------------------------------------------------------------------
+// Histories: src/History.cpp:8-39 (construction) and :98-140 (stitchCommonHistory).
+// A history is CSR: vertex v owns entries off[v] .. off[v+1]-1 with times ht[] and positions hp[3*].
+// ----------------------------------------------------------------------------------------------
+struct Hist {
+    const int64_t *off;
+    const double *ht;
+    const double *hp;
+};
+
+// History.cpp:98-140 for nv vertices.  Produces stitched times and nv positions per entry.
+void stitch(const Hist &h, const int *verts, int nv, std::vector<double> &times, std::vector<V3> &pos)
+{
+    times.clear();
+    pos.clear();
+    int64_t its[4];
+    for (int i = 0; i < nv; i++) its[i] = h.off[verts[i]];
+    double curtime = 0;
+    while (curtime <= 1.0) {
+        double newtime = std::numeric_limits<double>::infinity();
+        times.push_back(curtime);
+        for (int i = 0; i < nv; i++) {
+            int64_t end = h.off[verts[i] + 1];
+            int64_t next = its[i] + 1;
+            while (next != end && h.ht[next] <= curtime) {
+                ++next;
+                ++its[i];
+            }
+            V3 oldpos = load3(h.hp + 3 * its[i]);
+            if (next == end)
+                pos.push_back(oldpos);
+            else {
+                V3 newpos = load3(h.hp + 3 * next);
+                double dt = h.ht[next] - h.ht[its[i]];
+                double a = curtime - h.ht[its[i]];
+                double b = h.ht[next] - curtime;
+                V3 num = b * oldpos + a * newpos;
+                pos.push_back(V3{num.x / dt, num.y / dt, num.z / dt});
+                newtime = smin(newtime, h.ht[next]);
+            }
+        }
+        curtime = newtime;
+    }
+}
+
+// Per-stencil result record shared with the Python side (tests/ read it through ctypes).
+struct StencilResult {
+    int32_t hit;       // 1 if the reference's narrowphase reports the stencil
+    int32_t subtest;   // which primitive fired: 0 = VF/EE, 1.. = VE in reference order, then VV
+    int32_t window;    // stitched window index of the hit
+    int32_t pad;
+    double t;          // the firing primitive's time of impact (window-local, as the reference's t)
+    double margin;     // decision margin over every primitive executed for this stencil
+};
+
+// CTCDNarrowPhase.cpp:24-72
+bool check_vfs(const Hist &h, const int *key, double eta, StencilResult &res, Ctx *ctx)
+{
+    int verts[4] = {key[0], key[1], key[2], key[3]};
+    std::vector<double> times;
+    std::vector<V3> P;
+    stitch(h, verts, 4, times, P);
+    size_t ne = times.size();
+    for (size_t w = 0; w + 1 < ne; w++) {
+        const V3 *s = &P[4 * w];
+        const V3 *e = &P[4 * (w + 1)];
+        double t;
+        res.window = (int)w;
+        if (vertex_face_ctcd(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t, ctx)) {
+            res.subtest = 0;
+            res.t = t;
+            return true;
+        }
+        for (int edge = 0; edge < 3; edge++) {
+            int a = 1 + (edge % 3), b = 1 + ((edge + 1) % 3);
+            if (vertex_edge_ctcd(s[0], s[a], s[b], e[0], e[a], e[b], eta, t, ctx)) {
+                res.subtest = 1 + edge;
+                res.t = t;
+                return true;
+            }
+        }
+        for (int v = 0; v < 3; v++) {
+            if (vertex_vertex_ctcd(s[0], s[1 + v], e[0], e[1 + v], eta, t, ctx)) {
+                res.subtest = 4 + v;
+                res.t = t;
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+// CTCDNarrowPhase.cpp:74-135
+bool check_ees(const Hist &h, const int *key, double eta, StencilResult &res, Ctx *ctx)
+{
+    int verts[4] = {key[0], key[1], key[2], key[3]};
+    std::vector<double> times;
+    std::vector<V3> P;
+    stitch(h, verts, 4, times, P);
+    size_t ne = times.size();
+    static const int ve_idx[4][3] = {{0, 2, 3}, {1, 2, 3}, {2, 0, 1}, {3, 0, 1}};
+    static const int vv_idx[4][2] = {{0, 2}, {0, 3}, {1, 2}, {1, 3}};
+    for (size_t w = 0; w + 1 < ne; w++) {
+        const V3 *s = &P[4 * w];
+        const V3 *e = &P[4 * (w + 1)];
+        double t;
+        res.window = (int)w;
+        if (edge_edge_ctcd(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t, ctx)) {
+            res.subtest = 0;
+            res.t = t;
+            return true;
+        }
+        for (int k = 0; k < 4; k++) {
+            const int *q = ve_idx[k];
+            if (vertex_edge_ctcd(s[q[0]], s[q[1]], s[q[2]], e[q[0]], e[q[1]], e[q[2]], eta, t, ctx)) {
+                res.subtest = 1 + k;
+                res.t = t;
+                return true;
+            }
+        }
+        for (int k = 0; k < 4; k++) {
+            const int *q = vv_idx[k];
+            if (vertex_vertex_ctcd(s[q[0]], s[q[1]], e[q[0]], e[q[1]], eta, t, ctx)) {
+                res.subtest = 5 + k;
+                res.t = t;
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+// ----------------------------------------------------------------------------------------------
+// Stencil keys: src/Stencils.h:13-57 (VF) and :86-117 (EE)
+// ----------------------------------------------------------------------------------------------
+struct Key {
+    int32_t k[4];
+    bool operator<(const Key &o) const
+    {
+        for (int i = 0; i < 4; i++) {
+            if (k[i] < o.k[i]) return true;
+            if (k[i] > o.k[i]) return false;
+        }
+        return false;
+    }
+    bool operator==(const Key &o) const
+    {
+        return k[0] == o.k[0] && k[1] == o.k[1] && k[2] == o.k[2] && k[3] == o.k[3];
+    }
+};
+
+Key vf_key(int p, int a, int b, int c)   // Stencils.h:13-57: face vertices ascending
+{
+    int q[3] = {a, b, c};
+    std::sort(q, q + 3);
+    return Key{{p, q[0], q[1], q[2]}};
+}
+
+Key ee_key(int a, int b, int c, int d)   // Stencils.h:86-117
+{
+    int p0 = std::min(a, b), p1 = std::max(a, b);
+    int q0 = std::min(c, d), q1 = std::max(c, d);
+    if (p0 > q0) {
+        std::swap(p0, q0);
+        std::swap(p1, q1);
+    }
+    return Key{{p0, p1, q0, q1}};
+}
+
+// ----------------------------------------------------------------------------------------------
+// k-DOP broadphase: src/KDOPBroadPhase.{h,cpp}
+// ----------------------------------------------------------------------------------------------
+const int KAX = 13;
+struct Axes {
+    V3 a[KAX];
+};
+Axes make_axes()   // KDOPBroadPhase.cpp:10-32: 13 directions, each divided by its norm
+{
+    static const double raw[KAX][3] = {{1, 0, 0},  {0, 1, 0},  {0, 0, 1},  {1, 1, 0},  {1, -1, 0},
+                                       {0, 1, 1},  {0, 1, -1}, {1, 0, 1},  {1, 0, -1}, {1, 1, 1},
+                                       {1, 1, -1}, {1, -1, 1}, {1, -1, -1}};
+    Axes ax;
+    for (int i = 0; i < KAX; i++) {
+        V3 v = load3(raw[i]);
+        double n = norm(v);
+        ax.a[i] = V3{v.x / n, v.y / n, v.z / n};
+    }
+    return ax;
+}
+
+struct Dop {
+    double mn[KAX], mx[KAX];
+};
+
+// KDOPBroadPhase.cpp:43-74 (leaf bounds), generalised to `naxes` = 13 (k-DOP) or 3 (AABB,
+// AABBBroadPhase.cpp:9-40, which subtracts eta from the coordinate itself).
+void leaf_dops(const int32_t *faces, int64_t F, const Hist &h, double eta, bool aabb, std::vector<Dop> &out)
+{
+    Axes ax = make_axes();
+    int na = aabb ? 3 : KAX;
+    out.resize(F);
+    for (int64_t f = 0; f < F; f++) {
+        Dop &d = out[f];
+        for (int k = 0; k < KAX; k++) {
+            d.mn[k] = std::numeric_limits<double>::infinity();
+            d.mx[k] = -std::numeric_limits<double>::infinity();
+        }
+        for (int j = 0; j < 3; j++) {
+            int v = faces[3 * f + j];
+            for (int64_t e = h.off[v]; e < h.off[v + 1]; e++) {
+                V3 p = load3(h.hp + 3 * e);
+                for (int k = 0; k < na; k++) {
+                    double s = aabb ? (k == 0 ? p.x : (k == 1 ? p.y : p.z)) : dot(p, ax.a[k]);
+                    d.mn[k] = smin(s - eta, d.mn[k]);
+                    d.mx[k] = smax(s + eta, d.mx[k]);
+                }
+            }
+        }
+    }
+}
+
+inline bool dop_overlap(const Dop &a, const Dop &b, int na)   // KDOPBroadPhase.cpp:132-137
+{
+    for (int k = 0; k < na; k++)
+        if (a.mx[k] < b.mn[k] || b.mx[k] < a.mn[k]) return false;
+    return true;
+}
+
+inline bool neighboring(const int32_t *faces, int64_t f, int64_t g)   // Mesh.cpp:13-20
+{
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            if (faces[3 * f + i] == faces[3 * g + j]) return true;
+    return false;
+}
+
+struct CandSets {
+    std::vector<Key> vf, ee;
+};
+
+// KDOPBroadPhase.cpp:150-184: the 6 VF and 9 EE stencils of an overlapping, non-neighbouring leaf
+// pair, skipping those whose 4 vertices are all fixed.
+template <class Sink>
+void emit_pair(const int32_t *faces, int64_t f, int64_t g, const uint8_t *fixed, Sink &sink)
+{
+    const int32_t *F0 = faces + 3 * f;
+    const int32_t *G0 = faces + 3 * g;
+    auto fx = [&](int v) { return fixed && fixed[v]; };
+    for (int i = 0; i < 3; i++) {
+        bool lf = fx(F0[i]) && fx(G0[0]) && fx(G0[1]) && fx(G0[2]);
+        bool rf = fx(G0[i]) && fx(F0[0]) && fx(F0[1]) && fx(F0[2]);
+        if (!lf) sink.vf(vf_key(F0[i], G0[0], G0[1], G0[2]));
+        if (!rf) sink.vf(vf_key(G0[i], F0[0], F0[1], F0[2]));
+        for (int j = 0; j < 3; j++) {
+            int a = F0[i], b = F0[(i + 1) % 3], c = G0[j], d = G0[(j + 1) % 3];
+            bool ef = fx(a) && fx(b) && fx(c) && fx(d);
+            if (!ef) sink.ee(ee_key(a, b, c, d));
+        }
+    }
+}
+
+struct VecSink {
+    CandSets *cs;
+    void vf(const Key &k) { cs->vf.push_back(k); }
+    void ee(const Key &k) { cs->ee.push_back(k); }
+};
+struct SetSink {
+    std::set<Key> *vfs, *ees;
+    void vf(const Key &k) { vfs->insert(k); }
+    void ee(const Key &k) { ees->insert(k); }
+};
+
+void sort_unique(std::vector<Key> &v)
+{
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+}
+
+// Brute force over all face pairs f < g.  The reference's tree visits exactly the leaf pairs whose
+// bounds overlap (interior bounds are exact unions of child bounds, KDOPBroadPhase.cpp:85-98), in
+// both orders, and emission is symmetric, so this enumerates the same stencil set.
+void candidates_bruteforce(const int32_t *faces, int64_t F, const std::vector<Dop> &dops, int na,
+                           const uint8_t *fixed, CandSets &out, std::vector<int64_t> *pairs)
+{
+    VecSink sink{&out};
+    for (int64_t f = 0; f < F; f++)
+        for (int64_t g = f + 1; g < F; g++) {
+            if (!dop_overlap(dops[f], dops[g], na)) continue;
+            if (neighboring(faces, f, g)) continue;
+            if (pairs) {
+                pairs->push_back(f);
+                pairs->push_back(g);
+            }
+            emit_pair(faces, f, g, fixed, sink);
+        }
+    sort_unique(out.vf);
+    sort_unique(out.ee);
+}
+
+// The reference-structured tree: top-down split of the node's children at the median of the
+// longest-extent axis (KDOPBroadPhase.cpp:76-128; AABB variant AABBBroadPhase.cpp:42-89 with
+// >= ties), recursive dual traversal (:130-149) and std::set insertion (:150-184).
+struct Node {
+    Dop d;
+    int face;   // >= 0 for a leaf
+    int left, right;
+};
+
+struct Tree {
+    std::vector<Node> nodes;
+    int na;
+    int build(std::vector<int> &items)   // items: node indices (leaves) to group
+    {
+        int n = (int)items.size();
+        if (n == 1) return items[0];
+        Node node;
+        node.face = -1;
+        for (int k = 0; k < KAX; k++) {
+            node.d.mn[k] = std::numeric_limits<double>::infinity();
+            node.d.mx[k] = -std::numeric_limits<double>::infinity();
+        }
+        for (int it : items)
+            for (int k = 0; k < na; k++) {
+                node.d.mn[k] = smin(nodes[it].d.mn[k], node.d.mn[k]);
+                node.d.mx[k] = smax(nodes[it].d.mx[k], node.d.mx[k]);
+            }
+        int axis = -1;
+        if (na == KAX) {
+            double best = 0;
+            for (int k = 0; k < na; k++) {
+                double len = node.d.mx[k] - node.d.mn[k];
+                if (len > best) {
+                    best = len;
+                    axis = k;
+                }
+            }
+            if (axis < 0) axis = 0;   // all extents <= 0 is unreachable for eta > 0
+        } else {
+            double l0 = node.d.mx[0] - node.d.mn[0], l1 = node.d.mx[1] - node.d.mn[1],
+                   l2 = node.d.mx[2] - node.d.mn[2];
+            if (l0 >= l1 && l0 >= l2)
+                axis = 0;
+            else if (l1 >= l0 && l1 >= l2)
+                axis = 1;
+            else
+                axis = 2;
+        }
+        const std::vector<Node> &nd = nodes;
+        std::sort(items.begin(), items.end(),
+                  [&nd, axis](int a, int b) { return nd[a].d.mn[axis] < nd[b].d.mn[axis]; });
+        std::vector<int> L(items.begin(), items.begin() + n / 2), R(items.begin() + n / 2, items.end());
+        int l = build(L);
+        int r = build(R);
+        node.left = l;
+        node.right = r;
+        nodes.push_back(node);
+        return (int)nodes.size() - 1;
+    }
+};
+
+template <class Sink>
+void dual_traverse(const Tree &T, int a, int b, const int32_t *faces, const uint8_t *fixed, Sink &sink)
+{
+    const Node &A = T.nodes[a];
+    const Node &B = T.nodes[b];
+    if (!dop_overlap(A.d, B.d, T.na)) return;
+    if (A.face < 0) {
+        dual_traverse(T, A.left, b, faces, fixed, sink);
+        dual_traverse(T, A.right, b, faces, fixed, sink);
+    } else if (B.face < 0) {
+        dual_traverse(T, a, B.left, faces, fixed, sink);
+        dual_traverse(T, a, B.right, faces, fixed, sink);
+    } else {
+        if (neighboring(faces, A.face, B.face)) return;
+        emit_pair(faces, A.face, B.face, fixed, sink);
+    }
+}
+
+void candidates_tree(const int32_t *faces, int64_t F, const std::vector<Dop> &dops, int na,
+                     const uint8_t *fixed, CandSets &out, bool use_std_set)
+{
+    if (F == 0) return;
+    Tree T;
+    T.na = na;
+    T.nodes.reserve(2 * F);
+    std::vector<int> items(F);
+    for (int64_t f = 0; f < F; f++) {
+        Node n;
+        n.d = dops[f];
+        n.face = (int)f;
+        n.left = n.right = -1;
+        T.nodes.push_back(n);
+        items[f] = (int)f;
+    }
+    int root = T.build(items);
+    if (use_std_set) {
+        std::set<Key> vfs, ees;
+        SetSink sink{&vfs, &ees};
+        dual_traverse(T, root, root, faces, fixed, sink);
+        out.vf.assign(vfs.begin(), vfs.end());
+        out.ee.assign(ees.begin(), ees.end());
+    } else {
+        VecSink sink{&out};
+        dual_traverse(T, root, root, faces, fixed, sink);
+        sort_unique(out.vf);
+        sort_unique(out.ee);
+    }
+}
+
+// Result holder handed across the C boundary.
+struct Result {
+    CandSets cs;
+    std::vector<int64_t> pairs;
+};
+
+}   // namespace
+
+// ==============================================================================================
+// C interface (ctypes).  Positions are flat double[3V]; faces flat int32[3F] (a column of
+// Eigen::Matrix3Xi per face); histories CSR (off[V+1] int64, times[E], pos[3E]).
+// ==============================================================================================
+extern "C" {
+
+void orc_set_dot_order(int order) { g_dot_order = order; }
+void orc_set_rootfinder(void *fn) { g_rootfinder = (rootfinder_fn)fn; }
+
+int orc_aberth_roots(const double *op, int degree, double *zr, double *zi)
+{
+    return aberth_roots(op, degree, zr, zi);
+}
+
+int orc_find_roots(double *op, int degree, double *zr, double *zi) { return find_roots(op, degree, zr, zi); }
+
+void orc_kdop_axes(double *out39)
+{
+    Axes ax = make_axes();
+    for (int i = 0; i < KAX; i++) {
+        out39[3 * i] = ax.a[i].x;
+        out39[3 * i + 1] = ax.a[i].y;
+        out39[3 * i + 2] = ax.a[i].z;
+    }
+}
+
+int orc_get_quad_roots(double a, double b, double c, double *t0, double *t1)
+{
+    return get_quad_roots(a, b, c, *t0, *t1);
+}
+
+// findIntervals on a copy of op; writes up to cap intervals as (l,u) pairs; returns the count.
+int orc_find_intervals(const double *op, int n, int pos, double *out_lu, int cap, double *out_op,
+                       double *margin)
+{
+    double c[8];
+    for (int i = 0; i <= n; i++) c[i] = op[i];
+    std::vector<TI> iv;
+    Ctx ctx;
+    find_intervals(c, n, iv, pos != 0, &ctx);
+    int m = (int)iv.size();
+    for (int i = 0; i < m && i < cap; i++) {
+        out_lu[2 * i] = iv[i].l;
+        out_lu[2 * i + 1] = iv[i].u;
+    }
+    if (out_op)
+        for (int i = 0; i <= n; i++) out_op[i] = c[i];
+    if (margin) *margin = ctx.margin;
+    return m;
+}
+
+// The four primitives on 3-vectors packed as start positions then end positions.
+int orc_vertex_face_ctcd(const double *q, double eta, double *t, double *margin)
+{
+    Ctx ctx;
+    double tt = 0;
+    bool r = vertex_face_ctcd(load3(q), load3(q + 3), load3(q + 6), load3(q + 9), load3(q + 12),
+                              load3(q + 15), load3(q + 18), load3(q + 21), eta, tt, &ctx);
+    if (r) *t = tt;
+    if (margin) *margin = ctx.margin;
+    return r;
+}
+int orc_edge_edge_ctcd(const double *q, double eta, double *t, double *margin)
+{
+    Ctx ctx;
+    double tt = 0;
+    bool r = edge_edge_ctcd(load3(q), load3(q + 3), load3(q + 6), load3(q + 9), load3(q + 12), load3(q + 15),
+                            load3(q + 18), load3(q + 21), eta, tt, &ctx);
+    if (r) *t = tt;
+    if (margin) *margin = ctx.margin;
+    return r;
+}
+int orc_vertex_edge_ctcd(const double *q, double eta, double *t, double *margin)
+{
+    Ctx ctx;
+    double tt = 0;
+    bool r = vertex_edge_ctcd(load3(q), load3(q + 3), load3(q + 6), load3(q + 9), load3(q + 12), load3(q + 15),
+                              eta, tt, &ctx);
+    if (r) *t = tt;
+    if (margin) *margin = ctx.margin;
+    return r;
+}
+int orc_vertex_vertex_ctcd(const double *q, double eta, double *t, double *margin)
+{
+    Ctx ctx;
+    double tt = 0;
+    bool r = vertex_vertex_ctcd(load3(q), load3(q + 3), load3(q + 6), load3(q + 9), eta, tt, &ctx);
+    if (r) *t = tt;
+    if (margin) *margin = ctx.margin;
+    return r;
+}
+
+// Leaf bounds (13 mins then 13 maxs per face; AABB uses the first 3 of each).
+void orc_leaf_dops(const int32_t *faces, int64_t F, const int64_t *off, const double *ht, const double *hp,
+                   double eta, int aabb, double *out_mn, double *out_mx)
+{
+    Hist h{off, ht, hp};
+    std::vector<Dop> d;
+    leaf_dops(faces, F, h, eta, aabb != 0, d);
+    for (int64_t f = 0; f < F; f++)
+        for (int k = 0; k < KAX; k++) {
+            out_mn[KAX * f + k] = d[f].mn[k];
+            out_mx[KAX * f + k] = d[f].mx[k];
+        }
+}
+
+// Candidate stencil sets.  method: 0 = brute force, 1 = reference-structured tree with std::set,
+// 2 = tree with vector + sort/unique.  aabb selects AABBBroadPhase's 3-axis bounds.
+void *orc_candidates(const int32_t *faces, int64_t F, const int64_t *off, const double *ht, const double *hp,
+                     double eta, const uint8_t *fixed, int method, int aabb, int want_pairs)
+{
+    Hist h{off, ht, hp};
+    std::vector<Dop> d;
+    leaf_dops(faces, F, h, eta, aabb != 0, d);
+    Result *r = new Result;
+    int na = aabb ? 3 : KAX;
+    if (method == 0)
+        candidates_bruteforce(faces, F, d, na, fixed, r->cs, want_pairs ? &r->pairs : nullptr);
+    else
+        candidates_tree(faces, F, d, na, fixed, r->cs, method == 1);
+    return r;
+}
+int64_t orc_result_count(void *r, int which)
+{
+    Result *R = (Result *)r;
+    if (which == 0) return (int64_t)R->cs.vf.size();
+    if (which == 1) return (int64_t)R->cs.ee.size();
+    return (int64_t)R->pairs.size() / 2;
+}
+void orc_result_copy(void *r, int which, int32_t *out)
+{
+    Result *R = (Result *)r;
+    if (which == 2) {
+        for (size_t i = 0; i < R->pairs.size(); i++) out[i] = (int32_t)R->pairs[i];
+        return;
+    }
+    const std::vector<Key> &v = which == 0 ? R->cs.vf : R->cs.ee;
+    std::memcpy(out, v.data(), v.size() * sizeof(Key));
+}
+void orc_result_free(void *r) { delete (Result *)r; }
+
+// Narrowphase over stencil lists (CTCDNarrowPhase.cpp:9-22), one record per stencil.  is_ee selects
+// checkEES.  counts (optional) receives the ST_COUNT work counters summed over all stencils.
+void orc_narrowphase(const int64_t *off, const double *ht, const double *hp, const int32_t *keys, const double *eta,
+                     int64_t n, int is_ee, void *out_records, int64_t *counts)
+{
+    Hist h{off, ht, hp};
+    StencilResult *R = (StencilResult *)out_records;
+    for (int64_t i = 0; i < n; i++) {
+        Ctx ctx;
+        StencilResult res;
+        res.hit = 0;
+        res.subtest = -1;
+        res.window = -1;
+        res.pad = 0;
+        res.t = 0;
+        bool hit = is_ee ? check_ees(h, keys + 4 * i, eta[i], res, &ctx) : check_vfs(h, keys + 4 * i, eta[i], res, &ctx);
+        res.hit = hit ? 1 : 0;
+        if (!hit) {
+            res.subtest = -1;
+            res.window = -1;
+            res.t = 0;
+        }
+        res.margin = ctx.margin;
+        R[i] = res;
+        if (counts)
+            for (int k = 0; k < ST_COUNT; k++) counts[k] += ctx.counts[k];
+    }
+}
+
+int orc_stage_count() { return ST_COUNT; }
+
+}   // extern "C"

@@ -1,0 +1,77 @@
+// Test-only host build of the product's FP64 core (collisiondetection_amd/csrc/ccd_core.h).  It lets
+// the CPU test suite compare the core's algorithm with the oracle without a GPU, and the GPU tests
+// separate "the algorithm differs from the reference" from "the device arithmetic differs from the
+// host".  Built by tests/native/build.sh into tests/_build/; never shipped.
+#include <cstdint>
+
+#include "../../collisiondetection_amd/csrc/ccd_core.h"
+
+using ccd::V3;
+
+static V3 ld(const double *p) { return ccd::mk(p[0], p[1], p[2]); }
+
+extern "C" int core_primitive(int kind, const double *q, double eta, double *t)
+{
+    double tt = 0.0;
+    bool r = false;
+    if (kind == 0)
+        r = ccd::vertex_face(ld(q), ld(q + 3), ld(q + 6), ld(q + 9), ld(q + 12), ld(q + 15), ld(q + 18), ld(q + 21), eta, tt);
+    else if (kind == 1)
+        r = ccd::edge_edge(ld(q), ld(q + 3), ld(q + 6), ld(q + 9), ld(q + 12), ld(q + 15), ld(q + 18), ld(q + 21), eta, tt);
+    else if (kind == 2)
+        r = ccd::vertex_edge(ld(q), ld(q + 3), ld(q + 6), ld(q + 9), ld(q + 12), ld(q + 15), eta, tt);
+    else
+        r = ccd::vertex_vertex(ld(q), ld(q + 3), ld(q + 6), ld(q + 9), eta, tt);
+    *t = r ? tt : 0.0;
+    return r ? 1 : 0;
+}
+
+extern "C" void core_narrowphase(const double *qs, const double *qe, const int32_t *keys, int64_t n, int is_ee,
+                                 double eta, int32_t *hit, double *t, int32_t *sub)
+{
+    for (int64_t i = 0; i < n; i++) {
+        V3 s[4], e[4];
+        for (int j = 0; j < 4; j++) {
+            const double *a = qs + 3 * (int64_t)keys[4 * i + j];
+            const double *b = qe + 3 * (int64_t)keys[4 * i + j];
+            e[j] = ld(b);
+            s[j] = ccd::mk(ccd::stitch0(a[0], b[0]), ccd::stitch0(a[1], b[1]), ccd::stitch0(a[2], b[2]));
+        }
+        double tt = 0.0;
+        int r = is_ee ? ccd::check_ee_window(s, e, eta, tt) : ccd::check_vf_window(s, e, eta, tt);
+        hit[i] = r >= 0;
+        t[i] = r >= 0 ? tt : 0.0;
+        sub[i] = r;
+    }
+}
+
+// findIntervals on a copy of op (degree 2, 3, 4 or 6); returns the count, pieces in out[2k], out[2k+1].
+extern "C" int core_find_intervals(const double *op, int n, int pos, double *out)
+{
+    double c[7];
+    for (int i = 0; i <= n; i++) c[i] = op[i];
+    ccd::Ivls<8> L;
+    L.n = 0;
+    if (n == 2) ccd::find_intervals<2>(c, L, pos != 0);
+    else if (n == 3) ccd::find_intervals<3>(c, L, pos != 0);
+    else if (n == 4) ccd::find_intervals<4>(c, L, pos != 0);
+    else if (n == 6) ccd::find_intervals<6>(c, L, pos != 0);
+    else return -1;
+    for (int i = 0; i < L.n; i++) {
+        out[2 * i] = L.l[i];
+        out[2 * i + 1] = L.u[i];
+    }
+    return L.n;
+}
+
+// The core's real roots of c[0..n] (n = 3..6) in [0,1].
+extern "C" int core_roots01(const double *c, int n, double *r)
+{
+    switch (n) {
+    case 3: return ccd::RootIsolator<3>::roots01(c, r);
+    case 4: return ccd::RootIsolator<4>::roots01(c, r);
+    case 5: return ccd::RootIsolator<5>::roots01(c, r);
+    case 6: return ccd::RootIsolator<6>::roots01(c, r);
+    }
+    return -1;
+}
