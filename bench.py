@@ -1,0 +1,182 @@
+"""Benchmark: swept triangles per second of the full swept-primitive CCD path on one MI355X.
+
+Workload (default): the draped cloth self-CCD step -- a 256 x 256 vertex grid (65,536 vertices,
+130,050 triangles) folded as z = 0.15 sin(6x) cos(5y), end = start - 0.02 z + N(0, 0.004^2), eta = 1e-6,
+FP64.  One step = the reference's KDOPBroadPhase (leaf bounds, tree, traversal, stencil set) followed by
+CTCDNarrowPhase on every candidate stencil, with positions already resident in HBM; the colliding
+stencils and their times of impact are the output.
+
+Printed: one JSON line with the metric, the per-stage HIP-event timings, the roofline of the dominant
+kernel and a CPU baseline (the oracle's reference-structured CPU path on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene, sample_stencils=60000, seed=0):
+    """The oracle's reference-structured CPU path (top-down k-DOP tree, recursive traversal with std::set
+    insertion, then the per-stencil narrowphase), single-threaded, on a bounded sample: the broadphase
+    in full and the narrowphase on a random sample of its stencils, extrapolated per stencil."""
+    import numpy as np
+
+    import oracle
+
+    rf = oracle.use_default_rootfinder()
+    h = oracle.Hist.two_entry(scene["qstart"], scene["qend"])
+    t0 = time.perf_counter()
+    vf, ee = oracle.candidates(scene["faces"], h, scene["eta"], method="tree_set")
+    t_broad = time.perf_counter() - t0
+    rng = np.random.default_rng(seed)
+    nv = min(len(vf), sample_stencils * len(vf) // max(1, len(vf) + len(ee)))
+    ne = min(len(ee), sample_stencils - nv)
+    sv = vf[rng.choice(len(vf), size=nv, replace=False)] if nv else vf[:0]
+    se = ee[rng.choice(len(ee), size=ne, replace=False)] if ne else ee[:0]
+    t0 = time.perf_counter()
+    oracle.narrowphase(h, sv, scene["eta"], False)
+    t_vf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.narrowphase(h, se, scene["eta"], True)
+    t_ee = time.perf_counter() - t0
+    t_narrow = (t_vf / max(nv, 1)) * len(vf) + (t_ee / max(ne, 1)) * len(ee)
+    total = t_broad + t_narrow
+    F = len(scene["faces"])
+    return {"value": F / total, "unit": "swept tris/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle with root finder '{rf}'; full broadphase ({t_broad:.2f} s, {len(vf)} VF + {len(ee)} EE stencils) + narrowphase on "
+                       f"{nv} VF and {ne} EE sampled stencils ({t_vf + t_ee:.2f} s), extrapolated to all: "
+                       f"{total:.1f} s per step")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=int, default=256, help="cloth grid size (256 = the 130,050-triangle cloth)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from collisiondetection_amd import _native as N
+    from collisiondetection_amd import api, scenes
+
+    dev = torch.device("cuda", 0)
+    scene = scenes.cloth(args.grid)
+    F, V = len(scene["faces"]), len(scene["qstart"])
+    log(f"workload: cloth {args.grid}^2, {V} vertices, {F} triangles, eta {scene['eta']}")
+    ctx = api.Context(0)
+    ctx.bind_stream()
+    faces = api.to_device(scene["faces"], torch.int32, ctx).reshape(-1)
+    qs = api.to_device(scene["qstart"], torch.float64, ctx).reshape(-1)
+    qe = api.to_device(scene["qend"], torch.float64, ctx).reshape(-1)
+    eta = scene["eta"]
+    cap = 4 * F
+    vk = torch.empty(4 * cap, dtype=torch.int32, device=dev)
+    vt = torch.empty(cap, dtype=torch.float64, device=dev)
+    ek = torch.empty(4 * cap, dtype=torch.int32, device=dev)
+    et = torch.empty(cap, dtype=torch.float64, device=dev)
+    cnt = [ctypes.c_int64(0) for _ in range(4)]
+    L = N.lib()
+
+    def step():
+        rc = L.ccd_detect(ctx.ptr, faces.data_ptr(), F, qs.data_ptr(), qe.data_ptr(), V, eta, eta, None,
+                          vk.data_ptr(), vt.data_ptr(), cap, ctypes.byref(cnt[0]), ek.data_ptr(), et.data_ptr(), cap,
+                          ctypes.byref(cnt[1]), ctypes.byref(cnt[2]), ctypes.byref(cnt[3]))
+        ctx.check(rc)
+
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i + 1}/{args.warmup}: {cnt[2].value} VF + {cnt[3].value} EE candidates, "
+            f"{cnt[0].value} + {cnt[1].value} collisions")
+    torch.cuda.synchronize(dev)
+
+    # Work accounting for the narrowphase roofline (outside the timed region).
+    pv, pe = ctypes.c_void_p(), ctypes.c_void_p()
+    nv, ne = ctypes.c_int64(), ctypes.c_int64()
+    ctx.check(L.ccd_last_candidates(ctx.ptr, ctypes.byref(pv), ctypes.byref(nv), ctypes.byref(pe), ctypes.byref(ne)))
+    wv = (ctypes.c_int64 * N.NSTAGES)()
+    we = (ctypes.c_int64 * N.NSTAGES)()
+    ctx.check(L.ccd_narrowphase_work(ctx.ptr, qs.data_ptr(), qe.data_ptr(), V, pv, nv.value, pe, ne.value, eta, wv, we))
+    fl = (ctypes.c_int64 * N.NSTAGES)()
+    L.ccd_stage_flops(fl)
+    flops_vf = sum(wv[i] * fl[i] for i in range(N.NSTAGES))
+    flops_ee = sum(we[i] * fl[i] for i in range(N.NSTAGES))
+
+    # Timed region: K steps, HIP-event stage timers on the context's stream.
+    L.ccd_set_profiling(ctx.ptr, 1)
+    L.ccd_stage_times(ctx.ptr, None, None, 1)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize(dev)
+    wall = time.perf_counter() - t0
+    ms = (ctypes.c_double * N.NTIMERS)()
+    calls = (ctypes.c_int64 * N.NTIMERS)()
+    L.ccd_stage_times(ctx.ptr, ms, calls, 1)
+    L.ccd_set_profiling(ctx.ptr, 0)
+    stage_ms = {N.TIMER_NAMES[i]: ms[i] / max(calls[i], 1) for i in range(N.NTIMERS)}
+    ms_per_step = 1000.0 * wall / args.steps
+    value = F / (wall / args.steps)
+    nstencils = nv.value + ne.value
+    log("stage ms/step:", json.dumps({k: round(v, 4) for k, v in stage_ms.items()}))
+
+    # Roofline of the dominant kernel.
+    dom = max(stage_ms, key=stage_ms.get)
+    if dom in ("narrow_vf", "narrow_ee"):
+        flops = flops_vf if dom == "narrow_vf" else flops_ee
+        achieved = flops / (stage_ms[dom] * 1e-3) / 1e12
+        roof = {"kernel": "k_narrow2_compact<%s>" % ("VF" if dom == "narrow_vf" else "EE"), "bound": "mfma",
+                "pipe": "FP64 VALU (no dense contraction, so no MFMA; peak is MI355X's FP64 figure)",
+                "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                "traffic": None, "algorithmic_flops": flops}
+    else:
+        # Broadphase stages are bandwidth/latency bound: algorithmic bytes per swept triangle =
+        # 12 (face) + 48 V/F (start + end positions) + 16 S/F (unique stencil writes).
+        algo = 12 * F + 48 * V + 16 * nstencils
+        achieved = algo / (stage_ms[dom] * 1e-3) / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": algo}
+    roof["stage_ms"] = {k: round(v, 5) for k, v in stage_ms.items()}
+    roof["narrowphase_tflops"] = {
+        "vf": flops_vf / max(stage_ms["narrow_vf"], 1e-9) / 1e9, "ee": flops_ee / max(stage_ms["narrow_ee"], 1e-9) / 1e9}
+
+    out = {
+        "metric": "swept tris/sec on 1 GPU, batched vs plain, exact sets; p50/p99 CCD query latency",
+        "value": value, "unit": "swept tris/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"cloth self-CCD {args.grid}x{args.grid} ({V} V, {F} tris), eta 1e-6, seed 1",
+                   "global_batch": 1, "parallelism": "single GPU"},
+        "pair_tests_per_s": nstencils / ((stage_ms["narrow_vf"] + stage_ms["narrow_ee"]) * 1e-3),
+        "candidates": {"vf": nv.value, "ee": ne.value}, "collisions": {"vf": cnt[0].value, "ee": cnt[1].value},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle, 1 thread) ...")
+        out["cpu_baseline"] = cpu_baseline(scene)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

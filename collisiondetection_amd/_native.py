@@ -48,7 +48,18 @@ SIGNATURES = {
                                      _INT], _INT),
     "ccd_detect": ([_V, _V, _I64, _V, _V, _I64, _D, _D, _V, _V, _V, _I64, _P64, _V, _V, _I64, _P64, _P64, _P64],
                    _INT),
+    "ccd_set_profiling": ([_V, _INT], _INT),
+    "ccd_stage_times": ([_V, ctypes.POINTER(_D), _P64, _INT], _INT),
+    "ccd_narrowphase_work": ([_V, _V, _V, _I64, _V, _I64, _V, _I64, _D, _P64, _P64], _INT),
+    "ccd_stage_flops": ([_P64], _INT),
+    "ccd_last_candidates": ([_V, ctypes.POINTER(_V), _P64, ctypes.POINTER(_V), _P64], _INT),
 }
+
+NTIMERS = 8
+TIMER_NAMES = ("bounds", "sort", "build", "tables", "traverse", "expand", "narrow_vf", "narrow_ee")
+NSTAGES = 12
+STAGE_NAMES = ("vf_call", "vf_plane", "vf_dist", "ee_dist", "ee_par", "ee_bary", "ve_call", "ve_quad", "ve_quart",
+               "vv", "quad_root", "isolate")
 
 _lib = None
 

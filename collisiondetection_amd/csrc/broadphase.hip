@@ -739,6 +739,7 @@ int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, 
         return CCD_E_OOM;
     hipStream_t st = ctx->stream;
 
+    prof_begin(ctx, CCD_T_BOUNDS);
     hipLaunchKernelGGL(k_leaf_bounds, dim3(blocks_for(F, 256)), dim3(256), 0, st, faces, F, qs, qe, hist_off,
                        hist_pos, eta, axes, dop64, centre);
     CCD_LAUNCH_CHECK(ctx);
@@ -750,12 +751,16 @@ int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, 
     CCD_LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(F, 256)), dim3(256), 0, st, centre, F, box, keys, vals);
     CCD_LAUNCH_CHECK(ctx);
+    prof_end(ctx, CCD_T_BOUNDS);
+    prof_begin(ctx, CCD_T_SORT);
     size_t tmp = 0;
     CCD_HIP_CHECK(ctx, rocprim::radix_sort_pairs((void *)nullptr, tmp, keys, keys2, vals, order, (size_t)F, 0, 63,
                                                  st));
     void *stmp = buffer(ctx, "radix_tmp", tmp + 16);
     if (!stmp) return CCD_E_OOM;
     CCD_HIP_CHECK(ctx, rocprim::radix_sort_pairs(stmp, tmp, keys, keys2, vals, order, (size_t)F, 0, 63, st));
+    prof_end(ctx, CCD_T_SORT);
+    prof_begin(ctx, CCD_T_BUILD);
 
     hipLaunchKernelGGL(k_karras, dim3(blocks_for(F - 1, 256)), dim3(256), 0, st, keys2, F, left, rf, rl, parent);
     CCD_LAUNCH_CHECK(ctx);
@@ -768,6 +773,8 @@ int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, 
     hipLaunchKernelGGL(k_refit, dim3(blocks_for(F, 256)), dim3(256), 0, st, F, parent, left, escape, nodes,
                        arrivals);
     CCD_LAUNCH_CHECK(ctx);
+    prof_end(ctx, CCD_T_BUILD);
+    prof_begin(ctx, CCD_T_TABLES);
 
     // Vertex -> face table and mesh checks.
     CCD_HIP_CHECK(ctx, hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, st));
@@ -786,6 +793,7 @@ int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, 
     unsigned long long *c = (unsigned long long *)cnt;
     hipLaunchKernelGGL(k_mesh_check, dim3(blocks_for(F, 256)), dim3(256), 0, st, faces, F, voff, vlist, c + C_BAD);
     CCD_LAUNCH_CHECK(ctx);
+    prof_end(ctx, CCD_T_TABLES);
 
     // Traversal, with one retry if the pair buffer was too small.
     int64_t pcap = std::max<int64_t>(ctx->bufs.count("pairs") ? (int64_t)(ctx->bufs["pairs"].bytes / 8) : 0,
@@ -794,9 +802,11 @@ int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, 
     if (!pairs) return CCD_E_OOM;
     for (int attempt = 0; attempt < 2; attempt++) {
         CCD_HIP_CHECK(ctx, hipMemsetAsync(c + C_PAIRS, 0, 8, st));
+        prof_begin(ctx, CCD_T_TRAVERSE);
         hipLaunchKernelGGL(k_traverse, dim3(blocks_for(F, 256)), dim3(256), 0, st, F, nodes, sdop64, sfaces, pairs,
                            pcap, c + C_PAIRS);
         CCD_LAUNCH_CHECK(ctx);
+        prof_end(ctx, CCD_T_TRAVERSE);
         CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts, cnt, sizeof(int64_t) * NCOUNTERS, hipMemcpyDeviceToHost, st));
         CCD_HIP_CHECK(ctx, hipStreamSynchronize(st));
         if (ctx->h_counts[C_PAIRS] <= pcap) break;
@@ -819,6 +829,7 @@ int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, 
         int4 *eek = (int4 *)buffer(ctx, "ee_keys", (size_t)ecap * 16);
         if (!vfk || !eek) return CCD_E_OOM;
         CCD_HIP_CHECK(ctx, hipMemsetAsync(c + C_VF, 0, 16, st));
+        prof_begin(ctx, CCD_T_EXPAND);
         unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(P, 256), 65536));
         if (owner)
             hipLaunchKernelGGL(k_expand<true>, dim3(grid), dim3(256), 0, st, mv, pairs, c + C_PAIRS, pcap, vfk, vcap,
@@ -827,6 +838,7 @@ int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, 
             hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(256), 0, st, mv, pairs, c + C_PAIRS, pcap, vfk,
                                vcap, c + C_VF, eek, ecap, c + C_EE);
         CCD_LAUNCH_CHECK(ctx);
+        prof_end(ctx, CCD_T_EXPAND);
         CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts, cnt, sizeof(int64_t) * NCOUNTERS, hipMemcpyDeviceToHost, st));
         CCD_HIP_CHECK(ctx, hipStreamSynchronize(st));
         int64_t nv = ctx->h_counts[C_VF], ne = ctx->h_counts[C_EE];

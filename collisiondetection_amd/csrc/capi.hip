@@ -2,6 +2,7 @@
 #include <cstdarg>
 #include <new>
 
+#include "ccd_core.h"
 #include "ccd_internal.h"
 
 namespace ccdi {
@@ -17,6 +18,35 @@ int set_error(ccd_ctx *ctx, int code, const char *fmt, ...)
         ctx->err = buf;
     }
     return code;
+}
+
+void prof_begin(ccd_ctx *ctx, int timer)
+{
+    if (!ctx->profiling) return;
+    (void)hipEventRecord(ctx->ev[2 * timer], ctx->stream);
+}
+
+void prof_end(ccd_ctx *ctx, int timer)
+{
+    if (!ctx->profiling) return;
+    (void)hipEventRecord(ctx->ev[2 * timer + 1], ctx->stream);
+    ctx->pending[timer] = true;
+}
+
+void prof_collect(ccd_ctx *ctx)
+{
+    if (!ctx->profiling) return;
+    for (int t = 0; t < CCD_NTIMERS; t++) {
+        if (!ctx->pending[t]) continue;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev[2 * t], ctx->ev[2 * t + 1]) == hipSuccess) {
+            ctx->stage_ms[t] += ms;
+            ctx->stage_calls[t]++;
+        } else {
+            (void)hipGetLastError();
+        }
+        ctx->pending[t] = false;
+    }
 }
 
 void *buffer(ccd_ctx *ctx, const char *name, size_t bytes)
@@ -137,6 +167,8 @@ int ccd_destroy(ccd_ctx *ctx)
     for (auto &kv : ctx->bufs)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     if (ctx->d_counts) (void)hipFree(ctx->d_counts);
+    for (int i = 0; i < 2 * CCD_NTIMERS; i++)
+        if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->h_counts) (void)hipHostFree(ctx->h_counts);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
@@ -188,9 +220,19 @@ int ccd_ctcd_narrowphase(ccd_ctx *ctx, const double *qstart, const double *qend,
     if (rc) return rc;
     rc = validate_keys(ctx, ee_keys, nee, nverts, "edge-edge keys reference vertices outside [0, nverts)");
     if (rc) return rc;
+    prof_begin(ctx, CCD_T_NARROW_VF);
     rc = launch_narrow2(ctx, true, qstart, qend, vf_keys, vf_eta, eta_scalar, nvf, vf_hit, vf_t, vf_sub);
     if (rc) return rc;
-    return launch_narrow2(ctx, false, qstart, qend, ee_keys, ee_eta, eta_scalar, nee, ee_hit, ee_t, ee_sub);
+    prof_end(ctx, CCD_T_NARROW_VF);
+    prof_begin(ctx, CCD_T_NARROW_EE);
+    rc = launch_narrow2(ctx, false, qstart, qend, ee_keys, ee_eta, eta_scalar, nee, ee_hit, ee_t, ee_sub);
+    if (rc) return rc;
+    prof_end(ctx, CCD_T_NARROW_EE);
+    if (ctx->profiling) {
+        CCD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        prof_collect(ctx);
+    }
+    return CCD_OK;
 }
 
 int ccd_ctcd_narrowphase_history(ccd_ctx *ctx, const int64_t *hist_off, const double *hist_t,
@@ -227,6 +269,10 @@ static int candidates_common(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces,
     Candidates c;
     int rc = broadphase(ctx, faces, nfaces, qs, qe, hoff, hpos, nverts, outer_eta, fixed, c);
     if (rc) return rc;
+    ctx->last_vf = c.vf;
+    ctx->last_ee = c.ee;
+    ctx->last_nvf = c.nvf;
+    ctx->last_nee = c.nee;
     if (flags & CCD_SORTED) {
         rc = sort_stencils(ctx, c.vf, c.nvf);
         if (rc) return rc;
@@ -242,6 +288,7 @@ static int candidates_common(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces,
     if (ne > 0 && ee_out)
         CCD_HIP_CHECK(ctx, hipMemcpyAsync(ee_out, c.ee, (size_t)ne * 16, hipMemcpyDeviceToDevice, ctx->stream));
     CCD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
     if (c.nvf > vf_cap || c.nee > ee_cap)
         return set_error(ctx, CCD_E_CAPACITY, "candidate buffers too small: need %lld vertex-face, %lld edge-edge",
                          (long long)c.nvf, (long long)c.nee);
@@ -295,16 +342,25 @@ int ccd_detect(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces, const double 
     Candidates c;
     rc = broadphase(ctx, faces, nfaces, qstart, qend, nullptr, nullptr, nverts, outer_eta, fixed, c);
     if (rc) return rc;
+    ctx->last_vf = c.vf;
+    ctx->last_ee = c.ee;
+    ctx->last_nvf = c.nvf;
+    ctx->last_nee = c.nee;
     CCD_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_counts + C_VF_HITS, 0, 16, ctx->stream));
+    prof_begin(ctx, CCD_T_NARROW_VF);
     rc = launch_narrow2_compact(ctx, true, qstart, qend, c.vf, ctx->d_counts + C_VF, c.nvf, eta, vf_hits, vf_toi,
                                 vf_cap, ctx->d_counts + C_VF_HITS);
     if (rc) return rc;
+    prof_end(ctx, CCD_T_NARROW_VF);
+    prof_begin(ctx, CCD_T_NARROW_EE);
     rc = launch_narrow2_compact(ctx, false, qstart, qend, c.ee, ctx->d_counts + C_EE, c.nee, eta, ee_hits, ee_toi,
                                 ee_cap, ctx->d_counts + C_EE_HITS);
     if (rc) return rc;
+    prof_end(ctx, CCD_T_NARROW_EE);
     CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts, ctx->d_counts, sizeof(int64_t) * NCOUNTERS, hipMemcpyDeviceToHost,
                                       ctx->stream));
     CCD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
     int64_t nvh = ctx->h_counts[C_VF_HITS], neh = ctx->h_counts[C_EE_HITS];
     if (vf_hit_count) *vf_hit_count = nvh;
     if (ee_hit_count) *ee_hit_count = neh;
@@ -313,6 +369,78 @@ int ccd_detect(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces, const double 
     if (nvh > vf_cap || neh > ee_cap)
         return set_error(ctx, CCD_E_CAPACITY, "collision buffers too small: need %lld vertex-face, %lld edge-edge",
                          (long long)nvh, (long long)neh);
+    return CCD_OK;
+}
+
+int ccd_set_profiling(ccd_ctx *ctx, int on)
+{
+    if (!ctx) return CCD_E_INVALID;
+    if (on && !ctx->ev[0]) {
+        CCD_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+        for (int i = 0; i < 2 * CCD_NTIMERS; i++) CCD_HIP_CHECK(ctx, hipEventCreate(&ctx->ev[i]));
+    }
+    ctx->profiling = on != 0;
+    return CCD_OK;
+}
+
+int ccd_stage_times(ccd_ctx *ctx, double *ms, int64_t *calls, int reset)
+{
+    if (!ctx) return CCD_E_INVALID;
+    for (int t = 0; t < CCD_NTIMERS; t++) {
+        if (ms) ms[t] = ctx->stage_ms[t];
+        if (calls) calls[t] = ctx->stage_calls[t];
+        if (reset) {
+            ctx->stage_ms[t] = 0;
+            ctx->stage_calls[t] = 0;
+        }
+    }
+    return CCD_OK;
+}
+
+int ccd_stage_flops(int64_t *flops)
+{
+    if (!flops) return CCD_E_INVALID;
+    for (int s = 0; s < CCD_NSTAGES; s++) flops[s] = ccd::STAGE_FLOPS[s];
+    return CCD_OK;
+}
+
+int ccd_narrowphase_work(ccd_ctx *ctx, const double *qstart, const double *qend, int64_t nverts,
+                         const int32_t *vf_keys, int64_t nvf, const int32_t *ee_keys, int64_t nee, double eta,
+                         int64_t *vf_counts, int64_t *ee_counts)
+{
+    if (!ctx) return CCD_E_INVALID;
+    CCD_REQUIRE(ctx, nverts >= 0 && nvf >= 0 && nee >= 0, "negative size");
+    CCD_REQUIRE(ctx, (nvf == 0 || vf_keys) && (nee == 0 || ee_keys) && ((nvf == 0 && nee == 0) || (qstart && qend)),
+                "null input");
+    CCD_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    int rc = validate_keys(ctx, vf_keys, nvf, nverts, "vertex-face keys reference vertices outside [0, nverts)");
+    if (rc) return rc;
+    rc = validate_keys(ctx, ee_keys, nee, nverts, "edge-edge keys reference vertices outside [0, nverts)");
+    if (rc) return rc;
+    unsigned long long *d = (unsigned long long *)buffer(ctx, "work_counts", 2 * CCD_NSTAGES * 8);
+    if (!d) return CCD_E_OOM;
+    CCD_HIP_CHECK(ctx, hipMemsetAsync(d, 0, 2 * CCD_NSTAGES * 8, ctx->stream));
+    rc = launch_narrow2_count(ctx, true, qstart, qend, vf_keys, nvf, eta, d);
+    if (rc) return rc;
+    rc = launch_narrow2_count(ctx, false, qstart, qend, ee_keys, nee, eta, d + CCD_NSTAGES);
+    if (rc) return rc;
+    int64_t h[2 * CCD_NSTAGES];
+    CCD_HIP_CHECK(ctx, hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    CCD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int s = 0; s < CCD_NSTAGES; s++) {
+        if (vf_counts) vf_counts[s] = h[s];
+        if (ee_counts) ee_counts[s] = h[CCD_NSTAGES + s];
+    }
+    return CCD_OK;
+}
+
+int ccd_last_candidates(ccd_ctx *ctx, const int32_t **vf_keys, int64_t *nvf, const int32_t **ee_keys, int64_t *nee)
+{
+    if (!ctx) return CCD_E_INVALID;
+    if (vf_keys) *vf_keys = ctx->last_vf;
+    if (nvf) *nvf = ctx->last_nvf;
+    if (ee_keys) *ee_keys = ctx->last_ee;
+    if (nee) *nee = ctx->last_nee;
     return CCD_OK;
 }
 

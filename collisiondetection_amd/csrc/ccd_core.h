@@ -78,6 +78,33 @@ CCD_FHD V3 cross(V3 a, V3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x 
 CCD_FHD double norm(V3 a) { return dsqrt(dot(a, a)); }
 
 // ------------------------------------------------------------------------------------------------
+// Work accounting.  A counter functor is called once per executed stage; NoCount compiles away.
+// The FP64 operations (add, sub, mul, div, sqrt) each stage performs to build its coefficients are
+// listed in STAGE_FLOPS; root isolation is not counted (it depends on the method).
+// ------------------------------------------------------------------------------------------------
+enum Stage {
+    S_VF_CALL = 0,   // vertexFaceCTCD entry: 4 velocities                                   (CTCD.cpp:423-427)
+    S_VF_PLANE,      // one edge-side cubic: its 6 vectors, 4 coefficients, normalisation   (:433-464, :215-229)
+    S_VF_DIST,       // the coplanar-distance sextic                                        (:466-475, :231-257)
+    S_EE_DIST,       // edgeEdgeCTCD entry: velocities and the distance sextic              (:270-288)
+    S_EE_PAR,        // one parallel-edge check of a raw coplanar interval                  (:294-308)
+    S_EE_BARY,       // one barycentric quartic                                             (:313-350, :179-213)
+    S_VE_CALL,       // vertexEdgeCTCD entry: velocities and the 6 edge vectors             (:520-534)
+    S_VE_QUAD,       // one side quadratic                                                  (:536-555)
+    S_VE_QUART,      // the colinear-distance quartic                                       (:557-573)
+    S_VV,            // vertexVertexCTCD's quadratic coefficients                            (:610-621)
+    S_QUAD_ROOT,     // one getQuadRoots call with real roots                                (:38-56)
+    S_ISOLATE,       // one polynomial of reduced degree > 2 sent to root isolation          (:135-142)
+    S_COUNT
+};
+// Flops per stage (add/sub/mul/div/sqrt = 1 each), counted from the expressions below.
+constexpr int STAGE_FLOPS[S_COUNT] = {12, 126, 172, 184, 41, 130, 27, 19, 93, 68, 10, 0};
+
+struct NoCount {
+    CCD_FHD void operator()(int) const {}
+};
+
+// ------------------------------------------------------------------------------------------------
 // Interval lists (CTCD.h:7-26).  CAP is the most pieces a degree-D polynomial can produce.
 // ------------------------------------------------------------------------------------------------
 template <int CAP>
@@ -296,8 +323,8 @@ struct RootIsolator<2> {
 // ------------------------------------------------------------------------------------------------
 // findIntervals (CTCD.cpp:98-177) for a degree-N polynomial, op modified in place.
 // ------------------------------------------------------------------------------------------------
-template <int N, int CAP>
-CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos)
+template <int N, int CAP, class C = NoCount>
+CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos, C &&cnt = C())
 {
     out.n = 0;
     double maxval = 0;
@@ -332,6 +359,7 @@ CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos)
     int roots = 0;
     if (reduced > 2) {
         if (!could_have_roots(op, reduced, pos)) return;
+        cnt(S_ISOLATE);
         if (reduced == 3) roots = RootIsolator<3>::roots01(op, time);
         if (N >= 4 && reduced == 4) roots = RootIsolator<(N >= 4 ? 4 : 3)>::roots01(op, time);
         if (N >= 5 && reduced == 5) roots = RootIsolator<(N >= 5 ? 5 : 3)>::roots01(op, time);
@@ -353,6 +381,7 @@ CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos)
         return;
     } else if (reduced == 2) {
         roots = quad_roots(op[0], op[1], op[2], time[0], time[1]);
+        if (roots) cnt(S_QUAD_ROOT);
         // std::sort of two values: swaps only when time[1] < time[0].
         if (roots == 2 && time[1] < time[0]) {
             double s = time[0];
@@ -382,8 +411,8 @@ CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos)
 // ------------------------------------------------------------------------------------------------
 // Coefficient builders (CTCD.cpp:179-257)
 // ------------------------------------------------------------------------------------------------
-template <int CAP>
-CCD_HD void barycentric_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, Ivls<CAP> &out)
+template <int CAP, class Cn>
+CCD_HD void barycentric_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, Ivls<CAP> &out, Cn &cnt)
 {
     double A = dot(x10, x10);
     double B = 2 * dot(x10, v10);
@@ -403,11 +432,11 @@ CCD_HD void barycentric_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, Ivl
     op[2] = F * J + D * L + E * K - C * G - A * I - B * H;
     op[3] = D * K + E * J - A * H - B * G;
     op[4] = D * J - A * G;
-    find_intervals<4>(op, out, true);
+    find_intervals<4>(op, out, true, cnt);
 }
 
-template <int CAP>
-CCD_HD void plane_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, Ivls<CAP> &out)
+template <int CAP, class Cn>
+CCD_HD void plane_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, Ivls<CAP> &out, Cn &cnt)
 {
     double op[4];
     V3 v23 = cross(v20, v30);
@@ -418,11 +447,11 @@ CCD_HD void plane_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, Ivls<CAP>
     op[1] = dot(x10, v23) + dot(v10, x2v3) + dot(v10, v2x3);
     op[2] = dot(x10, x2v3) + dot(x10, v2x3) + dot(v10, x23);
     op[3] = dot(x10, x23);
-    find_intervals<3>(op, out, true);
+    find_intervals<3>(op, out, true, cnt);
 }
 
-template <int CAP>
-CCD_HD void distance_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, double minD, Ivls<CAP> &out)
+template <int CAP, class Cn>
+CCD_HD void distance_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, double minD, Ivls<CAP> &out, Cn &cnt)
 {
     V3 v23 = cross(v20, v30);
     V3 x2v3 = cross(x20, v30);
@@ -443,7 +472,7 @@ CCD_HD void distance_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, double
     op[4] = 2 * B * D + C * C - (2 * dot(G, E) + dot(F, F)) * minD;
     op[5] = 2 * C * D - 2 * dot(F, E) * minD;
     op[6] = D * D - dot(E, E) * minD;
-    find_intervals<6>(op, out, false);
+    find_intervals<6>(op, out, false, cnt);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -451,21 +480,25 @@ CCD_HD void distance_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, double
 // ------------------------------------------------------------------------------------------------
 
 // edgeEdgeCTCD (CTCD.cpp:259-411): edges (q0,p0) and (q1,p1).
-CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3 p1e, double eta, double &t)
+template <class C = NoCount>
+CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3 p1e, double eta, double &t,
+                      C &&cnt = C())
 {
+    cnt(S_EE_DIST);
     double minD = eta * eta;
     V3 vp0 = p0e - p0s;
     V3 vp1 = p1e - p1s;
     V3 vq0 = q0e - q0s;
     V3 vq1 = q1e - q1s;
     Ivls<7> raw;
-    distance_poly(p0s - p1s, p0s - q0s, p1s - q1s, vp0 - vp1, vp0 - vq0, vp1 - vq1, minD, raw);
+    distance_poly(p0s - p1s, p0s - q0s, p1s - q1s, vp0 - vp1, vp0 - vq0, vp1 - vq1, minD, raw, cnt);
 
     // Parallel-edge classification of each raw interval at its midpoint (CTCD.cpp:294-308).
     Ivls<7> coplane, parallel;
     coplane.n = 0;
     parallel.n = 0;
     for (int i = 0; i < raw.n; i++) {
+        cnt(S_EE_PAR);
         double midt = (raw.u[i] + raw.l[i]) / 2;
         V3 a = (q0s - p0s) + midt * (vq0 - vp0);
         V3 b = (q1s - p1s) + midt * (vq1 - vp1);
@@ -483,15 +516,19 @@ CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3
 
     Ivls<5> a0, a1, b0, b1;
     V3 x10 = p1s - q1s, v10 = vp1 - vq1;
-    barycentric_poly(x10, p0s - q0s, q0s - q1s, v10, vp0 - vq0, vq0 - vq1, a0);
+    cnt(S_EE_BARY);
+    barycentric_poly(x10, p0s - q0s, q0s - q1s, v10, vp0 - vq0, vq0 - vq1, a0, cnt);
     if (a0.n == 0) return false;
-    barycentric_poly(x10, q0s - p0s, p0s - q1s, v10, vq0 - vp0, vp0 - vq1, a1);
+    cnt(S_EE_BARY);
+    barycentric_poly(x10, q0s - p0s, p0s - q1s, v10, vq0 - vp0, vp0 - vq1, a1, cnt);
     if (a1.n == 0) return false;
     x10 = p0s - q0s;
     v10 = vp0 - vq0;
-    barycentric_poly(x10, p1s - q1s, q1s - q0s, v10, vp1 - vq1, vq1 - vq0, b0);
+    cnt(S_EE_BARY);
+    barycentric_poly(x10, p1s - q1s, q1s - q0s, v10, vp1 - vq1, vq1 - vq0, b0, cnt);
     if (b0.n == 0) return false;
-    barycentric_poly(x10, q1s - p1s, p1s - q0s, v10, vq1 - vp1, vp1 - vq0, b1);
+    cnt(S_EE_BARY);
+    barycentric_poly(x10, q1s - p1s, p1s - q0s, v10, vq1 - vp1, vp1 - vq0, b1, cnt);
     if (b1.n == 0) return false;
 
     // All combinations (CTCD.cpp:352-396): a combination counts if its five intervals overlap
@@ -531,22 +568,29 @@ CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3
 }
 
 // vertexFaceCTCD (CTCD.cpp:413-508): vertex q0 and face (q1,q2,q3).
-CCD_HD bool vertex_face(V3 q0s, V3 q1s, V3 q2s, V3 q3s, V3 q0e, V3 q1e, V3 q2e, V3 q3e, double eta, double &t)
+template <class C = NoCount>
+CCD_HD bool vertex_face(V3 q0s, V3 q1s, V3 q2s, V3 q3s, V3 q0e, V3 q1e, V3 q2e, V3 q3e, double eta, double &t,
+                        C &&cnt = C())
 {
+    cnt(S_VF_CALL);
     double minD = eta * eta;
     V3 v0 = q0e - q0s;
     V3 v1 = q1e - q1s;
     V3 v2 = q2e - q2s;
     V3 v3 = q3e - q3s;
     Ivls<4> e1, e2, e3;
-    plane_poly(q0s - q1s, cross(q3s - q1s, q2s - q1s), q3s - q1s, v0 - v1, cross(v3 - v1, v2 - v1), v3 - v1, e1);
+    cnt(S_VF_PLANE);
+    plane_poly(q0s - q1s, cross(q3s - q1s, q2s - q1s), q3s - q1s, v0 - v1, cross(v3 - v1, v2 - v1), v3 - v1, e1, cnt);
     if (e1.n == 0) return false;
-    plane_poly(q0s - q2s, cross(q1s - q2s, q3s - q2s), q1s - q2s, v0 - v2, cross(v1 - v2, v3 - v2), v1 - v2, e2);
+    cnt(S_VF_PLANE);
+    plane_poly(q0s - q2s, cross(q1s - q2s, q3s - q2s), q1s - q2s, v0 - v2, cross(v1 - v2, v3 - v2), v1 - v2, e2, cnt);
     if (e2.n == 0) return false;
-    plane_poly(q0s - q3s, cross(q2s - q3s, q1s - q3s), q2s - q3s, v0 - v3, cross(v2 - v3, v1 - v3), v2 - v3, e3);
+    cnt(S_VF_PLANE);
+    plane_poly(q0s - q3s, cross(q2s - q3s, q1s - q3s), q2s - q3s, v0 - v3, cross(v2 - v3, v1 - v3), v2 - v3, e3, cnt);
     if (e3.n == 0) return false;
     Ivls<7> cop;
-    distance_poly(q0s - q1s, q2s - q1s, q3s - q1s, v0 - v1, v2 - v1, v3 - v1, minD, cop);
+    cnt(S_VF_DIST);
+    distance_poly(q0s - q1s, q2s - q1s, q3s - q1s, v0 - v1, v2 - v1, v3 - v1, minD, cop, cnt);
     if (cop.n == 0) return false;
 
     bool col = false;
@@ -575,8 +619,10 @@ CCD_HD bool vertex_face(V3 q0s, V3 q1s, V3 q2s, V3 q3s, V3 q0e, V3 q1e, V3 q2e, 
 }
 
 // vertexEdgeCTCD (CTCD.cpp:511-602): vertex q0 and edge (q1,q2).
-CCD_HD bool vertex_edge(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double eta, double &t)
+template <class Cn = NoCount>
+CCD_HD bool vertex_edge(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double eta, double &t, Cn &&cnt = Cn())
 {
+    cnt(S_VE_CALL);
     double minD = eta * eta;
     V3 v0 = q0e - q0s;
     V3 v1 = q1e - q1s;
@@ -593,13 +639,16 @@ CCD_HD bool vertex_edge(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double e
     op[0] = dot(vab, vac);
     op[1] = dot(ac, vab) + dot(ab, vac);
     op[2] = dot(ab, ac);
-    find_intervals<2>(op, e1, true);
+    cnt(S_VE_QUAD);
+    find_intervals<2>(op, e1, true, cnt);
     if (e1.n == 0) return false;
     op[0] = dot(vab, vcb);
     op[1] = dot(cb, vab) + dot(ab, vcb);
     op[2] = dot(ab, cb);
-    find_intervals<2>(op, e2, true);
+    cnt(S_VE_QUAD);
+    find_intervals<2>(op, e2, true, cnt);
     if (e2.n == 0) return false;
+    cnt(S_VE_QUART);
     double A = dot(ab, ab);
     double B = 2 * dot(ab, vab);
     double C = dot(vab, vab);
@@ -614,7 +663,7 @@ CCD_HD bool vertex_edge(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double e
     op[2] = B * E + A * F + C * D - H * H - 2 * G * I - minD * C;
     op[1] = B * F + C * E - 2 * H * I;
     op[0] = C * F - I * I;
-    find_intervals<4>(op, colin, false);
+    find_intervals<4>(op, colin, false, cnt);
     if (colin.n == 0) return false;
 
     bool col = false;
@@ -642,8 +691,10 @@ CCD_HD bool vertex_edge(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double e
 }
 
 // vertexVertexCTCD (CTCD.cpp:604-692)
-CCD_HD bool vertex_vertex(V3 q1s, V3 q2s, V3 q1e, V3 q2e, double eta, double &t)
+template <class Cn = NoCount>
+CCD_HD bool vertex_vertex(V3 q1s, V3 q2s, V3 q1e, V3 q2e, double eta, double &t, Cn &&cnt = Cn())
 {
+    cnt(S_VV);
     double min_d = eta * eta;
     double t1 = 0, t2 = 0;
     int roots = 0;
@@ -654,6 +705,7 @@ CCD_HD bool vertex_vertex(V3 q1s, V3 q2s, V3 q1e, V3 q2e, double eta, double &t)
     double c = dot(q1s, q1s) + dot(q2s, q2s) - 2 * dot(q1s, q2s) - min_d;
     if (a != 0) {
         roots = quad_roots(a, b, c, t1, t2);
+        if (roots) cnt(S_QUAD_ROOT);
     } else if (b != 0) {
         t1 = -c / b;
         roots = 1;
@@ -710,29 +762,31 @@ CCD_HD bool vertex_vertex(V3 q1s, V3 q2s, V3 q1e, V3 q2e, double eta, double &t)
 // p[0..3] = start positions, e[0..3] = end positions of the stencil's vertices in key order.
 // Returns the firing sub-test (0 = VF/EE, then VE in reference order, then VV) or -1.
 // ------------------------------------------------------------------------------------------------
-CCD_HD int check_vf_window(const V3 *s, const V3 *e, double eta, double &t)
+template <class C = NoCount>
+CCD_HD int check_vf_window(const V3 *s, const V3 *e, double eta, double &t, C &&cnt = C())
 {
-    if (vertex_face(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t)) return 0;
-    if (vertex_edge(s[0], s[1], s[2], e[0], e[1], e[2], eta, t)) return 1;
-    if (vertex_edge(s[0], s[2], s[3], e[0], e[2], e[3], eta, t)) return 2;
-    if (vertex_edge(s[0], s[3], s[1], e[0], e[3], e[1], eta, t)) return 3;
-    if (vertex_vertex(s[0], s[1], e[0], e[1], eta, t)) return 4;
-    if (vertex_vertex(s[0], s[2], e[0], e[2], eta, t)) return 5;
-    if (vertex_vertex(s[0], s[3], e[0], e[3], eta, t)) return 6;
+    if (vertex_face(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t, cnt)) return 0;
+    if (vertex_edge(s[0], s[1], s[2], e[0], e[1], e[2], eta, t, cnt)) return 1;
+    if (vertex_edge(s[0], s[2], s[3], e[0], e[2], e[3], eta, t, cnt)) return 2;
+    if (vertex_edge(s[0], s[3], s[1], e[0], e[3], e[1], eta, t, cnt)) return 3;
+    if (vertex_vertex(s[0], s[1], e[0], e[1], eta, t, cnt)) return 4;
+    if (vertex_vertex(s[0], s[2], e[0], e[2], eta, t, cnt)) return 5;
+    if (vertex_vertex(s[0], s[3], e[0], e[3], eta, t, cnt)) return 6;
     return -1;
 }
 
-CCD_HD int check_ee_window(const V3 *s, const V3 *e, double eta, double &t)
+template <class C = NoCount>
+CCD_HD int check_ee_window(const V3 *s, const V3 *e, double eta, double &t, C &&cnt = C())
 {
-    if (edge_edge(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t)) return 0;
-    if (vertex_edge(s[0], s[2], s[3], e[0], e[2], e[3], eta, t)) return 1;
-    if (vertex_edge(s[1], s[2], s[3], e[1], e[2], e[3], eta, t)) return 2;
-    if (vertex_edge(s[2], s[0], s[1], e[2], e[0], e[1], eta, t)) return 3;
-    if (vertex_edge(s[3], s[0], s[1], e[3], e[0], e[1], eta, t)) return 4;
-    if (vertex_vertex(s[0], s[2], e[0], e[2], eta, t)) return 5;
-    if (vertex_vertex(s[0], s[3], e[0], e[3], eta, t)) return 6;
-    if (vertex_vertex(s[1], s[2], e[1], e[2], eta, t)) return 7;
-    if (vertex_vertex(s[1], s[3], e[1], e[3], eta, t)) return 8;
+    if (edge_edge(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t, cnt)) return 0;
+    if (vertex_edge(s[0], s[2], s[3], e[0], e[2], e[3], eta, t, cnt)) return 1;
+    if (vertex_edge(s[1], s[2], s[3], e[1], e[2], e[3], eta, t, cnt)) return 2;
+    if (vertex_edge(s[2], s[0], s[1], e[2], e[0], e[1], eta, t, cnt)) return 3;
+    if (vertex_edge(s[3], s[0], s[1], e[3], e[0], e[1], eta, t, cnt)) return 4;
+    if (vertex_vertex(s[0], s[2], e[0], e[2], eta, t, cnt)) return 5;
+    if (vertex_vertex(s[0], s[3], e[0], e[3], eta, t, cnt)) return 6;
+    if (vertex_vertex(s[1], s[2], e[1], e[2], eta, t, cnt)) return 7;
+    if (vertex_vertex(s[1], s[3], e[1], e[3], eta, t, cnt)) return 8;
     return -1;
 }
 

@@ -23,6 +23,15 @@ struct ccd_ctx {
     int64_t device_bytes = 0;
     int64_t *h_counts = nullptr;                  // pinned host mailbox for device counters
     int64_t *d_counts = nullptr;                  // device counters (CCD_NCOUNTERS int64)
+    // Stage timing with HIP events on the context's stream (ccd_set_profiling).
+    bool profiling = false;
+    hipEvent_t ev[2 * CCD_NTIMERS] = {};
+    bool pending[CCD_NTIMERS] = {};
+    double stage_ms[CCD_NTIMERS] = {};
+    int64_t stage_calls[CCD_NTIMERS] = {};
+    // The candidates of the last broadphase (ccd_last_candidates).
+    const int32_t *last_vf = nullptr, *last_ee = nullptr;
+    int64_t last_nvf = 0, last_nee = 0;
 };
 
 namespace ccdi {
@@ -40,6 +49,10 @@ enum Counter {
 };
 
 int set_error(ccd_ctx *ctx, int code, const char *fmt, ...);
+// Stage timers: no-ops unless profiling is on.  prof_collect needs the stream to be synchronised.
+void prof_begin(ccd_ctx *ctx, int timer);
+void prof_end(ccd_ctx *ctx, int timer);
+void prof_collect(ccd_ctx *ctx);
 // Grow-only named device buffer.  Returns nullptr (and sets the context error) on failure.
 void *buffer(ccd_ctx *ctx, const char *name, size_t bytes);
 
@@ -71,6 +84,9 @@ int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *
 int launch_narrow2_compact(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
                            const int64_t *n_dev, int64_t n_max, double eta, int32_t *out_keys, double *out_toi,
                            int64_t cap, int64_t *counter);
+// Work accounting: per-stage execution counts (ccd::Stage order) summed over the stencils into counts.
+int launch_narrow2_count(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
+                         int64_t n, double eta, unsigned long long *counts);
 // Validation: counts keys outside [0, nverts) into *bad.
 int launch_validate_keys(ccd_ctx *ctx, const int32_t *keys, int64_t n4, int64_t nverts, int64_t *bad);
 int launch_validate_history(ccd_ctx *ctx, const int64_t *off, const double *ht, int64_t nverts, int64_t nentries,

@@ -39,6 +39,20 @@ extern "C" {
 /* Flags for ccd_kdop_candidates. */
 #define CCD_SORTED 1 /* return stencils in the reference's std::set order (Stencils.h:59-77, :119-137) */
 
+/* Stage timers (ccd_stage_times), in pipeline order. */
+#define CCD_T_BOUNDS 0    /* leaf k-DOPs, scene box, Morton codes */
+#define CCD_T_SORT 1      /* Morton radix sort */
+#define CCD_T_BUILD 2     /* radix tree, escape links, leaf records, bottom-up refit */
+#define CCD_T_TABLES 3    /* vertex -> face table and mesh checks */
+#define CCD_T_TRAVERSE 4  /* face-pair traversal */
+#define CCD_T_EXPAND 5    /* stencil expansion with deduplication */
+#define CCD_T_NARROW_VF 6 /* vertex-face narrowphase */
+#define CCD_T_NARROW_EE 7 /* edge-edge narrowphase */
+#define CCD_NTIMERS 8
+
+/* Narrowphase work stages (ccd_narrowphase_work), in the order of ccd::Stage in ccd_core.h. */
+#define CCD_NSTAGES 12
+
 typedef struct ccd_ctx ccd_ctx;
 
 /* Context: owns a HIP stream and a grow-only device arena.  No counterpart in the reference, whose
@@ -129,6 +143,33 @@ int ccd_detect(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces, const double 
                int64_t nverts, double outer_eta, double eta, const uint8_t *fixed, int32_t *vf_hits, double *vf_toi,
                int64_t vf_cap, int64_t *vf_hit_count, int32_t *ee_hits, double *ee_toi, int64_t ee_cap,
                int64_t *ee_hit_count, int64_t *vf_candidates, int64_t *ee_candidates);
+
+/*
+ * Instrumentation (no counterpart in the reference, which times nothing).
+ * ccd_set_profiling: when on, every pipeline stage is bracketed by HIP events on the context's stream
+ * and calls that run a narrowphase synchronise at their end; ccd_stage_times returns the summed
+ * milliseconds and launch counts per CCD_T_* timer since the last reset (reset != 0 clears them).
+ */
+int ccd_set_profiling(ccd_ctx *ctx, int on);
+int ccd_stage_times(ccd_ctx *ctx, double *ms, int64_t *calls, int reset);
+
+/*
+ * Work accounting for the narrowphase's roofline: runs the two-entry narrowphase in a counting
+ * variant and writes, per stage, how many times it executed (vf_counts / ee_counts: host arrays of
+ * CCD_NSTAGES), and the FP64 operations each stage performs (ccd_stage_flops, CCD_NSTAGES values).
+ * Synchronous.
+ */
+int ccd_narrowphase_work(ccd_ctx *ctx, const double *qstart, const double *qend, int64_t nverts,
+                         const int32_t *vf_keys, int64_t nvf, const int32_t *ee_keys, int64_t nee, double eta,
+                         int64_t *vf_counts, int64_t *ee_counts);
+int ccd_stage_flops(int64_t *flops);
+
+/*
+ * The candidate stencils of the last ccd_detect / ccd_kdop_candidates call on this context, still on
+ * the device (owned by the context, valid until its next call): device addresses and counts.
+ */
+int ccd_last_candidates(ccd_ctx *ctx, const int32_t **vf_keys, int64_t *nvf, const int32_t **ee_keys,
+                        int64_t *nee);
 
 #ifdef __cplusplus
 }

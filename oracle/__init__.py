@@ -24,8 +24,8 @@ _lib = None
 _ref = None
 
 # Stage counters reported by orc_narrowphase (order of the Stage enum in ccd_oracle.cpp).
-STAGES = ("vf_plane", "vf_dist", "ee_dist", "ee_par", "ee_bary", "ve_quad", "ve_quart", "vv",
-          "quad_root", "isolate")
+STAGES = ("vf_call", "vf_plane", "vf_dist", "ee_dist", "ee_par", "ee_bary", "ve_call", "ve_quad", "ve_quart",
+          "vv", "quad_root", "isolate")
 
 RECORD_DTYPE = np.dtype([("hit", np.int32), ("subtest", np.int32), ("window", np.int32),
                          ("pad", np.int32), ("t", np.float64), ("margin", np.float64)])
@@ -98,15 +98,66 @@ def _ref_lib():
     return _ref
 
 
+_ROOTFN = ctypes.CFUNCTYPE(ctypes.c_int, _dp, ctypes.c_int, _dp, _dp)
+_mp_callback = None
+
+
+def _mp_roots(op, degree, zr, zi):
+    """Roots of the double polynomial op[0..degree] at 60 significant digits (mpmath), real parts and
+    imaginary parts rounded to double; exact zero roots first, as rpoly reports them."""
+    import mpmath as mp
+
+    c = [op[i] for i in range(degree + 1)]
+    if c[0] == 0.0:
+        return -1
+    nz = 0
+    while degree > 0 and c[degree] == 0.0:
+        zr[nz] = 0.0
+        zi[nz] = 0.0
+        nz += 1
+        degree -= 1
+        c = c[:-1]
+    if degree == 0:
+        return nz
+    with mp.workdps(60):
+        coeffs = [mp.mpf(x) for x in c]
+        roots = None
+        for steps in (200, 2000, 20000):
+            try:
+                roots = mp.polyroots(coeffs, maxsteps=steps, extraprec=400)
+                break
+            except mp.NoConvergence:
+                continue
+        if roots is None:
+            return nz
+        for k, z in enumerate(roots):
+            zr[nz + k] = float(mp.re(z))
+            zi[nz + k] = float(mp.im(z))
+    return nz + degree
+
+
 def use_rootfinder(which: str) -> None:
-    """Select the oracle's root finder: 'aberth' (built in) or 'rpoly' (the reference's, from _ref)."""
+    """Select the oracle's root finder: 'aberth' (built in), 'rpoly' (the reference's, from _ref) or
+    'mpmath' (the double polynomial's roots to 60 digits -- slow; the arbiter for disagreements)."""
+    global _mp_callback
     if which == "aberth":
         lib().orc_set_rootfinder(None)
     elif which == "rpoly":
         fn = ctypes.cast(_ref_lib().ref_rpoly, ctypes.c_void_p).value
         lib().orc_set_rootfinder(fn)
+    elif which == "mpmath":
+        if _mp_callback is None:
+            _mp_callback = _ROOTFN(_mp_roots)
+        lib().orc_set_rootfinder(ctypes.cast(_mp_callback, ctypes.c_void_p).value)
     else:
         raise ValueError(which)
+
+
+def use_default_rootfinder() -> str:
+    """The reference's own rpoly when oracle/_ref holds it, else the built-in Aberth iteration."""
+    which = "rpoly" if ref_rpoly_available() else "aberth"
+    use_rootfinder(which)
+    return which
 
 
 def set_dot_order(order: int) -> None:

@@ -67,11 +67,13 @@ inline double norm(V3 a) { return std::sqrt(dot(a, a)); }
 // Work counters (for the roofline's algorithmic flop count) and the decision margin.
 // ----------------------------------------------------------------------------------------------
 enum Stage {
-    ST_VF_PLANE = 0,   // one edge-side cubic of vertexFaceCTCD (CTCD.cpp:433-464)
+    ST_VF_CALL = 0,    // one vertexFaceCTCD call (CTCD.cpp:413-427)
+    ST_VF_PLANE,       // one edge-side cubic of vertexFaceCTCD (CTCD.cpp:433-464)
     ST_VF_DIST,        // the coplanar-distance sextic of vertexFaceCTCD (CTCD.cpp:466-475)
     ST_EE_DIST,        // the distance sextic of edgeEdgeCTCD (CTCD.cpp:275-288)
     ST_EE_PAR,         // one parallel-edge check of a raw coplanar interval (CTCD.cpp:294-308)
     ST_EE_BARY,        // one barycentric quartic of edgeEdgeCTCD (CTCD.cpp:313-350)
+    ST_VE_CALL,        // one vertexEdgeCTCD call (CTCD.cpp:511-534)
     ST_VE_QUAD,        // one side quadratic of vertexEdgeCTCD (CTCD.cpp:529-555)
     ST_VE_QUART,       // the colinear-distance quartic of vertexEdgeCTCD (CTCD.cpp:557-573)
     ST_VV,             // vertexVertexCTCD's quadratic (CTCD.cpp:604-639)
@@ -574,6 +576,7 @@ bool edge_edge_ctcd(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3 p
 bool vertex_face_ctcd(V3 q0s, V3 q1s, V3 q2s, V3 q3s, V3 q0e, V3 q1e, V3 q2e, V3 q3e, double eta, double &t,
                       Ctx *ctx)
 {
+    if (ctx) ctx->counts[ST_VF_CALL]++;
     double minD = eta * eta;
     V3 v0 = q0e - q0s;
     V3 v1 = q1e - q1s;
@@ -644,6 +647,7 @@ bool vertex_face_ctcd(V3 q0s, V3 q1s, V3 q2s, V3 q3s, V3 q0e, V3 q1e, V3 q2e, V3
 // CTCD.cpp:511-602.  Vertex q0 against edge (q1,q2).
 bool vertex_edge_ctcd(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double eta, double &t, Ctx *ctx)
 {
+    if (ctx) ctx->counts[ST_VE_CALL]++;
     double op[5];
     double minD = eta * eta;
     V3 v0 = q0e - q0s;

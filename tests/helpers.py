@@ -1,0 +1,98 @@
+"""Shared helpers for the test suite (not collected)."""
+import ctypes
+
+import numpy as np
+
+dp = ctypes.POINTER(ctypes.c_double)
+ip = ctypes.POINTER(ctypes.c_int32)
+
+
+def core_narrowphase(H, qs, qe, keys, is_ee, eta):
+    keys = np.ascontiguousarray(keys, np.int32).reshape(-1, 4)
+    n = len(keys)
+    hit = np.zeros(n, np.int32)
+    t = np.zeros(n)
+    sub = np.zeros(n, np.int32)
+    qs = np.ascontiguousarray(qs, dtype=np.float64)
+    qe = np.ascontiguousarray(qe, dtype=np.float64)
+    H.core_narrowphase(qs.ctypes.data_as(dp), qe.ctypes.data_as(dp), keys.ctypes.data_as(ip), ctypes.c_int64(n),
+                       int(is_ee), ctypes.c_double(eta), hit.ctypes.data_as(ip), t.ctypes.data_as(dp),
+                       sub.ctypes.data_as(ip))
+    return hit.astype(bool), t, sub
+
+
+def core_primitive(H, kind, q, eta):
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+    t = ctypes.c_double(0.0)
+    r = H.core_primitive(int(kind), q.ctypes.data_as(dp), ctypes.c_double(eta), ctypes.byref(t))
+    return bool(r), (t.value if r else None)
+
+
+def core_find_intervals(H, op, pos):
+    op = np.ascontiguousarray(op, dtype=np.float64)
+    out = np.zeros(32)
+    k = H.core_find_intervals(op.ctypes.data_as(dp), len(op) - 1, int(pos), out.ctypes.data_as(dp))
+    return [(out[2 * i], out[2 * i + 1]) for i in range(k)]
+
+
+def core_roots01(H, c):
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    r = np.zeros(8)
+    k = H.core_roots01(c.ctypes.data_as(dp), len(c) - 1, r.ctypes.data_as(dp))
+    return r[:k]
+
+
+def compare_results(name, hit_a, t_a, rec, tol_t=1e-9, margin_tol=1e-12, arbiter=None):
+    """Compare a product result (hit, t) with oracle records.
+
+    Flag mismatches whose oracle margin is below margin_tol are listed as borderline.  With an arbiter
+    (a function idx -> oracle records recomputed with exact roots, oracle root finder 'mpmath'), every
+    other disagreement -- a flag, or a time of impact more than tol_t apart -- is re-decided against
+    the exact-root answer: if the product agrees with it, the disagreement is the reference root
+    finder's error and is listed under ref_root_error; otherwise it is hard."""
+    hit_o = rec["hit"].astype(bool)
+    mism = np.where(hit_a != hit_o)[0]
+    border = mism[rec["margin"][mism] < margin_tol]
+    hard = mism[rec["margin"][mism] >= margin_tol]
+    both = hit_a & hit_o
+    tdiff = np.abs(t_a[both] - rec["t"][both])
+    idx_both = np.where(both)[0]
+    t_bad = idx_both[tdiff > tol_t]
+    t_bad_hard = t_bad[rec["margin"][t_bad] >= margin_tol]
+    ref_err = []
+    if arbiter is not None and (len(hard) or len(t_bad_hard)):
+        cand = np.unique(np.concatenate([hard, t_bad_hard])).astype(np.int64)
+        ex = arbiter(cand)
+        keep_hard, keep_t = [], []
+        for j, i in enumerate(cand):
+            eh = bool(ex["hit"][j])
+            ok = (eh == bool(hit_a[i])) and (not eh or abs(ex["t"][j] - t_a[i]) <= tol_t)
+            if ok:
+                ref_err.append((int(i), float(rec["t"][i]), float(ex["t"][j]), float(t_a[i])))
+            elif ex["margin"][j] < margin_tol:
+                pass
+            elif i in set(hard.tolist()):
+                keep_hard.append(int(i))
+            else:
+                keep_t.append(int(i))
+        hard = np.array(keep_hard, dtype=np.int64)
+        t_bad_hard = np.array(keep_t, dtype=np.int64)
+    return {"name": name, "n": len(hit_a), "hits": int(hit_o.sum()), "mismatch": len(mism),
+            "borderline": border.tolist()[:20], "hard": hard.tolist()[:20],
+            "max_tdiff": float(tdiff.max()) if len(tdiff) else 0.0, "t_bad": t_bad.tolist()[:20],
+            "t_bad_hard": t_bad_hard.tolist()[:20], "ref_root_error": ref_err[:20], "n_ref_root_error": len(ref_err)}
+
+
+def mpmath_arbiter(hist, keys, eta, is_ee):
+    """idx -> oracle records for keys[idx] computed with exact (60-digit) roots."""
+    import oracle
+
+    def run(idx):
+        eta_arr = np.broadcast_to(np.asarray(eta, dtype=np.float64), (len(keys),))
+        try:
+            oracle.use_rootfinder("mpmath")
+            return oracle.narrowphase(hist, keys[idx], eta_arr[idx], is_ee)
+        finally:
+            oracle.use_default_rootfinder()
+
+    return run

@@ -1,0 +1,113 @@
+"""The product's FP64 core (collisiondetection_amd/csrc/ccd_core.h), compiled for the host, against the
+oracle: the algorithm itself, before any device arithmetic is involved."""
+import numpy as np
+import pytest
+
+import oracle
+from collisiondetection_amd import scenes
+from tests.helpers import (compare_results, core_find_intervals, core_narrowphase, core_primitive, core_roots01,
+                           mpmath_arbiter)
+from tests.test_oracle import ETA, TESTCTCD, crossing_cases, expected_testctcd
+
+KIND = {"vf": 0, "ee": 1, "ve": 2, "vv": 3}
+
+
+def test_core_testctcd_cases(core_harness):
+    exp = expected_testctcd()
+    for k in ("vv", "ve", "vf", "ee"):
+        r, t = core_primitive(core_harness, KIND[k], TESTCTCD[k], ETA)
+        o = {"vv": oracle.vertex_vertex_ctcd, "ve": oracle.vertex_edge_ctcd, "vf": oracle.vertex_face_ctcd,
+             "ee": oracle.edge_edge_ctcd}[k](TESTCTCD[k], ETA)
+        assert r and o[0]
+        assert abs(t - o[1]) < 1e-12
+        assert abs(t - exp[k]) < 1e-10
+    # closed-form (degree <= 2) answers are bit-identical
+    assert core_primitive(core_harness, 3, TESTCTCD["vv"], ETA)[1] == exp["vv"]
+    assert core_primitive(core_harness, 2, TESTCTCD["ve"], ETA)[1] == exp["ve"]
+
+
+def test_core_crossing_cases(core_harness):
+    for kind, q, hit, t in crossing_cases(ETA):
+        r, tt = core_primitive(core_harness, KIND[kind], q, ETA)
+        assert r == hit, kind
+        if hit:
+            assert abs(tt - t) < 1e-10
+
+
+def test_core_quadratic_intervals_bit_exact(core_harness):
+    rng = np.random.default_rng(3)
+    for _ in range(3000):
+        op = rng.normal(size=3)
+        op[rng.random(3) < 0.15] = 0.0
+        for pos in (True, False):
+            a = core_find_intervals(core_harness, op, pos)
+            b, _, _ = oracle.find_intervals(op, pos)
+            assert a == b, (op, pos, a, b)
+
+
+@pytest.mark.parametrize("deg", [3, 4, 6])
+def test_core_intervals_vs_oracle(core_harness, deg):
+    rng = np.random.default_rng(deg)
+    bad = 0
+    for _ in range(3000):
+        op = rng.normal(size=deg + 1) * np.exp(2 * rng.normal(size=deg + 1))
+        for pos in (True, False):
+            a = core_find_intervals(core_harness, op, pos)
+            b, _, m = oracle.find_intervals(op, pos)
+            # union of the intervals: equal up to the root-finder accuracy
+            ua = _union(a)
+            ub = _union(b)
+            if len(ua) != len(ub) or any(abs(x - y) > 1e-9 for p, q in zip(ua, ub) for x, y in zip(p, q)):
+                if m >= 1e-12:
+                    bad += 1
+    assert bad == 0
+
+
+def _union(iv):
+    iv = sorted(iv)
+    out = []
+    for l, u in iv:
+        if out and l <= out[-1][1]:
+            out[-1] = (out[-1][0], max(out[-1][1], u))
+        else:
+            out.append((l, u))
+    return out
+
+
+def test_core_roots_vs_mpmath(core_harness):
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 50
+    rng = np.random.default_rng(21)
+    for _ in range(300):
+        n = int(rng.integers(3, 7))
+        # polynomials with known roots, some inside [0,1]
+        roots = rng.uniform(-0.5, 1.5, size=n)
+        c = np.poly(roots) * rng.uniform(0.5, 2)
+        got = core_roots01(core_harness, c)
+        ref = sorted(float(mp.re(z)) for z in mp.polyroots([mp.mpf(float(x)) for x in c], maxsteps=300, extraprec=300)
+                     if abs(mp.im(z)) < 1e-20 and 0 <= mp.re(z) <= 1)
+        gaps = np.diff(np.sort(roots)) if n > 1 else [1]
+        if min(gaps) < 1e-6:
+            continue
+        assert len(got) == len(ref)
+        # forward error of a root of the double polynomial: eps * sum|c_i||r|^(n-i) / |p'(r)|
+        for g, r in zip(got, ref):
+            cond = sum(abs(ci) * abs(r) ** (n - i) for i, ci in enumerate(c)) / abs(np.polyval(np.polyder(c), r))
+            assert abs(g - r) <= 8 * 2.2e-16 * cond + 1e-15, (g, r, cond)
+
+
+@pytest.mark.parametrize("scene", ["two_spheres", "converging", "cloth64"])
+def test_core_narrowphase_vs_oracle(core_harness, scene):
+    s = {"two_spheres": scenes.two_spheres, "converging": scenes.converging_spheres,
+         "cloth64": lambda: scenes.cloth(64, drop=0.05, noise=0.01)}[scene]()
+    h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+    vf, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
+    for keys, is_ee in ((vf, False), (ee, True)):
+        rec = oracle.narrowphase(h, keys, s["eta"], is_ee)
+        hit, t, sub = core_narrowphase(core_harness, s["qstart"], s["qend"], keys, is_ee, s["eta"])
+        rep = compare_results(scene, hit, t, rec, arbiter=mpmath_arbiter(h, keys, s["eta"], is_ee))
+        print(rep)
+        assert not rep["hard"], rep
+        assert not rep["t_bad_hard"], rep
+        both = hit & rec["hit"].astype(bool)
+        assert np.array_equal(sub[both], rec["subtest"][both])

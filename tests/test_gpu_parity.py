@@ -1,0 +1,281 @@
+"""GPU parity: libccd_amd.so (HIP, gfx950) against the CPU oracle and the host-built core.
+
+Bars: candidate sets bit-exact; collision flags equal except stencils whose oracle decision margin is
+below 1e-12 (listed, not hidden); times of impact within 1e-9 absolute; the GPU core bit-identical to
+the same core compiled for the host (device FP64 arithmetic: no contraction, IEEE div/sqrt).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from collisiondetection_amd import scenes
+from tests.helpers import compare_results, core_narrowphase, core_primitive, mpmath_arbiter
+from tests.test_oracle import ETA, TESTCTCD, crossing_cases, expected_testctcd
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def api():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    from collisiondetection_amd import api as A
+
+    return A
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+KIND = {"vf": 0, "ee": 1, "ve": 2, "vv": 3}
+WIDTH = {"vf": 24, "ee": 24, "ve": 18, "vv": 12}
+
+
+def test_gpu_testctcd_and_crossings(api):
+    say("case: testCTCD + crossings")
+    exp = expected_testctcd()
+    for k in ("vv", "ve", "vf", "ee"):
+        hit, t = api.primitives(KIND[k], TESTCTCD[k].reshape(1, -1), ETA)
+        assert hit[0]
+        assert abs(t[0] - exp[k]) < 1e-10
+    assert api.primitives(3, TESTCTCD["vv"].reshape(1, -1), ETA)[1][0] == exp["vv"]
+    assert api.primitives(2, TESTCTCD["ve"].reshape(1, -1), ETA)[1][0] == exp["ve"]
+    for kind, q, hit, t in crossing_cases(ETA):
+        h, tt = api.primitives(KIND[kind], q.reshape(1, -1), ETA)
+        assert bool(h[0]) == hit
+        if hit:
+            assert abs(tt[0] - t) < 1e-10
+
+
+def _random_primitives(kind, n, rng):
+    """Random primitives drawn so that many come close: vertices near a common point, moving across."""
+    w = WIDTH[kind] // 6
+    s = rng.normal(size=(n, w, 3)) * 0.5
+    v = rng.normal(size=(n, w, 3)) * 0.7
+    return np.concatenate([s, s + v], axis=1).reshape(n, -1)
+
+
+@pytest.mark.parametrize("kind", ["vf", "ee", "ve", "vv"])
+def test_gpu_primitives_bit_identical_to_host_core(api, core_harness, kind):
+    say("case: primitives bit-identical", kind)
+    rng = np.random.default_rng(100 + KIND[kind])
+    q = _random_primitives(kind, 4000, rng)
+    eta = 0.05
+    hit, t = api.primitives(KIND[kind], q, eta)
+    nh = 0
+    for i in range(len(q)):
+        r, tt = core_primitive(core_harness, KIND[kind], q[i], eta)
+        assert bool(hit[i]) == r, (kind, i)
+        if r:
+            nh += 1
+            assert t[i] == tt, (kind, i, t[i], tt)
+    assert nh > 50
+
+
+@pytest.mark.parametrize("kind", ["vf", "ee", "ve", "vv"])
+def test_gpu_primitives_vs_oracle(api, kind):
+    say("case: primitives vs oracle", kind)
+    rng = np.random.default_rng(200 + KIND[kind])
+    q = _random_primitives(kind, 3000, rng)
+    eta = 0.05
+    hit, t = api.primitives(KIND[kind], q, eta)
+    fn = {"vf": oracle.vertex_face_ctcd, "ee": oracle.edge_edge_ctcd, "ve": oracle.vertex_edge_ctcd,
+          "vv": oracle.vertex_vertex_ctcd}[kind]
+    bad = []
+    for i in range(len(q)):
+        r, tt, m = fn(q[i], eta)
+        if (bool(hit[i]) != r and m >= 1e-12) or (r and hit[i] and abs(t[i] - tt) > 1e-9 and m >= 1e-12):
+            bad.append(i)
+    # Arbitrate the disagreements against exact (60-digit) roots of the same polynomials.
+    hard, ref_err = [], []
+    try:
+        oracle.use_rootfinder("mpmath")
+        for i in bad:
+            r, tt, m = fn(q[i], eta)
+            if bool(hit[i]) == r and (not r or abs(t[i] - tt) <= 1e-9):
+                ref_err.append(i)
+            elif m >= 1e-12:
+                hard.append(i)
+    finally:
+        oracle.use_default_rootfinder()
+    say(kind, "reference root-finder errors (GPU agrees with exact roots):", len(ref_err))
+    assert not hard, hard[:10]
+
+
+SCENES = {
+    "two_spheres": lambda: scenes.two_spheres(),
+    "converging": lambda: scenes.converging_spheres(),
+    "cloth64": lambda: scenes.cloth(64, drop=0.05, noise=0.01),
+}
+
+
+@pytest.mark.parametrize("scene", list(SCENES))
+def test_gpu_candidates_equal_oracle(api, scene):
+    say("case: candidates", scene)
+    s = SCENES[scene]()
+    h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+    for fixed in (None, set(range(0, len(s["qstart"]), 5))):
+        ovf, oee = oracle.candidates(s["faces"], h, s["eta"], fixed=fixed, method="tree")
+        gvf, gee = api.kdop_candidates(s["faces"], s["qstart"], s["qend"], s["eta"], fixed=fixed, sort=True)
+        assert np.array_equal(ovf, gvf), (scene, len(ovf), len(gvf))
+        assert np.array_equal(oee, gee), (scene, len(oee), len(gee))
+    assert len(ovf) > 0
+
+
+def test_gpu_candidates_non_manifold_fallback(api):
+    """Degenerate and duplicate faces switch the broadphase to sort-based deduplication."""
+    say("case: candidates with duplicate and degenerate faces")
+    s = scenes.converging_spheres(levels=1)
+    faces = np.concatenate([s["faces"], s["faces"][:7], np.array([[0, 0, 5], [3, 9, 3]], np.int32)])
+    h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+    ovf, oee = oracle.candidates(faces, h, s["eta"], method="brute")
+    gvf, gee = api.kdop_candidates(faces, s["qstart"], s["qend"], s["eta"], sort=True)
+    assert np.array_equal(ovf, gvf) and np.array_equal(oee, gee)
+
+
+def test_gpu_candidates_small_meshes(api):
+    say("case: candidates on 0, 1 and 2 faces")
+    q = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 0.1], [1, 0, 0.1], [0, 1, 0.1]], float)
+    for faces in (np.zeros((0, 3), np.int32), np.array([[0, 1, 2]], np.int32), np.array([[0, 1, 2], [3, 4, 5]], np.int32)):
+        gvf, gee = api.kdop_candidates(faces, q, q, 0.5)
+        ovf, oee = oracle.candidates(faces, oracle.Hist.two_entry(q, q), 0.5, method="brute")
+        assert np.array_equal(ovf, gvf) and np.array_equal(oee, gee)
+
+
+@pytest.mark.parametrize("scene", list(SCENES))
+def test_gpu_narrowphase_vs_oracle_and_core(api, core_harness, scene):
+    say("case: narrowphase", scene)
+    s = SCENES[scene]()
+    h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+    vf, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
+    r = api.narrowphase(s["qstart"], s["qend"], vf, ee, s["eta"])
+    for keys, is_ee, pre in ((vf, False, "vf"), (ee, True, "ee")):
+        rec = oracle.narrowphase(h, keys, s["eta"], is_ee)
+        rep = compare_results(scene + pre, r[pre + "_hit"], r[pre + "_t"], rec,
+                              arbiter=mpmath_arbiter(h, keys, s["eta"], is_ee))
+        say(rep)
+        assert not rep["hard"] and not rep["t_bad_hard"], rep
+        ch, ct, cs = core_narrowphase(core_harness, s["qstart"], s["qend"], keys, is_ee, s["eta"])
+        assert np.array_equal(ch, r[pre + "_hit"])
+        assert np.array_equal(ct.view(np.int64), r[pre + "_t"].view(np.int64))
+
+
+def test_gpu_narrowphase_per_stencil_eta(api):
+    say("case: per-stencil eta")
+    s = scenes.converging_spheres()
+    h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+    vf, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
+    rng = np.random.default_rng(4)
+    ve = 10.0 ** rng.uniform(-7, -2, size=len(vf))
+    eev = 10.0 ** rng.uniform(-7, -2, size=len(ee))
+    r = api.narrowphase(s["qstart"], s["qend"], vf, ee, 0.0, vf_eta=ve, ee_eta=eev)
+    for keys, etas, is_ee, pre in ((vf, ve, False, "vf"), (ee, eev, True, "ee")):
+        rec = oracle.narrowphase(h, keys, etas, is_ee)
+        rep = compare_results("eta" + pre, r[pre + "_hit"], r[pre + "_t"], rec,
+                              arbiter=mpmath_arbiter(h, keys, etas, is_ee))
+        assert not rep["hard"] and not rep["t_bad_hard"], rep
+
+
+def test_gpu_narrowphase_history(api):
+    """Three-entry histories (a bent path through a midpoint): stitched windows."""
+    say("case: multi-entry history")
+    s = scenes.converging_spheres()
+    rng = np.random.default_rng(9)
+    V = len(s["qstart"])
+    qm = 0.5 * (s["qstart"] + s["qend"]) + rng.normal(0, 0.02, size=(V, 3))
+    tmid = rng.uniform(0.2, 0.8, size=V)   # per-vertex breakpoints: windows differ per stencil
+    off = np.arange(0, 3 * V + 1, 3, dtype=np.int64)
+    ht = np.stack([np.zeros(V), tmid, np.ones(V)], axis=1).reshape(-1)
+    hp = np.stack([s["qstart"], qm, s["qend"]], axis=1).reshape(-1)
+    h3 = oracle.Hist(off, ht, hp)
+    ovf, oee = oracle.candidates(s["faces"], h3, s["eta"], method="tree")
+    gvf, gee = api.kdop_candidates(s["faces"], None, None, s["eta"], hist=(off, ht, hp))
+    assert np.array_equal(ovf, gvf) and np.array_equal(oee, gee)
+    r = api.narrowphase_history(off, ht, hp, ovf, oee, s["eta"])
+    for keys, is_ee, pre in ((ovf, False, "vf"), (oee, True, "ee")):
+        rec = oracle.narrowphase(h3, keys, s["eta"], is_ee)
+        rep = compare_results("hist" + pre, r[pre + "_hit"], r[pre + "_t"], rec,
+                              arbiter=mpmath_arbiter(h3, keys, s["eta"], is_ee))
+        assert not rep["hard"] and not rep["t_bad_hard"], rep
+        hit = r[pre + "_hit"]
+        assert np.array_equal(r[pre + "_win"][hit & rec["hit"].astype(bool)],
+                              rec["window"][hit & rec["hit"].astype(bool)])
+    assert r["vf_hit"].sum() + r["ee_hit"].sum() > 0
+
+
+def test_gpu_detect_equals_oracle(api):
+    say("case: detect")
+    s = scenes.converging_spheres()
+    h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+    vf, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
+    rv = oracle.narrowphase(h, vf, s["eta"], False)
+    re = oracle.narrowphase(h, ee, s["eta"], True)
+    d = api.detect(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"])
+    assert d["vf_candidates"] == len(vf) and d["ee_candidates"] == len(ee)
+    for got, toi, keys, rec, is_ee in ((d["vf"], d["vf_toi"], vf, rv, False), (d["ee"], d["ee_toi"], ee, re, True)):
+        g = {tuple(k): t for k, t in zip(got.tolist(), toi)}
+        ghit = np.array([tuple(k) in g for k in keys.tolist()])
+        gt = np.array([g.get(tuple(k), 0.0) for k in keys.tolist()])
+        rep = compare_results("detect", ghit, gt, rec, arbiter=mpmath_arbiter(h, keys, s["eta"], is_ee))
+        assert not rep["hard"] and not rep["t_bad_hard"], rep
+    assert len(d["vf"]) + len(d["ee"]) > 0
+    # capacity handling: a too-small collision buffer is retried with the reported size
+    d2 = api.detect(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"], cap=1)
+    assert len(d2["vf"]) == len(d["vf"]) and len(d2["ee"]) == len(d["ee"])
+
+
+def test_gpu_work_counts_match_oracle(api):
+    say("case: work counts")
+    import ctypes
+    from collisiondetection_amd import _native as N
+
+    s = scenes.converging_spheres()
+    h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+    vf, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
+    _, cv = oracle.narrowphase(h, vf, s["eta"], False, with_counts=True)
+    _, ce = oracle.narrowphase(h, ee, s["eta"], True, with_counts=True)
+    ctx = api.default_context()
+    import torch
+
+    dv = lambda x, dt: api.to_device(x, dt, ctx)
+    qs, qe = dv(s["qstart"], torch.float64), dv(s["qend"], torch.float64)
+    vk, ek = dv(vf, torch.int32), dv(ee, torch.int32)
+    a = (ctypes.c_int64 * 12)()
+    b = (ctypes.c_int64 * 12)()
+    ctx.bind_stream()
+    ctx.check(N.lib().ccd_narrowphase_work(ctx.ptr, qs.data_ptr(), qe.data_ptr(), len(s["qstart"]), vk.data_ptr(),
+                                           len(vf), ek.data_ptr(), len(ee), s["eta"], a, b))
+    for i, name in enumerate(N.STAGE_NAMES):
+        if name == "ee_par":   # per raw interval: the reference also splits at complex roots' real parts
+            assert b[i] <= ce[name]
+            continue
+        assert a[i] == cv[name], name
+        assert b[i] == ce[name], name
+
+
+@pytest.mark.parametrize("n", [256])
+def test_gpu_cloth_full_size(api, n):
+    """The 256^2 cloth: candidate counts match the oracle; flags on a sample and on every GPU hit."""
+    say("case: cloth", n)
+    s = scenes.cloth(n)
+    h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+    ovf, oee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
+    gvf, gee = api.kdop_candidates(s["faces"], s["qstart"], s["qend"], s["eta"], sort=True)
+    assert np.array_equal(ovf, gvf) and np.array_equal(oee, gee)
+    say("candidates", len(ovf), len(oee))
+    r = api.narrowphase(s["qstart"], s["qend"], gvf, gee, s["eta"])
+    rng = np.random.default_rng(1)
+    for keys, pre, is_ee in ((gvf, "vf", False), (gee, "ee", True)):
+        hit = r[pre + "_hit"]
+        idx = np.unique(np.concatenate([rng.choice(len(keys), size=min(20000, len(keys)), replace=False),
+                                        np.where(hit)[0]]))
+        rec = oracle.narrowphase(h, keys[idx], s["eta"], is_ee)
+        rep = compare_results("cloth" + pre, hit[idx], r[pre + "_t"][idx], rec,
+                              arbiter=mpmath_arbiter(h, keys[idx], s["eta"], is_ee))
+        say(rep)
+        assert not rep["hard"] and not rep["t_bad_hard"], rep
