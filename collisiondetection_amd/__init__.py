@@ -7,11 +7,3 @@ KDOPBroadPhase -> CTCDNarrowPhase -> CTCD primitives, behind the C ABI of includ
 from . import scenes  # noqa: F401
 
 __all__ = ["scenes", "api"]
-
-
-def __getattr__(name):
-    if name == "api":
-        from . import api
-
-        return api
-    raise AttributeError(name)

@@ -48,6 +48,10 @@ SIGNATURES = {
                                      _INT], _INT),
     "ccd_detect": ([_V, _V, _I64, _V, _V, _I64, _D, _D, _V, _V, _V, _I64, _P64, _V, _V, _I64, _P64, _P64, _P64],
                    _INT),
+    "ccd_device_alloc": ([_V, _I64, ctypes.POINTER(_V)], _INT),
+    "ccd_device_free": ([_V, _V], _INT),
+    "ccd_copy_to_device": ([_V, _V, _V, _I64], _INT),
+    "ccd_copy_to_host": ([_V, _V, _V, _I64], _INT),
     "ccd_set_profiling": ([_V, _INT], _INT),
     "ccd_stage_times": ([_V, ctypes.POINTER(_D), _P64, _INT], _INT),
     "ccd_narrowphase_work": ([_V, _V, _V, _I64, _V, _I64, _V, _I64, _D, _P64, _P64], _INT),

@@ -372,6 +372,46 @@ int ccd_detect(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces, const double 
     return CCD_OK;
 }
 
+int ccd_device_alloc(ccd_ctx *ctx, int64_t bytes, void **ptr)
+{
+    if (!ctx || !ptr || bytes < 0) return CCD_E_INVALID;
+    *ptr = nullptr;
+    if (bytes == 0) return CCD_OK;
+    CCD_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (hipMalloc(ptr, (size_t)bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(ctx, CCD_E_OOM, "device allocation of %lld bytes failed", (long long)bytes);
+    }
+    return CCD_OK;
+}
+
+int ccd_device_free(ccd_ctx *ctx, void *ptr)
+{
+    if (!ctx) return CCD_E_INVALID;
+    if (!ptr) return CCD_OK;
+    CCD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    CCD_HIP_CHECK(ctx, hipFree(ptr));
+    return CCD_OK;
+}
+
+int ccd_copy_to_device(ccd_ctx *ctx, void *dst, const void *src, int64_t bytes)
+{
+    if (!ctx || bytes < 0 || (bytes > 0 && (!dst || !src))) return CCD_E_INVALID;
+    if (bytes == 0) return CCD_OK;
+    CCD_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream));
+    CCD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return CCD_OK;
+}
+
+int ccd_copy_to_host(ccd_ctx *ctx, void *dst, const void *src, int64_t bytes)
+{
+    if (!ctx || bytes < 0 || (bytes > 0 && (!dst || !src))) return CCD_E_INVALID;
+    if (bytes == 0) return CCD_OK;
+    CCD_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream));
+    CCD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return CCD_OK;
+}
+
 int ccd_set_profiling(ccd_ctx *ctx, int on)
 {
     if (!ctx) return CCD_E_INVALID;

@@ -145,6 +145,15 @@ int ccd_detect(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces, const double 
                int64_t *ee_hit_count, int64_t *vf_candidates, int64_t *ee_candidates);
 
 /*
+ * Device memory helpers for host callers that do not use the HIP API themselves (the C++ classes in
+ * include/ccdamd/ use them to stage host data).  Copies are synchronous on the context's stream.
+ */
+int ccd_device_alloc(ccd_ctx *ctx, int64_t bytes, void **ptr);
+int ccd_device_free(ccd_ctx *ctx, void *ptr);
+int ccd_copy_to_device(ccd_ctx *ctx, void *dst, const void *src, int64_t bytes);
+int ccd_copy_to_host(ccd_ctx *ctx, void *dst, const void *src, int64_t bytes);
+
+/*
  * Instrumentation (no counterpart in the reference, which times nothing).
  * ccd_set_profiling: when on, every pipeline stage is bracketed by HIP events on the context's stream
  * and calls that run a narrowphase synchronise at their end; ccd_stage_times returns the summed

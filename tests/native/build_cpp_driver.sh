@@ -1,0 +1,8 @@
+#!/bin/bash
+# Builds the C++ interface driver against include/ccdamd and libccd_amd.so (no HIP headers needed).
+set -e
+cd "$(dirname "$0")"
+mkdir -p ../_build
+ROOT=$(cd ../.. && pwd)
+g++ -O2 -std=c++17 -Wall -I"$ROOT/include/ccdamd" -o ../_build/cpp_interface_driver cpp_interface_driver.cpp \
+    -L"$ROOT/collisiondetection_amd" -lccd_amd -Wl,-rpath,"$ROOT/collisiondetection_amd"
