@@ -26,6 +26,11 @@ PRIM_VERTEX_VERTEX = 3
 
 CCD_SORTED = 1
 
+DIST_VERTEX_PLANE_LT = 0
+DIST_LINE_LINE_LT = 1
+DIST_VERTEX_FACE = 2
+DIST_EDGE_EDGE = 3
+
 # Every exported symbol of include/ccd_amd.h with its ctypes signature.
 _V = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -48,6 +53,8 @@ SIGNATURES = {
                                      _INT], _INT),
     "ccd_detect": ([_V, _V, _I64, _V, _V, _I64, _D, _D, _V, _V, _V, _I64, _P64, _V, _V, _I64, _P64, _P64, _P64],
                    _INT),
+    "ccd_distance_primitives": ([_V, _INT, _V, _D, _I64, _V, _V, _V], _INT),
+    "ccd_mesh_self_distance": ([_V, _V, _I64, _V, _I64, _V, ctypes.POINTER(_D), _P64, _P64], _INT),
     "ccd_device_alloc": ([_V, _I64, ctypes.POINTER(_V)], _INT),
     "ccd_device_free": ([_V, _V], _INT),
     "ccd_copy_to_device": ([_V, _V, _V, _I64], _INT),

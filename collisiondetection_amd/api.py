@@ -414,3 +414,40 @@ class CTCD:
     @staticmethod
     def vertexVertexCTCD(q1start, q2start, q1end, q2end, eta):
         return CTCD._one(N.PRIM_VERTEX_VERTEX, (q1start, q2start, q1end, q2end), eta)
+
+
+# ------------------------------------------------------------------------------------------------
+# Proximity
+# ------------------------------------------------------------------------------------------------
+def distance_primitives(kind: int, q, eta: float = 0.0, ctx: Context | None = None):
+    """The Distance statics on n items (q: (n, 12)); returns (flag, vec (n,3), bary (n,4)) as numpy."""
+    torch = _torch()
+    ctx = ctx or default_context()
+    ctx.bind_stream()
+    qd = to_device(q, torch.float64, ctx).reshape(-1)
+    n = qd.numel() // 12
+    dev = ctx.torch_device
+    flag = torch.zeros(n, dtype=torch.uint8, device=dev)
+    vec = torch.zeros(3 * n, dtype=torch.float64, device=dev)
+    bary = torch.zeros(4 * n, dtype=torch.float64, device=dev)
+    ctx.check(N.lib().ccd_distance_primitives(ctx.ptr, int(kind), _ptr(qd), float(eta), n, _ptr(flag), _ptr(vec),
+                                              _ptr(bary)))
+    torch.cuda.synchronize(dev)
+    return flag.cpu().numpy().astype(bool), vec.cpu().numpy().reshape(-1, 3), bary.cpu().numpy().reshape(-1, 4)
+
+
+def mesh_self_distance(verts, faces, fixed=None, ctx: Context | None = None):
+    """Distance::meshSelfDistance -> (distance, n_vf_candidates, n_ee_candidates)."""
+    torch = _torch()
+    ctx = ctx or default_context()
+    ctx.bind_stream()
+    v = to_device(verts, torch.float64, ctx).reshape(-1)
+    f = to_device(faces, torch.int32, ctx).reshape(-1)
+    V, F = v.numel() // 3, f.numel() // 3
+    fx = _fixed_mask(fixed, V, ctx)
+    d = ctypes.c_double(0.0)
+    nv = ctypes.c_int64(0)
+    ne = ctypes.c_int64(0)
+    ctx.check(N.lib().ccd_mesh_self_distance(ctx.ptr, _ptr(v), V, _ptr(f), F, _ptr(fx), ctypes.byref(d),
+                                             ctypes.byref(nv), ctypes.byref(ne)))
+    return d.value, nv.value, ne.value

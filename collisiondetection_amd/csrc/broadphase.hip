@@ -103,11 +103,24 @@ __device__ __forceinline__ void dop_point(const Axes &ax, double x, double y, do
     }
 }
 
+// AABBBroadPhase.cpp:31-36: per coordinate, min/max of pos[k] -/+ eta.
+__device__ __forceinline__ void aabb_point(double x, double y, double z, double eta, double *mn, double *mx)
+{
+    double p[3] = {x, y, z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        mn[k] = ccd::smin(p[k] - eta, mn[k]);
+        mx[k] = ccd::smax(p[k] + eta, mx[k]);
+    }
+}
+
 // Two-entry histories (qs = entry at t=0, qe = entry at t=1) or CSR histories (hoff/hpos).
+// aabb != 0 gives AABBBroadPhase's bounds (AABBBroadPhase.cpp:9-40): the three coordinates -/+ eta,
+// with the other ten slabs unbounded so that the 13-axis overlap test reduces to the 3-axis one.
 __global__ __launch_bounds__(256) void k_leaf_bounds(const int32_t *__restrict__ faces, int64_t F,
                                                      const double *__restrict__ qs, const double *__restrict__ qe,
                                                      const int64_t *__restrict__ hoff,
-                                                     const double *__restrict__ hpos, double eta, Axes ax,
+                                                     const double *__restrict__ hpos, double eta, Axes ax, int aabb,
                                                      double *__restrict__ dop64, double *__restrict__ centre)
 {
     int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -121,13 +134,29 @@ __global__ __launch_bounds__(256) void k_leaf_bounds(const int32_t *__restrict__
     for (int j = 0; j < 3; j++) {
         int v = faces[3 * f + j];
         if (hoff) {
-            for (int64_t e = hoff[v]; e < hoff[v + 1]; e++)
-                dop_point(ax, hpos[3 * e], hpos[3 * e + 1], hpos[3 * e + 2], eta, mn, mx);
+            for (int64_t e = hoff[v]; e < hoff[v + 1]; e++) {
+                if (aabb)
+                    aabb_point(hpos[3 * e], hpos[3 * e + 1], hpos[3 * e + 2], eta, mn, mx);
+                else
+                    dop_point(ax, hpos[3 * e], hpos[3 * e + 1], hpos[3 * e + 2], eta, mn, mx);
+            }
         } else {
             const double *a = qs + 3 * (int64_t)v;
             const double *b = qe + 3 * (int64_t)v;
-            dop_point(ax, a[0], a[1], a[2], eta, mn, mx);
-            dop_point(ax, b[0], b[1], b[2], eta, mn, mx);
+            if (aabb) {
+                aabb_point(a[0], a[1], a[2], eta, mn, mx);
+                aabb_point(b[0], b[1], b[2], eta, mn, mx);
+            } else {
+                dop_point(ax, a[0], a[1], a[2], eta, mn, mx);
+                dop_point(ax, b[0], b[1], b[2], eta, mn, mx);
+            }
+        }
+    }
+    if (aabb) {
+#pragma unroll
+        for (int k = 3; k < KAX; k++) {
+            mn[k] = -__builtin_inf();
+            mx[k] = __builtin_inf();
         }
     }
     double *o = dop64 + NB * f;
@@ -700,9 +729,42 @@ static int sort_unique(ccd_ctx *ctx, int32_t *keys, int64_t n, const char *tag, 
     return CCD_OK;
 }
 
+int build_vertex_face_table(ccd_ctx *ctx, const int32_t *faces, int64_t F, int64_t V)
+{
+    hipStream_t st = ctx->stream;
+    int32_t *vcnt = (int32_t *)buffer(ctx, "vf_cnt", (size_t)(V + 1) * 4);
+    int32_t *voff = (int32_t *)buffer(ctx, "vf_off", (size_t)(V + 1) * 4);
+    int32_t *vlist = (int32_t *)buffer(ctx, "vf_list", (size_t)(3 * F + 1) * 4);
+    if (!vcnt || !voff || !vlist) return CCD_E_OOM;
+    CCD_HIP_CHECK(ctx, hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, st));
+    if (F > 0) {
+        hipLaunchKernelGGL(k_vf_count, dim3(blocks_for(3 * F, 256)), dim3(256), 0, st, faces, F, vcnt);
+        CCD_LAUNCH_CHECK(ctx);
+    }
+    size_t stmp2 = 0;
+    CCD_HIP_CHECK(ctx, rocprim::exclusive_scan((void *)nullptr, stmp2, vcnt, voff, 0, (size_t)(V + 1),
+                                               rocprim::plus<int32_t>(), st));
+    void *scan_tmp = buffer(ctx, "scan_tmp", stmp2 + 16);
+    if (!scan_tmp) return CCD_E_OOM;
+    CCD_HIP_CHECK(ctx, rocprim::exclusive_scan(scan_tmp, stmp2, vcnt, voff, 0, (size_t)(V + 1),
+                                               rocprim::plus<int32_t>(), st));
+    CCD_HIP_CHECK(ctx, hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, st));
+    if (F > 0) {
+        hipLaunchKernelGGL(k_vf_fill, dim3(blocks_for(3 * F, 256)), dim3(256), 0, st, faces, F, voff, vcnt, vlist);
+        CCD_LAUNCH_CHECK(ctx);
+    }
+    return CCD_OK;
+}
+
+void vertex_face_table(ccd_ctx *ctx, const int32_t **off, const int32_t **list)
+{
+    *off = (const int32_t *)ctx->bufs["vf_off"].ptr;
+    *list = (const int32_t *)ctx->bufs["vf_list"].ptr;
+}
+
 int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, const double *qe,
                const int64_t *hist_off, const double *hist_pos, int64_t V, double eta, const uint8_t *fixed,
-               Candidates &out)
+               Candidates &out, bool aabb)
 {
     out = Candidates();
     int64_t *cnt = ctx->d_counts;
@@ -731,17 +793,14 @@ int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, 
     NodeRec *nodes = (NodeRec *)buffer(ctx, "nodes", (size_t)(2 * F) * sizeof(NodeRec));
     double *sdop64 = (double *)buffer(ctx, "sdop64", (size_t)F * NB * 8);
     int4 *sfaces = (int4 *)buffer(ctx, "sfaces", (size_t)F * 16);
-    int32_t *vcnt = (int32_t *)buffer(ctx, "vf_cnt", (size_t)(V + 1) * 4);
-    int32_t *voff = (int32_t *)buffer(ctx, "vf_off", (size_t)(V + 1) * 4);
-    int32_t *vlist = (int32_t *)buffer(ctx, "vf_list", (size_t)(3 * F) * 4);
     if (!dop64 || !centre || !partial || !keys || !keys2 || !vals || !order || !left || !rf || !rl || !parent ||
-        !escape || !arrivals || !nodes || !sdop64 || !sfaces || !vcnt || !voff || !vlist)
+        !escape || !arrivals || !nodes || !sdop64 || !sfaces)
         return CCD_E_OOM;
     hipStream_t st = ctx->stream;
 
     prof_begin(ctx, CCD_T_BOUNDS);
     hipLaunchKernelGGL(k_leaf_bounds, dim3(blocks_for(F, 256)), dim3(256), 0, st, faces, F, qs, qe, hist_off,
-                       hist_pos, eta, axes, dop64, centre);
+                       hist_pos, eta, axes, aabb ? 1 : 0, dop64, centre);
     CCD_LAUNCH_CHECK(ctx);
     int nb = (int)std::min<int64_t>(nparts, blocks_for(F, 256));
     hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(256), 0, st, centre, F, partial);
@@ -777,19 +836,10 @@ int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, 
     prof_begin(ctx, CCD_T_TABLES);
 
     // Vertex -> face table and mesh checks.
-    CCD_HIP_CHECK(ctx, hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, st));
-    hipLaunchKernelGGL(k_vf_count, dim3(blocks_for(3 * F, 256)), dim3(256), 0, st, faces, F, vcnt);
-    CCD_LAUNCH_CHECK(ctx);
-    size_t stmp2 = 0;
-    CCD_HIP_CHECK(ctx, rocprim::exclusive_scan((void *)nullptr, stmp2, vcnt, voff, 0, (size_t)(V + 1),
-                                               rocprim::plus<int32_t>(), st));
-    void *scan_tmp = buffer(ctx, "scan_tmp", stmp2 + 16);
-    if (!scan_tmp) return CCD_E_OOM;
-    CCD_HIP_CHECK(ctx, rocprim::exclusive_scan(scan_tmp, stmp2, vcnt, voff, 0, (size_t)(V + 1),
-                                               rocprim::plus<int32_t>(), st));
-    CCD_HIP_CHECK(ctx, hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, st));
-    hipLaunchKernelGGL(k_vf_fill, dim3(blocks_for(3 * F, 256)), dim3(256), 0, st, faces, F, voff, vcnt, vlist);
-    CCD_LAUNCH_CHECK(ctx);
+    int rc0 = build_vertex_face_table(ctx, faces, F, V);
+    if (rc0) return rc0;
+    const int32_t *voff = nullptr, *vlist = nullptr;
+    vertex_face_table(ctx, &voff, &vlist);
     unsigned long long *c = (unsigned long long *)cnt;
     hipLaunchKernelGGL(k_mesh_check, dim3(blocks_for(F, 256)), dim3(256), 0, st, faces, F, voff, vlist, c + C_BAD);
     CCD_LAUNCH_CHECK(ctx);

@@ -372,6 +372,33 @@ int ccd_detect(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces, const double 
     return CCD_OK;
 }
 
+int ccd_distance_primitives(ccd_ctx *ctx, int kind, const double *q, double eta, int64_t n, uint8_t *flag,
+                            double *vec, double *bary)
+{
+    if (!ctx) return CCD_E_INVALID;
+    CCD_REQUIRE(ctx, kind >= 0 && kind <= 3, "unknown proximity kind");
+    CCD_REQUIRE(ctx, n >= 0, "negative item count");
+    if (n == 0) return CCD_OK;
+    CCD_REQUIRE(ctx, q, "null q");
+    CCD_REQUIRE(ctx, kind >= CCD_DIST_VERTEX_FACE || flag, "null flag output");
+    CCD_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    return launch_distance_primitives(ctx, kind, q, eta, n, flag, vec, bary);
+}
+
+int ccd_mesh_self_distance(ccd_ctx *ctx, const double *verts, int64_t nverts, const int32_t *faces, int64_t nfaces,
+                           const uint8_t *fixed, double *distance, int64_t *nvf, int64_t *nee)
+{
+    if (!ctx) return CCD_E_INVALID;
+    CCD_REQUIRE(ctx, nverts >= 0 && nfaces >= 0, "negative size");
+    CCD_REQUIRE(ctx, nfaces < (int64_t)1 << 30 && nverts < (int64_t)1 << 31, "mesh too large for 32-bit indices");
+    CCD_REQUIRE(ctx, distance, "null distance output");
+    CCD_REQUIRE(ctx, (nfaces == 0 || faces) && (nverts == 0 || verts), "null mesh arrays");
+    CCD_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    int rc = validate_faces(ctx, faces, nfaces, nverts);
+    if (rc) return rc;
+    return mesh_self_distance(ctx, verts, nverts, faces, nfaces, fixed, distance, nvf, nee);
+}
+
 int ccd_device_alloc(ccd_ctx *ctx, int64_t bytes, void **ptr)
 {
     if (!ctx || !ptr || bytes < 0) return CCD_E_INVALID;

@@ -790,6 +790,130 @@ CCD_HD int check_ee_window(const V3 *s, const V3 *e, double eta, double &t, C &&
     return -1;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Proximity tests (include/Distance.h:14-174), straight-line FP64 in the reference's expression order.
+// ------------------------------------------------------------------------------------------------
+
+// Distance.h:14-19: is p closer than eta to the plane of (q0, q1, q2)?  No division.
+CCD_HD bool vertex_plane_distance_less_than(V3 p, V3 q0, V3 q1, V3 q2, double eta)
+{
+    V3 c = cross(q1 - q0, q2 - q0);
+    return dot(c, p - q0) * dot(c, p - q0) < eta * eta * dot(c, c);
+}
+
+// Distance.h:23-28: is the line (p0,p1) closer than eta to the line (q0,q1)?  No division.
+CCD_HD bool line_line_distance_less_than(V3 p0, V3 p1, V3 q0, V3 q1, double eta)
+{
+    V3 c = cross(p1 - p0, q1 - q0);
+    return dot(c, q0 - p0) * dot(c, q0 - p0) < eta * eta * dot(c, c);
+}
+
+CCD_FHD double clamp01(double u) { return smin(1.0, smax(u, 0.0)); }   // Distance.h:183-186
+
+// Distance.h:32-115: vector from p to its closest point on the triangle (q0,q1,q2), by Voronoi region,
+// with the closest point's barycentric coordinates.
+CCD_HD V3 vertex_face_distance(V3 p, V3 q0, V3 q1, V3 q2, double &b0, double &b1, double &b2)
+{
+    V3 ab = q1 - q0;
+    V3 ac = q2 - q0;
+    V3 ap = p - q0;
+    double d1 = dot(ab, ap);
+    double d2 = dot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) {
+        b0 = 1.0;
+        b1 = 0.0;
+        b2 = 0.0;
+        return q0 - p;
+    }
+    V3 bp = p - q1;
+    double d3 = dot(ab, bp);
+    double d4 = dot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) {
+        b0 = 0.0;
+        b1 = 1.0;
+        b2 = 0.0;
+        return q1 - p;
+    }
+    double vc = d1 * d4 - d3 * d2;
+    if ((vc <= 0) && (d1 >= 0) && (d3 <= 0)) {
+        double v = d1 / (d1 - d3);
+        b0 = 1.0 - v;
+        b1 = v;
+        b2 = 0;
+        return (q0 + v * ab) - p;
+    }
+    V3 cp = p - q2;
+    double d5 = dot(ab, cp);
+    double d6 = dot(ac, cp);
+    if (d6 >= 0 && d5 <= d6) {
+        b0 = 0;
+        b1 = 0;
+        b2 = 1.0;
+        return q2 - p;
+    }
+    double vb = d5 * d2 - d1 * d6;
+    if ((vb <= 0) && (d2 >= 0) && (d6 <= 0)) {
+        double w = d2 / (d2 - d6);
+        b0 = 1 - w;
+        b1 = 0;
+        b2 = w;
+        return (q0 + w * ac) - p;
+    }
+    double va = d3 * d6 - d5 * d4;
+    if ((va <= 0) && (d4 - d3 >= 0) && (d5 - d6 >= 0)) {
+        double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        b0 = 0;
+        b1 = 1.0 - w;
+        b2 = w;
+        return (q1 + w * (q2 - q1)) - p;
+    }
+    double denom = 1.0 / (va + vb + vc);
+    double v = vb * denom;
+    double w = vc * denom;
+    double u = 1.0 - v - w;
+    b0 = u;
+    b1 = v;
+    b2 = w;
+    return ((u * q0 + v * q1) + w * q2) - p;
+}
+
+// Distance.h:119-174: shortest vector from segment (p0,p1) to segment (q0,q1), with the closest
+// points' barycentric coordinates (parallel or degenerate segments handled as the reference does).
+CCD_HD V3 edge_edge_distance(V3 p0, V3 p1, V3 q0, V3 q1, double &p0b, double &p1b, double &q0b, double &q1b)
+{
+    V3 d1 = p1 - p0;
+    V3 d2 = q1 - q0;
+    V3 r = p0 - q0;
+    double a = dot(d1, d1);
+    double e = dot(d2, d2);
+    double f = dot(d2, r);
+    double s, t;
+    double c = dot(d1, r);
+    double b = dot(d1, d2);
+    double denom = a * e - b * b;
+    if (denom != 0.0)
+        s = clamp01((b * f - c * e) / denom);
+    else
+        s = 0;
+    double tnom = b * s + f;
+    if (tnom < 0 || e == 0) {
+        t = 0;
+        s = (a == 0) ? 0 : clamp01(-c / a);
+    } else if (tnom > e) {
+        t = 1.0;
+        s = (a == 0) ? 0 : clamp01((b - c) / a);
+    } else {
+        t = tnom / e;
+    }
+    V3 c1 = p0 + s * d1;
+    V3 c2 = q0 + t * d2;
+    p0b = 1.0 - s;
+    p1b = s;
+    q0b = 1.0 - t;
+    q1b = t;
+    return c2 - c1;
+}
+
 // The position History::stitchCommonHistory assigns at t = 0 for a two-entry history
 // (History.cpp:127-133 with dt = 1, a = 0, b = 1): (1*start + 0*end) / 1, kept literally so that
 // signed zeros come out as in the reference.

@@ -102,8 +102,18 @@ struct Candidates {
     int64_t nvf = 0, nee = 0, npairs = 0;
     int64_t vf_cap = 0, ee_cap = 0;
 };
+// aabb selects AABBBroadPhase's 3-axis bounds (used by the mesh self-distance), else the 13-axis k-DOP.
 int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, const double *qe,
                const int64_t *hist_off, const double *hist_pos, int64_t V, double eta, const uint8_t *fixed,
-               Candidates &out);
+               Candidates &out, bool aabb = false);
+// Vertex -> face CSR table (device buffers "vf_off" / "vf_list"), and its current addresses.
+int build_vertex_face_table(ccd_ctx *ctx, const int32_t *faces, int64_t F, int64_t V);
+void vertex_face_table(ccd_ctx *ctx, const int32_t **off, const int32_t **list);
 int sort_stencils(ccd_ctx *ctx, int32_t *keys, int64_t n);
+
+// Proximity (distance.hip)
+int launch_distance_primitives(ccd_ctx *ctx, int kind, const double *q, double eta, int64_t n, uint8_t *flag,
+                               double *vec, double *bary);
+int mesh_self_distance(ccd_ctx *ctx, const double *verts, int64_t V, const int32_t *faces, int64_t F,
+                       const uint8_t *fixed, double *out, int64_t *nvf, int64_t *nee);
 }   // namespace ccdi

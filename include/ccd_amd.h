@@ -36,6 +36,12 @@ extern "C" {
 #define CCD_PRIM_VERTEX_EDGE 2   /* CTCD::vertexEdgeCTCD   (include/CTCD.h:63-70)  18 doubles per item */
 #define CCD_PRIM_VERTEX_VERTEX 3 /* CTCD::vertexVertexCTCD (include/CTCD.h:75-79)  12 doubles per item */
 
+/* Proximity kinds for ccd_distance_primitives (include/Distance.h of the reference); 12 doubles per item. */
+#define CCD_DIST_VERTEX_PLANE_LT 0 /* Distance::vertexPlaneDistanceLessThan(p, q0, q1, q2, eta)  (Distance.h:14-19) */
+#define CCD_DIST_LINE_LINE_LT 1    /* Distance::lineLineDistanceLessThan(p0, p1, q0, q1, eta)     (Distance.h:23-28) */
+#define CCD_DIST_VERTEX_FACE 2     /* Distance::vertexFaceDistance(p, q0, q1, q2, b0, b1, b2)    (Distance.h:32-115) */
+#define CCD_DIST_EDGE_EDGE 3       /* Distance::edgeEdgeDistance(p0, p1, q0, q1, b0..b3)          (Distance.h:119-174) */
+
 /* Flags for ccd_kdop_candidates. */
 #define CCD_SORTED 1 /* return stencils in the reference's std::set order (Stencils.h:59-77, :119-137) */
 
@@ -143,6 +149,26 @@ int ccd_detect(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces, const double 
                int64_t nverts, double outer_eta, double eta, const uint8_t *fixed, int32_t *vf_hits, double *vf_toi,
                int64_t vf_cap, int64_t *vf_hit_count, int32_t *ee_hits, double *ee_toi, int64_t ee_cap,
                int64_t *ee_hit_count, int64_t *vf_candidates, int64_t *ee_candidates);
+
+/*
+ * Proximity tests on n items -- the Distance statics (include/Distance.h:14-174).  Item i's points are
+ * q[12*i .. 12*i+12) (device): (p, q0, q1, q2) or (p0, p1, q0, q1) in the reference's argument order.
+ * For the two LessThan kinds flag[i] (device) receives the boolean; for the two closest-point kinds
+ * vec (device, 3n, may be NULL) receives the returned vector and bary (device, 4n, may be NULL) the
+ * barycentric outputs (3 used for vertex-face).
+ */
+int ccd_distance_primitives(ccd_ctx *ctx, int kind, const double *q, double eta, int64_t n, uint8_t *flag,
+                            double *vec, double *bary);
+
+/*
+ * Distance::meshSelfDistance (src/Distance.cpp:12-66): the shortest distance between a vertex and a face
+ * not containing it, or between two edges sharing no vertex, over the mesh (verts: device, 3*nverts;
+ * faces: device, 3*nfaces; fixed: device, nverts bytes or NULL).  *distance (host) receives it
+ * (+inf when no such pair exists); *nvf / *nee (host, may be NULL) the numbers of candidate stencils
+ * checked, the counts the reference prints (Distance.cpp:47).  Synchronous.
+ */
+int ccd_mesh_self_distance(ccd_ctx *ctx, const double *verts, int64_t nverts, const int32_t *faces, int64_t nfaces,
+                           const uint8_t *fixed, double *distance, int64_t *nvf, int64_t *nee);
 
 /*
  * Device memory helpers for host callers that do not use the HIP API themselves (the C++ classes in

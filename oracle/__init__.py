@@ -78,6 +78,10 @@ def lib():
         L.orc_result_free.argtypes = [ctypes.c_void_p]
         L.orc_narrowphase.argtypes = [_lp, _dp, _dp, _ip, _dp, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, _lp]
         L.orc_stage_count.restype = ctypes.c_int
+        L.orc_distance.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp, _dp]
+        L.orc_distance.restype = ctypes.c_int
+        L.orc_mesh_self_distance.argtypes = [_dp, ctypes.c_int64, _ip, ctypes.c_int64, _up, _lp, _lp]
+        L.orc_mesh_self_distance.restype = ctypes.c_double
         assert L.orc_stage_count() == len(STAGES)
         _lib = L
     return _lib
@@ -331,3 +335,33 @@ def narrowphase(hist: Hist, keys, eta, is_ee: bool, with_counts=False):
     if with_counts:
         return rec, dict(zip(STAGES, counts.tolist()))
     return rec
+
+
+# ------------------------------------------------------------------------------------------------
+# Proximity (include/Distance.h, src/Distance.cpp)
+# ------------------------------------------------------------------------------------------------
+def distance(kind: int, q, eta: float = 0.0):
+    """kind 0/1: (flag,) ; kind 2/3: (vec[3], bary[4])."""
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+    vec = np.zeros(3)
+    bary = np.zeros(4)
+    r = lib().orc_distance(int(kind), _d(q), float(eta), _d(vec), _d(bary))
+    if kind < 2:
+        return bool(r)
+    return vec, bary
+
+
+def mesh_self_distance(verts, faces, fixed=None):
+    """Distance::meshSelfDistance -> (distance, n_vf_candidates, n_ee_candidates)."""
+    v = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1)
+    f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+    V, F = len(v) // 3, len(f) // 3
+    fx = None
+    if fixed is not None:
+        fx_arr = np.zeros(V, dtype=np.uint8)
+        fx_arr[np.asarray(list(fixed), dtype=np.int64)] = 1
+        fx = fx_arr.ctypes.data_as(_up)
+    nv = ctypes.c_int64(0)
+    ne = ctypes.c_int64(0)
+    d = lib().orc_mesh_self_distance(_d(v), V, _i(f), F, fx, ctypes.byref(nv), ctypes.byref(ne))
+    return d, nv.value, ne.value

@@ -1196,6 +1196,138 @@ void candidates_tree(const int32_t *faces, int64_t F, const std::vector<Dop> &do
     }
 }
 
+// ----------------------------------------------------------------------------------------------
+// Proximity: include/Distance.h:14-174 and src/Distance.cpp:12-66
+// ----------------------------------------------------------------------------------------------
+bool vertex_plane_lt(V3 p, V3 q0, V3 q1, V3 q2, double eta)   // Distance.h:14-19
+{
+    V3 c = cross(q1 - q0, q2 - q0);
+    return dot(c, p - q0) * dot(c, p - q0) < eta * eta * dot(c, c);
+}
+bool line_line_lt(V3 p0, V3 p1, V3 q0, V3 q1, double eta)   // Distance.h:23-28
+{
+    V3 c = cross(p1 - p0, q1 - q0);
+    return dot(c, q0 - p0) * dot(c, q0 - p0) < eta * eta * dot(c, c);
+}
+double clamp_unit(double u) { return smin(1.0, smax(u, 0.0)); }   // Distance.h:183-186
+
+V3 vertex_face_distance(V3 p, V3 q0, V3 q1, V3 q2, double *bary)   // Distance.h:32-115
+{
+    V3 ab = q1 - q0, ac = q2 - q0, ap = p - q0;
+    double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) {
+        bary[0] = 1.0; bary[1] = 0.0; bary[2] = 0.0;
+        return q0 - p;
+    }
+    V3 bp = p - q1;
+    double d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) {
+        bary[0] = 0.0; bary[1] = 1.0; bary[2] = 0.0;
+        return q1 - p;
+    }
+    double vc = d1 * d4 - d3 * d2;
+    if ((vc <= 0) && (d1 >= 0) && (d3 <= 0)) {
+        double v = d1 / (d1 - d3);
+        bary[0] = 1.0 - v; bary[1] = v; bary[2] = 0;
+        return (q0 + v * ab) - p;
+    }
+    V3 cp = p - q2;
+    double d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0 && d5 <= d6) {
+        bary[0] = 0; bary[1] = 0; bary[2] = 1.0;
+        return q2 - p;
+    }
+    double vb = d5 * d2 - d1 * d6;
+    if ((vb <= 0) && (d2 >= 0) && (d6 <= 0)) {
+        double w = d2 / (d2 - d6);
+        bary[0] = 1 - w; bary[1] = 0; bary[2] = w;
+        return (q0 + w * ac) - p;
+    }
+    double va = d3 * d6 - d5 * d4;
+    if ((va <= 0) && (d4 - d3 >= 0) && (d5 - d6 >= 0)) {
+        double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        bary[0] = 0; bary[1] = 1.0 - w; bary[2] = w;
+        return (q1 + w * (q2 - q1)) - p;
+    }
+    double denom = 1.0 / (va + vb + vc);
+    double v = vb * denom, w = vc * denom;
+    double u = 1.0 - v - w;
+    bary[0] = u; bary[1] = v; bary[2] = w;
+    return ((u * q0 + v * q1) + w * q2) - p;
+}
+
+V3 edge_edge_distance(V3 p0, V3 p1, V3 q0, V3 q1, double *bary)   // Distance.h:119-174
+{
+    V3 d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
+    double a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+    double s, t;
+    double c = dot(d1, r), b = dot(d1, d2);
+    double denom = a * e - b * b;
+    if (denom != 0.0)
+        s = clamp_unit((b * f - c * e) / denom);
+    else
+        s = 0;
+    double tnom = b * s + f;
+    if (tnom < 0 || e == 0) {
+        t = 0;
+        if (a == 0) s = 0; else s = clamp_unit(-c / a);
+    } else if (tnom > e) {
+        t = 1.0;
+        if (a == 0) s = 0; else s = clamp_unit((b - c) / a);
+    } else
+        t = tnom / e;
+    V3 c1 = p0 + s * d1, c2 = q0 + t * d2;
+    bary[0] = 1.0 - s; bary[1] = s; bary[2] = 1.0 - t; bary[3] = t;
+    return c2 - c1;
+}
+
+// Distance.cpp:12-66
+double mesh_self_distance(const double *verts, int64_t V, const int32_t *faces, int64_t F, const uint8_t *fixed,
+                          int64_t *nvf, int64_t *nee)
+{
+    double closest = std::numeric_limits<double>::infinity();
+    for (int64_t i = 0; i < V; i++)
+        for (int64_t j = 0; j < F; j++) {
+            const int32_t *fj = faces + 3 * j;
+            if (fj[0] == i || fj[1] == i || fj[2] == i) continue;   // Mesh::vertexOfFace
+            for (int k = 0; k < 3; k++) {
+                V3 d = load3(verts + 3 * i) - load3(verts + 3 * (int64_t)fj[k]);
+                double dist = dot(d, d);
+                if (dist < closest) closest = dist;
+            }
+        }
+    closest = std::sqrt(closest);
+    // History(verts) + finishHistory(verts): two identical entries per vertex.
+    std::vector<int64_t> off(V + 1);
+    std::vector<double> ht(2 * V), hp(6 * V);
+    for (int64_t v = 0; v <= V; v++) off[v] = 2 * v;
+    for (int64_t v = 0; v < V; v++) {
+        ht[2 * v] = 0.0;
+        ht[2 * v + 1] = 1.0;
+        for (int k = 0; k < 3; k++) hp[6 * v + k] = hp[6 * v + 3 + k] = verts[3 * v + k];
+    }
+    Hist h{off.data(), ht.data(), hp.data()};
+    std::vector<Dop> d;
+    leaf_dops(faces, F, h, closest, true, d);
+    CandSets cs;
+    candidates_bruteforce(faces, F, d, 3, fixed, cs, nullptr);
+    *nvf = (int64_t)cs.vf.size();
+    *nee = (int64_t)cs.ee.size();
+    closest = std::numeric_limits<double>::infinity();
+    double bary[4];
+    for (const Key &k : cs.vf) {
+        double dist = norm(vertex_face_distance(load3(verts + 3 * (int64_t)k.k[0]), load3(verts + 3 * (int64_t)k.k[1]),
+                                                load3(verts + 3 * (int64_t)k.k[2]), load3(verts + 3 * (int64_t)k.k[3]), bary));
+        if (dist < closest) closest = dist;
+    }
+    for (const Key &k : cs.ee) {
+        double dist = norm(edge_edge_distance(load3(verts + 3 * (int64_t)k.k[0]), load3(verts + 3 * (int64_t)k.k[1]),
+                                              load3(verts + 3 * (int64_t)k.k[2]), load3(verts + 3 * (int64_t)k.k[3]), bary));
+        if (dist < closest) closest = dist;
+    }
+    return closest;
+}
+
 // Result holder handed across the C boundary.
 struct Result {
     CandSets cs;
@@ -1375,5 +1507,24 @@ void orc_narrowphase(const int64_t *off, const double *ht, const double *hp, con
 }
 
 int orc_stage_count() { return ST_COUNT; }
+
+// Proximity statics on one item: q = 4 points (12 doubles).  kind as CCD_DIST_* in include/ccd_amd.h.
+int orc_distance(int kind, const double *q, double eta, double *vec, double *bary)
+{
+    V3 a = load3(q), b = load3(q + 3), c = load3(q + 6), d = load3(q + 9);
+    if (kind == 0) return vertex_plane_lt(a, b, c, d, eta);
+    if (kind == 1) return line_line_lt(a, b, c, d, eta);
+    V3 r = kind == 2 ? vertex_face_distance(a, b, c, d, bary) : edge_edge_distance(a, b, c, d, bary);
+    vec[0] = r.x;
+    vec[1] = r.y;
+    vec[2] = r.z;
+    return 1;
+}
+
+double orc_mesh_self_distance(const double *verts, int64_t V, const int32_t *faces, int64_t F, const uint8_t *fixed,
+                              int64_t *nvf, int64_t *nee)
+{
+    return mesh_self_distance(verts, V, faces, F, fixed, nvf, nee);
+}
 
 }   // extern "C"

@@ -96,3 +96,14 @@ def mpmath_arbiter(hist, keys, eta, is_ee):
             oracle.use_default_rootfinder()
 
     return run
+
+
+def core_distance(H, kind, q, eta=0.0):
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+    vec = np.zeros(3)
+    bary = np.zeros(4)
+    r = H.core_distance(int(kind), q.ctypes.data_as(dp), ctypes.c_double(eta), vec.ctypes.data_as(dp),
+                        bary.ctypes.data_as(dp))
+    if kind < 2:
+        return bool(r)
+    return vec, bary

@@ -75,3 +75,17 @@ extern "C" int core_roots01(const double *c, int n, double *r)
     }
     return -1;
 }
+
+// Proximity statics (kind as CCD_DIST_*): returns the flag for kinds 0/1, else writes vec[3], bary[4].
+extern "C" int core_distance(int kind, const double *q, double eta, double *vec, double *bary)
+{
+    V3 a = ld(q), b = ld(q + 3), c = ld(q + 6), d = ld(q + 9);
+    if (kind == 0) return ccd::vertex_plane_distance_less_than(a, b, c, d, eta);
+    if (kind == 1) return ccd::line_line_distance_less_than(a, b, c, d, eta);
+    V3 r = kind == 2 ? ccd::vertex_face_distance(a, b, c, d, bary[0], bary[1], bary[2])
+                     : ccd::edge_edge_distance(a, b, c, d, bary[0], bary[1], bary[2], bary[3]);
+    vec[0] = r.x;
+    vec[1] = r.y;
+    vec[2] = r.z;
+    return 1;
+}

@@ -1,0 +1,131 @@
+"""Proximity (include/Distance.h, src/Distance.cpp:12-66): the oracle pinned by closed forms, the host
+core against the oracle bit for bit, and (GPU) the device path against both."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from collisiondetection_amd import scenes
+from tests.helpers import core_distance
+
+TRI = np.array([[0, 0, 0], [2, 0, 0], [0, 2, 0]], float)
+
+
+def closed_form_vertex_face():
+    """(point, expected distance, expected closest point) with p in each Voronoi region of TRI."""
+    return [
+        ([0.5, 0.5, 3.0], 3.0, [0.5, 0.5, 0]),            # face interior
+        ([-1.0, -1.0, 0.0], math.sqrt(2), [0, 0, 0]),     # vertex q0 region
+        ([3.0, -1.0, 0.0], math.sqrt(2), [2, 0, 0]),      # vertex q1 region
+        ([-1.0, 3.0, 0.0], math.sqrt(2), [0, 2, 0]),      # vertex q2 region
+        ([1.0, -2.0, 0.0], 2.0, [1, 0, 0]),               # edge q0q1
+        ([-2.0, 1.0, 1.0], math.sqrt(5), [0, 1, 0]),      # edge q0q2
+        ([2.0, 2.0, 0.0], math.sqrt(2), [1, 1, 0]),       # edge q1q2
+    ]
+
+
+def test_vertex_face_distance_closed_form():
+    for p, d, c in closed_form_vertex_face():
+        vec, bary = oracle.distance(2, np.concatenate([p, TRI.reshape(-1)]))
+        assert abs(np.linalg.norm(vec) - d) < 1e-14
+        assert np.allclose(np.array(p) + vec, c, atol=1e-14)
+        assert abs(bary[:3].sum() - 1) < 1e-14
+        assert np.allclose(bary[:3] @ TRI, c, atol=1e-14)
+
+
+def test_edge_edge_distance_closed_form():
+    cases = [
+        # skew perpendicular segments: x-axis segment and a y-parallel segment 3 above: distance 3
+        ([0, 0, 0], [2, 0, 0], [1, -1, 3], [1, 1, 3], 3.0, (0.5, 0.5)),
+        # parallel segments 1 apart (denominator 0: s = 0, then t from the projection)
+        ([0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0], 1.0, None),
+        # endpoint-to-endpoint beyond both segments
+        ([0, 0, 0], [1, 0, 0], [3, 0, 0], [4, 0, 0], 2.0, (1.0, 0.0)),
+        # degenerate (point) segment against a segment
+        ([1, 5, 0], [1, 5, 0], [0, 0, 0], [2, 0, 0], 5.0, (0.0, 0.5)),
+    ]
+    for p0, p1, q0, q1, d, st in cases:
+        vec, bary = oracle.distance(3, np.concatenate([p0, p1, q0, q1]).astype(float))
+        assert abs(np.linalg.norm(vec) - d) < 1e-14
+        if st is not None:
+            assert abs(bary[1] - st[0]) < 1e-14 and abs(bary[3] - st[1]) < 1e-14
+
+
+def test_less_than_tests_closed_form():
+    plane = np.concatenate([[0.3, 0.2, 0.5], TRI.reshape(-1)])
+    assert oracle.distance(0, plane, 0.51) and not oracle.distance(0, plane, 0.49)
+    lines = np.array([0, 0, 0, 1, 0, 0, 0, 0, 2, 0, 1, 2], float)   # x-axis vs y-line at z = 2
+    assert oracle.distance(1, lines, 2.01) and not oracle.distance(1, lines, 1.99)
+
+
+def test_mesh_self_distance_closed_form():
+    # Two unit icospheres 2.05 apart along x: the subdivided meshes have vertices at (+-1, 0, 0), so the
+    # closest pair is the two vertices on the x axis, 0.05 apart.
+    for levels in (1, 2):
+        s = scenes.two_spheres(levels=levels, offset=2.05)
+        d, nvf, nee = oracle.mesh_self_distance(s["qstart"], s["faces"])
+        assert abs(d - 0.05) < 1e-14
+        assert nvf > 0 and nee > 0
+    # a single triangle has no vertex-face or edge-edge pair: +inf
+    d, _, _ = oracle.mesh_self_distance(TRI, np.array([[0, 1, 2]], np.int32))
+    assert d == math.inf
+    # two parallel triangles 0.25 apart
+    v = np.concatenate([TRI, TRI + [0, 0, 0.25]])
+    d, _, _ = oracle.mesh_self_distance(v, np.array([[0, 1, 2], [3, 4, 5]], np.int32))
+    assert abs(d - 0.25) < 1e-15
+
+
+def _random_items(rng, n):
+    return rng.normal(size=(n, 12))
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_core_distance_bit_identical_to_oracle(core_harness, kind):
+    rng = np.random.default_rng(50 + kind)
+    q = _random_items(rng, 3000)
+    q[::7, 3:6] = q[::7, 0:3]   # degenerate segments / triangles
+    for i in range(len(q)):
+        eta = abs(rng.normal())
+        a = core_distance(core_harness, kind, q[i], eta)
+        b = oracle.distance(kind, q[i], eta)
+        if kind < 2:
+            assert a == b
+        else:
+            assert np.array_equal(a[0].view(np.int64), b[0].view(np.int64))
+            assert np.array_equal(a[1].view(np.int64), b[1].view(np.int64))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_gpu_distance_primitives(core_harness, kind):
+    from collisiondetection_amd import api
+
+    print("case: distance primitives", kind, flush=True)
+    rng = np.random.default_rng(60 + kind)
+    q = _random_items(rng, 4000)
+    q[::7, 3:6] = q[::7, 0:3]
+    eta = 0.7
+    flag, vec, bary = api.distance_primitives(kind, q, eta)
+    for i in range(len(q)):
+        a = core_distance(core_harness, kind, q[i], eta)
+        if kind < 2:
+            assert bool(flag[i]) == a == oracle.distance(kind, q[i], eta)
+        else:
+            assert np.array_equal(vec[i].view(np.int64), a[0].view(np.int64))
+            assert np.array_equal(bary[i].view(np.int64), a[1].view(np.int64))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scene", ["two_spheres", "converging", "cloth64"])
+def test_gpu_mesh_self_distance(scene):
+    from collisiondetection_amd import api
+
+    print("case: mesh self distance", scene, flush=True)
+    s = {"two_spheres": scenes.two_spheres, "converging": scenes.converging_spheres,
+         "cloth64": lambda: scenes.cloth(64)}[scene]()
+    for fixed in (None, set(range(0, len(s["qstart"]), 3))):
+        d, nvf, nee = oracle.mesh_self_distance(s["qstart"], s["faces"], fixed=fixed)
+        g, gvf, gee = api.mesh_self_distance(s["qstart"], s["faces"], fixed=fixed)
+        assert g == d, (g, d)
+        assert (gvf, gee) == (nvf, nee)

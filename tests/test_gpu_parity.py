@@ -56,6 +56,8 @@ def _random_primitives(kind, n, rng):
     """Random primitives drawn so that many come close: vertices near a common point, moving across."""
     w = WIDTH[kind] // 6
     s = rng.normal(size=(n, w, 3)) * 0.5
+    if kind == "vv":   # two points that pass near the origin at about the same time
+        return np.concatenate([s, -s + rng.normal(size=(n, w, 3)) * 0.05], axis=1).reshape(n, -1)
     v = rng.normal(size=(n, w, 3)) * 0.7
     return np.concatenate([s, s + v], axis=1).reshape(n, -1)
 
