@@ -25,8 +25,6 @@
 // assumptions; for them the raw stencils are emitted and deduplicated by sorting.
 #include <cstring>
 #include <rocprim/device/device_merge_sort.hpp>
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_select.hpp>
 
 #include "ccd_core.h"
@@ -253,12 +251,16 @@ __device__ __forceinline__ uint64_t spread21(uint64_t x)
     return x;
 }
 
+// Keys are (instance id << (63 - inst_bits)) | Morton code with (63 - inst_bits) / 3 bits per axis:
+// sorting groups each instance's faces, so every instance owns one subtree of the radix tree.
 __global__ __launch_bounds__(256) void k_morton(const double *__restrict__ centre, int64_t F,
-                                                const double *__restrict__ box, uint64_t *__restrict__ keys,
-                                                int32_t *__restrict__ vals)
+                                                const double *__restrict__ box, const int32_t *__restrict__ face_inst,
+                                                int inst_bits, uint64_t *__restrict__ keys, int32_t *__restrict__ vals)
 {
     int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (f >= F) return;
+    const int axis_bits = (63 - inst_bits) / 3;
+    const double scale = (double)((1u << axis_bits) - 1);
     uint64_t code = 0;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -267,11 +269,29 @@ __global__ __launch_bounds__(256) void k_morton(const double *__restrict__ centr
         double u = ext > 0 ? (centre[3 * f + c] - lo) / ext : 0.0;
         u = u < 0 ? 0 : (u > 1 ? 1 : u);
         if (!(u == u)) u = 0;   // NaN positions sort first; bounds still decide every pair
-        uint64_t q = (uint64_t)(u * 2097151.0);
+        uint64_t q = (uint64_t)(u * scale);
         code |= spread21(q) << c;
     }
+    if (face_inst) code |= (uint64_t)face_inst[f] << (3 * axis_bits);
     keys[f] = code;
     vals[f] = (int32_t)f;
+}
+
+// Instance of each face: the last instance whose first face is <= f.
+__global__ __launch_bounds__(256) void k_face_instance(const int64_t *__restrict__ inst_off, int ninst, int64_t F,
+                                                       int32_t *__restrict__ face_inst)
+{
+    int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (inst_off[mid] <= f)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    face_inst[f] = lo;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -427,9 +447,14 @@ __device__ __forceinline__ bool neighbours(int4 a, int4 b)   // Mesh.cpp:13-20
            a.z == b.y || a.z == b.z;
 }
 
+// inst_end: for batched instances, the sorted index one past the query's instance (its faces are the
+// sorted range [inst_off[k], inst_off[k+1])), else null (one instance).  An escape link always leads
+// to a node whose first leaf is its index (internal) or its leaf number: once that reaches inst_end
+// the walk has left the instance's subtree and stops.
 __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__restrict__ nodes,
                                                   const double *__restrict__ sdop64,
-                                                  const int4 *__restrict__ sfaces, int2 *__restrict__ pairs,
+                                                  const int4 *__restrict__ sfaces, const int32_t *__restrict__ face_inst,
+                                                  const int64_t *__restrict__ inst_off, int2 *__restrict__ pairs,
                                                   int64_t cap, unsigned long long *counter)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -438,6 +463,7 @@ __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__re
     double q64[NB];
     int4 qf = make_int4(0, 0, 0, 0);
     uint32_t node = NIL;
+    int64_t iend = F;
     if (active) {
         const NodeRec &me = nodes[F - 1 + i];
 #pragma unroll
@@ -447,7 +473,11 @@ __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__re
         }
         qf = sfaces[i];
         node = me.esc;
+        if (face_inst) iend = inst_off[face_inst[qf.w] + 1];
     }
+    const uint32_t nint = (uint32_t)(F - 1);
+    auto first_leaf = [nint](uint32_t n) -> int64_t { return n < nint ? (int64_t)n : (int64_t)(n - nint); };
+    if (node != NIL && first_leaf(node) >= iend) node = NIL;
     const int lane = __lane_id();
     while (true) {
         bool emit = false;
@@ -458,19 +488,23 @@ __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__re
 #pragma unroll
             for (int k = 0; k < NB; k++) b[k] = n.b[k];
             uint32_t child = n.child, esc = n.esc;
+            bool descend = false;
             if (overlap32(q32, b)) {
-                if (node < (uint32_t)(F - 1)) {
-                    node = child;
+                if (node < nint) {
+                    descend = true;
                 } else {
                     int4 of = sfaces[child];
                     if (overlap64(q64, sdop64 + NB * (int64_t)child) && !neighbours(qf, of)) {
                         emit = true;
                         other = of.w;
                     }
-                    node = esc;
                 }
+            }
+            if (descend) {
+                node = child;
             } else {
                 node = esc;
+                if (node != NIL && first_leaf(node) >= iend) node = NIL;
             }
         }
         uint64_t m = __ballot(emit);
@@ -624,13 +658,16 @@ __device__ __forceinline__ void append(bool want, int4 key, int4 *out, int64_t c
     }
 }
 
-template <bool OWNER>
+// raw_dev: the mesh check's count of degenerate or duplicate faces; nonzero switches every lane to raw
+// emission (all 15 stencils of a pair), deduplicated afterwards by sorting.
 __global__ __launch_bounds__(256) void k_expand(MeshView m, const int2 *__restrict__ pairs,
                                                 const unsigned long long *__restrict__ npairs_dev, int64_t pairs_cap,
+                                                const unsigned long long *__restrict__ raw_dev,
                                                 int4 *__restrict__ vf_out, int64_t vf_cap,
                                                 unsigned long long *vf_counter, int4 *__restrict__ ee_out,
                                                 int64_t ee_cap, unsigned long long *ee_counter)
 {
+    const bool OWNER = *raw_dev == 0;
     int64_t np = (int64_t)*npairs_dev;
     if (np > pairs_cap) np = pairs_cap;
     for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < np; base += (int64_t)gridDim.x * blockDim.x) {
@@ -729,26 +766,33 @@ static int sort_unique(ccd_ctx *ctx, int32_t *keys, int64_t n, const char *tag, 
     return CCD_OK;
 }
 
+int build_vertex_face_table_buffers(ccd_ctx *ctx, int64_t F, int64_t V)
+{
+    if (!buffer(ctx, "vf_cnt", (size_t)(V + 1) * 4) || !buffer(ctx, "vf_off", (size_t)(V + 1) * 4) ||
+        !buffer(ctx, "vf_list", (size_t)(3 * F + 1) * 4) || !buffer(ctx, "scan_tmp", scan_temp_bytes(V + 1)))
+        return CCD_E_OOM;
+    return CCD_OK;
+}
+
 int build_vertex_face_table(ccd_ctx *ctx, const int32_t *faces, int64_t F, int64_t V)
 {
     hipStream_t st = ctx->stream;
-    int32_t *vcnt = (int32_t *)buffer(ctx, "vf_cnt", (size_t)(V + 1) * 4);
-    int32_t *voff = (int32_t *)buffer(ctx, "vf_off", (size_t)(V + 1) * 4);
-    int32_t *vlist = (int32_t *)buffer(ctx, "vf_list", (size_t)(3 * F + 1) * 4);
-    if (!vcnt || !voff || !vlist) return CCD_E_OOM;
-    CCD_HIP_CHECK(ctx, hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, st));
+    int rc = build_vertex_face_table_buffers(ctx, F, V);
+    if (rc) return rc;
+    int32_t *vcnt = (int32_t *)ctx->bufs["vf_cnt"].ptr;
+    int32_t *voff = (int32_t *)ctx->bufs["vf_off"].ptr;
+    int32_t *vlist = (int32_t *)ctx->bufs["vf_list"].ptr;
+    void *scan_tmp = ctx->bufs["scan_tmp"].ptr;
+    rc = fill_u32(ctx, (uint32_t *)vcnt, V + 1, 0);
+    if (rc) return rc;
     if (F > 0) {
         hipLaunchKernelGGL(k_vf_count, dim3(blocks_for(3 * F, 256)), dim3(256), 0, st, faces, F, vcnt);
         CCD_LAUNCH_CHECK(ctx);
     }
-    size_t stmp2 = 0;
-    CCD_HIP_CHECK(ctx, rocprim::exclusive_scan((void *)nullptr, stmp2, vcnt, voff, 0, (size_t)(V + 1),
-                                               rocprim::plus<int32_t>(), st));
-    void *scan_tmp = buffer(ctx, "scan_tmp", stmp2 + 16);
-    if (!scan_tmp) return CCD_E_OOM;
-    CCD_HIP_CHECK(ctx, rocprim::exclusive_scan(scan_tmp, stmp2, vcnt, voff, 0, (size_t)(V + 1),
-                                               rocprim::plus<int32_t>(), st));
-    CCD_HIP_CHECK(ctx, hipMemsetAsync(vcnt, 0, (size_t)(V + 1) * 4, st));
+    rc = exclusive_scan_i32(ctx, vcnt, voff, V + 1, scan_tmp, nullptr);
+    if (rc) return rc;
+    rc = fill_u32(ctx, (uint32_t *)vcnt, V + 1, 0);
+    if (rc) return rc;
     if (F > 0) {
         hipLaunchKernelGGL(k_vf_fill, dim3(blocks_for(3 * F, 256)), dim3(256), 0, st, faces, F, voff, vcnt, vlist);
         CCD_LAUNCH_CHECK(ctx);
@@ -762,148 +806,187 @@ void vertex_face_table(ccd_ctx *ctx, const int32_t **off, const int32_t **list)
     *list = (const int32_t *)ctx->bufs["vf_list"].ptr;
 }
 
-int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, const double *qe,
-               const int64_t *hist_off, const double *hist_pos, int64_t V, double eta, const uint8_t *fixed,
-               Candidates &out, bool aabb)
+// Device buffers of one broadphase, sized for F faces and V vertices and the capacities in caps.
+struct BPBufs {
+    double *dop64, *centre, *partial, *sdop64;
+    uint64_t *keys, *keys2;
+    int32_t *vals, *order, *face_inst;
+    uint32_t *left, *rf, *rl, *parent, *escape, *arrivals;
+    NodeRec *nodes;
+    int4 *sfaces;
+    void *sort_tmp;
+    int2 *pairs;
+    int4 *vfk, *eek;
+};
+constexpr int NPARTS = 512;
+
+static int bp_buffers(ccd_ctx *ctx, int64_t F, int64_t V, bool batched, BPBufs &b)
 {
-    out = Candidates();
-    int64_t *cnt = ctx->d_counts;
-    CCD_HIP_CHECK(ctx, hipMemsetAsync(cnt, 0, sizeof(int64_t) * NCOUNTERS, ctx->stream));
-    if (F < 2) {
-        CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts, cnt, sizeof(int64_t) * NCOUNTERS, hipMemcpyDeviceToHost,
-                                          ctx->stream));
-        CCD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-        return CCD_OK;
-    }
-    static const Axes axes = make_axes();
-    double *dop64 = (double *)buffer(ctx, "dop64", (size_t)F * NB * 8);
-    double *centre = (double *)buffer(ctx, "centre", (size_t)F * 3 * 8);
-    const int nparts = 512;
-    double *partial = (double *)buffer(ctx, "bpartial", (size_t)nparts * 6 * 8 + 64);
-    uint64_t *keys = (uint64_t *)buffer(ctx, "mkeys", (size_t)F * 8);
-    uint64_t *keys2 = (uint64_t *)buffer(ctx, "mkeys2", (size_t)F * 8);
-    int32_t *vals = (int32_t *)buffer(ctx, "mvals", (size_t)F * 4);
-    int32_t *order = (int32_t *)buffer(ctx, "order", (size_t)F * 4);
-    uint32_t *left = (uint32_t *)buffer(ctx, "left", (size_t)F * 4);
-    uint32_t *rf = (uint32_t *)buffer(ctx, "rfirst", (size_t)F * 4);
-    uint32_t *rl = (uint32_t *)buffer(ctx, "rlast", (size_t)F * 4);
-    uint32_t *parent = (uint32_t *)buffer(ctx, "parent", (size_t)(2 * F) * 4);
-    uint32_t *escape = (uint32_t *)buffer(ctx, "escape", (size_t)(2 * F) * 4);
-    unsigned int *arrivals = (unsigned int *)buffer(ctx, "arrivals", (size_t)F * 4);
-    NodeRec *nodes = (NodeRec *)buffer(ctx, "nodes", (size_t)(2 * F) * sizeof(NodeRec));
-    double *sdop64 = (double *)buffer(ctx, "sdop64", (size_t)F * NB * 8);
-    int4 *sfaces = (int4 *)buffer(ctx, "sfaces", (size_t)F * 16);
-    if (!dop64 || !centre || !partial || !keys || !keys2 || !vals || !order || !left || !rf || !rl || !parent ||
-        !escape || !arrivals || !nodes || !sdop64 || !sfaces)
+    b.dop64 = (double *)buffer(ctx, "dop64", (size_t)F * NB * 8);
+    b.centre = (double *)buffer(ctx, "centre", (size_t)F * 3 * 8);
+    b.partial = (double *)buffer(ctx, "bpartial", (size_t)NPARTS * 6 * 8 + 64);
+    b.keys = (uint64_t *)buffer(ctx, "mkeys", (size_t)F * 8);
+    b.keys2 = (uint64_t *)buffer(ctx, "mkeys2", (size_t)F * 8);
+    b.vals = (int32_t *)buffer(ctx, "mvals", (size_t)F * 4);
+    b.order = (int32_t *)buffer(ctx, "order", (size_t)F * 4);
+    b.face_inst = batched ? (int32_t *)buffer(ctx, "face_inst", (size_t)F * 4) : nullptr;
+    b.left = (uint32_t *)buffer(ctx, "left", (size_t)F * 4);
+    b.rf = (uint32_t *)buffer(ctx, "rfirst", (size_t)F * 4);
+    b.rl = (uint32_t *)buffer(ctx, "rlast", (size_t)F * 4);
+    b.parent = (uint32_t *)buffer(ctx, "parent", (size_t)(2 * F) * 4);
+    b.escape = (uint32_t *)buffer(ctx, "escape", (size_t)(2 * F) * 4);
+    b.arrivals = (uint32_t *)buffer(ctx, "arrivals", (size_t)F * 4);
+    b.nodes = (NodeRec *)buffer(ctx, "nodes", (size_t)(2 * F) * sizeof(NodeRec));
+    b.sdop64 = (double *)buffer(ctx, "sdop64", (size_t)F * NB * 8);
+    b.sfaces = (int4 *)buffer(ctx, "sfaces", (size_t)F * 16);
+    b.sort_tmp = buffer(ctx, "radix_tmp", radix_sort_temp_bytes(F));
+    b.pairs = (int2 *)buffer(ctx, "pairs", (size_t)ctx->cap_pairs * 8);
+    b.vfk = (int4 *)buffer(ctx, "vf_keys", (size_t)ctx->cap_vf * 16);
+    b.eek = (int4 *)buffer(ctx, "ee_keys", (size_t)ctx->cap_ee * 16);
+    if (!b.dop64 || !b.centre || !b.partial || !b.keys || !b.keys2 || !b.vals || !b.order ||
+        (batched && !b.face_inst) || !b.left || !b.rf || !b.rl || !b.parent || !b.escape || !b.arrivals ||
+        !b.nodes || !b.sdop64 || !b.sfaces || !b.sort_tmp || !b.pairs || !b.vfk || !b.eek)
         return CCD_E_OOM;
+    return CCD_OK;
+}
+
+int broadphase_prepare(ccd_ctx *ctx, int64_t F, int64_t V, bool batched)
+{
+    // Capacities: grow-only, first guesses from the face count (the 256^2 cloth has 9.3 face pairs,
+    // 21 vertex-face and 38 edge-edge stencils per face); an overflow is detected after the run.
+    ctx->cap_pairs = std::max<int64_t>(ctx->cap_pairs, 16 * F + 1024);
+    ctx->cap_vf = std::max<int64_t>(ctx->cap_vf, 24 * F + 1024);
+    ctx->cap_ee = std::max<int64_t>(ctx->cap_ee, 40 * F + 1024);
+    BPBufs b;
+    int rc = bp_buffers(ctx, F, V, batched, b);
+    if (rc) return rc;
+    rc = build_vertex_face_table_buffers(ctx, F, V);
+    return rc;
+}
+
+int broadphase_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, const double *qe,
+                      const int64_t *hist_off, const double *hist_pos, int64_t V, double eta, const uint8_t *fixed,
+                      bool aabb, const int64_t *inst_off, int ninst)
+{
     hipStream_t st = ctx->stream;
+    int64_t *cnt = ctx->d_counts;
+    unsigned long long *c = (unsigned long long *)cnt;
+    int rc = fill_u64(ctx, (uint64_t *)cnt, NCOUNTERS, 0);
+    if (rc) return rc;
+    if (F < 2) return CCD_OK;
+    const bool batched = inst_off && ninst > 1;
+    BPBufs b;
+    rc = bp_buffers(ctx, F, V, batched, b);
+    if (rc) return rc;
+    static const Axes axes = make_axes();
+    int inst_bits = 0;
+    if (batched)
+        while ((1ll << inst_bits) < ninst) inst_bits++;
 
     prof_begin(ctx, CCD_T_BOUNDS);
     hipLaunchKernelGGL(k_leaf_bounds, dim3(blocks_for(F, 256)), dim3(256), 0, st, faces, F, qs, qe, hist_off,
-                       hist_pos, eta, axes, aabb ? 1 : 0, dop64, centre);
+                       hist_pos, eta, axes, aabb ? 1 : 0, b.dop64, b.centre);
     CCD_LAUNCH_CHECK(ctx);
-    int nb = (int)std::min<int64_t>(nparts, blocks_for(F, 256));
-    hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(256), 0, st, centre, F, partial);
+    int nb = (int)std::min<int64_t>(NPARTS, blocks_for(F, 256));
+    hipLaunchKernelGGL(k_bounds_partial, dim3(nb), dim3(256), 0, st, b.centre, F, b.partial);
     CCD_LAUNCH_CHECK(ctx);
-    double *box = partial + 6 * nparts;
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, st, partial, nb, box);
+    double *box = b.partial + 6 * NPARTS;
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, st, b.partial, nb, box);
     CCD_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_morton, dim3(blocks_for(F, 256)), dim3(256), 0, st, centre, F, box, keys, vals);
+    if (batched) {
+        hipLaunchKernelGGL(k_face_instance, dim3(blocks_for(F, 256)), dim3(256), 0, st, inst_off, ninst, F,
+                           b.face_inst);
+        CCD_LAUNCH_CHECK(ctx);
+    }
+    hipLaunchKernelGGL(k_morton, dim3(blocks_for(F, 256)), dim3(256), 0, st, b.centre, F, box,
+                       batched ? b.face_inst : (const int32_t *)nullptr, inst_bits, b.keys, b.vals);
     CCD_LAUNCH_CHECK(ctx);
     prof_end(ctx, CCD_T_BOUNDS);
-    prof_begin(ctx, CCD_T_SORT);
-    size_t tmp = 0;
-    CCD_HIP_CHECK(ctx, rocprim::radix_sort_pairs((void *)nullptr, tmp, keys, keys2, vals, order, (size_t)F, 0, 63,
-                                                 st));
-    void *stmp = buffer(ctx, "radix_tmp", tmp + 16);
-    if (!stmp) return CCD_E_OOM;
-    CCD_HIP_CHECK(ctx, rocprim::radix_sort_pairs(stmp, tmp, keys, keys2, vals, order, (size_t)F, 0, 63, st));
-    prof_end(ctx, CCD_T_SORT);
-    prof_begin(ctx, CCD_T_BUILD);
 
-    hipLaunchKernelGGL(k_karras, dim3(blocks_for(F - 1, 256)), dim3(256), 0, st, keys2, F, left, rf, rl, parent);
+    prof_begin(ctx, CCD_T_SORT);
+    bool in_alt = false;
+    rc = radix_sort_pairs(ctx, b.keys, b.vals, b.keys2, b.order, F, 63, b.sort_tmp, in_alt);
+    if (rc) return rc;
+    const uint64_t *skeys = in_alt ? b.keys2 : b.keys;
+    const int32_t *order = in_alt ? b.order : b.vals;
+    prof_end(ctx, CCD_T_SORT);
+
+    prof_begin(ctx, CCD_T_BUILD);
+    hipLaunchKernelGGL(k_karras, dim3(blocks_for(F - 1, 256)), dim3(256), 0, st, skeys, F, b.left, b.rf, b.rl,
+                       b.parent);
     CCD_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_escape, dim3(blocks_for(2 * F - 1, 256)), dim3(256), 0, st, rf, rl, F, escape);
+    hipLaunchKernelGGL(k_escape, dim3(blocks_for(2 * F - 1, 256)), dim3(256), 0, st, b.rf, b.rl, F, b.escape);
     CCD_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_leaf_records, dim3(blocks_for(F, 256)), dim3(256), 0, st, order, dop64, faces, F, escape,
-                       nodes, sdop64, sfaces);
+    hipLaunchKernelGGL(k_leaf_records, dim3(blocks_for(F, 256)), dim3(256), 0, st, order, b.dop64, faces, F,
+                       b.escape, b.nodes, b.sdop64, b.sfaces);
     CCD_LAUNCH_CHECK(ctx);
-    CCD_HIP_CHECK(ctx, hipMemsetAsync(arrivals, 0, (size_t)F * 4, st));
-    hipLaunchKernelGGL(k_refit, dim3(blocks_for(F, 256)), dim3(256), 0, st, F, parent, left, escape, nodes,
-                       arrivals);
+    rc = fill_u32(ctx, b.arrivals, F, 0);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_refit, dim3(blocks_for(F, 256)), dim3(256), 0, st, F, b.parent, b.left, b.escape, b.nodes,
+                       b.arrivals);
     CCD_LAUNCH_CHECK(ctx);
     prof_end(ctx, CCD_T_BUILD);
-    prof_begin(ctx, CCD_T_TABLES);
 
-    // Vertex -> face table and mesh checks.
-    int rc0 = build_vertex_face_table(ctx, faces, F, V);
-    if (rc0) return rc0;
+    prof_begin(ctx, CCD_T_TABLES);
+    rc = build_vertex_face_table(ctx, faces, F, V);
+    if (rc) return rc;
     const int32_t *voff = nullptr, *vlist = nullptr;
     vertex_face_table(ctx, &voff, &vlist);
-    unsigned long long *c = (unsigned long long *)cnt;
     hipLaunchKernelGGL(k_mesh_check, dim3(blocks_for(F, 256)), dim3(256), 0, st, faces, F, voff, vlist, c + C_BAD);
     CCD_LAUNCH_CHECK(ctx);
     prof_end(ctx, CCD_T_TABLES);
 
-    // Traversal, with one retry if the pair buffer was too small.
-    int64_t pcap = std::max<int64_t>(ctx->bufs.count("pairs") ? (int64_t)(ctx->bufs["pairs"].bytes / 8) : 0,
-                                     16 * F + 1024);
-    int2 *pairs = (int2 *)buffer(ctx, "pairs", (size_t)pcap * 8);
-    if (!pairs) return CCD_E_OOM;
-    for (int attempt = 0; attempt < 2; attempt++) {
-        CCD_HIP_CHECK(ctx, hipMemsetAsync(c + C_PAIRS, 0, 8, st));
-        prof_begin(ctx, CCD_T_TRAVERSE);
-        hipLaunchKernelGGL(k_traverse, dim3(blocks_for(F, 256)), dim3(256), 0, st, F, nodes, sdop64, sfaces, pairs,
-                           pcap, c + C_PAIRS);
-        CCD_LAUNCH_CHECK(ctx);
-        prof_end(ctx, CCD_T_TRAVERSE);
-        CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts, cnt, sizeof(int64_t) * NCOUNTERS, hipMemcpyDeviceToHost, st));
-        CCD_HIP_CHECK(ctx, hipStreamSynchronize(st));
-        if (ctx->h_counts[C_PAIRS] <= pcap) break;
-        pcap = ctx->h_counts[C_PAIRS] + ctx->h_counts[C_PAIRS] / 8 + 1024;
-        pairs = (int2 *)buffer(ctx, "pairs", (size_t)pcap * 8);
-        if (!pairs) return CCD_E_OOM;
-    }
-    int64_t P = ctx->h_counts[C_PAIRS];
-    bool owner = ctx->h_counts[C_BAD] == 0;
-    out.npairs = P;
+    prof_begin(ctx, CCD_T_TRAVERSE);
+    hipLaunchKernelGGL(k_traverse, dim3(blocks_for(F, 256)), dim3(256), 0, st, F, b.nodes, b.sdop64, b.sfaces,
+                       batched ? b.face_inst : (const int32_t *)nullptr, inst_off, b.pairs, ctx->cap_pairs,
+                       c + C_PAIRS);
+    CCD_LAUNCH_CHECK(ctx);
+    prof_end(ctx, CCD_T_TRAVERSE);
 
-    // Expansion, with one retry if a stencil buffer was too small.
-    int64_t vcap = std::max<int64_t>(ctx->bufs.count("vf_keys") ? (int64_t)(ctx->bufs["vf_keys"].bytes / 16) : 0,
-                                     owner ? (P * 7) / 2 + 1024 : 6 * P + 1024);
-    int64_t ecap = std::max<int64_t>(ctx->bufs.count("ee_keys") ? (int64_t)(ctx->bufs["ee_keys"].bytes / 16) : 0,
-                                     owner ? (P * 11) / 2 + 1024 : 9 * P + 1024);
-    MeshView mv{faces, voff, vlist, dop64, fixed};
-    for (int attempt = 0; attempt < 2; attempt++) {
-        int4 *vfk = (int4 *)buffer(ctx, "vf_keys", (size_t)vcap * 16);
-        int4 *eek = (int4 *)buffer(ctx, "ee_keys", (size_t)ecap * 16);
-        if (!vfk || !eek) return CCD_E_OOM;
-        CCD_HIP_CHECK(ctx, hipMemsetAsync(c + C_VF, 0, 16, st));
-        prof_begin(ctx, CCD_T_EXPAND);
-        unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(P, 256), 65536));
-        if (owner)
-            hipLaunchKernelGGL(k_expand<true>, dim3(grid), dim3(256), 0, st, mv, pairs, c + C_PAIRS, pcap, vfk, vcap,
-                               c + C_VF, eek, ecap, c + C_EE);
-        else
-            hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(256), 0, st, mv, pairs, c + C_PAIRS, pcap, vfk,
-                               vcap, c + C_VF, eek, ecap, c + C_EE);
-        CCD_LAUNCH_CHECK(ctx);
-        prof_end(ctx, CCD_T_EXPAND);
-        CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts, cnt, sizeof(int64_t) * NCOUNTERS, hipMemcpyDeviceToHost, st));
+    prof_begin(ctx, CCD_T_EXPAND);
+    MeshView mv{faces, voff, vlist, b.dop64, fixed};
+    unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(ctx->cap_pairs, 256), 16384));
+    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(256), 0, st, mv, b.pairs, c + C_PAIRS, ctx->cap_pairs, c + C_BAD,
+                       b.vfk, ctx->cap_vf, c + C_VF, b.eek, ctx->cap_ee, c + C_EE);
+    CCD_LAUNCH_CHECK(ctx);
+    prof_end(ctx, CCD_T_EXPAND);
+    return CCD_OK;
+}
+
+int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, const double *qe,
+               const int64_t *hist_off, const double *hist_pos, int64_t V, double eta, const uint8_t *fixed,
+               Candidates &out, bool aabb, const int64_t *inst_off, int ninst)
+{
+    out = Candidates();
+    hipStream_t st = ctx->stream;
+    int rc = broadphase_prepare(ctx, F, V, inst_off && ninst > 1);
+    if (rc) return rc;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        rc = broadphase_launch(ctx, faces, F, qs, qe, hist_off, hist_pos, V, eta, fixed, aabb, inst_off, ninst);
+        if (rc) return rc;
+        CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts, ctx->d_counts, sizeof(int64_t) * NCOUNTERS,
+                                          hipMemcpyDeviceToHost, st));
         CCD_HIP_CHECK(ctx, hipStreamSynchronize(st));
-        int64_t nv = ctx->h_counts[C_VF], ne = ctx->h_counts[C_EE];
-        out.vf = (int32_t *)vfk;
-        out.ee = (int32_t *)eek;
-        out.vf_cap = vcap;
-        out.ee_cap = ecap;
-        if (nv <= vcap && ne <= ecap) break;
-        vcap = std::max(vcap, nv + nv / 8 + 1024);
-        ecap = std::max(ecap, ne + ne / 8 + 1024);
+        int64_t P = ctx->h_counts[C_PAIRS], nv = ctx->h_counts[C_VF], ne = ctx->h_counts[C_EE];
+        if (P <= ctx->cap_pairs && nv <= ctx->cap_vf && ne <= ctx->cap_ee) break;
+        // Too small: grow to the counts seen (the stencil counts are exact only once the pairs fit).
+        ctx->cap_pairs = std::max(ctx->cap_pairs, P + P / 8 + 1024);
+        ctx->cap_vf = std::max(ctx->cap_vf, nv + nv / 8 + 1024);
+        ctx->cap_ee = std::max(ctx->cap_ee, ne + ne / 8 + 1024);
+        rc = broadphase_prepare(ctx, F, V, inst_off && ninst > 1);
+        if (rc) return rc;
+        if (attempt == 2) return set_error(ctx, CCD_E_CAPACITY, "broadphase buffers did not converge");
     }
+    out.npairs = ctx->h_counts[C_PAIRS];
     out.nvf = ctx->h_counts[C_VF];
     out.nee = ctx->h_counts[C_EE];
-    if (!owner) {
-        int rc = sort_unique(ctx, out.vf, out.nvf, "vf", cnt + C_VF);
+    out.vf = (int32_t *)ctx->bufs["vf_keys"].ptr;
+    out.ee = (int32_t *)ctx->bufs["ee_keys"].ptr;
+    out.vf_cap = ctx->cap_vf;
+    out.ee_cap = ctx->cap_ee;
+    if (F >= 2 && ctx->h_counts[C_BAD] != 0) {   // raw emission: deduplicate by sorting
+        int64_t *cnt = ctx->d_counts;
+        rc = sort_unique(ctx, out.vf, out.nvf, "vf", cnt + C_VF);
         if (rc) return rc;
         rc = sort_unique(ctx, out.ee, out.nee, "ee", cnt + C_EE);
         if (rc) return rc;

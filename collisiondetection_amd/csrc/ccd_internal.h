@@ -29,6 +29,8 @@ struct ccd_ctx {
     bool pending[CCD_NTIMERS] = {};
     double stage_ms[CCD_NTIMERS] = {};
     int64_t stage_calls[CCD_NTIMERS] = {};
+    // Grow-only capacities of the broadphase outputs (face pairs, stencils).
+    int64_t cap_pairs = 0, cap_vf = 0, cap_ee = 0;
     // The candidates of the last broadphase (ccd_last_candidates).
     const int32_t *last_vf = nullptr, *last_ee = nullptr;
     int64_t last_nvf = 0, last_nee = 0;
@@ -103,13 +105,32 @@ struct Candidates {
     int64_t vf_cap = 0, ee_cap = 0;
 };
 // aabb selects AABBBroadPhase's 3-axis bounds (used by the mesh self-distance), else the 13-axis k-DOP.
+// inst_off (device, ninst+1 face offsets) runs independent instances in one pass: no pair crosses
+// instances.  Synchronous: checks capacities and reruns after growing them.
 int broadphase(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, const double *qe,
                const int64_t *hist_off, const double *hist_pos, int64_t V, double eta, const uint8_t *fixed,
-               Candidates &out, bool aabb = false);
+               Candidates &out, bool aabb = false, const int64_t *inst_off = nullptr, int ninst = 1);
+// The same pipeline as launches only (no host synchronisation, no allocation once prepared): it can be
+// captured in a HIP graph.  Counts land in ctx->d_counts; buffers must come from broadphase_prepare.
+int broadphase_prepare(ccd_ctx *ctx, int64_t F, int64_t V, bool batched);
+int broadphase_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, const double *qe,
+                      const int64_t *hist_off, const double *hist_pos, int64_t V, double eta, const uint8_t *fixed,
+                      bool aabb, const int64_t *inst_off, int ninst);
+int build_vertex_face_table_buffers(ccd_ctx *ctx, int64_t F, int64_t V);
 // Vertex -> face CSR table (device buffers "vf_off" / "vf_list"), and its current addresses.
 int build_vertex_face_table(ccd_ctx *ctx, const int32_t *faces, int64_t F, int64_t V);
 void vertex_face_table(ccd_ctx *ctx, const int32_t **off, const int32_t **list);
 int sort_stencils(ccd_ctx *ctx, int32_t *keys, int64_t n);
+
+// Device-wide primitives (devprims.hip): plain launches, safe inside a captured graph.
+int fill_u32(ccd_ctx *ctx, uint32_t *p, int64_t n, uint32_t v);
+int fill_u64(ccd_ctx *ctx, uint64_t *p, int64_t n, uint64_t v);
+size_t scan_temp_bytes(int64_t n);
+int exclusive_scan_i32(ccd_ctx *ctx, const int32_t *in, int32_t *out, int64_t n, void *temp, int64_t *total);
+size_t radix_sort_temp_bytes(int64_t n);
+// Sorts by the low `bits` bits of the keys; the result is in (keys_alt, vals_alt) when result_in_alt.
+int radix_sort_pairs(ccd_ctx *ctx, uint64_t *keys, int32_t *vals, uint64_t *keys_alt, int32_t *vals_alt, int64_t n,
+                     int bits, void *temp, bool &result_in_alt);
 
 // Proximity (distance.hip)
 int launch_distance_primitives(ccd_ctx *ctx, int kind, const double *q, double eta, int64_t n, uint8_t *flag,
