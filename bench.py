@@ -59,6 +59,80 @@ def cpu_baseline(scene, sample_stencils=60000, seed=0):
                        f"{total:.1f} s per step")}
 
 
+def bench_batched(steps, log):
+    """The 512-instance mixed workload: plain (ccd_detect per instance, the same kernels launched per
+    instance) vs batched (one ccd_batch_detect pipeline) vs batched replayed from a captured HIP graph."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from collisiondetection_amd import _native as N
+    from collisiondetection_amd import api, scenes
+
+    inst = scenes.batched_instances()
+    b = api.Batch(inst)
+    ctx = b.ctx
+    L = N.lib()
+    F = b.F
+    log(f"batched workload: {b.ninst} instances, {F} triangles, {b.V} vertices")
+    # per-instance face arrays with local vertex indices (the plain path's input)
+    local_faces = [api.to_device(s["faces"], torch.int32, ctx).reshape(-1) for s in inst]
+    cap = max(1024, 2 * max(len(s["faces"]) for s in inst))
+    dev = ctx.torch_device
+    vk = torch.empty(4 * cap, dtype=torch.int32, device=dev)
+    vt = torch.empty(cap, dtype=torch.float64, device=dev)
+    ek = torch.empty(4 * cap, dtype=torch.int32, device=dev)
+    et = torch.empty(cap, dtype=torch.float64, device=dev)
+    c = [ctypes.c_int64(0) for _ in range(4)]
+    qs0, qe0 = b.qs.data_ptr(), b.qe.data_ptr()
+
+    def plain():
+        hits = 0
+        for k, s in enumerate(inst):
+            off = int(b.vert_off[k]) * 24
+            rc = L.ccd_detect(ctx.ptr, local_faces[k].data_ptr(), len(s["faces"]), qs0 + off, qe0 + off,
+                              len(s["qstart"]), s["eta"], s["eta"], None, vk.data_ptr(), vt.data_ptr(), cap,
+                              ctypes.byref(c[0]), ek.data_ptr(), et.data_ptr(), cap, ctypes.byref(c[1]),
+                              ctypes.byref(c[2]), ctypes.byref(c[3]))
+            ctx.check(rc)
+            hits += c[0].value + c[1].value
+        return hits
+
+    ctx.bind_stream()
+    plain()
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    np_ = max(1, steps // 4)
+    for _ in range(np_):
+        h_plain = plain()
+    torch.cuda.synchronize(dev)
+    t_plain = (time.perf_counter() - t0) / np_
+    r = b.detect(1e-6, 1e-6)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = b.detect(1e-6, 1e-6)
+    torch.cuda.synchronize(dev)
+    t_batch = (time.perf_counter() - t0) / steps
+    h_batch = len(r["vf"]) + len(r["ee"])
+    b.detect(1e-6, 1e-6, graph=True)   # synchronous run + capture
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rg = b.detect(1e-6, 1e-6, graph=True)
+    torch.cuda.synchronize(dev)
+    t_graph = (time.perf_counter() - t0) / steps
+    h_graph = len(rg["vf"]) + len(rg["ee"])
+    log(f"batched: plain {t_plain * 1e3:.2f} ms, batched {t_batch * 1e3:.2f} ms, graph {t_graph * 1e3:.2f} ms; "
+        f"collisions {h_plain} / {h_batch} / {h_graph}")
+    return {"instances": b.ninst, "triangles": F, "plain_tris_per_s": F / t_plain, "batched_tris_per_s": F / t_batch,
+            "graph_tris_per_s": F / t_graph, "speedup_vs_plain": t_plain / t_graph,
+            "ms": {"plain": t_plain * 1e3, "batched": t_batch * 1e3, "graph": t_graph * 1e3},
+            "collisions_equal": h_plain == h_batch == h_graph, "collisions": h_graph,
+            "candidates": {"vf": rg["vf_candidates"], "ee": rg["ee_candidates"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +140,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=256, help="cloth grid size (256 = the 130,050-triangle cloth)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -172,6 +247,8 @@ def main():
         "candidates": {"vf": nv.value, "ee": ne.value}, "collisions": {"vf": cnt[0].value, "ee": cnt[1].value},
         "roofline": roof,
     }
+    if not args.no_batched:
+        out["batched"] = bench_batched(max(3, args.steps // 2), log)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, 1 thread) ...")
         out["cpu_baseline"] = cpu_baseline(scene)

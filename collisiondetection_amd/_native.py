@@ -25,6 +25,7 @@ PRIM_VERTEX_EDGE = 2
 PRIM_VERTEX_VERTEX = 3
 
 CCD_SORTED = 1
+CCD_GRAPH = 2
 
 DIST_VERTEX_PLANE_LT = 0
 DIST_LINE_LINE_LT = 1
@@ -59,6 +60,10 @@ SIGNATURES = {
     "ccd_device_free": ([_V, _V], _INT),
     "ccd_copy_to_device": ([_V, _V, _V, _I64], _INT),
     "ccd_copy_to_host": ([_V, _V, _V, _I64], _INT),
+    "ccd_batch_detect": ([_V, _I64, _V, _V, _I64, _V, _V, _I64, _D, _D, _V, _V, _I64, _P64, _V, _V, _I64, _P64, _P64,
+                          _P64, _INT], _INT),
+    "ccd_autotune": ([_V, _V, _V, _D, ctypes.POINTER(_INT)], _INT),
+    "ccd_set_queue_shape": ([_V, ctypes.POINTER(_INT)], _INT),
     "ccd_set_profiling": ([_V, _INT], _INT),
     "ccd_stage_times": ([_V, ctypes.POINTER(_D), _P64, _INT], _INT),
     "ccd_narrowphase_work": ([_V, _V, _V, _I64, _V, _I64, _V, _I64, _D, _P64, _P64], _INT),
@@ -66,8 +71,8 @@ SIGNATURES = {
     "ccd_last_candidates": ([_V, ctypes.POINTER(_V), _P64, ctypes.POINTER(_V), _P64], _INT),
 }
 
-NTIMERS = 8
-TIMER_NAMES = ("bounds", "sort", "build", "tables", "traverse", "expand", "narrow_vf", "narrow_ee")
+NTIMERS = 9
+TIMER_NAMES = ("bounds", "sort", "build", "tables", "traverse", "expand", "narrow_vf", "narrow_ee", "pipeline")
 NSTAGES = 12
 STAGE_NAMES = ("vf_call", "vf_plane", "vf_dist", "ee_dist", "ee_par", "ee_bary", "ve_call", "ve_quad", "ve_quart",
                "vv", "quad_root", "isolate")

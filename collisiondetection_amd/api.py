@@ -451,3 +451,79 @@ def mesh_self_distance(verts, faces, fixed=None, ctx: Context | None = None):
     ctx.check(N.lib().ccd_mesh_self_distance(ctx.ptr, _ptr(v), V, _ptr(f), F, _ptr(fx), ctypes.byref(d),
                                              ctypes.byref(nv), ctypes.byref(ne)))
     return d.value, nv.value, ne.value
+
+
+# ------------------------------------------------------------------------------------------------
+# Batched instances
+# ------------------------------------------------------------------------------------------------
+class Batch:
+    """Independent meshes packed for ccd_batch_detect: faces with global vertex indices, face offsets
+    per instance and vertex offsets (to map results back)."""
+
+    def __init__(self, scenes_list, ctx: Context | None = None):
+        torch = _torch()
+        self.ctx = ctx or default_context()
+        fs, qs, qe, foff, voff = [], [], [], [0], [0]
+        for s in scenes_list:
+            fs.append(np.asarray(s["faces"], dtype=np.int32) + voff[-1])
+            qs.append(np.asarray(s["qstart"], dtype=np.float64))
+            qe.append(np.asarray(s["qend"], dtype=np.float64))
+            foff.append(foff[-1] + len(s["faces"]))
+            voff.append(voff[-1] + len(s["qstart"]))
+        self.ninst = len(scenes_list)
+        self.face_off = np.asarray(foff, dtype=np.int64)
+        self.vert_off = np.asarray(voff, dtype=np.int64)
+        self.F, self.V = int(foff[-1]), int(voff[-1])
+        c = self.ctx
+        self.faces = to_device(np.concatenate(fs) if fs else np.zeros((0, 3), np.int32), torch.int32, c).reshape(-1)
+        self.qs = to_device(np.concatenate(qs) if qs else np.zeros((0, 3)), torch.float64, c).reshape(-1)
+        self.qe = to_device(np.concatenate(qe) if qe else np.zeros((0, 3)), torch.float64, c).reshape(-1)
+        self.off_d = to_device(self.face_off, torch.int64, c)
+        self.cap = max(1024, 2 * self.F)
+        self._alloc()
+
+    def _alloc(self):
+        torch = _torch()
+        dev = self.ctx.torch_device
+        self.vk = torch.empty(4 * self.cap, dtype=torch.int32, device=dev)
+        self.vt = torch.empty(self.cap, dtype=torch.float64, device=dev)
+        self.ek = torch.empty(4 * self.cap, dtype=torch.int32, device=dev)
+        self.et = torch.empty(self.cap, dtype=torch.float64, device=dev)
+
+    def instance_of_vertex(self, v):
+        return np.searchsorted(self.vert_off, v, side="right") - 1
+
+    def detect(self, outer_eta, eta, graph=False):
+        """One ccd_batch_detect call -> dict(vf, vf_toi, ee, ee_toi, vf_candidates, ee_candidates) with global
+        vertex indices (use instance_of_vertex to split)."""
+        ctx = self.ctx
+        ctx.bind_stream()
+        c = [ctypes.c_int64(0) for _ in range(4)]
+        for _ in range(2):
+            rc = N.lib().ccd_batch_detect(ctx.ptr, self.ninst, _ptr(self.off_d), _ptr(self.faces), self.F, _ptr(self.qs),
+                                          _ptr(self.qe), self.V, float(outer_eta), float(eta), _ptr(self.vk),
+                                          _ptr(self.vt), self.cap, ctypes.byref(c[0]), _ptr(self.ek), _ptr(self.et),
+                                          self.cap, ctypes.byref(c[1]), ctypes.byref(c[2]), ctypes.byref(c[3]),
+                                          N.CCD_GRAPH if graph else 0)
+            if rc == N.CCD_E_CAPACITY:
+                self.cap = max(c[0].value, c[1].value) + 1024
+                self._alloc()
+                continue
+            ctx.check(rc)
+            break
+        nv, ne = c[0].value, c[1].value
+        return {"vf": self.vk[: 4 * nv].reshape(-1, 4).cpu().numpy(), "vf_toi": self.vt[:nv].cpu().numpy(),
+                "ee": self.ek[: 4 * ne].reshape(-1, 4).cpu().numpy(), "ee_toi": self.et[:ne].cpu().numpy(),
+                "vf_candidates": c[2].value, "ee_candidates": c[3].value}
+
+
+def autotune(qstart, qend, eta, ctx: Context | None = None):
+    """Tune the persistent narrowphase on the last detection's candidates; returns the 6-int shape."""
+    torch = _torch()
+    ctx = ctx or default_context()
+    ctx.bind_stream()
+    qs = to_device(qstart, torch.float64, ctx).reshape(-1)
+    qe = to_device(qend, torch.float64, ctx).reshape(-1)
+    shape = (ctypes.c_int * 6)()
+    ctx.check(N.lib().ccd_autotune(ctx.ptr, _ptr(qs), _ptr(qe), float(eta), shape))
+    return list(shape)

@@ -1,6 +1,8 @@
 // C ABI of libccd_amd.so (declarations and contracts in include/ccd_amd.h).
 #include <cstdarg>
+#include <cstring>
 #include <new>
+#include <vector>
 
 #include "ccd_core.h"
 #include "ccd_internal.h"
@@ -22,13 +24,13 @@ int set_error(ccd_ctx *ctx, int code, const char *fmt, ...)
 
 void prof_begin(ccd_ctx *ctx, int timer)
 {
-    if (!ctx->profiling) return;
+    if (!ctx->profiling || ctx->capturing) return;
     (void)hipEventRecord(ctx->ev[2 * timer], ctx->stream);
 }
 
 void prof_end(ccd_ctx *ctx, int timer)
 {
-    if (!ctx->profiling) return;
+    if (!ctx->profiling || ctx->capturing) return;
     (void)hipEventRecord(ctx->ev[2 * timer + 1], ctx->stream);
     ctx->pending[timer] = true;
 }
@@ -148,6 +150,9 @@ int ccd_create(ccd_ctx **out, int device)
         return CCD_E_HIP;
     }
     ctx->stream = ctx->own_stream;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        ctx->num_cus = cus;
     if (hipHostMalloc((void **)&ctx->h_counts, sizeof(int64_t) * NCOUNTERS, hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void **)&ctx->d_counts, sizeof(int64_t) * NCOUNTERS) != hipSuccess) {
         (void)hipStreamDestroy(ctx->own_stream);
@@ -170,6 +175,8 @@ int ccd_destroy(ccd_ctx *ctx)
     for (int i = 0; i < 2 * CCD_NTIMERS; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->h_counts) (void)hipHostFree(ctx->h_counts);
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return CCD_OK;
@@ -326,6 +333,169 @@ int ccd_kdop_candidates_history(ccd_ctx *ctx, const int32_t *faces, int64_t nfac
                              vf_out, vf_cap, vf_count, ee_out, ee_cap, ee_count, flags);
 }
 
+// ----------------------------------------------------------------------------------------------
+// Detection core shared by ccd_detect and ccd_batch_detect.
+// ----------------------------------------------------------------------------------------------
+struct DetectArgs {
+    const int64_t *inst_off;
+    int64_t ninst;
+    const int32_t *faces;
+    int64_t F;
+    const double *qs, *qe;
+    int64_t V;
+    double outer_eta, eta;
+    const uint8_t *fixed;
+    int32_t *vf_hits;
+    double *vf_toi;
+    int64_t vf_cap;
+    int32_t *ee_hits;
+    double *ee_toi;
+    int64_t ee_cap;
+};
+
+// The narrowphase half: counters reset by fill kernels, then the two persistent queue kernels.  Pure
+// launches (capturable).  n_max bounds the grids: the stencil counts are read on the device.
+static int narrow_launches(ccd_ctx *ctx, const DetectArgs &a, int64_t nvf_max, int64_t nee_max)
+{
+    unsigned long long *work = (unsigned long long *)buffer(ctx, "queue_work", 16);
+    if (!work) return CCD_E_OOM;
+    int rc = fill_u64(ctx, (uint64_t *)(ctx->d_counts + C_VF_HITS), 2, 0);
+    if (rc) return rc;
+    rc = fill_u64(ctx, (uint64_t *)work, 2, 0);
+    if (rc) return rc;
+    const int32_t *vfk = (const int32_t *)ctx->bufs["vf_keys"].ptr;
+    const int32_t *eek = (const int32_t *)ctx->bufs["ee_keys"].ptr;
+    prof_begin(ctx, CCD_T_NARROW_VF);
+    rc = launch_narrow2_queue(ctx, true, a.qs, a.qe, vfk, ctx->d_counts + C_VF, nvf_max, a.eta, a.vf_hits, a.vf_toi,
+                              a.vf_cap, ctx->d_counts + C_VF_HITS, work, ctx->vf_shape);
+    if (rc) return rc;
+    prof_end(ctx, CCD_T_NARROW_VF);
+    prof_begin(ctx, CCD_T_NARROW_EE);
+    rc = launch_narrow2_queue(ctx, false, a.qs, a.qe, eek, ctx->d_counts + C_EE, nee_max, a.eta, a.ee_hits, a.ee_toi,
+                              a.ee_cap, ctx->d_counts + C_EE_HITS, work + 1, ctx->ee_shape);
+    if (rc) return rc;
+    prof_end(ctx, CCD_T_NARROW_EE);
+    return CCD_OK;
+}
+
+// Synchronous run: broadphase with capacity checks, then the narrowphase.  Counts end in h_counts.
+static int detect_sync(ccd_ctx *ctx, const DetectArgs &a)
+{
+    Candidates c;
+    int rc = broadphase(ctx, a.faces, a.F, a.qs, a.qe, nullptr, nullptr, a.V, a.outer_eta, a.fixed, c, false,
+                        a.inst_off, (int)a.ninst);
+    if (rc) return rc;
+    ctx->last_vf = c.vf;
+    ctx->last_ee = c.ee;
+    ctx->last_nvf = c.nvf;
+    ctx->last_nee = c.nee;
+    rc = narrow_launches(ctx, a, c.nvf, c.nee);
+    if (rc) return rc;
+    CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts, ctx->d_counts, sizeof(int64_t) * NCOUNTERS, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+    CCD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+    return CCD_OK;
+}
+
+static void graph_drop(ccd_ctx *ctx)
+{
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    ctx->graph_valid = false;
+}
+
+// Graph path: replay when the key matches; otherwise run synchronously (which sizes every buffer and
+// gives this call's result) and capture the pipeline for the next calls.
+static int detect_graph(ccd_ctx *ctx, const DetectArgs &a)
+{
+    unsigned char key[256] = {};
+    static_assert(sizeof(DetectArgs) + 3 * sizeof(int64_t) <= sizeof(key), "graph key");
+    memcpy(key, &a, sizeof(DetectArgs));
+    int64_t caps[3] = {ctx->cap_pairs, ctx->cap_vf, ctx->cap_ee};
+    memcpy(key + sizeof(DetectArgs), caps, sizeof(caps));
+    if (ctx->graph_valid && memcmp(key, ctx->graph_key, sizeof(key)) == 0) {
+        if (ctx->profiling) (void)hipEventRecord(ctx->ev[2 * CCD_T_PIPELINE], ctx->stream);
+        CCD_HIP_CHECK(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+        if (ctx->profiling) {
+            (void)hipEventRecord(ctx->ev[2 * CCD_T_PIPELINE + 1], ctx->stream);
+            ctx->pending[CCD_T_PIPELINE] = true;
+        }
+        CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts, ctx->d_counts, sizeof(int64_t) * NCOUNTERS,
+                                          hipMemcpyDeviceToHost, ctx->stream));
+        CCD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        prof_collect(ctx);
+        ctx->graph_replays++;
+        int64_t *h = ctx->h_counts;
+        if (h[C_PAIRS] <= ctx->cap_pairs && h[C_VF] <= ctx->cap_vf && h[C_EE] <= ctx->cap_ee && h[C_BAD] == 0) {
+            ctx->last_vf = (const int32_t *)ctx->bufs["vf_keys"].ptr;
+            ctx->last_ee = (const int32_t *)ctx->bufs["ee_keys"].ptr;
+            ctx->last_nvf = h[C_VF];
+            ctx->last_nee = h[C_EE];
+            return CCD_OK;
+        }
+        graph_drop(ctx);   // outgrew its buffers (or the mesh needs sort-based deduplication)
+    }
+    graph_drop(ctx);
+    int rc = detect_sync(ctx, a);
+    if (rc) return rc;
+    if (ctx->h_counts[C_BAD] != 0) return CCD_OK;   // raw emission needs a host-driven sort: no graph
+    int64_t saved[NCOUNTERS];
+    memcpy(saved, ctx->h_counts, sizeof(saved));
+    // Capture with the capacities the synchronous run settled on.
+    int64_t caps2[3] = {ctx->cap_pairs, ctx->cap_vf, ctx->cap_ee};
+    memcpy(key + sizeof(DetectArgs), caps2, sizeof(caps2));
+    ctx->capturing = true;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        ctx->capturing = false;
+        return set_error(ctx, CCD_E_HIP, "hipStreamBeginCapture failed: %s", hipGetErrorString(e));
+    }
+    rc = broadphase_launch(ctx, a.faces, a.F, a.qs, a.qe, nullptr, nullptr, a.V, a.outer_eta, a.fixed, false,
+                           a.inst_off, (int)a.ninst);
+    if (!rc) rc = narrow_launches(ctx, a, ctx->cap_vf, ctx->cap_ee);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(ctx->stream, &g);
+    ctx->capturing = false;
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess || !g) return set_error(ctx, CCD_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return set_error(ctx, CCD_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    ctx->graph = g;
+    ctx->graph_exec = ge;
+    ctx->graph_valid = true;
+    memcpy(ctx->graph_key, key, sizeof(key));
+    memcpy(ctx->h_counts, saved, sizeof(saved));   // this call's result is the synchronous run's
+    return CCD_OK;
+}
+
+static int detect_common(ccd_ctx *ctx, const DetectArgs &a, bool graph, int64_t *vf_hit_count, int64_t *ee_hit_count,
+                         int64_t *vf_candidates, int64_t *ee_candidates)
+{
+    int rc = validate_faces(ctx, a.faces, a.F, a.V);
+    if (rc) return rc;
+    rc = graph ? detect_graph(ctx, a) : detect_sync(ctx, a);
+    if (rc) return rc;
+    int64_t nvh = ctx->h_counts[C_VF_HITS], neh = ctx->h_counts[C_EE_HITS];
+    if (vf_hit_count) *vf_hit_count = nvh;
+    if (ee_hit_count) *ee_hit_count = neh;
+    if (vf_candidates) *vf_candidates = ctx->last_nvf;
+    if (ee_candidates) *ee_candidates = ctx->last_nee;
+    if (nvh > a.vf_cap || neh > a.ee_cap)
+        return set_error(ctx, CCD_E_CAPACITY, "collision buffers too small: need %lld vertex-face, %lld edge-edge",
+                         (long long)nvh, (long long)neh);
+    return CCD_OK;
+}
+
 int ccd_detect(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces, const double *qstart, const double *qend,
                int64_t nverts, double outer_eta, double eta, const uint8_t *fixed, int32_t *vf_hits, double *vf_toi,
                int64_t vf_cap, int64_t *vf_hit_count, int32_t *ee_hits, double *ee_toi, int64_t ee_cap,
@@ -337,38 +507,121 @@ int ccd_detect(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces, const double 
     CCD_REQUIRE(ctx, nfaces == 0 || (faces && qstart && qend), "null mesh arrays");
     CCD_REQUIRE(ctx, (vf_cap == 0 || vf_hits) && (ee_cap == 0 || ee_hits), "null collision buffers");
     CCD_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-    int rc = validate_faces(ctx, faces, nfaces, nverts);
-    if (rc) return rc;
-    Candidates c;
-    rc = broadphase(ctx, faces, nfaces, qstart, qend, nullptr, nullptr, nverts, outer_eta, fixed, c);
-    if (rc) return rc;
-    ctx->last_vf = c.vf;
-    ctx->last_ee = c.ee;
-    ctx->last_nvf = c.nvf;
-    ctx->last_nee = c.nee;
-    CCD_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_counts + C_VF_HITS, 0, 16, ctx->stream));
-    prof_begin(ctx, CCD_T_NARROW_VF);
-    rc = launch_narrow2_compact(ctx, true, qstart, qend, c.vf, ctx->d_counts + C_VF, c.nvf, eta, vf_hits, vf_toi,
-                                vf_cap, ctx->d_counts + C_VF_HITS);
-    if (rc) return rc;
-    prof_end(ctx, CCD_T_NARROW_VF);
-    prof_begin(ctx, CCD_T_NARROW_EE);
-    rc = launch_narrow2_compact(ctx, false, qstart, qend, c.ee, ctx->d_counts + C_EE, c.nee, eta, ee_hits, ee_toi,
-                                ee_cap, ctx->d_counts + C_EE_HITS);
-    if (rc) return rc;
-    prof_end(ctx, CCD_T_NARROW_EE);
-    CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts, ctx->d_counts, sizeof(int64_t) * NCOUNTERS, hipMemcpyDeviceToHost,
-                                      ctx->stream));
+    DetectArgs a{nullptr, 1, faces, nfaces, qstart, qend, nverts, outer_eta, eta, fixed,
+                 vf_hits, vf_toi, vf_cap, ee_hits, ee_toi, ee_cap};
+    return detect_common(ctx, a, false, vf_hit_count, ee_hit_count, vf_candidates, ee_candidates);
+}
+
+int ccd_batch_detect(ccd_ctx *ctx, int64_t ninst, const int64_t *inst_face_off, const int32_t *faces, int64_t nfaces,
+                     const double *qstart, const double *qend, int64_t nverts, double outer_eta, double eta,
+                     int32_t *vf_hits, double *vf_toi, int64_t vf_cap, int64_t *vf_hit_count, int32_t *ee_hits,
+                     double *ee_toi, int64_t ee_cap, int64_t *ee_hit_count, int64_t *vf_candidates,
+                     int64_t *ee_candidates, int flags)
+{
+    if (!ctx) return CCD_E_INVALID;
+    CCD_REQUIRE(ctx, ninst >= 1 && ninst < (1 << 20), "instance count must be in [1, 2^20)");
+    CCD_REQUIRE(ctx, inst_face_off, "null instance offsets");
+    CCD_REQUIRE(ctx, nfaces >= 0 && nverts >= 0 && vf_cap >= 0 && ee_cap >= 0, "negative size");
+    CCD_REQUIRE(ctx, nfaces < (int64_t)1 << 30 && nverts < (int64_t)1 << 31, "mesh too large for 32-bit indices");
+    CCD_REQUIRE(ctx, nfaces == 0 || (faces && qstart && qend), "null mesh arrays");
+    CCD_REQUIRE(ctx, (vf_cap == 0 || vf_hits) && (ee_cap == 0 || ee_hits), "null collision buffers");
+    CCD_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    // The offsets must run 0 = off[0] <= off[1] <= ... <= off[ninst] = nfaces (checked on the host copy).
+    std::vector<int64_t> off(ninst + 1);
+    CCD_HIP_CHECK(ctx, hipMemcpyAsync(off.data(), inst_face_off, (ninst + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
     CCD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    prof_collect(ctx);
-    int64_t nvh = ctx->h_counts[C_VF_HITS], neh = ctx->h_counts[C_EE_HITS];
-    if (vf_hit_count) *vf_hit_count = nvh;
-    if (ee_hit_count) *ee_hit_count = neh;
-    if (vf_candidates) *vf_candidates = c.nvf;
-    if (ee_candidates) *ee_candidates = c.nee;
-    if (nvh > vf_cap || neh > ee_cap)
-        return set_error(ctx, CCD_E_CAPACITY, "collision buffers too small: need %lld vertex-face, %lld edge-edge",
-                         (long long)nvh, (long long)neh);
+    bool ok = off[0] == 0 && off[ninst] == nfaces;
+    for (int64_t k = 0; k < ninst && ok; k++) ok = off[k] <= off[k + 1];
+    CCD_REQUIRE(ctx, ok, "instance face offsets must be nondecreasing from 0 to nfaces");
+    DetectArgs a{ninst > 1 ? inst_face_off : nullptr, ninst, faces, nfaces, qstart, qend, nverts, outer_eta, eta,
+                 nullptr, vf_hits, vf_toi, vf_cap, ee_hits, ee_toi, ee_cap};
+    return detect_common(ctx, a, (flags & CCD_GRAPH) != 0, vf_hit_count, ee_hit_count, vf_candidates, ee_candidates);
+}
+
+int ccd_set_queue_shape(ccd_ctx *ctx, const int *shape)
+{
+    if (!ctx || !shape) return CCD_E_INVALID;
+    QueueShape s[2];
+    for (int k = 0; k < 2; k++) {
+        s[k].block = shape[3 * k];
+        s[k].tile = shape[3 * k + 1];
+        s[k].waves_per_cu = shape[3 * k + 2];
+        if ((s[k].block != 64 && s[k].block != 128 && s[k].block != 256) || s[k].tile < 64 || s[k].tile % 64 ||
+            s[k].waves_per_cu < 1 || s[k].waves_per_cu > 32)
+            return set_error(ctx, CCD_E_INVALID, "invalid queue shape");
+    }
+    ctx->vf_shape = s[0];
+    ctx->ee_shape = s[1];
+    graph_drop(ctx);   // captured launches carry the old shape
+    return CCD_OK;
+}
+
+int ccd_autotune(ccd_ctx *ctx, const double *qstart, const double *qend, double eta, int *shape)
+{
+    if (!ctx) return CCD_E_INVALID;
+    CCD_REQUIRE(ctx, qstart && qend, "null positions");
+    CCD_REQUIRE(ctx, ctx->last_vf || ctx->last_ee, "no candidates yet: run a detection on this context first");
+    CCD_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int blocks[3] = {64, 128, 256};
+    const int tiles[5] = {64, 128, 256, 512, 1024};
+    const int waves[2] = {4, 8};
+    int64_t nmax[2] = {ctx->last_nvf, ctx->last_nee};
+    const int32_t *keys[2] = {ctx->last_vf, ctx->last_ee};
+    int32_t *sink = (int32_t *)buffer(ctx, "tune_sink", 16 * 4096 + 64);
+    double *sink_t = (double *)buffer(ctx, "tune_sink_t", 8 * 4096 + 64);
+    unsigned long long *work = (unsigned long long *)buffer(ctx, "tune_work", 16);
+    int64_t *cnt = (int64_t *)buffer(ctx, "tune_cnt", 16);
+    int64_t *ndev = (int64_t *)buffer(ctx, "tune_n", 16);
+    if (!sink || !sink_t || !work || !cnt || !ndev) return CCD_E_OOM;
+    hipEvent_t e0, e1;
+    CCD_HIP_CHECK(ctx, hipEventCreate(&e0));
+    CCD_HIP_CHECK(ctx, hipEventCreate(&e1));
+    QueueShape best[2];
+    for (int kind = 0; kind < 2; kind++) {
+        if (nmax[kind] == 0) continue;
+        CCD_HIP_CHECK(ctx, hipMemcpyAsync(ndev, &nmax[kind], 8, hipMemcpyHostToDevice, st));
+        float best_ms = 1e30f;
+        for (int b : blocks)
+            for (int t : tiles)
+                for (int w : waves) {
+                    QueueShape s;
+                    s.block = b;
+                    s.tile = t;
+                    s.waves_per_cu = w;
+                    float total = 0.f;
+                    for (int rep = 0; rep < 2; rep++) {   // first launch warms caches
+                        int rc = fill_u64(ctx, (uint64_t *)work, 1, 0);
+                        if (!rc) rc = fill_u64(ctx, (uint64_t *)cnt, 1, 0);
+                        if (rc) return rc;
+                        CCD_HIP_CHECK(ctx, hipEventRecord(e0, st));
+                        rc = launch_narrow2_queue(ctx, kind == 0, qstart, qend, keys[kind], ndev, nmax[kind], eta, sink,
+                                                  sink_t, 0, cnt, work, s);
+                        if (rc) return rc;
+                        CCD_HIP_CHECK(ctx, hipEventRecord(e1, st));
+                        CCD_HIP_CHECK(ctx, hipEventSynchronize(e1));
+                        float ms = 0.f;
+                        CCD_HIP_CHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
+                        if (rep == 1) total = ms;
+                    }
+                    if (total < best_ms) {
+                        best_ms = total;
+                        best[kind] = s;
+                    }
+                }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->vf_shape = best[0];
+    ctx->ee_shape = best[1];
+    graph_drop(ctx);
+    if (shape) {
+        for (int k = 0; k < 2; k++) {
+            shape[3 * k] = best[k].block;
+            shape[3 * k + 1] = best[k].tile;
+            shape[3 * k + 2] = best[k].waves_per_cu;
+        }
+    }
     return CCD_OK;
 }
 

@@ -10,8 +10,17 @@
 
 #include "../../include/ccd_amd.h"
 
+// Launch shape of the persistent work-queue narrowphase (tuned by ccd_autotune).
+struct QueueShape {
+    int block = 256;       // threads per block: 64, 128 or 256
+    int tile = 256;        // stencils a wavefront takes per atomic
+    int waves_per_cu = 8;  // resident wavefronts per compute unit in the persistent grid
+};
+
 struct ccd_ctx {
     int device = 0;
+    int num_cus = 256;
+    QueueShape vf_shape, ee_shape;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     std::string err;
@@ -31,6 +40,13 @@ struct ccd_ctx {
     int64_t stage_calls[CCD_NTIMERS] = {};
     // Grow-only capacities of the broadphase outputs (face pairs, stencils).
     int64_t cap_pairs = 0, cap_vf = 0, cap_ee = 0;
+    // Captured detection pipeline (ccd_batch_detect with CCD_GRAPH): replayed while its arguments match.
+    bool capturing = false;
+    bool graph_valid = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    unsigned char graph_key[256] = {};
+    int64_t graph_replays = 0;
     // The candidates of the last broadphase (ccd_last_candidates).
     const int32_t *last_vf = nullptr, *last_ee = nullptr;
     int64_t last_nvf = 0, last_nee = 0;
@@ -86,6 +102,10 @@ int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *
 int launch_narrow2_compact(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
                            const int64_t *n_dev, int64_t n_max, double eta, int32_t *out_keys, double *out_toi,
                            int64_t cap, int64_t *counter);
+// Persistent work-queue narrowphase; *work must be zero before the launch.
+int launch_narrow2_queue(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
+                         const int64_t *n_dev, int64_t n_max, double eta, int32_t *out_keys, double *out_toi,
+                         int64_t cap, int64_t *counter, unsigned long long *work, const QueueShape &shape);
 // Work accounting: per-stage execution counts (ccd::Stage order) summed over the stencils into counts.
 int launch_narrow2_count(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
                          int64_t n, double eta, unsigned long long *counts);

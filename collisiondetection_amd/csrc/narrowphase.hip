@@ -6,6 +6,8 @@
 // touch.  The work per stencil is the reference's fixed sub-test order with early exit
 // (CTCDNarrowPhase.cpp:43-69, :91-132), so lanes diverge; the kernel is FP64-VALU bound.
 #include "ccd_core.h"
+#include <algorithm>
+
 #include "ccd_internal.h"
 
 namespace {
@@ -117,6 +119,53 @@ __global__ __launch_bounds__(256) void k_narrow2_compact(const double *__restric
         if (pos < cap) {
             out_keys[pos] = k;
             if (out_toi) out_toi[pos] = t;
+        }
+    }
+}
+
+// Persistent work-queue form of the compacting narrowphase: a grid sized to the machine, each
+// wavefront repeatedly takes the next `tile` stencils with one atomic on *work and tests them 64 at a
+// time.  Per-stencil cost varies by orders of magnitude (early exits vs. full root isolation), so
+// dynamic tiles keep every SIMD busy to the end; tile and block size are tuning parameters.
+template <bool VF, int BS>
+__global__ __launch_bounds__(BS) void k_narrow2_queue(const double *__restrict__ qs, const double *__restrict__ qe,
+                                                      const int4 *__restrict__ keys, const int64_t *__restrict__ n_dev,
+                                                      double eta, int4 *__restrict__ out_keys,
+                                                      double *__restrict__ out_toi, int64_t cap,
+                                                      unsigned long long *counter, unsigned long long *work, int tile)
+{
+    const int64_t n = *n_dev;
+    const int lane = __lane_id();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    while (true) {
+        unsigned long long start = 0;
+        if (lane == 0) start = atomicAdd(work, (unsigned long long)tile);
+        start = __shfl(start, 0);
+        if ((int64_t)start >= n) break;
+        int64_t end = (int64_t)start + tile < n ? (int64_t)start + tile : n;
+        for (int64_t j0 = (int64_t)start; j0 < end; j0 += 64) {
+            int64_t i = j0 + lane;
+            bool hit = false;
+            double t = 0.0;
+            int4 k = make_int4(0, 0, 0, 0);
+            if (i < end) {
+                k = keys[i];
+                hit = stencil2<VF>(qs, qe, k, eta, t) >= 0;
+            }
+            uint64_t mask = __ballot(hit);
+            if (mask) {
+                int leader = __ffsll((unsigned long long)mask) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+                base = __shfl(base, leader);
+                if (hit) {
+                    int64_t pos = (int64_t)base + __popcll(mask & lt);
+                    if (pos < cap) {
+                        out_keys[pos] = k;
+                        if (out_toi) out_toi[pos] = t;
+                    }
+                }
+            }
         }
     }
 }
@@ -319,6 +368,42 @@ int launch_narrow2_count(ccd_ctx *ctx, bool vf, const double *qs, const double *
                            (const int4 *)keys, n, eta, counts);
     CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
+}
+
+template <bool VF>
+static int launch_queue_bs(ccd_ctx *ctx, int bs, unsigned grid, const double *qs, const double *qe,
+                           const int32_t *keys, const int64_t *n_dev, double eta, int32_t *out_keys, double *out_toi,
+                           int64_t cap, int64_t *counter, unsigned long long *work, int tile)
+{
+#define CCD_QUEUE_LAUNCH(B)                                                                                         \
+    hipLaunchKernelGGL((k_narrow2_queue<VF, B>), dim3(grid), dim3(B), 0, ctx->stream, qs, qe, (const int4 *)keys,  \
+                       n_dev, eta, (int4 *)out_keys, out_toi, cap, (unsigned long long *)counter, work, tile)
+    if (bs == 64)
+        CCD_QUEUE_LAUNCH(64);
+    else if (bs == 128)
+        CCD_QUEUE_LAUNCH(128);
+    else
+        CCD_QUEUE_LAUNCH(256);
+#undef CCD_QUEUE_LAUNCH
+    CCD_LAUNCH_CHECK(ctx);
+    return CCD_OK;
+}
+
+int launch_narrow2_queue(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
+                         const int64_t *n_dev, int64_t n_max, double eta, int32_t *out_keys, double *out_toi,
+                         int64_t cap, int64_t *counter, unsigned long long *work, const QueueShape &shape)
+{
+    if (n_max == 0) return CCD_OK;
+    int bs = shape.block == 64 || shape.block == 128 ? shape.block : 256;
+    int64_t waves = (int64_t)ctx->num_cus * shape.waves_per_cu;
+    int64_t need = (n_max + shape.tile - 1) / shape.tile;   // no more waves than tiles
+    waves = std::max<int64_t>(1, std::min(waves, need));
+    unsigned grid = (unsigned)((waves * 64 + bs - 1) / bs);
+    if (vf)
+        return launch_queue_bs<true>(ctx, bs, grid, qs, qe, keys, n_dev, eta, out_keys, out_toi, cap, counter, work,
+                                     shape.tile);
+    return launch_queue_bs<false>(ctx, bs, grid, qs, qe, keys, n_dev, eta, out_keys, out_toi, cap, counter, work,
+                                  shape.tile);
 }
 
 int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *ht, const double *hp,

@@ -10,6 +10,7 @@ Random draws use numpy.random.default_rng(seed) (PCG64), so a scene is fixed by 
 """
 from __future__ import annotations
 
+import functools
 import hashlib
 import math
 
@@ -35,6 +36,21 @@ def icosahedron():
 
 def icosphere(levels: int, radius: float = 1.0):
     """Icosahedron with `levels` midpoint subdivisions, projected to the sphere: 10*4^k+2 V, 20*4^k F."""
+    v, f = _unit_icosphere(levels)
+    return v * radius, f.copy()
+
+
+@functools.lru_cache(maxsize=None)
+def _unit_icosphere_cached(levels: int):
+    return _build_icosphere(levels)
+
+
+def _unit_icosphere(levels: int):
+    v, f = _unit_icosphere_cached(levels)
+    return v.copy(), f
+
+
+def _build_icosphere(levels: int):
     v, f = icosahedron()
     verts = [tuple(x) for x in v]
     for _ in range(levels):
@@ -55,7 +71,7 @@ def icosphere(levels: int, radius: float = 1.0):
             ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
             nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
         f = np.asarray(nf, dtype=np.int32)
-    return np.asarray(verts, dtype=np.float64) * radius, f
+    return np.asarray(verts, dtype=np.float64), f
 
 
 def grid(n: int, x0=0.0, x1=1.0, y0=0.0, y1=1.0):

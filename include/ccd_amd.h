@@ -42,8 +42,10 @@ extern "C" {
 #define CCD_DIST_VERTEX_FACE 2     /* Distance::vertexFaceDistance(p, q0, q1, q2, b0, b1, b2)    (Distance.h:32-115) */
 #define CCD_DIST_EDGE_EDGE 3       /* Distance::edgeEdgeDistance(p0, p1, q0, q1, b0..b3)          (Distance.h:119-174) */
 
-/* Flags for ccd_kdop_candidates. */
+/* Flags for ccd_kdop_candidates and ccd_batch_detect. */
 #define CCD_SORTED 1 /* return stencils in the reference's std::set order (Stencils.h:59-77, :119-137) */
+#define CCD_GRAPH 2  /* ccd_batch_detect: capture the pipeline in a HIP graph and replay it while the
+                        arguments (pointers, sizes, eta) stay the same; the data behind them may change */
 
 /* Stage timers (ccd_stage_times), in pipeline order. */
 #define CCD_T_BOUNDS 0    /* leaf k-DOPs, scene box, Morton codes */
@@ -54,7 +56,8 @@ extern "C" {
 #define CCD_T_EXPAND 5    /* stencil expansion with deduplication */
 #define CCD_T_NARROW_VF 6 /* vertex-face narrowphase */
 #define CCD_T_NARROW_EE 7 /* edge-edge narrowphase */
-#define CCD_NTIMERS 8
+#define CCD_T_PIPELINE 8  /* a whole captured pipeline (graph replays) */
+#define CCD_NTIMERS 9
 
 /* Narrowphase work stages (ccd_narrowphase_work), in the order of ccd::Stage in ccd_core.h. */
 #define CCD_NSTAGES 12
@@ -205,6 +208,33 @@ int ccd_stage_flops(int64_t *flops);
  */
 int ccd_last_candidates(ccd_ctx *ctx, const int32_t **vf_keys, int64_t *nvf, const int32_t **ee_keys,
                         int64_t *nee);
+
+/*
+ * Batched detection: ninst independent meshes in one pass.  Instance k owns faces
+ * inst_face_off[k] .. inst_face_off[k+1]-1 (inst_face_off: device, ninst+1 int64, starting at 0 and
+ * ending at nfaces); faces hold global vertex indices and no vertex is shared between instances.  The
+ * result is the union of ccd_detect over the instances -- no stencil pairs primitives of two
+ * instances -- computed by one pipeline: one forest of radix trees (instance id in the high bits of
+ * the sort keys), one traversal, one persistent work-queue narrowphase.  flags: CCD_GRAPH replays a
+ * captured HIP graph of the whole pipeline on calls with the same arguments (after the first call has
+ * sized every buffer); a capacity overflow found after a replay falls back to a fresh synchronous run.
+ * Output and counts as ccd_detect.  Synchronous.  No counterpart in the reference (one mesh per call).
+ */
+int ccd_batch_detect(ccd_ctx *ctx, int64_t ninst, const int64_t *inst_face_off, const int32_t *faces, int64_t nfaces,
+                     const double *qstart, const double *qend, int64_t nverts, double outer_eta, double eta,
+                     int32_t *vf_hits, double *vf_toi, int64_t vf_cap, int64_t *vf_hit_count, int32_t *ee_hits,
+                     double *ee_toi, int64_t ee_cap, int64_t *ee_hit_count, int64_t *vf_candidates,
+                     int64_t *ee_candidates, int flags);
+
+/*
+ * Tuning of the persistent narrowphase: times the work-queue kernel on the candidates of the last
+ * detection on this context over block sizes {64, 128, 256}, tiles {64, 128, 256, 512, 1024} and
+ * {4, 8} resident wavefronts per CU, keeps the fastest shape per stencil kind and returns it in
+ * shape[0..5] = {vf block, vf tile, vf waves/CU, ee block, ee tile, ee waves/CU} (host, may be NULL).
+ * ccd_set_queue_shape installs a shape (e.g. one cached from an earlier tuning run).
+ */
+int ccd_autotune(ccd_ctx *ctx, const double *qstart, const double *qend, double eta, int *shape);
+int ccd_set_queue_shape(ccd_ctx *ctx, const int *shape);
 
 #ifdef __cplusplus
 }

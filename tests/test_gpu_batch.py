@@ -1,0 +1,105 @@
+"""Batched detection (ccd_batch_detect): many independent instances in one pipeline equal the union of
+per-instance ccd_detect calls and the oracle; the captured HIP graph replays give the same results,
+also after the positions behind the captured pointers change."""
+import numpy as np
+import pytest
+
+import oracle
+from collisiondetection_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _as_set(keys, toi):
+    return {tuple(k): t for k, t in zip(keys.tolist(), toi)}
+
+
+def _instances(count=24, seed=7):
+    return scenes.batched_instances(count=count, seed=seed, tri_lo=80, tri_hi=1500)
+
+
+def test_batch_equals_per_instance_detect():
+    from collisiondetection_amd import api
+
+    print("case: batch vs per-instance", flush=True)
+    inst = _instances()
+    b = api.Batch(inst)
+    r = b.detect(1e-6, 1e-6)
+    per_vf, per_ee, nvf, nee = {}, {}, 0, 0
+    for k, s in enumerate(inst):
+        d = api.detect(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"])
+        off = b.vert_off[k]
+        per_vf.update(_as_set(d["vf"] + off, d["vf_toi"]))
+        per_ee.update(_as_set(d["ee"] + off, d["ee_toi"]))
+        nvf += d["vf_candidates"]
+        nee += d["ee_candidates"]
+    assert r["vf_candidates"] == nvf and r["ee_candidates"] == nee
+    gv, ge = _as_set(r["vf"], r["vf_toi"]), _as_set(r["ee"], r["ee_toi"])
+    assert gv == per_vf and ge == per_ee   # same stencils, same TOI bits
+    assert len(gv) + len(ge) > 0
+    # no stencil spans two instances
+    for keys in (r["vf"], r["ee"]):
+        inst_ids = b.instance_of_vertex(keys)
+        assert np.all(inst_ids == inst_ids[:, :1])
+
+
+def test_batch_instances_vs_oracle():
+    from collisiondetection_amd import api
+
+    print("case: batch vs oracle", flush=True)
+    inst = _instances(count=6, seed=11)
+    b = api.Batch(inst)
+    r = b.detect(1e-6, 1e-6)
+    gv, ge = _as_set(r["vf"], r["vf_toi"]), _as_set(r["ee"], r["ee_toi"])
+    for k, s in enumerate(inst):
+        h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+        vf, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
+        off = b.vert_off[k]
+        for keys, is_ee, got in ((vf, False, gv), (ee, True, ge)):
+            rec = oracle.narrowphase(h, keys, s["eta"], is_ee)
+            o = {tuple(x + off): t for x, t, hit in zip(keys.tolist(), rec["t"], rec["hit"]) if hit}
+            border = {tuple(x + off) for x, m in zip(keys.tolist(), rec["margin"]) if m < 1e-12}
+            mine = {kk: v for kk, v in got.items() if b.instance_of_vertex(kk[0]) == k}
+            assert set(mine) ^ set(o) <= border
+            for kk in set(mine) & set(o):
+                assert abs(mine[kk] - o[kk]) < 1e-9 or kk in border
+
+
+def test_batch_graph_replay_and_refit():
+    import torch
+
+    from collisiondetection_amd import api
+
+    print("case: graph capture and replay", flush=True)
+    inst = _instances(count=16, seed=13)
+    b = api.Batch(inst)
+    r0 = b.detect(1e-6, 1e-6, graph=False)
+    r1 = b.detect(1e-6, 1e-6, graph=True)    # synchronous run + capture
+    r2 = b.detect(1e-6, 1e-6, graph=True)    # replay
+    for r in (r1, r2):
+        assert _as_set(r["vf"], r["vf_toi"]) == _as_set(r0["vf"], r0["vf_toi"])
+        assert _as_set(r["ee"], r["ee_toi"]) == _as_set(r0["ee"], r0["ee_toi"])
+    # new end positions behind the same pointers: the replay must equal a fresh synchronous run
+    rng = np.random.default_rng(3)
+    for step in range(3):
+        noise = torch.from_numpy(rng.normal(0, 0.002, size=b.V * 3)).to(b.qe.device)
+        b.qe.add_(noise)
+        torch.cuda.synchronize()
+        rg = b.detect(1e-6, 1e-6, graph=True)
+        rs = b.detect(1e-6, 1e-6, graph=False)
+        assert _as_set(rg["vf"], rg["vf_toi"]) == _as_set(rs["vf"], rs["vf_toi"]), step
+        assert _as_set(rg["ee"], rg["ee_toi"]) == _as_set(rs["ee"], rs["ee_toi"]), step
+        assert rg["vf_candidates"] == rs["vf_candidates"] and rg["ee_candidates"] == rs["ee_candidates"]
+
+
+def test_autotune_keeps_results():
+    from collisiondetection_amd import api
+
+    print("case: autotune", flush=True)
+    s = scenes.cloth(96, drop=0.05)
+    d0 = api.detect(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"])
+    shape = api.autotune(s["qstart"], s["qend"], s["eta"])
+    assert len(shape) == 6 and shape[0] in (64, 128, 256) and shape[3] in (64, 128, 256)
+    d1 = api.detect(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"])
+    assert _as_set(d0["vf"], d0["vf_toi"]) == _as_set(d1["vf"], d1["vf_toi"])
+    assert _as_set(d0["ee"], d0["ee_toi"]) == _as_set(d1["ee"], d1["ee_toi"])
