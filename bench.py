@@ -133,6 +133,82 @@ def bench_batched(steps, log):
             "candidates": {"vf": rg["vf_candidates"], "ee": rg["ee_candidates"]}}
 
 
+def bench_served(seconds, rate, log, sessions=256, budget_gib=32, max_inflight=16):
+    """The served workload: `sessions` device-resident sessions (log-uniform 1k-500k triangles, seed 4),
+    Poisson arrivals at `rate` requests/s for `seconds`, each to a uniform session, supplying new end
+    positions = previous end + N(0, 0.002^2).  Latency = request submission to result on the host."""
+    import numpy as np
+
+    from collisiondetection_amd import api, scenes
+
+    sess = scenes.served_sessions(sessions)
+    nf = np.array([len(s["faces"]) for s in sess])
+    log(f"served workload: {len(sess)} sessions, {int(nf.sum())} resident triangles")
+    srv = api.Server(hbm_budget_bytes=budget_gib << 30, max_inflight=max_inflight)
+    t0 = time.perf_counter()
+    ids = []
+    for j, s in enumerate(sess):
+        ids.append(srv.open_session(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"]))
+        if j % 64 == 63:
+            log(f"opened {j + 1} sessions")
+    log(f"sessions opened in {time.perf_counter() - t0:.1f} s; stats {srv.stats()}")
+    ends = [s["qend"].reshape(-1).copy() for s in sess]
+    noise = scenes.NoiseBank(0.002, seed=4)
+    rng = np.random.default_rng(4)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 1.2) + 16))
+    arrivals = arrivals[arrivals < seconds]
+    targets = rng.integers(0, len(sess), size=len(arrivals))
+    results = []
+    submitted = 0
+    backlog_cap = int(rate * 2)   # stop offering load once 2 s of requests wait (overload)
+    overloaded = False
+    start = time.perf_counter()
+    for i, (ta, k) in enumerate(zip(arrivals, targets)):
+        while True:
+            now = time.perf_counter() - start
+            if now >= ta:
+                break
+            results += srv.poll()
+            if ta - (time.perf_counter() - start) > 0.0005:
+                time.sleep(0.0002)
+        ends[k] = ends[k] + noise.draw(len(ends[k]))
+        srv.submit(ids[k], ends[k], i)
+        submitted += 1
+        if i % 256 == 0:
+            st = srv.stats()
+            if st["pending"] > backlog_cap:
+                overloaded = True
+                log(f"overloaded at request {i} ({ta:.2f} s): {st['pending']} pending")
+                break
+        if i % 4000 == 0:
+            log(f"served: {i} requests submitted at {ta:.1f} s, {len(results)} done")
+    offered_s = time.perf_counter() - start
+    srv.drain()
+    results += srv.poll(1 << 20)
+    end = time.perf_counter() - start
+    st = srv.stats()
+    srv.close()
+    ok = [r for r in results if r["status"] == 0]
+    lat = np.array([r["t_done"] - r["t_submit"] for r in ok]) * 1e3
+    tris = np.array([nf[r["session"]] for r in ok])
+    small = lat[tris <= 100000]
+    t_first = min(r["t_submit"] for r in ok)
+    t_last = max(r["t_done"] for r in ok)
+    out = {"sessions": len(sess), "resident_triangles": int(nf.sum()), "rate_offered": rate, "seconds": seconds,
+           "submitted": submitted, "completed": len(ok), "failed": len(results) - len(ok), "overloaded": overloaded,
+           "offered_wall_s": offered_s, "drain_wall_s": end - offered_s,
+           "swept_tris_per_s": float(tris.sum() / (t_last - t_first)),
+           "requests_per_s": len(ok) / (t_last - t_first),
+           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                          "p50_le100k": float(np.percentile(small, 50)) if len(small) else None,
+                          "p99_le100k": float(np.percentile(small, 99)) if len(small) else None},
+           "reruns": st["reruns"], "pool_bytes_peak_note": "pool = admission budget", "budget_gib": budget_gib,
+           "max_inflight": max_inflight,
+           "client_noise": "one bank of 2^24 N(0, 0.002^2) draws, read at a random offset per request"}
+    log("served:", json.dumps(out))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +217,9 @@ def main():
     ap.add_argument("--grid", type=int, default=256, help="cloth grid size (256 = the 130,050-triangle cloth)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -249,6 +328,8 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = bench_batched(max(3, args.steps // 2), log)
+    if not args.no_served:
+        out["served"] = bench_served(args.serve_seconds, args.serve_rate, log)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, 1 thread) ...")
         out["cpu_baseline"] = cpu_baseline(scene)

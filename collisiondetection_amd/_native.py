@@ -77,6 +77,26 @@ NSTAGES = 12
 STAGE_NAMES = ("vf_call", "vf_plane", "vf_dist", "ee_dist", "ee_par", "ee_bary", "ve_call", "ve_quad", "ve_quart",
                "vv", "quad_root", "isolate")
 
+class RequestResult(ctypes.Structure):
+    """ccd_request_result (include/ccd_amd.h)."""
+    _fields_ = [("request_id", ctypes.c_int64), ("session", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("vf_hits", ctypes.c_int64), ("ee_hits", ctypes.c_int64), ("vf_candidates", ctypes.c_int64),
+                ("ee_candidates", ctypes.c_int64), ("checksum_vf", ctypes.c_uint64), ("checksum_ee", ctypes.c_uint64),
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+
+SIGNATURES.update({
+    "ccd_server_create": ([_INT, _I64, _INT, ctypes.POINTER(_V)], _INT),
+    "ccd_server_open_session": ([_V, _V, _I64, _V, _V, _I64, _D, _D, ctypes.POINTER(_INT)], _INT),
+    "ccd_server_submit": ([_V, _INT, _V, _I64], _INT),
+    "ccd_server_poll": ([_V, ctypes.POINTER(RequestResult), _INT, ctypes.POINTER(_INT)], _INT),
+    "ccd_server_drain": ([_V], _INT),
+    "ccd_server_stats": ([_V, _P64], _INT),
+    "ccd_server_keep_hits": ([_V, _INT], _INT),
+    "ccd_server_fetch_hits": ([_V, _I64, _V, _I64, _P64], _INT),
+    "ccd_server_destroy": ([_V], _INT),
+})
+
 _lib = None
 
 

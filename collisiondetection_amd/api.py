@@ -527,3 +527,89 @@ def autotune(qstart, qend, eta, ctx: Context | None = None):
     shape = (ctypes.c_int * 6)()
     ctx.check(N.lib().ccd_autotune(ctx.ptr, _ptr(qs), _ptr(qe), float(eta), shape))
     return list(shape)
+
+
+# ------------------------------------------------------------------------------------------------
+# Serving
+# ------------------------------------------------------------------------------------------------
+class Server:
+    """Device-resident sessions behind a continuous-batching request loop (ccd_server_* in
+    include/ccd_amd.h).  Sessions refit their tree per request; requests of different sessions
+    overlap on the device."""
+
+    def __init__(self, hbm_budget_bytes=8 << 30, max_inflight=8, device=0):
+        _torch()
+        self._p = ctypes.c_void_p()
+        rc = N.lib().ccd_server_create(device, int(hbm_budget_bytes), int(max_inflight), ctypes.byref(self._p))
+        if rc != N.CCD_OK:
+            raise N.CCDError(rc, "ccd_server_create failed")
+        self.nverts = {}
+        self._vf_hits = {}
+
+    def close(self):
+        if self._p is not None and self._p.value:
+            N.lib().ccd_server_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def open_session(self, faces, qstart, qend, outer_eta, eta) -> int:
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+        a = np.ascontiguousarray(qstart, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(qend, dtype=np.float64).reshape(-1)
+        sid = ctypes.c_int(-1)
+        rc = N.lib().ccd_server_open_session(self._p, f.ctypes.data_as(ctypes.c_void_p), len(f) // 3,
+                                             a.ctypes.data_as(ctypes.c_void_p), b.ctypes.data_as(ctypes.c_void_p),
+                                             len(a) // 3, float(outer_eta), float(eta), ctypes.byref(sid))
+        if rc != N.CCD_OK:
+            raise N.CCDError(rc, "ccd_server_open_session failed")
+        self.nverts[sid.value] = len(a) // 3
+        return sid.value
+
+    def submit(self, session: int, qend_next, request_id: int):
+        q = np.ascontiguousarray(qend_next, dtype=np.float64).reshape(-1)
+        assert len(q) == 3 * self.nverts[session]
+        rc = N.lib().ccd_server_submit(self._p, int(session), q.ctypes.data_as(ctypes.c_void_p), int(request_id))
+        if rc != N.CCD_OK:
+            raise N.CCDError(rc, "ccd_server_submit failed")
+
+    def poll(self, max_results=4096):
+        buf = (N.RequestResult * max_results)()
+        n = ctypes.c_int(0)
+        N.lib().ccd_server_poll(self._p, buf, max_results, ctypes.byref(n))
+        out = [{f: getattr(buf[i], f) for f, _ in N.RequestResult._fields_} for i in range(n.value)]
+        for r in out:
+            self._vf_hits[r["request_id"]] = r["vf_hits"]
+        return out
+
+    def drain(self):
+        N.lib().ccd_server_drain(self._p)
+
+    def stats(self):
+        s = (ctypes.c_int64 * 10)()
+        N.lib().ccd_server_stats(self._p, s)
+        return dict(zip(("submitted", "completed", "pending", "inflight", "pool_bytes", "resident_bytes",
+                         "reruns", "sessions", "mem_waits", "max_request_bytes"), list(s)))
+
+    def keep_hits(self, on=True):
+        N.lib().ccd_server_keep_hits(self._p, int(on))
+
+    def fetch_hits(self, request_id):
+        """-> (vf keys (n,4), vf toi, ee keys (m,4), ee toi) of a request that completed while keep_hits
+        was on and has been returned by poll()."""
+        n = ctypes.c_int64(0)
+        N.lib().ccd_server_fetch_hits(self._p, int(request_id), None, 0, ctypes.byref(n))
+        buf = np.zeros(n.value, dtype=np.int32)
+        rc = N.lib().ccd_server_fetch_hits(self._p, int(request_id), buf.ctypes.data_as(ctypes.c_void_p), n.value,
+                                           ctypes.byref(n))
+        if rc != N.CCD_OK:
+            raise N.CCDError(rc, "ccd_server_fetch_hits failed")
+        m = n.value // 6   # per stencil: 4 key ints, then (after all keys) 2 ints of FP64 TOI
+        nv = self._vf_hits[request_id]
+        keys = buf[: 4 * m].reshape(-1, 4)
+        toi = buf[4 * m:].view(np.float64)
+        return keys[:nv], toi[:nv], keys[nv:], toi[nv:]

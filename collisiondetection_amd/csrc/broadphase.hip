@@ -909,6 +909,7 @@ int broadphase_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const doubl
     if (rc) return rc;
     const uint64_t *skeys = in_alt ? b.keys2 : b.keys;
     const int32_t *order = in_alt ? b.order : b.vals;
+    ctx->tree_order = order;
     prof_end(ctx, CCD_T_SORT);
 
     prof_begin(ctx, CCD_T_BUILD);
@@ -943,6 +944,57 @@ int broadphase_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const doubl
     CCD_LAUNCH_CHECK(ctx);
     prof_end(ctx, CCD_T_TRAVERSE);
 
+    prof_begin(ctx, CCD_T_EXPAND);
+    MeshView mv{faces, voff, vlist, b.dop64, fixed};
+    unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(ctx->cap_pairs, 256), 16384));
+    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(256), 0, st, mv, b.pairs, c + C_PAIRS, ctx->cap_pairs, c + C_BAD,
+                       b.vfk, ctx->cap_vf, c + C_VF, b.eek, ctx->cap_ee, c + C_EE);
+    CCD_LAUNCH_CHECK(ctx);
+    prof_end(ctx, CCD_T_EXPAND);
+    return CCD_OK;
+}
+
+// Refit: the tree of the last full build on this context (same faces, same instance layout) with new
+// positions -- new leaf bounds and records, then the bottom-up refit; the Morton order, the radix tree,
+// the escape links and the vertex->face table are kept.  Candidate sets do not depend on the tree's
+// shape, so this is exact; only culling efficiency depends on how far the mesh moved since the build.
+int broadphase_refit_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, const double *qe,
+                            int64_t V, double eta, const uint8_t *fixed, const int64_t *inst_off, int ninst)
+{
+    hipStream_t st = ctx->stream;
+    int64_t *cnt = ctx->d_counts;
+    unsigned long long *c = (unsigned long long *)cnt;
+    int rc = fill_u64(ctx, (uint64_t *)cnt, NCOUNTERS, 0);
+    if (rc) return rc;
+    if (F < 2) return CCD_OK;
+    const bool batched = inst_off && ninst > 1;
+    BPBufs b;
+    rc = bp_buffers(ctx, F, V, batched, b);
+    if (rc) return rc;
+    static const Axes axes = make_axes();
+    prof_begin(ctx, CCD_T_BOUNDS);
+    hipLaunchKernelGGL(k_leaf_bounds, dim3(blocks_for(F, 256)), dim3(256), 0, st, faces, F, qs, qe,
+                       (const int64_t *)nullptr, (const double *)nullptr, eta, axes, 0, b.dop64, b.centre);
+    CCD_LAUNCH_CHECK(ctx);
+    prof_end(ctx, CCD_T_BOUNDS);
+    prof_begin(ctx, CCD_T_BUILD);
+    hipLaunchKernelGGL(k_leaf_records, dim3(blocks_for(F, 256)), dim3(256), 0, st, ctx->tree_order, b.dop64, faces,
+                       F, b.escape, b.nodes, b.sdop64, b.sfaces);
+    CCD_LAUNCH_CHECK(ctx);
+    rc = fill_u32(ctx, b.arrivals, F, 0);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_refit, dim3(blocks_for(F, 256)), dim3(256), 0, st, F, b.parent, b.left, b.escape, b.nodes,
+                       b.arrivals);
+    CCD_LAUNCH_CHECK(ctx);
+    prof_end(ctx, CCD_T_BUILD);
+    const int32_t *voff = nullptr, *vlist = nullptr;
+    vertex_face_table(ctx, &voff, &vlist);
+    prof_begin(ctx, CCD_T_TRAVERSE);
+    hipLaunchKernelGGL(k_traverse, dim3(blocks_for(F, 256)), dim3(256), 0, st, F, b.nodes, b.sdop64, b.sfaces,
+                       batched ? b.face_inst : (const int32_t *)nullptr, inst_off, b.pairs, ctx->cap_pairs,
+                       c + C_PAIRS);
+    CCD_LAUNCH_CHECK(ctx);
+    prof_end(ctx, CCD_T_TRAVERSE);
     prof_begin(ctx, CCD_T_EXPAND);
     MeshView mv{faces, voff, vlist, b.dop64, fixed};
     unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(ctx->cap_pairs, 256), 16384));

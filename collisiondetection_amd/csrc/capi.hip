@@ -56,6 +56,10 @@ void *buffer(ccd_ctx *ctx, const char *name, size_t bytes)
     if (bytes == 0) bytes = 16;
     ccd_ctx::Buf &b = ctx->bufs[name];
     if (b.bytes >= bytes) return b.ptr;
+    if (b.external) {
+        set_error(ctx, CCD_E_CAPACITY, "external buffer %s too small (%zu < %zu bytes)", name, b.bytes, bytes);
+        return nullptr;
+    }
     if (b.ptr) {
         // The old allocation may still be read by queued work on the stream.
         if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
@@ -170,7 +174,7 @@ int ccd_destroy(ccd_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &kv : ctx->bufs)
-        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+        if (kv.second.ptr && !kv.second.external) (void)hipFree(kv.second.ptr);
     if (ctx->d_counts) (void)hipFree(ctx->d_counts);
     for (int i = 0; i < 2 * CCD_NTIMERS; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);

@@ -27,6 +27,7 @@ struct ccd_ctx {
     struct Buf {
         void *ptr = nullptr;
         size_t bytes = 0;
+        bool external = false;   // memory owned by someone else (a server's pool): never freed here
     };
     std::unordered_map<std::string, Buf> bufs;   // named grow-only device buffers
     int64_t device_bytes = 0;
@@ -40,6 +41,8 @@ struct ccd_ctx {
     int64_t stage_calls[CCD_NTIMERS] = {};
     // Grow-only capacities of the broadphase outputs (face pairs, stencils).
     int64_t cap_pairs = 0, cap_vf = 0, cap_ee = 0;
+    // Sorted face order of the last full broadphase build (kept for refits).
+    const int32_t *tree_order = nullptr;
     // Captured detection pipeline (ccd_batch_detect with CCD_GRAPH): replayed while its arguments match.
     bool capturing = false;
     bool graph_valid = false;
@@ -137,6 +140,9 @@ int broadphase_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const doubl
                       const int64_t *hist_off, const double *hist_pos, int64_t V, double eta, const uint8_t *fixed,
                       bool aabb, const int64_t *inst_off, int ninst);
 int build_vertex_face_table_buffers(ccd_ctx *ctx, int64_t F, int64_t V);
+// Refit of the last full build's tree with new positions (launches only; see broadphase.hip).
+int broadphase_refit_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const double *qs, const double *qe,
+                            int64_t V, double eta, const uint8_t *fixed, const int64_t *inst_off, int ninst);
 // Vertex -> face CSR table (device buffers "vf_off" / "vf_list"), and its current addresses.
 int build_vertex_face_table(ccd_ctx *ctx, const int32_t *faces, int64_t F, int64_t V);
 void vertex_face_table(ccd_ctx *ctx, const int32_t **off, const int32_t **list);

@@ -218,3 +218,48 @@ def digest(scene) -> str:
     h.update(np.ascontiguousarray(scene["qend"], dtype="<f8").tobytes())
     h.update(np.float64(scene["eta"]).astype("<f8").tobytes())
     return h.hexdigest()
+
+
+def served_sessions(count: int = 256, seed: int = 4, tri_lo: float = 1000, tri_hi: float = 500000,
+                    spacing: float = 0.02, eta: float = 1e-6):
+    """Resident sessions for the served workload: `count` sessions with log-uniform triangle counts in
+    [tri_lo, tri_hi]; each is (50/50) a self-colliding draped sheet (as `cloth`) or a pair of noisy
+    sheets 2*spacing apart.  Grid spacing is fixed at `spacing` so that density does not grow with
+    size.  Start = end = the rest shape; requests move the end (see `served_requests`)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        target = math.exp(rng.uniform(math.log(tri_lo), math.log(tri_hi)))
+        kind = int(rng.integers(0, 2))
+        if kind == 0:
+            n = max(3, int(round(math.sqrt(target / 2.0))) + 1)
+            L = spacing * (n - 1)
+            v, f = grid(n, 0.0, L, 0.0, L)
+            v[:, 2] = 0.15 * L * np.sin(6.0 * v[:, 0] / L) * np.cos(5.0 * v[:, 1] / L)
+        else:
+            n = max(3, int(round(math.sqrt(target / 4.0))) + 1)
+            L = spacing * (n - 1)
+            a, fa = grid(n, 0.0, L, 0.0, L)
+            b = a.copy()
+            a[:, 2] = rng.normal(0.0, 0.1 * spacing, size=len(a))
+            b[:, 2] = 2.0 * spacing + rng.normal(0.0, 0.1 * spacing, size=len(b))
+            v = np.concatenate([a, b])
+            f = np.concatenate([fa, fa + len(a)])
+        out.append(_scene(f"session{k}_s{seed}", f, v, v.copy(), eta))
+    return out
+
+
+class NoiseBank:
+    """N(0, sigma^2) draws for request generation at serving rates: one bank of 2^24 normals (seed),
+    read at a random offset per request (a fresh draw per request would cost the client more than
+    the server spends on it)."""
+
+    def __init__(self, sigma: float = 0.002, seed: int = 4, size: int = 1 << 24):
+        self.rng = np.random.default_rng(seed)
+        self.bank = self.rng.normal(0.0, sigma, size=size)
+
+    def draw(self, n: int):
+        if n > len(self.bank):
+            return self.rng.normal(0.0, self.bank.std(), size=n)
+        o = int(self.rng.integers(0, len(self.bank) - n + 1))
+        return self.bank[o:o + n]

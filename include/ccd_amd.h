@@ -236,6 +236,48 @@ int ccd_batch_detect(ccd_ctx *ctx, int64_t ninst, const int64_t *inst_face_off, 
 int ccd_autotune(ccd_ctx *ctx, const double *qstart, const double *qend, double eta, int *shape);
 int ccd_set_queue_shape(ccd_ctx *ctx, const int *shape);
 
+/*
+ * Serving (no counterpart in the reference): device-resident sessions and a continuous-batching loop.
+ *
+ * ccd_server_create: a server on `device` with an HBM pool of hbm_budget_bytes for the transient
+ *   workspace of in-flight requests (face pairs, stencils, collisions) and at most max_inflight
+ *   requests on the device at once.  A worker thread admits requests in arrival order when their
+ *   session is idle, a slot is free and their workspace fits in the pool.
+ * ccd_server_open_session: uploads a mesh (host arrays: faces 3F, qstart/qend 3V) and builds its tree
+ *   (one full detection qstart -> qend).  Fails with CCD_E_BUDGET if less than 2 GiB of device
+ *   memory would remain.  Synchronous; returns the session id.
+ * ccd_server_submit: a request with new end positions (host, 3V, copied): the session's previous end
+ *   becomes the start and its tree is refit, not rebuilt.  Asynchronous.
+ * ccd_server_poll: completed requests (counts, checksums of the colliding stencils, timestamps).
+ * ccd_server_drain: waits until every submitted request has completed.
+ * ccd_server_stats: 10 int64 = {submitted, completed, pending, in flight, pool bytes in use, resident
+ *   bytes, reruns after a workspace overflow, sessions, admissions that waited for pool memory, largest
+ *   request workspace in bytes}.  A request whose workspace exceeds the whole pool completes with
+ *   CCD_E_BUDGET.
+ * ccd_server_keep_hits / ccd_server_fetch_hits: keep each request's colliding stencils (4 int32 keys,
+ *   then the TOIs as doubles, vertex-face first) for retrieval by request id.
+ */
+typedef struct ccd_server ccd_server;
+typedef struct {
+    int64_t request_id;
+    int32_t session;
+    int32_t status;
+    int64_t vf_hits, ee_hits, vf_candidates, ee_candidates;
+    uint64_t checksum_vf, checksum_ee; /* order-independent hashes of (key, TOI bits) of the collisions */
+    double t_submit, t_start, t_done;   /* seconds, steady clock */
+} ccd_request_result;
+
+int ccd_server_create(int device, int64_t hbm_budget_bytes, int max_inflight, ccd_server **out);
+int ccd_server_open_session(ccd_server *server, const int32_t *faces, int64_t nfaces, const double *qstart,
+                            const double *qend, int64_t nverts, double outer_eta, double eta, int *session);
+int ccd_server_submit(ccd_server *server, int session, const double *qend_next, int64_t request_id);
+int ccd_server_poll(ccd_server *server, ccd_request_result *out, int max, int *n);
+int ccd_server_drain(ccd_server *server);
+int ccd_server_stats(ccd_server *server, int64_t *stats);
+int ccd_server_keep_hits(ccd_server *server, int on);
+int ccd_server_fetch_hits(ccd_server *server, int64_t request_id, int32_t *out, int64_t cap, int64_t *n);
+int ccd_server_destroy(ccd_server *server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,10 +46,11 @@ def compare_results(name, hit_a, t_a, rec, tol_t=1e-9, margin_tol=1e-12, arbiter
     """Compare a product result (hit, t) with oracle records.
 
     Flag mismatches whose oracle margin is below margin_tol are listed as borderline.  With an arbiter
-    (a function idx -> oracle records recomputed with exact roots, oracle root finder 'mpmath'), every
-    other disagreement -- a flag, or a time of impact more than tol_t apart -- is re-decided against
-    the exact-root answer: if the product agrees with it, the disagreement is the reference root
-    finder's error and is listed under ref_root_error; otherwise it is hard."""
+    (a function idx -> oracle records recomputed with exact roots, oracle root finder 'mpmath', or a
+    list of such functions tried in turn), every other disagreement -- a flag, or a time of impact
+    more than tol_t apart -- is re-decided against the exact-root answer: if the product agrees with
+    it, the disagreement is the reference root finder's error and is listed under ref_root_error;
+    otherwise it is hard."""
     hit_o = rec["hit"].astype(bool)
     mism = np.where(hit_a != hit_o)[0]
     border = mism[rec["margin"][mism] < margin_tol]
@@ -60,27 +61,58 @@ def compare_results(name, hit_a, t_a, rec, tol_t=1e-9, margin_tol=1e-12, arbiter
     t_bad = idx_both[tdiff > tol_t]
     t_bad_hard = t_bad[rec["margin"][t_bad] >= margin_tol]
     ref_err = []
-    if arbiter is not None and (len(hard) or len(t_bad_hard)):
+    arbiters = [] if arbiter is None else (list(arbiter) if isinstance(arbiter, (list, tuple)) else [arbiter])
+    if arbiters and (len(hard) or len(t_bad_hard)):
         cand = np.unique(np.concatenate([hard, t_bad_hard])).astype(np.int64)
-        ex = arbiter(cand)
+        hard_set = set(hard.tolist())
         keep_hard, keep_t = [], []
-        for j, i in enumerate(cand):
-            eh = bool(ex["hit"][j])
-            ok = (eh == bool(hit_a[i])) and (not eh or abs(ex["t"][j] - t_a[i]) <= tol_t)
-            if ok:
-                ref_err.append((int(i), float(rec["t"][i]), float(ex["t"][j]), float(t_a[i])))
-            elif ex["margin"][j] < margin_tol:
-                pass
-            elif i in set(hard.tolist()):
-                keep_hard.append(int(i))
-            else:
-                keep_t.append(int(i))
+        # Each arbiter re-decides what the previous ones left open; the last one's verdict stands.
+        for level, arb in enumerate(arbiters):
+            ex = arb(cand)
+            left = []
+            for j, i in enumerate(cand):
+                eh = bool(ex["hit"][j])
+                ok = (eh == bool(hit_a[i])) and (not eh or abs(ex["t"][j] - t_a[i]) <= tol_t)
+                if ok:
+                    ref_err.append((int(i), float(rec["t"][i]), float(ex["t"][j]), float(t_a[i])))
+                elif ex["margin"][j] < margin_tol:
+                    pass
+                elif level + 1 < len(arbiters):
+                    left.append(int(i))
+                elif i in hard_set:
+                    keep_hard.append(int(i))
+                else:
+                    keep_t.append(int(i))
+            cand = np.array(left, dtype=np.int64)
+            if not len(cand):
+                break
         hard = np.array(keep_hard, dtype=np.int64)
         t_bad_hard = np.array(keep_t, dtype=np.int64)
     return {"name": name, "n": len(hit_a), "hits": int(hit_o.sum()), "mismatch": len(mism),
             "borderline": border.tolist()[:20], "hard": hard.tolist()[:20],
             "max_tdiff": float(tdiff.max()) if len(tdiff) else 0.0, "t_bad": t_bad.tolist()[:20],
             "t_bad_hard": t_bad_hard.tolist()[:20], "ref_root_error": ref_err[:20], "n_ref_root_error": len(ref_err)}
+
+
+def aberth_arbiter(hist, keys, eta, is_ee):
+    """idx -> oracle records for keys[idx] computed with the oracle's long-double Aberth roots (accurate
+    to ~1e-16 on these polynomials, and fast): a first arbiter, backed by mpmath_arbiter."""
+    import oracle
+
+    def run(idx):
+        eta_arr = np.broadcast_to(np.asarray(eta, dtype=np.float64), (len(keys),))
+        try:
+            oracle.use_rootfinder("aberth")
+            return oracle.narrowphase(hist, keys[idx], eta_arr[idx], is_ee)
+        finally:
+            oracle.use_default_rootfinder()
+
+    return run
+
+
+def exact_arbiters(hist, keys, eta, is_ee):
+    """Aberth first, 60-digit mpmath for whatever Aberth leaves in dispute."""
+    return [aberth_arbiter(hist, keys, eta, is_ee), mpmath_arbiter(hist, keys, eta, is_ee)]
 
 
 def mpmath_arbiter(hist, keys, eta, is_ee):

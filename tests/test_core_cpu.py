@@ -6,7 +6,7 @@ import pytest
 import oracle
 from collisiondetection_amd import scenes
 from tests.helpers import (compare_results, core_find_intervals, core_narrowphase, core_primitive, core_roots01,
-                           mpmath_arbiter)
+                           exact_arbiters)
 from tests.test_oracle import ETA, TESTCTCD, crossing_cases, expected_testctcd
 
 KIND = {"vf": 0, "ee": 1, "ve": 2, "vv": 3}
@@ -105,7 +105,7 @@ def test_core_narrowphase_vs_oracle(core_harness, scene):
     for keys, is_ee in ((vf, False), (ee, True)):
         rec = oracle.narrowphase(h, keys, s["eta"], is_ee)
         hit, t, sub = core_narrowphase(core_harness, s["qstart"], s["qend"], keys, is_ee, s["eta"])
-        rep = compare_results(scene, hit, t, rec, arbiter=mpmath_arbiter(h, keys, s["eta"], is_ee))
+        rep = compare_results(scene, hit, t, rec, arbiter=exact_arbiters(h, keys, s["eta"], is_ee))
         print(rep)
         assert not rep["hard"], rep
         assert not rep["t_bad_hard"], rep

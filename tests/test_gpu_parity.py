@@ -11,7 +11,7 @@ import pytest
 
 import oracle
 from collisiondetection_amd import scenes
-from tests.helpers import compare_results, core_narrowphase, core_primitive, mpmath_arbiter
+from tests.helpers import compare_results, core_narrowphase, core_primitive, exact_arbiters
 from tests.test_oracle import ETA, TESTCTCD, crossing_cases, expected_testctcd
 
 pytestmark = pytest.mark.gpu
@@ -159,7 +159,7 @@ def test_gpu_narrowphase_vs_oracle_and_core(api, core_harness, scene):
     for keys, is_ee, pre in ((vf, False, "vf"), (ee, True, "ee")):
         rec = oracle.narrowphase(h, keys, s["eta"], is_ee)
         rep = compare_results(scene + pre, r[pre + "_hit"], r[pre + "_t"], rec,
-                              arbiter=mpmath_arbiter(h, keys, s["eta"], is_ee))
+                              arbiter=exact_arbiters(h, keys, s["eta"], is_ee))
         say(rep)
         assert not rep["hard"] and not rep["t_bad_hard"], rep
         ch, ct, cs = core_narrowphase(core_harness, s["qstart"], s["qend"], keys, is_ee, s["eta"])
@@ -179,7 +179,7 @@ def test_gpu_narrowphase_per_stencil_eta(api):
     for keys, etas, is_ee, pre in ((vf, ve, False, "vf"), (ee, eev, True, "ee")):
         rec = oracle.narrowphase(h, keys, etas, is_ee)
         rep = compare_results("eta" + pre, r[pre + "_hit"], r[pre + "_t"], rec,
-                              arbiter=mpmath_arbiter(h, keys, etas, is_ee))
+                              arbiter=exact_arbiters(h, keys, etas, is_ee))
         assert not rep["hard"] and not rep["t_bad_hard"], rep
 
 
@@ -202,7 +202,7 @@ def test_gpu_narrowphase_history(api):
     for keys, is_ee, pre in ((ovf, False, "vf"), (oee, True, "ee")):
         rec = oracle.narrowphase(h3, keys, s["eta"], is_ee)
         rep = compare_results("hist" + pre, r[pre + "_hit"], r[pre + "_t"], rec,
-                              arbiter=mpmath_arbiter(h3, keys, s["eta"], is_ee))
+                              arbiter=exact_arbiters(h3, keys, s["eta"], is_ee))
         assert not rep["hard"] and not rep["t_bad_hard"], rep
         hit = r[pre + "_hit"]
         assert np.array_equal(r[pre + "_win"][hit & rec["hit"].astype(bool)],
@@ -223,7 +223,7 @@ def test_gpu_detect_equals_oracle(api):
         g = {tuple(k): t for k, t in zip(got.tolist(), toi)}
         ghit = np.array([tuple(k) in g for k in keys.tolist()])
         gt = np.array([g.get(tuple(k), 0.0) for k in keys.tolist()])
-        rep = compare_results("detect", ghit, gt, rec, arbiter=mpmath_arbiter(h, keys, s["eta"], is_ee))
+        rep = compare_results("detect", ghit, gt, rec, arbiter=exact_arbiters(h, keys, s["eta"], is_ee))
         assert not rep["hard"] and not rep["t_bad_hard"], rep
     assert len(d["vf"]) + len(d["ee"]) > 0
     # capacity handling: a too-small collision buffer is retried with the reported size
@@ -267,17 +267,19 @@ def test_gpu_cloth_full_size(api, n):
     s = scenes.cloth(n)
     h = oracle.Hist.two_entry(s["qstart"], s["qend"])
     ovf, oee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
+    say("oracle candidates", len(ovf), len(oee))
     gvf, gee = api.kdop_candidates(s["faces"], s["qstart"], s["qend"], s["eta"], sort=True)
     assert np.array_equal(ovf, gvf) and np.array_equal(oee, gee)
-    say("candidates", len(ovf), len(oee))
+    say("GPU candidates equal")
     r = api.narrowphase(s["qstart"], s["qend"], gvf, gee, s["eta"])
     rng = np.random.default_rng(1)
     for keys, pre, is_ee in ((gvf, "vf", False), (gee, "ee", True)):
         hit = r[pre + "_hit"]
         idx = np.unique(np.concatenate([rng.choice(len(keys), size=min(20000, len(keys)), replace=False),
                                         np.where(hit)[0]]))
+        say(f"cloth {pre}: comparing {len(idx)} stencils ({int(hit.sum())} GPU hits)")
         rec = oracle.narrowphase(h, keys[idx], s["eta"], is_ee)
         rep = compare_results("cloth" + pre, hit[idx], r[pre + "_t"][idx], rec,
-                              arbiter=mpmath_arbiter(h, keys[idx], s["eta"], is_ee))
+                              arbiter=exact_arbiters(h, keys[idx], s["eta"], is_ee))
         say(rep)
         assert not rep["hard"] and not rep["t_bad_hard"], rep

@@ -1,0 +1,98 @@
+"""Serving loop (ccd_server_*): each request's result is the same alone and in the mix (bit for bit:
+the colliding stencils and their TOIs), refit equals a fresh build, and admission respects the pool."""
+import numpy as np
+import pytest
+
+from collisiondetection_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _sessions():
+    out = []
+    for k, s in enumerate(scenes.batched_instances(count=6, seed=21, tri_lo=200, tri_hi=6000)):
+        out.append(s)
+    out.append(scenes.cloth(64, drop=0.03))
+    return out
+
+
+def _trace(sess, steps, seed=4):
+    rng = np.random.default_rng(seed)
+    ends = [s["qend"].copy() for s in sess]
+    reqs = []
+    for r in range(steps):
+        for k in rng.permutation(len(sess)):
+            ends[k] = ends[k] + rng.normal(0.0, 0.002, size=ends[k].shape)
+            reqs.append((int(k), ends[k].copy()))
+    return reqs
+
+
+def _run(sess, reqs, only=None, max_inflight=8, budget=2 << 30):
+    from collisiondetection_amd import api
+
+    srv = api.Server(hbm_budget_bytes=budget, max_inflight=max_inflight)
+    srv.keep_hits(True)
+    ids = [srv.open_session(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"]) for s in sess]
+    out = {}
+    for rid, (k, q) in enumerate(reqs):
+        if only is not None and k != only:
+            continue
+        srv.submit(ids[k], q, rid)
+    srv.drain()
+    for r in srv.poll(1 << 16):
+        assert r["status"] == 0
+        vk, vt, ek, et = srv.fetch_hits(r["request_id"])
+        hits = {("vf",) + tuple(x): t for x, t in zip(vk.tolist(), vt)}
+        hits.update({("ee",) + tuple(x): t for x, t in zip(ek.tolist(), et)})
+        out[r["request_id"]] = (r, hits)
+    st = srv.stats()
+    srv.close()
+    return out, st
+
+
+def test_server_alone_equals_in_mix():
+    print("case: server alone vs in mix", flush=True)
+    sess = _sessions()
+    reqs = _trace(sess, steps=3)
+    mixed, st = _run(sess, reqs)
+    assert len(mixed) == len(reqs) and st["pending"] == 0 and st["inflight"] == 0
+    for k in range(len(sess)):
+        alone, _ = _run(sess, reqs, only=k, max_inflight=1)
+        for rid, (r, hits) in alone.items():
+            rm, hm = mixed[rid]
+            assert hits == hm, (k, rid)
+            assert (r["checksum_vf"], r["checksum_ee"]) == (rm["checksum_vf"], rm["checksum_ee"])
+            assert (r["vf_candidates"], r["ee_candidates"]) == (rm["vf_candidates"], rm["ee_candidates"])
+
+
+def test_server_refit_equals_fresh_detect():
+    from collisiondetection_amd import api
+
+    print("case: server refit vs fresh build", flush=True)
+    sess = _sessions()[:3] + [_sessions()[-1]]
+    reqs = _trace(sess, steps=2, seed=9)
+    res, _ = _run(sess, reqs)
+    prev = [s["qend"].copy() for s in sess]
+    for rid, (k, q) in enumerate(reqs):
+        s = sess[k]
+        d = api.detect(s["faces"], prev[k], q, s["eta"], s["eta"])
+        r, hits = res[rid]
+        assert (r["vf_candidates"], r["ee_candidates"]) == (d["vf_candidates"], d["ee_candidates"])
+        fresh = {("vf",) + tuple(x): t for x, t in zip(d["vf"].tolist(), d["vf_toi"])}
+        fresh.update({("ee",) + tuple(x): t for x, t in zip(d["ee"].tolist(), d["ee_toi"])})
+        assert hits == fresh, rid
+        prev[k] = q
+
+
+def test_server_small_pool_admission():
+    print("case: server admission with a small pool", flush=True)
+    sess = _sessions()
+    reqs = _trace(sess, steps=2, seed=5)
+    big, st_big = _run(sess, reqs)
+    # a pool that holds about one of the largest requests at a time makes requests wait for memory
+    budget = st_big["max_request_bytes"] * 3 // 2 + 4096
+    small, st = _run(sess, reqs, budget=budget)
+    print("small pool", budget, st, flush=True)
+    assert st["completed"] == len(reqs)
+    for rid in big:
+        assert big[rid][1] == small[rid][1]
