@@ -258,49 +258,61 @@ def main():
                           ctypes.byref(cnt[1]), ctypes.byref(cnt[2]), ctypes.byref(cnt[3]))
         ctx.check(rc)
 
-    for i in range(args.warmup):
-        step()
-        log(f"warmup {i + 1}/{args.warmup}: {cnt[2].value} VF + {cnt[3].value} EE candidates, "
-            f"{cnt[0].value} + {cnt[1].value} collisions")
-    torch.cuda.synchronize(dev)
+    def run_mode(mode):
+        """Warmup, work accounting and the timed region for one root mode; returns its figures."""
+        ctx.set_root_mode(mode)
+        for i in range(args.warmup):
+            step()
+            log(f"[{N.ROOT_MODE_NAMES[mode]}] warmup {i + 1}/{args.warmup}: {cnt[2].value} VF + {cnt[3].value} EE "
+                f"candidates, {cnt[0].value} + {cnt[1].value} collisions")
+        torch.cuda.synchronize(dev)
 
-    # Work accounting for the narrowphase roofline (outside the timed region).
-    pv, pe = ctypes.c_void_p(), ctypes.c_void_p()
-    nv, ne = ctypes.c_int64(), ctypes.c_int64()
-    ctx.check(L.ccd_last_candidates(ctx.ptr, ctypes.byref(pv), ctypes.byref(nv), ctypes.byref(pe), ctypes.byref(ne)))
-    wv = (ctypes.c_int64 * N.NSTAGES)()
-    we = (ctypes.c_int64 * N.NSTAGES)()
-    ctx.check(L.ccd_narrowphase_work(ctx.ptr, qs.data_ptr(), qe.data_ptr(), V, pv, nv.value, pe, ne.value, eta, wv, we))
-    fl = (ctypes.c_int64 * N.NSTAGES)()
-    L.ccd_stage_flops(fl)
-    flops_vf = sum(wv[i] * fl[i] for i in range(N.NSTAGES))
-    flops_ee = sum(we[i] * fl[i] for i in range(N.NSTAGES))
+        # Work accounting for the narrowphase roofline (outside the timed region).
+        pv, pe = ctypes.c_void_p(), ctypes.c_void_p()
+        nv, ne = ctypes.c_int64(), ctypes.c_int64()
+        ctx.check(L.ccd_last_candidates(ctx.ptr, ctypes.byref(pv), ctypes.byref(nv), ctypes.byref(pe),
+                                        ctypes.byref(ne)))
+        wv = (ctypes.c_int64 * N.NSTAGES)()
+        we = (ctypes.c_int64 * N.NSTAGES)()
+        ctx.check(L.ccd_narrowphase_work(ctx.ptr, qs.data_ptr(), qe.data_ptr(), V, pv, nv.value, pe, ne.value, eta,
+                                         wv, we))
+        fl = (ctypes.c_int64 * N.NSTAGES)()
+        L.ccd_stage_flops(fl)
+        flops_vf = sum(wv[i] * fl[i] for i in range(N.NSTAGES))
+        flops_ee = sum(we[i] * fl[i] for i in range(N.NSTAGES))
 
-    # Timed region: K steps, HIP-event stage timers on the context's stream.
-    L.ccd_set_profiling(ctx.ptr, 1)
-    L.ccd_stage_times(ctx.ptr, None, None, 1)
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize(dev)
-    wall = time.perf_counter() - t0
-    ms = (ctypes.c_double * N.NTIMERS)()
-    calls = (ctypes.c_int64 * N.NTIMERS)()
-    L.ccd_stage_times(ctx.ptr, ms, calls, 1)
-    L.ccd_set_profiling(ctx.ptr, 0)
-    stage_ms = {N.TIMER_NAMES[i]: ms[i] / max(calls[i], 1) for i in range(N.NTIMERS)}
+        # Timed region: K steps, HIP-event stage timers on the context's stream.
+        L.ccd_set_profiling(ctx.ptr, 1)
+        L.ccd_stage_times(ctx.ptr, None, None, 1)
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize(dev)
+        wall = time.perf_counter() - t0
+        ms = (ctypes.c_double * N.NTIMERS)()
+        calls = (ctypes.c_int64 * N.NTIMERS)()
+        L.ccd_stage_times(ctx.ptr, ms, calls, 1)
+        L.ccd_set_profiling(ctx.ptr, 0)
+        stage_ms = {N.TIMER_NAMES[i]: ms[i] / max(calls[i], 1) for i in range(N.NTIMERS)}
+        log(f"[{N.ROOT_MODE_NAMES[mode]}] stage ms/step:", json.dumps({k: round(v, 4) for k, v in stage_ms.items()}))
+        return {"wall": wall, "stage_ms": stage_ms, "nv": nv.value, "ne": ne.value, "flops_vf": flops_vf,
+                "flops_ee": flops_ee, "hits": (cnt[0].value, cnt[1].value)}
+
+    alt = run_mode(N.ROOTS_ISOLATE)
+    res = run_mode(N.ROOTS_REFERENCE)   # the default mode: the reference's results bit for bit
+    wall, stage_ms, flops_vf, flops_ee = res["wall"], res["stage_ms"], res["flops_vf"], res["flops_ee"]
+    nv, ne = ctypes.c_int64(res["nv"]), ctypes.c_int64(res["ne"])
     ms_per_step = 1000.0 * wall / args.steps
     value = F / (wall / args.steps)
     nstencils = nv.value + ne.value
-    log("stage ms/step:", json.dumps({k: round(v, 4) for k, v in stage_ms.items()}))
 
     # Roofline of the dominant kernel.
-    dom = max(stage_ms, key=stage_ms.get)
+    dom = max((k for k in stage_ms if k != "pipeline"), key=stage_ms.get)
     if dom in ("narrow_vf", "narrow_ee"):
         flops = flops_vf if dom == "narrow_vf" else flops_ee
         achieved = flops / (stage_ms[dom] * 1e-3) / 1e12
-        roof = {"kernel": "k_narrow2_compact<%s>" % ("VF" if dom == "narrow_vf" else "EE"), "bound": "mfma",
+        roof = {"kernel": "k_narrow2_queue<%s>" % ("VF" if dom == "narrow_vf" else "EE"), "bound": "mfma",
                 "pipe": "FP64 VALU (no dense contraction, so no MFMA; peak is MI355X's FP64 figure)",
                 "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                 "traffic": None, "algorithmic_flops": flops}
@@ -314,6 +326,11 @@ def main():
     roof["stage_ms"] = {k: round(v, 5) for k, v in stage_ms.items()}
     roof["narrowphase_tflops"] = {
         "vf": flops_vf / max(stage_ms["narrow_vf"], 1e-9) / 1e9, "ee": flops_ee / max(stage_ms["narrow_ee"], 1e-9) / 1e9}
+    isolate = {"value": F / (alt["wall"] / args.steps), "ms_per_step": 1000.0 * alt["wall"] / args.steps,
+               "stage_ms": {k: round(v, 5) for k, v in alt["stage_ms"].items()},
+               "narrowphase_tflops": {"vf": alt["flops_vf"] / max(alt["stage_ms"]["narrow_vf"], 1e-9) / 1e9,
+                                      "ee": alt["flops_ee"] / max(alt["stage_ms"]["narrow_ee"], 1e-9) / 1e9},
+               "collisions": {"vf": alt["hits"][0], "ee": alt["hits"][1]}}
 
     out = {
         "metric": "swept tris/sec on 1 GPU, batched vs plain, exact sets; p50/p99 CCD query latency",
@@ -323,8 +340,10 @@ def main():
         "config": {"workload": f"cloth self-CCD {args.grid}x{args.grid} ({V} V, {F} tris), eta 1e-6, seed 1",
                    "global_batch": 1, "parallelism": "single GPU"},
         "pair_tests_per_s": nstencils / ((stage_ms["narrow_vf"] + stage_ms["narrow_ee"]) * 1e-3),
-        "candidates": {"vf": nv.value, "ee": ne.value}, "collisions": {"vf": cnt[0].value, "ee": cnt[1].value},
+        "candidates": {"vf": nv.value, "ee": ne.value}, "collisions": {"vf": res["hits"][0], "ee": res["hits"][1]},
+        "root_mode": "reference (the reference's Jenkins-Traub root finder restated; its results bit for bit)",
         "roofline": roof,
+        "isolating_root_mode": isolate,
     }
     if not args.no_batched:
         out["batched"] = bench_batched(max(3, args.steps // 2), log)

@@ -560,6 +560,25 @@ int ccd_set_queue_shape(ccd_ctx *ctx, const int *shape)
     return CCD_OK;
 }
 
+int ccd_set_root_mode(ccd_ctx *ctx, int mode)
+{
+    if (!ctx) return CCD_E_INVALID;
+    if (mode != CCD_ROOTS_REFERENCE && mode != CCD_ROOTS_ISOLATE)
+        return set_error(ctx, CCD_E_INVALID, "unknown root mode %d", mode);
+    if (mode != ctx->root_mode) {
+        ctx->root_mode = mode;
+        graph_drop(ctx);   // captured launches carry the old kernels
+    }
+    return CCD_OK;
+}
+
+int ccd_root_mode(ccd_ctx *ctx, int *mode)
+{
+    if (!ctx || !mode) return CCD_E_INVALID;
+    *mode = ctx->root_mode;
+    return CCD_OK;
+}
+
 int ccd_autotune(ccd_ctx *ctx, const double *qstart, const double *qend, double eta, int *shape)
 {
     if (!ctx) return CCD_E_INVALID;

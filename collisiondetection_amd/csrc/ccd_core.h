@@ -14,15 +14,22 @@
 //   * the sign of each piece between roots is decided at its midpoint with plain Horner on the
 //     normalised coefficients (CTCD.cpp:59-79), and std::max / std::min keep their (a < b) ? b : a
 //     and (b < a) ? b : a argument order.
-// Only the roots of degree 3..6 polynomials come from a different method: the reference calls a
-// Jenkins-Traub solver, this file isolates the real roots in [0,1] by recursion on derivatives
-// (critical points split [0,1] into monotone pieces; each piece with a sign change holds exactly one
-// root, found by bracketed Newton iteration).  The real parts of complex roots, which the reference
-// also uses as split points, never change the union of the sign intervals, so vertex-face and
-// vertex-edge results agree except where a root lies within rounding distance of a decision.
+// Roots of degree 3..6 polynomials come from one of two methods, chosen per call by the counter
+// type (see RefRoots):
+//   * default -- this file isolates the real roots in [0,1] by recursion on derivatives (critical
+//     points split [0,1] into monotone pieces; each piece with a sign change holds exactly one root,
+//     found by bracketed Newton iteration).  The real parts of complex roots, which the reference
+//     also uses as split points, never change the union of the sign intervals, so vertex-face and
+//     vertex-edge results agree except where a root lies within rounding distance of a decision.
+//     Times of impact are the exact roots of the double polynomials to ~1e-15.
+//   * reference roots -- the reference's own Jenkins-Traub solver restated operation for operation
+//     (ccd_rpoly.h), splitting at the real parts of all its roots as CTCD.cpp:135-176 does: the
+//     reference's outputs bit for bit, its root-finding error included.
 #pragma once
 
 #include <stdint.h>
+
+#include <type_traits>
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
@@ -103,6 +110,33 @@ constexpr int STAGE_FLOPS[S_COUNT] = {12, 126, 172, 184, 41, 130, 27, 19, 93, 68
 struct NoCount {
     CCD_FHD void operator()(int) const {}
 };
+
+// A counter type whose `ref_roots` member is true selects the reference-root mode in find_intervals.
+struct RefRoots {
+    static constexpr bool ref_roots = true;
+    CCD_FHD void operator()(int) const {}
+};
+template <class Base>
+struct WithRefRoots : Base {
+    static constexpr bool ref_roots = true;
+};
+template <class C, class = void>
+struct uses_ref_roots : std::false_type {};
+template <class C>
+struct uses_ref_roots<C, std::void_t<decltype(C::ref_roots)>> : std::integral_constant<bool, C::ref_roots> {};
+
+CCD_FHD double bits_to_double(long long v)
+{
+    double d;
+    __builtin_memcpy(&d, &v, 8);
+    return d;
+}
+CCD_FHD long long double_to_bits(double d)
+{
+    long long v;
+    __builtin_memcpy(&v, &d, 8);
+    return v;
+}
 
 // ------------------------------------------------------------------------------------------------
 // Interval lists (CTCD.h:7-26).  CAP is the most pieces a degree-D polynomial can produce.
@@ -320,6 +354,10 @@ struct RootIsolator<2> {
     }
 };
 
+}   // namespace ccd
+#include "ccd_rpoly.h"
+namespace ccd {
+
 // ------------------------------------------------------------------------------------------------
 // findIntervals (CTCD.cpp:98-177) for a degree-N polynomial, op modified in place.
 // ------------------------------------------------------------------------------------------------
@@ -355,11 +393,30 @@ CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos, C &&cnt = C())
         }
     }
 
-    double time[N > 2 ? N : 2];
+    double time[N > 2 ? N : 2] = {};
     int roots = 0;
     if (reduced > 2) {
         if (!could_have_roots(op, reduced, pos)) return;
         cnt(S_ISOLATE);
+        if constexpr (uses_ref_roots<std::decay_t<C>>::value) {
+            // CTCD.cpp:139-141 and :160-176: all roots the solver returns, sorted, real parts of
+            // complex ones included.
+            double zr[6], zi[6];
+            jt::Solver js;
+            roots = js.rpoly(op, reduced, zr, zi);
+            if (roots > 0) {
+                jt::sort_small(zr, roots);
+                if (zr[0] >= 0) check_interval<N>(0, zr[0], op, reduced, out, pos);
+                for (int i = 0; i < roots - 1; i++) {
+                    if (!((zr[i] < 0 && zr[i + 1] < 0) || (zr[i] > 1.0 && zr[i + 1] > 1.0)))
+                        check_interval<N>(zr[i], zr[i + 1], op, reduced, out, pos);
+                }
+                if (zr[roots - 1] <= 1.0) check_interval<N>(zr[roots - 1], 1.0, op, reduced, out, pos);
+            } else {
+                check_interval<N>(0.0, 1.0, op, reduced, out, pos);
+            }
+            return;
+        }
         if (reduced == 3) roots = RootIsolator<3>::roots01(op, time);
         if (N >= 4 && reduced == 4) roots = RootIsolator<(N >= 4 ? 4 : 3)>::roots01(op, time);
         if (N >= 5 && reduced == 5) roots = RootIsolator<(N >= 5 ? 5 : 3)>::roots01(op, time);

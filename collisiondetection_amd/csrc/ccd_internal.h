@@ -20,6 +20,7 @@ struct QueueShape {
 struct ccd_ctx {
     int device = 0;
     int num_cus = 256;
+    int root_mode = CCD_ROOTS_REFERENCE;
     QueueShape vf_shape, ee_shape;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;

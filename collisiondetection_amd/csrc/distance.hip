@@ -123,11 +123,11 @@ __global__ __launch_bounds__(256) void k_closest_vertex_pair(const double *__res
 // Step 3: minimum distance over the candidate stencils; the stencil count is read on the device.
 template <bool VF>
 __global__ __launch_bounds__(256) void k_stencil_min_distance(const double *__restrict__ v, const int4 *__restrict__ keys,
-                                                              const int64_t *__restrict__ n_dev,
+                                                              const int64_t *__restrict__ n_dev, int64_t n_max,
                                                               unsigned long long *best_bits)
 {
     __shared__ double red[256];
-    int64_t n = *n_dev;
+    int64_t n = *n_dev < n_max ? *n_dev : n_max;   // never past the stored keys
     double best = __builtin_inf();
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         int4 k = keys[i];
@@ -201,13 +201,13 @@ int mesh_self_distance(ccd_ctx *ctx, const double *verts, int64_t V, const int32
     if (c.nvf) {
         unsigned g = (unsigned)std::min<int64_t>(blocks_for(c.nvf, 256), 4096);
         hipLaunchKernelGGL(k_stencil_min_distance<true>, dim3(g), dim3(256), 0, st, verts, (const int4 *)c.vf,
-                           ctx->d_counts + C_VF, best + 1);
+                           ctx->d_counts + C_VF, c.nvf, best + 1);
         CCD_LAUNCH_CHECK(ctx);
     }
     if (c.nee) {
         unsigned g = (unsigned)std::min<int64_t>(blocks_for(c.nee, 256), 4096);
         hipLaunchKernelGGL(k_stencil_min_distance<false>, dim3(g), dim3(256), 0, st, verts, (const int4 *)c.ee,
-                           ctx->d_counts + C_EE, best + 1);
+                           ctx->d_counts + C_EE, c.nee, best + 1);
         CCD_LAUNCH_CHECK(ctx);
     }
     CCD_HIP_CHECK(ctx, hipMemcpyAsync(&hb, best + 1, 8, hipMemcpyDeviceToHost, st));

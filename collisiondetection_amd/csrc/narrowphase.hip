@@ -20,6 +20,7 @@ __device__ __forceinline__ V3 load3(const double *__restrict__ p, int v)
     return ccd::mk(q[0], q[1], q[2]);
 }
 
+template <class M>
 __global__ __launch_bounds__(256) void k_primitives(int kind, const double *__restrict__ q,
                                                     const double *__restrict__ eta, double eta_s, int64_t n,
                                                     uint8_t *__restrict__ hit, double *__restrict__ tout)
@@ -35,28 +36,28 @@ __global__ __launch_bounds__(256) void k_primitives(int kind, const double *__re
 #pragma unroll
         for (int k = 0; k < 8; k++) a[k] = ccd::mk(p[3 * k], p[3 * k + 1], p[3 * k + 2]);
         if (kind == CCD_PRIM_VERTEX_FACE)
-            r = ccd::vertex_face(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], e, t);
+            r = ccd::vertex_face(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], e, t, M());
         else
-            r = ccd::edge_edge(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], e, t);
+            r = ccd::edge_edge(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], e, t, M());
     } else if (kind == CCD_PRIM_VERTEX_EDGE) {
         const double *p = q + 18 * i;
         V3 a[6];
 #pragma unroll
         for (int k = 0; k < 6; k++) a[k] = ccd::mk(p[3 * k], p[3 * k + 1], p[3 * k + 2]);
-        r = ccd::vertex_edge(a[0], a[1], a[2], a[3], a[4], a[5], e, t);
+        r = ccd::vertex_edge(a[0], a[1], a[2], a[3], a[4], a[5], e, t, M());
     } else {
         const double *p = q + 12 * i;
         V3 a[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) a[k] = ccd::mk(p[3 * k], p[3 * k + 1], p[3 * k + 2]);
-        r = ccd::vertex_vertex(a[0], a[1], a[2], a[3], e, t);
+        r = ccd::vertex_vertex(a[0], a[1], a[2], a[3], e, t, M());
     }
     hit[i] = r ? 1 : 0;
     if (tout) tout[i] = r ? t : 0.0;
 }
 
 // Two-entry histories: window (stitch0(start), end), CTCDNarrowPhase.cpp:24-36 with History.cpp:127-133.
-template <bool VF>
+template <bool VF, class M>
 __device__ __forceinline__ int stencil2(const double *__restrict__ qs, const double *__restrict__ qe, int4 k,
                                         double eta, double &t)
 {
@@ -69,10 +70,10 @@ __device__ __forceinline__ int stencil2(const double *__restrict__ qs, const dou
         s[j] = ccd::mk(ccd::stitch0(a.x, b.x), ccd::stitch0(a.y, b.y), ccd::stitch0(a.z, b.z));
         e[j] = b;
     }
-    return VF ? ccd::check_vf_window(s, e, eta, t) : ccd::check_ee_window(s, e, eta, t);
+    return VF ? ccd::check_vf_window(s, e, eta, t, M()) : ccd::check_ee_window(s, e, eta, t, M());
 }
 
-template <bool VF>
+template <bool VF, class M>
 __global__ __launch_bounds__(256) void k_narrow2(const double *__restrict__ qs, const double *__restrict__ qe,
                                                  const int4 *__restrict__ keys, const double *__restrict__ eta,
                                                  double eta_s, int64_t n, uint8_t *__restrict__ hit,
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(256) void k_narrow2(const double *__restrict__ qs, 
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     double t = 0.0;
-    int r = stencil2<VF>(qs, qe, keys[i], eta ? eta[i] : eta_s, t);
+    int r = stencil2<VF, M>(qs, qe, keys[i], eta ? eta[i] : eta_s, t);
     hit[i] = r >= 0 ? 1 : 0;
     if (tout) tout[i] = r >= 0 ? t : 0.0;
     if (sub) sub[i] = r >= 0 ? (uint8_t)r : (uint8_t)255;
@@ -89,21 +90,24 @@ __global__ __launch_bounds__(256) void k_narrow2(const double *__restrict__ qs, 
 
 // Narrowphase + compaction of the colliding stencils.  The stencil count is read from the device
 // (n_dev) so the launch needs no host round trip; the grid covers n_max.
-template <bool VF>
+template <bool VF, class M>
 __global__ __launch_bounds__(256) void k_narrow2_compact(const double *__restrict__ qs, const double *__restrict__ qe,
                                                          const int4 *__restrict__ keys,
-                                                         const int64_t *__restrict__ n_dev, double eta,
-                                                         int4 *__restrict__ out_keys, double *__restrict__ out_toi,
-                                                         int64_t cap, unsigned long long *counter)
+                                                         const int64_t *__restrict__ n_dev, int64_t n_max,
+                                                         double eta, int4 *__restrict__ out_keys,
+                                                         double *__restrict__ out_toi, int64_t cap,
+                                                         unsigned long long *counter)
 {
-    int64_t n = *n_dev;
+    // The device count may exceed the key buffer (an overflowing broadphase counts every stencil but
+    // stores only n_max); never read past what was stored.
+    int64_t n = *n_dev < n_max ? *n_dev : n_max;
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     bool hit = false;
     double t = 0.0;
     int4 k = make_int4(0, 0, 0, 0);
     if (i < n) {
         k = keys[i];
-        hit = stencil2<VF>(qs, qe, k, eta, t) >= 0;
+        hit = stencil2<VF, M>(qs, qe, k, eta, t) >= 0;
     }
     // Wave-aggregated append: one atomic per wavefront.
     uint64_t mask = __ballot(hit);
@@ -127,14 +131,15 @@ __global__ __launch_bounds__(256) void k_narrow2_compact(const double *__restric
 // wavefront repeatedly takes the next `tile` stencils with one atomic on *work and tests them 64 at a
 // time.  Per-stencil cost varies by orders of magnitude (early exits vs. full root isolation), so
 // dynamic tiles keep every SIMD busy to the end; tile and block size are tuning parameters.
-template <bool VF, int BS>
+template <bool VF, int BS, class M>
 __global__ __launch_bounds__(BS) void k_narrow2_queue(const double *__restrict__ qs, const double *__restrict__ qe,
                                                       const int4 *__restrict__ keys, const int64_t *__restrict__ n_dev,
-                                                      double eta, int4 *__restrict__ out_keys,
+                                                      int64_t n_max, double eta, int4 *__restrict__ out_keys,
                                                       double *__restrict__ out_toi, int64_t cap,
                                                       unsigned long long *counter, unsigned long long *work, int tile)
 {
-    const int64_t n = *n_dev;
+    // Clamp to the stored keys: an overflowing broadphase counts more stencils than it stores.
+    const int64_t n = *n_dev < n_max ? *n_dev : n_max;
     const int lane = __lane_id();
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     while (true) {
@@ -150,7 +155,7 @@ __global__ __launch_bounds__(BS) void k_narrow2_queue(const double *__restrict__
             int4 k = make_int4(0, 0, 0, 0);
             if (i < end) {
                 k = keys[i];
-                hit = stencil2<VF>(qs, qe, k, eta, t) >= 0;
+                hit = stencil2<VF, M>(qs, qe, k, eta, t) >= 0;
             }
             uint64_t mask = __ballot(hit);
             if (mask) {
@@ -176,13 +181,13 @@ struct LaneCount {
     __device__ __forceinline__ void operator()(int s) { n[s]++; }
 };
 
-template <bool VF>
+template <bool VF, class LC>
 __global__ __launch_bounds__(256) void k_narrow2_count(const double *__restrict__ qs, const double *__restrict__ qe,
                                                        const int4 *__restrict__ keys, int64_t n, double eta,
                                                        unsigned long long *counts)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    LaneCount c;
+    LC c;
 #pragma unroll
     for (int k = 0; k < ccd::S_COUNT; k++) c.n[k] = 0;
     if (i < n) {
@@ -211,7 +216,7 @@ __global__ __launch_bounds__(256) void k_narrow2_count(const double *__restrict_
 }
 
 // Multi-entry histories: stitch on the fly (History.cpp:98-140) and test window by window.
-template <bool VF>
+template <bool VF, class M>
 __global__ __launch_bounds__(256) void k_narrow_hist(const int64_t *__restrict__ off, const double *__restrict__ ht,
                                                      const double *__restrict__ hp, const int4 *__restrict__ keys,
                                                      const double *__restrict__ eta, double eta_s, int64_t n,
@@ -261,7 +266,7 @@ __global__ __launch_bounds__(256) void k_narrow_hist(const int64_t *__restrict__
             }
         }
         if (w >= 0) {
-            r = VF ? ccd::check_vf_window(prev, cur, e, t) : ccd::check_ee_window(prev, cur, e, t);
+            r = VF ? ccd::check_vf_window(prev, cur, e, t, M()) : ccd::check_ee_window(prev, cur, e, t, M());
             if (r >= 0) break;
         }
 #pragma unroll
@@ -315,12 +320,25 @@ inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) /
 
 namespace ccdi {
 
+// Every narrowphase launch instantiates its kernel for the context's root mode: ccd::NoCount is the
+// default root isolation, ccd::RefRoots the reference's Jenkins-Traub solver (CCD_ROOTS_REFERENCE).
+#define CCD_MODE_DISPATCH(ctx, LAUNCH)                                                                              \
+    do {                                                                                                            \
+        if ((ctx)->root_mode == CCD_ROOTS_REFERENCE) {                                                              \
+            using M = ccd::RefRoots;                                                                                \
+            LAUNCH;                                                                                                 \
+        } else {                                                                                                    \
+            using M = ccd::NoCount;                                                                                 \
+            LAUNCH;                                                                                                 \
+        }                                                                                                           \
+    } while (0)
+
 int launch_primitives(ccd_ctx *ctx, int kind, const double *q, const double *eta, double eta_s, int64_t n,
                       uint8_t *hit, double *t)
 {
     if (n == 0) return CCD_OK;
-    hipLaunchKernelGGL(k_primitives, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, kind, q, eta, eta_s, n,
-                       hit, t);
+    CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL(k_primitives<M>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream,
+                                              kind, q, eta, eta_s, n, hit, t));
     CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
 }
@@ -330,11 +348,11 @@ int launch_narrow2(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, co
 {
     if (n == 0) return CCD_OK;
     if (vf)
-        hipLaunchKernelGGL(k_narrow2<true>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
-                           (const int4 *)keys, eta, eta_s, n, hit, t, sub);
+        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow2<true, M>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                                                  ctx->stream, qs, qe, (const int4 *)keys, eta, eta_s, n, hit, t, sub));
     else
-        hipLaunchKernelGGL(k_narrow2<false>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
-                           (const int4 *)keys, eta, eta_s, n, hit, t, sub);
+        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow2<false, M>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                                                  ctx->stream, qs, qe, (const int4 *)keys, eta, eta_s, n, hit, t, sub));
     CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
 }
@@ -345,13 +363,13 @@ int launch_narrow2_compact(ccd_ctx *ctx, bool vf, const double *qs, const double
 {
     if (n_max == 0) return CCD_OK;
     if (vf)
-        hipLaunchKernelGGL(k_narrow2_compact<true>, dim3(blocks_for(n_max, 256)), dim3(256), 0, ctx->stream, qs,
-                           qe, (const int4 *)keys, n_dev, eta, (int4 *)out_keys, out_toi, cap,
-                           (unsigned long long *)counter);
+        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow2_compact<true, M>), dim3(blocks_for(n_max, 256)),
+                                                  dim3(256), 0, ctx->stream, qs, qe, (const int4 *)keys, n_dev, n_max,
+                                                  eta, (int4 *)out_keys, out_toi, cap, (unsigned long long *)counter));
     else
-        hipLaunchKernelGGL(k_narrow2_compact<false>, dim3(blocks_for(n_max, 256)), dim3(256), 0, ctx->stream, qs,
-                           qe, (const int4 *)keys, n_dev, eta, (int4 *)out_keys, out_toi, cap,
-                           (unsigned long long *)counter);
+        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow2_compact<false, M>), dim3(blocks_for(n_max, 256)),
+                                                  dim3(256), 0, ctx->stream, qs, qe, (const int4 *)keys, n_dev, n_max,
+                                                  eta, (int4 *)out_keys, out_toi, cap, (unsigned long long *)counter));
     CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
 }
@@ -360,24 +378,32 @@ int launch_narrow2_count(ccd_ctx *ctx, bool vf, const double *qs, const double *
                          int64_t n, double eta, unsigned long long *counts)
 {
     if (n == 0) return CCD_OK;
-    if (vf)
-        hipLaunchKernelGGL(k_narrow2_count<true>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
-                           (const int4 *)keys, n, eta, counts);
+    const bool ref = ctx->root_mode == CCD_ROOTS_REFERENCE;
+#define CCD_COUNT_LAUNCH(V, LCT)                                                                                    \
+    hipLaunchKernelGGL((k_narrow2_count<V, LCT>), dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,     \
+                       (const int4 *)keys, n, eta, counts)
+    if (vf && ref)
+        CCD_COUNT_LAUNCH(true, ccd::WithRefRoots<LaneCount>);
+    else if (vf)
+        CCD_COUNT_LAUNCH(true, LaneCount);
+    else if (ref)
+        CCD_COUNT_LAUNCH(false, ccd::WithRefRoots<LaneCount>);
     else
-        hipLaunchKernelGGL(k_narrow2_count<false>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
-                           (const int4 *)keys, n, eta, counts);
+        CCD_COUNT_LAUNCH(false, LaneCount);
+#undef CCD_COUNT_LAUNCH
     CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
 }
 
-template <bool VF>
-static int launch_queue_bs(ccd_ctx *ctx, int bs, unsigned grid, const double *qs, const double *qe,
-                           const int32_t *keys, const int64_t *n_dev, double eta, int32_t *out_keys, double *out_toi,
-                           int64_t cap, int64_t *counter, unsigned long long *work, int tile)
+template <bool VF, class M>
+static void launch_queue_bs(ccd_ctx *ctx, int bs, unsigned grid, const double *qs, const double *qe,
+                            const int32_t *keys, const int64_t *n_dev, int64_t n_max, double eta, int32_t *out_keys,
+                            double *out_toi, int64_t cap, int64_t *counter, unsigned long long *work, int tile)
 {
 #define CCD_QUEUE_LAUNCH(B)                                                                                         \
-    hipLaunchKernelGGL((k_narrow2_queue<VF, B>), dim3(grid), dim3(B), 0, ctx->stream, qs, qe, (const int4 *)keys,  \
-                       n_dev, eta, (int4 *)out_keys, out_toi, cap, (unsigned long long *)counter, work, tile)
+    hipLaunchKernelGGL((k_narrow2_queue<VF, B, M>), dim3(grid), dim3(B), 0, ctx->stream, qs, qe,                 \
+                       (const int4 *)keys, n_dev, n_max, eta, (int4 *)out_keys, out_toi, cap,                      \
+                       (unsigned long long *)counter, work, tile)
     if (bs == 64)
         CCD_QUEUE_LAUNCH(64);
     else if (bs == 128)
@@ -385,8 +411,6 @@ static int launch_queue_bs(ccd_ctx *ctx, int bs, unsigned grid, const double *qs
     else
         CCD_QUEUE_LAUNCH(256);
 #undef CCD_QUEUE_LAUNCH
-    CCD_LAUNCH_CHECK(ctx);
-    return CCD_OK;
 }
 
 int launch_narrow2_queue(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
@@ -400,10 +424,13 @@ int launch_narrow2_queue(ccd_ctx *ctx, bool vf, const double *qs, const double *
     waves = std::max<int64_t>(1, std::min(waves, need));
     unsigned grid = (unsigned)((waves * 64 + bs - 1) / bs);
     if (vf)
-        return launch_queue_bs<true>(ctx, bs, grid, qs, qe, keys, n_dev, eta, out_keys, out_toi, cap, counter, work,
-                                     shape.tile);
-    return launch_queue_bs<false>(ctx, bs, grid, qs, qe, keys, n_dev, eta, out_keys, out_toi, cap, counter, work,
-                                  shape.tile);
+        CCD_MODE_DISPATCH(ctx, (launch_queue_bs<true, M>(ctx, bs, grid, qs, qe, keys, n_dev, n_max, eta, out_keys,
+                                                         out_toi, cap, counter, work, shape.tile)));
+    else
+        CCD_MODE_DISPATCH(ctx, (launch_queue_bs<false, M>(ctx, bs, grid, qs, qe, keys, n_dev, n_max, eta, out_keys,
+                                                          out_toi, cap, counter, work, shape.tile)));
+    CCD_LAUNCH_CHECK(ctx);
+    return CCD_OK;
 }
 
 int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *ht, const double *hp,
@@ -412,11 +439,13 @@ int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *
 {
     if (n == 0) return CCD_OK;
     if (vf)
-        hipLaunchKernelGGL(k_narrow_hist<true>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, off, ht, hp,
-                           (const int4 *)keys, eta, eta_s, n, hit, t, sub, win);
+        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow_hist<true, M>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                                                  ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, hit, t,
+                                                  sub, win));
     else
-        hipLaunchKernelGGL(k_narrow_hist<false>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, off, ht, hp,
-                           (const int4 *)keys, eta, eta_s, n, hit, t, sub, win);
+        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow_hist<false, M>), dim3(blocks_for(n, 256)), dim3(256), 0,
+                                                  ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, hit, t,
+                                                  sub, win));
     CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
 }

@@ -30,7 +30,7 @@ def core_harness():
 
     so = os.path.join(ROOT, "tests", "_build", "core_harness.so")
     src = os.path.join(ROOT, "tests", "native", "core_harness.cpp")
-    hdr = os.path.join(ROOT, "collisiondetection_amd", "csrc", "ccd_core.h")
-    if not os.path.exists(so) or os.path.getmtime(so) < max(os.path.getmtime(src), os.path.getmtime(hdr)):
+    hdrs = [os.path.join(ROOT, "collisiondetection_amd", "csrc", h) for h in ("ccd_core.h", "ccd_rpoly.h")]
+    if not os.path.exists(so) or os.path.getmtime(so) < max(os.path.getmtime(x) for x in [src] + hdrs):
         subprocess.run(["bash", os.path.join(ROOT, "tests", "native", "build.sh")], check=True)
     return ctypes.CDLL(so)

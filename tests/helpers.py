@@ -94,6 +94,24 @@ def compare_results(name, hit_a, t_a, rec, tol_t=1e-9, margin_tol=1e-12, arbiter
             "t_bad_hard": t_bad_hard.tolist()[:20], "ref_root_error": ref_err[:20], "n_ref_root_error": len(ref_err)}
 
 
+def compare_reference(name, hit_a, t_a, rec, tol_t=1e-9, margin_tol=1e-12):
+    """Reference-root mode against the oracle running the reference's own root finder: flags equal
+    (a disagreement is tolerated only with an oracle margin below margin_tol, and listed), times of
+    impact within tol_t, and the count of times of impact that are not bit-identical reported."""
+    hit_o = rec["hit"].astype(bool)
+    mism = np.where(hit_a != hit_o)[0]
+    border = mism[rec["margin"][mism] < margin_tol]
+    hard = mism[rec["margin"][mism] >= margin_tol]
+    both = np.where(hit_a & hit_o)[0]
+    ta, to = t_a[both], rec["t"][both]
+    not_bits = both[ta.view(np.int64) != to.view(np.int64)]
+    t_bad = both[np.abs(ta - to) > tol_t]
+    return {"name": name, "n": len(hit_a), "hits": int(hit_o.sum()), "borderline": border.tolist()[:20],
+            "hard": hard.tolist()[:20], "t_bad": t_bad.tolist()[:20], "n_not_bit_identical": len(not_bits),
+            "not_bit_identical": [(int(i), float(t_a[i]), float(rec["t"][i])) for i in not_bits[:10]],
+            "max_tdiff": float(np.abs(ta - to).max()) if len(both) else 0.0}
+
+
 def aberth_arbiter(hist, keys, eta, is_ee):
     """idx -> oracle records for keys[idx] computed with the oracle's long-double Aberth roots (accurate
     to ~1e-16 on these polynomials, and fast): a first arbiter, backed by mpmath_arbiter."""

@@ -10,18 +10,25 @@ using ccd::V3;
 
 static V3 ld(const double *p) { return ccd::mk(p[0], p[1], p[2]); }
 
+// 0 = the core's own root isolation, 1 = the reference-root mode (ccd_rpoly.h).
+static int g_ref_roots = 0;
+extern "C" void core_set_ref_roots(int on) { g_ref_roots = on; }
+
+template <class C>
+static bool primitive(int kind, const double *q, double eta, double &tt, C c)
+{
+    if (kind == 0)
+        return ccd::vertex_face(ld(q), ld(q + 3), ld(q + 6), ld(q + 9), ld(q + 12), ld(q + 15), ld(q + 18), ld(q + 21), eta, tt, c);
+    if (kind == 1)
+        return ccd::edge_edge(ld(q), ld(q + 3), ld(q + 6), ld(q + 9), ld(q + 12), ld(q + 15), ld(q + 18), ld(q + 21), eta, tt, c);
+    if (kind == 2) return ccd::vertex_edge(ld(q), ld(q + 3), ld(q + 6), ld(q + 9), ld(q + 12), ld(q + 15), eta, tt, c);
+    return ccd::vertex_vertex(ld(q), ld(q + 3), ld(q + 6), ld(q + 9), eta, tt, c);
+}
+
 extern "C" int core_primitive(int kind, const double *q, double eta, double *t)
 {
     double tt = 0.0;
-    bool r = false;
-    if (kind == 0)
-        r = ccd::vertex_face(ld(q), ld(q + 3), ld(q + 6), ld(q + 9), ld(q + 12), ld(q + 15), ld(q + 18), ld(q + 21), eta, tt);
-    else if (kind == 1)
-        r = ccd::edge_edge(ld(q), ld(q + 3), ld(q + 6), ld(q + 9), ld(q + 12), ld(q + 15), ld(q + 18), ld(q + 21), eta, tt);
-    else if (kind == 2)
-        r = ccd::vertex_edge(ld(q), ld(q + 3), ld(q + 6), ld(q + 9), ld(q + 12), ld(q + 15), eta, tt);
-    else
-        r = ccd::vertex_vertex(ld(q), ld(q + 3), ld(q + 6), ld(q + 9), eta, tt);
+    bool r = g_ref_roots ? primitive(kind, q, eta, tt, ccd::RefRoots()) : primitive(kind, q, eta, tt, ccd::NoCount());
     *t = r ? tt : 0.0;
     return r ? 1 : 0;
 }
@@ -38,7 +45,11 @@ extern "C" void core_narrowphase(const double *qs, const double *qe, const int32
             s[j] = ccd::mk(ccd::stitch0(a[0], b[0]), ccd::stitch0(a[1], b[1]), ccd::stitch0(a[2], b[2]));
         }
         double tt = 0.0;
-        int r = is_ee ? ccd::check_ee_window(s, e, eta, tt) : ccd::check_vf_window(s, e, eta, tt);
+        int r;
+        if (g_ref_roots)
+            r = is_ee ? ccd::check_ee_window(s, e, eta, tt, ccd::RefRoots()) : ccd::check_vf_window(s, e, eta, tt, ccd::RefRoots());
+        else
+            r = is_ee ? ccd::check_ee_window(s, e, eta, tt) : ccd::check_vf_window(s, e, eta, tt);
         hit[i] = r >= 0;
         t[i] = r >= 0 ? tt : 0.0;
         sub[i] = r;
@@ -52,11 +63,19 @@ extern "C" int core_find_intervals(const double *op, int n, int pos, double *out
     for (int i = 0; i <= n; i++) c[i] = op[i];
     ccd::Ivls<8> L;
     L.n = 0;
-    if (n == 2) ccd::find_intervals<2>(c, L, pos != 0);
-    else if (n == 3) ccd::find_intervals<3>(c, L, pos != 0);
-    else if (n == 4) ccd::find_intervals<4>(c, L, pos != 0);
-    else if (n == 6) ccd::find_intervals<6>(c, L, pos != 0);
-    else return -1;
+    if (g_ref_roots) {
+        if (n == 2) ccd::find_intervals<2>(c, L, pos != 0, ccd::RefRoots());
+        else if (n == 3) ccd::find_intervals<3>(c, L, pos != 0, ccd::RefRoots());
+        else if (n == 4) ccd::find_intervals<4>(c, L, pos != 0, ccd::RefRoots());
+        else if (n == 6) ccd::find_intervals<6>(c, L, pos != 0, ccd::RefRoots());
+        else return -1;
+    } else {
+        if (n == 2) ccd::find_intervals<2>(c, L, pos != 0);
+        else if (n == 3) ccd::find_intervals<3>(c, L, pos != 0);
+        else if (n == 4) ccd::find_intervals<4>(c, L, pos != 0);
+        else if (n == 6) ccd::find_intervals<6>(c, L, pos != 0);
+        else return -1;
+    }
     for (int i = 0; i < L.n; i++) {
         out[2 * i] = L.l[i];
         out[2 * i + 1] = L.u[i];
@@ -89,3 +108,13 @@ extern "C" int core_distance(int kind, const double *q, double eta, double *vec,
     vec[2] = r.z;
     return 1;
 }
+
+// The reference-root mode's solver on its own: op[0..degree] descending -> count, zeror, zeroi.
+extern "C" int core_rpoly(const double *op, int degree, double *zr, double *zi)
+{
+    ccd::jt::Solver js;
+    return js.rpoly(op, degree, zr, zi);
+}
+
+extern "C" double core_log_cr(double x) { return ccd::jt::log_cr(x); }
+extern "C" double core_exp_cr(double x) { return ccd::jt::exp_cr(x); }

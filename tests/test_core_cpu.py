@@ -1,11 +1,13 @@
 """The product's FP64 core (collisiondetection_amd/csrc/ccd_core.h), compiled for the host, against the
 oracle: the algorithm itself, before any device arithmetic is involved."""
+import ctypes
+
 import numpy as np
 import pytest
 
 import oracle
 from collisiondetection_amd import scenes
-from tests.helpers import (compare_results, core_find_intervals, core_narrowphase, core_primitive, core_roots01,
+from tests.helpers import (compare_results, core_find_intervals, core_narrowphase, core_primitive, core_roots01, dp,
                            exact_arbiters)
 from tests.test_oracle import ETA, TESTCTCD, crossing_cases, expected_testctcd
 
@@ -111,3 +113,100 @@ def test_core_narrowphase_vs_oracle(core_harness, scene):
         assert not rep["t_bad_hard"], rep
         both = hit & rec["hit"].astype(bool)
         assert np.array_equal(sub[both], rec["subtest"][both])
+
+
+# ------------------------------------------------------------------------------------------------
+# Reference-root mode (ccd_rpoly.h): the reference's Jenkins-Traub solver restated.
+# ------------------------------------------------------------------------------------------------
+def _core_fn(H, name, restype=ctypes.c_double):
+    f = getattr(H, name)
+    f.restype = restype
+    f.argtypes = [ctypes.c_double]
+    return f
+
+
+def test_core_log_exp_correctly_rounded(core_harness):
+    """The restated solver's log/exp are correctly rounded (checked against 200-bit mpmath), so host and
+    device agree bit for bit; the C library's log/exp agree with them except where it misrounds."""
+    import math
+
+    import mpmath
+
+    mpmath.mp.prec = 200
+    lg, ex = _core_fn(core_harness, "core_log_cr"), _core_fn(core_harness, "core_exp_cr")
+    rng = np.random.default_rng(11)
+    xs = np.concatenate([np.exp(rng.uniform(-700, 700, 1500)), rng.uniform(0.5, 2.0, 1500),
+                         2.0 ** rng.integers(-1070, 1020, 100).astype(float), [4.9e-324, 1e-310, 1.0, 2.0]])
+    for x in xs:
+        assert lg(x) == float(mpmath.log(mpmath.mpf(x))), x
+    ys = np.concatenate([rng.uniform(-700, 700, 1500), rng.uniform(-1, 1, 1500), [0.0, 1.0, -1.0, 700.0]])
+    for y in ys:
+        assert ex(y) == float(mpmath.exp(mpmath.mpf(y))), y
+    # glibc agreement rate (informational): misrounded cases are rare
+    agree = np.mean([lg(x) == math.log(x) for x in xs])
+    assert agree > 0.99
+
+
+def _core_rpoly(H, op):
+    op = np.ascontiguousarray(op, dtype=np.float64)
+    zr, zi = np.zeros(8), np.zeros(8)
+    H.core_rpoly.restype = ctypes.c_int
+    k = H.core_rpoly(op.ctypes.data_as(dp), len(op) - 1, zr.ctypes.data_as(dp), zi.ctypes.data_as(dp))
+    return zr[:max(k, 0)], zi[:max(k, 0)], k
+
+
+@pytest.mark.skipif(not oracle.ref_rpoly_available(), reason="the reference's root finder (oracle/_ref) is not built")
+def test_core_rpoly_restatement_matches_reference(core_harness):
+    """Random, clustered, wide-range and sparse polynomials of degree 3..6: the restated solver returns
+    the reference's roots bit for bit (the rare exception is a C-library log/exp misrounding)."""
+    rng = np.random.default_rng(5)
+    tot = bad = 0
+    for trial in range(8000):
+        n = int(rng.integers(3, 7))
+        kind = trial % 4
+        if kind == 0:
+            op = rng.normal(size=n + 1)
+        elif kind == 1:
+            r = rng.uniform(-0.2, 1.2, size=n)
+            r[1] = r[0] + rng.uniform(-1e-6, 1e-6)
+            op = np.poly(r) * rng.uniform(0.1, 10)
+        elif kind == 2:
+            op = rng.normal(size=n + 1) * 10.0 ** rng.uniform(-30, 5, size=n + 1)
+        else:
+            op = rng.normal(size=n + 1)
+            op[rng.integers(1, n + 1)] = 0.0
+        op = op / np.abs(op).max()
+        if op[0] == 0:
+            continue
+        a, b = oracle.ref_rpoly(op), _core_rpoly(core_harness, op)
+        tot += 1
+        if a[2] != b[2] or not (np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])):
+            bad += 1
+    print("rpoly restatement: mismatches", bad, "of", tot)
+    assert bad <= tot // 2000
+
+
+@pytest.mark.skipif(not oracle.ref_rpoly_available(), reason="the reference's root finder (oracle/_ref) is not built")
+@pytest.mark.parametrize("scene", ["converging", "cloth64"])
+def test_core_reference_mode_vs_oracle(core_harness, scene):
+    """Reference-root mode of the host-built core against the oracle running the reference's solver."""
+    from tests.helpers import compare_reference
+
+    s = {"converging": scenes.converging_spheres, "cloth64": lambda: scenes.cloth(64, drop=0.05, noise=0.01)}[scene]()
+    h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+    vf, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
+    oracle.use_rootfinder("rpoly")
+    core_harness.core_set_ref_roots(1)
+    try:
+        for keys, is_ee in ((vf, False), (ee, True)):
+            rec = oracle.narrowphase(h, keys, s["eta"], is_ee)
+            hit, t, sub = core_narrowphase(core_harness, s["qstart"], s["qend"], keys, is_ee, s["eta"])
+            rep = compare_reference(scene, hit, t, rec)
+            print(rep)
+            assert not rep["hard"] and not rep["t_bad"], rep
+            assert rep["n_not_bit_identical"] <= max(1, rep["hits"] // 1000), rep
+            both = hit & rec["hit"].astype(bool)
+            assert np.array_equal(sub[both], rec["subtest"][both])
+    finally:
+        core_harness.core_set_ref_roots(0)
+        oracle.use_default_rootfinder()

@@ -1,8 +1,13 @@
 """GPU parity: libccd_amd.so (HIP, gfx950) against the CPU oracle and the host-built core.
 
-Bars: candidate sets bit-exact; collision flags equal except stencils whose oracle decision margin is
-below 1e-12 (listed, not hidden); times of impact within 1e-9 absolute; the GPU core bit-identical to
-the same core compiled for the host (device FP64 arithmetic: no contraction, IEEE div/sqrt).
+Bars: candidate sets bit-exact; the GPU core bit-identical to the same core compiled for the host
+(device FP64 arithmetic: no contraction, IEEE div/sqrt), in both root modes.  Against the oracle:
+  * reference-root mode (the default): the oracle runs the reference's own root finder; flags equal
+    (a disagreement only with an oracle margin below 1e-12, listed), times of impact within 1e-9, and
+    the count of times of impact that are not bit-identical reported;
+  * isolating mode: flags equal except margin < 1e-12 (listed); times of impact within 1e-9 of the
+    reference or, where they are not, within 1e-9 of the exact-root answer that the reference's root
+    finder misses (listed as ref_root_error).
 """
 import sys
 
@@ -11,7 +16,7 @@ import pytest
 
 import oracle
 from collisiondetection_amd import scenes
-from tests.helpers import compare_results, core_narrowphase, core_primitive, exact_arbiters
+from tests.helpers import compare_reference, compare_results, core_narrowphase, core_primitive, exact_arbiters
 from tests.test_oracle import ETA, TESTCTCD, crossing_cases, expected_testctcd
 
 pytestmark = pytest.mark.gpu
@@ -30,6 +35,36 @@ def api():
 def say(*a):
     print(*a, flush=True)
     sys.stdout.flush()
+
+
+@pytest.fixture(params=["reference", "isolate"])
+def mode(request, api, core_harness):
+    """Runs a test in each root mode, on the GPU's default context and the host-built core alike."""
+    from collisiondetection_amd import _native as N
+
+    if request.param == "reference" and not oracle.ref_rpoly_available():
+        pytest.skip("the reference's root finder (oracle/_ref) is not built")
+    m = N.ROOTS_REFERENCE if request.param == "reference" else N.ROOTS_ISOLATE
+    ctx = api.default_context()
+    old = ctx.root_mode
+    ctx.set_root_mode(m)
+    core_harness.core_set_ref_roots(1 if request.param == "reference" else 0)
+    oracle.use_default_rootfinder()
+    yield request.param
+    ctx.set_root_mode(old)
+    core_harness.core_set_ref_roots(0)
+
+
+def check_vs_oracle(mode, name, hit, t, rec, hist, keys, eta, is_ee):
+    if mode == "reference":
+        rep = compare_reference(name, hit, t, rec)
+        say(rep)
+        assert not rep["hard"] and not rep["t_bad"], rep
+    else:
+        rep = compare_results(name, hit, t, rec, arbiter=exact_arbiters(hist, keys, eta, is_ee))
+        say(rep)
+        assert not rep["hard"] and not rep["t_bad_hard"], rep
+    return rep
 
 
 KIND = {"vf": 0, "ee": 1, "ve": 2, "vv": 3}
@@ -63,8 +98,8 @@ def _random_primitives(kind, n, rng):
 
 
 @pytest.mark.parametrize("kind", ["vf", "ee", "ve", "vv"])
-def test_gpu_primitives_bit_identical_to_host_core(api, core_harness, kind):
-    say("case: primitives bit-identical", kind)
+def test_gpu_primitives_bit_identical_to_host_core(api, core_harness, mode, kind):
+    say("case: primitives bit-identical", kind, mode)
     rng = np.random.default_rng(100 + KIND[kind])
     q = _random_primitives(kind, 4000, rng)
     eta = 0.05
@@ -80,17 +115,24 @@ def test_gpu_primitives_bit_identical_to_host_core(api, core_harness, kind):
 
 
 @pytest.mark.parametrize("kind", ["vf", "ee", "ve", "vv"])
-def test_gpu_primitives_vs_oracle(api, kind):
-    say("case: primitives vs oracle", kind)
+def test_gpu_primitives_vs_oracle(api, mode, kind):
+    say("case: primitives vs oracle", kind, mode)
     rng = np.random.default_rng(200 + KIND[kind])
     q = _random_primitives(kind, 3000, rng)
     eta = 0.05
     hit, t = api.primitives(KIND[kind], q, eta)
     fn = {"vf": oracle.vertex_face_ctcd, "ee": oracle.edge_edge_ctcd, "ve": oracle.vertex_edge_ctcd,
           "vv": oracle.vertex_vertex_ctcd}[kind]
+    recs = [fn(q[i], eta) for i in range(len(q))]
+    if mode == "reference":
+        rec = {"hit": np.array([r[0] for r in recs], dtype=np.int32), "t": np.array([r[1] for r in recs]),
+               "margin": np.array([r[2] for r in recs])}
+        rep = compare_reference(kind, hit.astype(bool), t, rec)
+        say(rep)
+        assert not rep["hard"] and not rep["t_bad"], rep
+        return
     bad = []
-    for i in range(len(q)):
-        r, tt, m = fn(q[i], eta)
+    for i, (r, tt, m) in enumerate(recs):
         if (bool(hit[i]) != r and m >= 1e-12) or (r and hit[i] and abs(t[i] - tt) > 1e-9 and m >= 1e-12):
             bad.append(i)
     # Arbitrate the disagreements against exact (60-digit) roots of the same polynomials.
@@ -150,18 +192,15 @@ def test_gpu_candidates_small_meshes(api):
 
 
 @pytest.mark.parametrize("scene", list(SCENES))
-def test_gpu_narrowphase_vs_oracle_and_core(api, core_harness, scene):
-    say("case: narrowphase", scene)
+def test_gpu_narrowphase_vs_oracle_and_core(api, core_harness, mode, scene):
+    say("case: narrowphase", scene, mode)
     s = SCENES[scene]()
     h = oracle.Hist.two_entry(s["qstart"], s["qend"])
     vf, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
     r = api.narrowphase(s["qstart"], s["qend"], vf, ee, s["eta"])
     for keys, is_ee, pre in ((vf, False, "vf"), (ee, True, "ee")):
         rec = oracle.narrowphase(h, keys, s["eta"], is_ee)
-        rep = compare_results(scene + pre, r[pre + "_hit"], r[pre + "_t"], rec,
-                              arbiter=exact_arbiters(h, keys, s["eta"], is_ee))
-        say(rep)
-        assert not rep["hard"] and not rep["t_bad_hard"], rep
+        check_vs_oracle(mode, scene + pre, r[pre + "_hit"], r[pre + "_t"], rec, h, keys, s["eta"], is_ee)
         ch, ct, cs = core_narrowphase(core_harness, s["qstart"], s["qend"], keys, is_ee, s["eta"])
         assert np.array_equal(ch, r[pre + "_hit"])
         assert np.array_equal(ct.view(np.int64), r[pre + "_t"].view(np.int64))
@@ -261,9 +300,10 @@ def test_gpu_work_counts_match_oracle(api):
 
 
 @pytest.mark.parametrize("n", [256])
-def test_gpu_cloth_full_size(api, n):
-    """The 256^2 cloth: candidate counts match the oracle; flags on a sample and on every GPU hit."""
-    say("case: cloth", n)
+def test_gpu_cloth_full_size(api, mode, n):
+    """The 256^2 cloth: candidate sets equal the oracle's; flags and times of impact on a sample and on
+    every GPU hit."""
+    say("case: cloth", n, mode)
     s = scenes.cloth(n)
     h = oracle.Hist.two_entry(s["qstart"], s["qend"])
     ovf, oee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
@@ -279,7 +319,4 @@ def test_gpu_cloth_full_size(api, n):
                                         np.where(hit)[0]]))
         say(f"cloth {pre}: comparing {len(idx)} stencils ({int(hit.sum())} GPU hits)")
         rec = oracle.narrowphase(h, keys[idx], s["eta"], is_ee)
-        rep = compare_results("cloth" + pre, hit[idx], r[pre + "_t"][idx], rec,
-                              arbiter=exact_arbiters(h, keys[idx], s["eta"], is_ee))
-        say(rep)
-        assert not rep["hard"] and not rep["t_bad_hard"], rep
+        check_vs_oracle(mode, "cloth" + pre, hit[idx], r[pre + "_t"][idx], rec, h, keys[idx], s["eta"], is_ee)
