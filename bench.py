@@ -215,9 +215,13 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=256, help="cloth grid size (256 = the 130,050-triangle cloth)")
+    ap.add_argument("--workload", choices=["cloth", "sheet"], default="cloth",
+                    help="cloth: the draped cloth (default); sheet: the crumpled sheet (--grid 1024 = 2,093,058 tris)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 256 MiB device copy 5 times (scale reference for HBM byte counters)")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     args = ap.parse_args()
@@ -235,27 +239,36 @@ def main():
     from collisiondetection_amd import api, scenes
 
     dev = torch.device("cuda", 0)
-    scene = scenes.cloth(args.grid)
+    scene = scenes.cloth(args.grid) if args.workload == "cloth" else scenes.crumpled_sheet(args.grid)
     F, V = len(scene["faces"]), len(scene["qstart"])
-    log(f"workload: cloth {args.grid}^2, {V} vertices, {F} triangles, eta {scene['eta']}")
+    log(f"workload: {args.workload} {args.grid}^2, {V} vertices, {F} triangles, eta {scene['eta']}")
     ctx = api.Context(0)
     ctx.bind_stream()
     faces = api.to_device(scene["faces"], torch.int32, ctx).reshape(-1)
     qs = api.to_device(scene["qstart"], torch.float64, ctx).reshape(-1)
     qe = api.to_device(scene["qend"], torch.float64, ctx).reshape(-1)
     eta = scene["eta"]
-    cap = 4 * F
-    vk = torch.empty(4 * cap, dtype=torch.int32, device=dev)
-    vt = torch.empty(cap, dtype=torch.float64, device=dev)
-    ek = torch.empty(4 * cap, dtype=torch.int32, device=dev)
-    et = torch.empty(cap, dtype=torch.float64, device=dev)
+    bufs = {}
+
+    def alloc(cap):
+        bufs.update(cap=cap, vk=torch.empty(4 * cap, dtype=torch.int32, device=dev),
+                    vt=torch.empty(cap, dtype=torch.float64, device=dev),
+                    ek=torch.empty(4 * cap, dtype=torch.int32, device=dev),
+                    et=torch.empty(cap, dtype=torch.float64, device=dev))
+
+    alloc(4 * F)
     cnt = [ctypes.c_int64(0) for _ in range(4)]
     L = N.lib()
 
     def step():
+        b = bufs
         rc = L.ccd_detect(ctx.ptr, faces.data_ptr(), F, qs.data_ptr(), qe.data_ptr(), V, eta, eta, None,
-                          vk.data_ptr(), vt.data_ptr(), cap, ctypes.byref(cnt[0]), ek.data_ptr(), et.data_ptr(), cap,
-                          ctypes.byref(cnt[1]), ctypes.byref(cnt[2]), ctypes.byref(cnt[3]))
+                          b["vk"].data_ptr(), b["vt"].data_ptr(), b["cap"], ctypes.byref(cnt[0]), b["ek"].data_ptr(),
+                          b["et"].data_ptr(), b["cap"], ctypes.byref(cnt[1]), ctypes.byref(cnt[2]),
+                          ctypes.byref(cnt[3]))
+        if rc == N.CCD_E_CAPACITY:   # collision buffers too small (warmup only): grow and repeat
+            alloc(max(cnt[0].value, cnt[1].value) * 5 // 4 + 1024)
+            return step()
         ctx.check(rc)
 
     def run_mode(mode):
@@ -337,7 +350,9 @@ def main():
         "value": value, "unit": "swept tris/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"cloth self-CCD {args.grid}x{args.grid} ({V} V, {F} tris), eta 1e-6, seed 1",
+        "config": {"workload": (f"cloth self-CCD {args.grid}x{args.grid} ({V} V, {F} tris), eta 1e-6, seed 1"
+                                if args.workload == "cloth" else
+                                f"crumpled sheet self-CCD {args.grid}x{args.grid} ({V} V, {F} tris), eta 1e-7, seed 3"),
                    "global_batch": 1, "parallelism": "single GPU"},
         "pair_tests_per_s": nstencils / ((stage_ms["narrow_vf"] + stage_ms["narrow_ee"]) * 1e-3),
         "candidates": {"vf": nv.value, "ee": ne.value}, "collisions": {"vf": res["hits"][0], "ee": res["hits"][1]},
@@ -345,6 +360,13 @@ def main():
         "roofline": roof,
         "isolating_root_mode": isolate,
     }
+    if args.copy_probe:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize(dev)
+        out["copy_probe_bytes"] = {"read": 256 << 20, "written": 256 << 20, "dispatches": 5}
     if not args.no_batched:
         out["batched"] = bench_batched(max(3, args.steps // 2), log)
     if not args.no_served:

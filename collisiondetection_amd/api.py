@@ -226,10 +226,11 @@ def _fixed_mask(fixed, V, ctx):
 
 
 def kdop_candidates(faces, qstart, qend, outer_eta, fixed=None, sort=True, ctx: Context | None = None,
-                    hist=None):
+                    hist=None, as_numpy=True):
     """KDOPBroadPhase candidate sets -> (vf, ee) int32 arrays of shape (n, 4).
 
-    hist=(off, t, pos) selects the multi-entry form (qstart/qend are then ignored)."""
+    hist=(off, t, pos) selects the multi-entry form (qstart/qend are then ignored).  as_numpy=False
+    returns device tensors (for sets too large to copy to the host)."""
     torch = _torch()
     ctx = ctx or default_context()
     ctx.bind_stream()
@@ -265,7 +266,10 @@ def kdop_candidates(faces, qstart, qend, outer_eta, fixed=None, sort=True, ctx: 
         ctx.check(rc)
         break
     torch.cuda.synchronize(ctx.torch_device)
-    return (vf[: 4 * nv.value].reshape(-1, 4).cpu().numpy(), ee[: 4 * ne.value].reshape(-1, 4).cpu().numpy())
+    vf, ee = vf[: 4 * nv.value].reshape(-1, 4), ee[: 4 * ne.value].reshape(-1, 4)
+    if not as_numpy:
+        return vf, ee
+    return vf.cpu().numpy(), ee.cpu().numpy()
 
 
 def detect(faces, qstart, qend, outer_eta, eta, fixed=None, ctx: Context | None = None, cap=None):

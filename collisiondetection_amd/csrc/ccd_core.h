@@ -147,6 +147,37 @@ struct Ivls {
     int n;
 };
 
+// Static-index access.  On the GPU an array indexed by a runtime value lives in scratch memory (per-lane
+// private memory, cached but far slower than registers).  Every list here is small, so reads and writes
+// at a runtime index go through these unrolled selects and the arrays stay in registers.
+template <int N>
+CCD_FHD double sel(const double (&a)[N], int i)
+{
+    double v = a[0];
+#pragma unroll
+    for (int k = 1; k < N; k++)
+        if (k == i) v = a[k];
+    return v;
+}
+template <int N>
+CCD_FHD void put(double (&a)[N], int i, double v)
+{
+#pragma unroll
+    for (int k = 0; k < N; k++)
+        if (k == i) a[k] = v;
+}
+
+// Append without the TimeInterval constructor (a copy of an existing interval).
+template <int CAP>
+CCD_FHD void append_ti(Ivls<CAP> &L, double tl, double tu)
+{
+    if (L.n < CAP) {
+        put(L.l, L.n, tl);
+        put(L.u, L.n, tu);
+        L.n++;
+    }
+}
+
 // TimeInterval(tl, tu): order, then clamp (CTCD.h:9-14).
 template <int CAP>
 CCD_FHD void push_ti(Ivls<CAP> &L, double tl, double tu)
@@ -158,11 +189,7 @@ CCD_FHD void push_ti(Ivls<CAP> &L, double tl, double tu)
     }
     tl = smax(tl, 0.0);
     tu = smin(tu, 1.0);
-    if (L.n < CAP) {
-        L.l[L.n] = tl;
-        L.u[L.n] = tu;
-        L.n++;
-    }
+    append_ti(L, tl, tu);
 }
 
 // Horner in the reference's order (CTCD.cpp:68-73), no fused multiply-add.
@@ -193,17 +220,23 @@ CCD_FHD void check_interval(double t1, double t2, const double *op, int degree, 
     if ((pos && f >= 0) || (!pos && f <= 0)) push_ti(out, t1, t2);
 }
 
-// couldHaveRoots (CTCD.cpp:81-94)
+// couldHaveRoots (CTCD.cpp:81-94) on op[0..degree], degree <= D.
+template <int D>
 CCD_FHD bool could_have_roots(const double *op, int degree, bool pos)
 {
     double result = 0;
     if ((pos && op[0] > 0) || (!pos && op[0] < 0)) result = op[0];
-    for (int i = 1; i < degree; i++) {
-        result *= 1.0;
-        if ((pos && op[i] > 0) || (!pos && op[i] < 0)) result += op[i];
+    double last = op[0];
+#pragma unroll
+    for (int i = 1; i <= D; i++) {
+        if (i < degree) {
+            result *= 1.0;
+            if ((pos && op[i] > 0) || (!pos && op[i] < 0)) result += op[i];
+        }
+        if (i == degree) last = op[i];
     }
     result *= 1.0;
-    result += op[degree];
+    result += last;
     return !((pos && result < 0) || (!pos && result > 0));
 }
 
@@ -284,17 +317,21 @@ CCD_HD double bracketed_root(const double *c, double a, double b, double fa, dou
 
 template <int D>
 struct RootIsolator {
-    CCD_HD static int roots01(const double *c, double *r)
+    CCD_HD static int roots01(const double *c, double (&r)[D])
     {
         double d[D];
 #pragma unroll
         for (int i = 0; i < D; i++) d[i] = c[i] * (double)(D - i);
-        double cr[D - 1 > 0 ? D - 1 : 1];
+        double cr[D - 1];
         int nc = RootIsolator<D - 1>::roots01(d, cr);
         int nr = 0;
         double a = 0.0;
         double fa = peval<D>(c, 0.0);
-        if (fa == 0.0) r[nr++] = 0.0;
+        double last = 0.0;   // r[nr - 1]
+        if (fa == 0.0) {
+            put(r, nr++, 0.0);
+            last = 0.0;
+        }
 #pragma unroll
         for (int k = 0; k < D; k++) {
             if (k <= nc) {
@@ -302,9 +339,13 @@ struct RootIsolator {
                 if (b > a) {
                     double fb = peval<D>(c, b);
                     if (fb == 0.0) {
-                        if (nr == 0 || r[nr - 1] != b) r[nr++] = b;
+                        if (nr == 0 || last != b) {
+                            put(r, nr++, b);
+                            last = b;
+                        }
                     } else if (fa != 0.0 && ((fa < 0) != (fb < 0))) {
-                        r[nr++] = bracketed_root<D>(c, a, b, fa, fb);
+                        last = bracketed_root<D>(c, a, b, fa, fb);
+                        put(r, nr++, last);
                     }
                     a = b;
                     fa = fb;
@@ -316,7 +357,7 @@ struct RootIsolator {
 };
 template <>
 struct RootIsolator<1> {
-    CCD_HD static int roots01(const double *c, double *r)
+    CCD_HD static int roots01(const double *c, double (&r)[1])
     {
         double x = -c[1] / c[0];
         if (x >= 0.0 && x <= 1.0) {
@@ -328,7 +369,7 @@ struct RootIsolator<1> {
 };
 template <>
 struct RootIsolator<2> {
-    CCD_HD static int roots01(const double *c, double *r)
+    CCD_HD static int roots01(const double *c, double (&r)[2])
     {
         double a = c[0], b = c[1], cc = c[2];
         double disc = b * b - 4.0 * a * cc;
@@ -348,8 +389,17 @@ struct RootIsolator<2> {
             x1 = s;
         }
         int n = 0;
-        if (x0 >= 0.0 && x0 <= 1.0) r[n++] = x0;
-        if (x1 >= 0.0 && x1 <= 1.0 && !(n > 0 && r[n - 1] == x1)) r[n++] = x1;
+        if (x0 >= 0.0 && x0 <= 1.0) {
+            r[0] = x0;
+            n = 1;
+        }
+        if (x1 >= 0.0 && x1 <= 1.0 && !(n > 0 && x0 == x1)) {
+            if (n == 0)
+                r[0] = x1;
+            else
+                r[1] = x1;
+            n++;
+        }
         return n;
     }
 };
@@ -357,6 +407,17 @@ struct RootIsolator<2> {
 }   // namespace ccd
 #include "ccd_rpoly.h"
 namespace ccd {
+
+// Roots of the degree-D polynomial op in [0,1] into time[0..), D <= M.
+template <int D, int M>
+CCD_HD int isolate_into(const double *op, double (&time)[M])
+{
+    double r[D];
+    int n = RootIsolator<D>::roots01(op, r);
+#pragma unroll
+    for (int k = 0; k < D && k < M; k++) time[k] = r[k];
+    return n;
+}
 
 // ------------------------------------------------------------------------------------------------
 // findIntervals (CTCD.cpp:98-177) for a degree-N polynomial, op modified in place.
@@ -396,31 +457,35 @@ CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos, C &&cnt = C())
     double time[N > 2 ? N : 2] = {};
     int roots = 0;
     if (reduced > 2) {
-        if (!could_have_roots(op, reduced, pos)) return;
+        if (!could_have_roots<N>(op, reduced, pos)) return;
         cnt(S_ISOLATE);
         if constexpr (uses_ref_roots<std::decay_t<C>>::value) {
             // CTCD.cpp:139-141 and :160-176: all roots the solver returns, sorted, real parts of
             // complex ones included.
-            double zr[6], zi[6];
+            double zr[6], zi[6], opc[7];
+#pragma unroll
+            for (int i = 0; i < 7; i++) opc[i] = i <= N ? op[i] : 0.0;
             jt::Solver js;
-            roots = js.rpoly(op, reduced, zr, zi);
+            roots = js.rpoly(opc, reduced, zr, zi);
             if (roots > 0) {
                 jt::sort_small(zr, roots);
                 if (zr[0] >= 0) check_interval<N>(0, zr[0], op, reduced, out, pos);
-                for (int i = 0; i < roots - 1; i++) {
-                    if (!((zr[i] < 0 && zr[i + 1] < 0) || (zr[i] > 1.0 && zr[i + 1] > 1.0)))
+#pragma unroll
+                for (int i = 0; i < 5; i++) {
+                    if (i < roots - 1 && !((zr[i] < 0 && zr[i + 1] < 0) || (zr[i] > 1.0 && zr[i + 1] > 1.0)))
                         check_interval<N>(zr[i], zr[i + 1], op, reduced, out, pos);
                 }
-                if (zr[roots - 1] <= 1.0) check_interval<N>(zr[roots - 1], 1.0, op, reduced, out, pos);
+                double zl = sel(zr, roots - 1);
+                if (zl <= 1.0) check_interval<N>(zl, 1.0, op, reduced, out, pos);
             } else {
                 check_interval<N>(0.0, 1.0, op, reduced, out, pos);
             }
             return;
         }
-        if (reduced == 3) roots = RootIsolator<3>::roots01(op, time);
-        if (N >= 4 && reduced == 4) roots = RootIsolator<(N >= 4 ? 4 : 3)>::roots01(op, time);
-        if (N >= 5 && reduced == 5) roots = RootIsolator<(N >= 5 ? 5 : 3)>::roots01(op, time);
-        if (N >= 6 && reduced == 6) roots = RootIsolator<(N >= 6 ? 6 : 3)>::roots01(op, time);
+        if (reduced == 3) roots = isolate_into<3>(op, time);
+        if (N >= 4 && reduced == 4) roots = isolate_into<(N >= 4 ? 4 : 3)>(op, time);
+        if (N >= 5 && reduced == 5) roots = isolate_into<(N >= 5 ? 5 : 3)>(op, time);
+        if (N >= 6 && reduced == 6) roots = isolate_into<(N >= 6 ? 6 : 3)>(op, time);
         // Pieces between consecutive roots (all in [0,1]) and the ends, as CTCD.cpp:160-176.
         if (roots == 0) {
             check_interval<N>(0.0, 1.0, op, reduced, out, pos);
@@ -554,19 +619,17 @@ CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3
     Ivls<7> coplane, parallel;
     coplane.n = 0;
     parallel.n = 0;
-    for (int i = 0; i < raw.n; i++) {
-        cnt(S_EE_PAR);
-        double midt = (raw.u[i] + raw.l[i]) / 2;
-        V3 a = (q0s - p0s) + midt * (vq0 - vp0);
-        V3 b = (q1s - p1s) + midt * (vq1 - vp1);
-        if (norm(cross(a, b)) < 1e-8) {
-            parallel.l[parallel.n] = raw.l[i];
-            parallel.u[parallel.n] = raw.u[i];
-            parallel.n++;
-        } else {
-            coplane.l[coplane.n] = raw.l[i];
-            coplane.u[coplane.n] = raw.u[i];
-            coplane.n++;
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+        if (i < raw.n) {
+            cnt(S_EE_PAR);
+            double midt = (raw.u[i] + raw.l[i]) / 2;
+            V3 a = (q0s - p0s) + midt * (vq0 - vp0);
+            V3 b = (q1s - p1s) + midt * (vq1 - vp1);
+            if (norm(cross(a, b)) < 1e-8)
+                append_ti(parallel, raw.l[i], raw.u[i]);
+            else
+                append_ti(coplane, raw.l[i], raw.u[i]);
         }
     }
     if (coplane.n == 0) return false;
@@ -597,8 +660,8 @@ CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3
             for (int k = 0; k < a1.n; k++)
                 for (int l = 0; l < b0.n; l++)
                     for (int m = 0; m < b1.n; m++) {
-                        double L[5] = {coplane.l[i], a0.l[j], a1.l[k], b0.l[l], b1.l[m]};
-                        double U[5] = {coplane.u[i], a0.u[j], a1.u[k], b0.u[l], b1.u[m]};
+                        double L[5] = {sel(coplane.l, i), sel(a0.l, j), sel(a1.l, k), sel(b0.l, l), sel(b1.l, m)};
+                        double U[5] = {sel(coplane.u, i), sel(a0.u, j), sel(a1.u, k), sel(b0.u, l), sel(b1.u, m)};
                         bool ov = true;
 #pragma unroll
                         for (int x = 0; x < 5; x++)
@@ -613,8 +676,9 @@ CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3
                             iu = smin(U[x], iu);
                         }
                         bool skip = false;
-                        for (int p = 0; p < parallel.n; p++)
-                            if (!(il > parallel.u[p] || parallel.l[p] > iu)) skip = true;
+#pragma unroll
+                        for (int p = 0; p < 7; p++)
+                            if (p < parallel.n && !(il > parallel.u[p] || parallel.l[p] > iu)) skip = true;
                         if (!skip) {
                             mint = smin(mint, il);
                             col = true;
@@ -656,8 +720,8 @@ CCD_HD bool vertex_face(V3 q0s, V3 q1s, V3 q2s, V3 q3s, V3 q0e, V3 q1e, V3 q2e, 
         for (int j = 0; j < e1.n; j++)
             for (int k = 0; k < e2.n; k++)
                 for (int l = 0; l < e3.n; l++) {
-                    double L[4] = {cop.l[i], e1.l[j], e2.l[k], e3.l[l]};
-                    double U[4] = {cop.u[i], e1.u[j], e2.u[k], e3.u[l]};
+                    double L[4] = {sel(cop.l, i), sel(e1.l, j), sel(e2.l, k), sel(e3.l, l)};
+                    double U[4] = {sel(cop.u, i), sel(e1.u, j), sel(e2.u, k), sel(e3.u, l)};
                     bool ov = true;
 #pragma unroll
                     for (int x = 0; x < 4; x++)
@@ -728,8 +792,8 @@ CCD_HD bool vertex_edge(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double e
     for (int i = 0; i < colin.n; i++)
         for (int j = 0; j < e1.n; j++)
             for (int k = 0; k < e2.n; k++) {
-                double L[3] = {colin.l[i], e1.l[j], e2.l[k]};
-                double U[3] = {colin.u[i], e1.u[j], e2.u[k]};
+                double L[3] = {sel(colin.l, i), sel(e1.l, j), sel(e2.l, k)};
+                double U[3] = {sel(colin.u, i), sel(e1.u, j), sel(e2.u, k)};
                 bool ov = true;
 #pragma unroll
                 for (int x = 0; x < 3; x++)

@@ -206,23 +206,30 @@ constexpr double INFIN = 3.4e38, SMALNO = 1.2e-38;
 constexpr double COSR = -0x1.1db902ec147ep-4;   // cos(94.0 * 0.017453293), rpoly.h:67-70
 constexpr double SINR = 0x1.fec0b6fc05f2p-1;    // sin(94.0 * 0.017453293)
 
+// Every array access below uses a compile-time index (loops are unrolled over the 7 slots with a guard
+// on the live degree n) or goes through ccd::sel / ccd::put, so on the GPU the solver's arrays stay in
+// registers instead of per-lane scratch memory.  The arithmetic and its order are unchanged.
 struct Solver {
     double p[7], qp[7], k[7], qk[7], svk[7];
     double sr, si, u, v, a, b, c, d, a1, a3, a7, e, f, g, h, szr, szi, lzr, lzi;
     int n;
 
     // rpoly.h:665-680: divide pp by the quadratic 1,u,v; quotient to q, remainder to (a, b).
-    CCD_HD static void quadsd(int nn, double uu, double vv, const double *pp, double *q, double &aa, double &bb)
+    CCD_HD static void quadsd(int nn, double uu, double vv, const double (&pp)[7], double (&q)[7], double &aa,
+                              double &bb)
     {
         bb = pp[0];
         q[0] = bb;
         aa = pp[1] - bb * uu;
         q[1] = aa;
-        for (int i = 2; i <= nn; i++) {
-            double cc = pp[i] - aa * uu - bb * vv;
-            q[i] = cc;
-            bb = aa;
-            aa = cc;
+#pragma unroll
+        for (int i = 2; i < 7; i++) {
+            if (i <= nn) {
+                double cc = pp[i] - aa * uu - bb * vv;
+                q[i] = cc;
+                bb = aa;
+                aa = cc;
+            }
         }
     }
 
@@ -272,7 +279,7 @@ struct Solver {
     CCD_HD int calcsc()
     {
         quadsd(n - 1, u, v, k, qk, c, d);
-        if (!(dabs(c) > dabs(k[n - 1] * 100.0 * ETA)) && !(dabs(d) > dabs(k[n - 2] * 100.0 * ETA))) return 3;
+        if (!(dabs(c) > dabs(sel(k, n - 1) * 100.0 * ETA)) && !(dabs(d) > dabs(sel(k, n - 2) * 100.0 * ETA))) return 3;
         if (dabs(d) < dabs(c)) {
             e = a / c;
             f = d / c;
@@ -299,21 +306,27 @@ struct Solver {
         if (type == 3) {
             k[0] = 0.0;
             k[1] = 0.0;
-            for (int i = 2; i < n; i++) k[i] = qk[i - 2];
+#pragma unroll
+            for (int i = 2; i < 7; i++)
+                if (i < n) k[i] = qk[i - 2];
             return;
         }
         double temp = type == 1 ? b : a;
         if (dabs(a1) <= dabs(temp) * ETA * 10.0) {
             k[0] = 0.0;
             k[1] = -a7 * qp[0];
-            for (int i = 2; i < n; i++) k[i] = a3 * qk[i - 2] - a7 * qp[i - 1];
+#pragma unroll
+            for (int i = 2; i < 7; i++)
+                if (i < n) k[i] = a3 * qk[i - 2] - a7 * qp[i - 1];
             return;
         }
         a7 /= a1;
         a3 /= a1;
         k[0] = qp[0];
         k[1] = qp[1] - a7 * qp[0];
-        for (int i = 2; i < n; i++) k[i] = a3 * qk[i - 2] - a7 * qp[i - 1] + qp[i];
+#pragma unroll
+        for (int i = 2; i < 7; i++)
+            if (i < n) k[i] = a3 * qk[i - 2] - a7 * qp[i - 1] + qp[i];
     }
 
     // rpoly.h:638-661
@@ -332,8 +345,9 @@ struct Solver {
             a4 = a + u * b + h * f;
             a5 = c + (u + v * f) * d;
         }
-        double b1 = -k[n - 1] / p[n];
-        double b2 = -(k[n - 2] + b1 * p[n - 1]) / p[n];
+        const double pn = sel(p, n);
+        double b1 = -sel(k, n - 1) / pn;
+        double b2 = -(sel(k, n - 2) + b1 * sel(p, n - 1)) / pn;
         double c1 = v * b2 * a1;
         double c2 = b1 * a7;
         double c3 = b1 * b1 * a3;
@@ -364,7 +378,9 @@ struct Solver {
             zm = dsqrt(dabs(v));
             ee = 2.0 * dabs(qp[0]);
             t = -szr * b;
-            for (int i = 1; i < n; i++) ee = ee * zm + dabs(qp[i]);
+#pragma unroll
+            for (int i = 1; i < 7; i++)
+                if (i < n) ee = ee * zm + dabs(qp[i]);
             ee = ee * zm + dabs(a + t);
             ee *= (5.0 * MRE + 4.0 * ARE);
             ee = ee - (5.0 * MRE + 2.0 * ARE) * (dabs(a + t) + dabs(b) * zm) + 2.0 * ARE * dabs(t);
@@ -407,14 +423,19 @@ struct Solver {
         while (true) {
             pv = p[0];
             qp[0] = pv;
-            for (int i = 1; i <= n; i++) {
-                pv = pv * s + p[i];
-                qp[i] = pv;
+#pragma unroll
+            for (int i = 1; i < 7; i++) {
+                if (i <= n) {
+                    pv = pv * s + p[i];
+                    qp[i] = pv;
+                }
             }
             mp = dabs(pv);
             ms = dabs(s);
             ee = (MRE / (ARE + MRE)) * dabs(qp[0]);
-            for (int i = 1; i <= n; i++) ee = ee * ms + dabs(qp[i]);
+#pragma unroll
+            for (int i = 1; i < 7; i++)
+                if (i <= n) ee = ee * ms + dabs(qp[i]);
             if (mp <= 20.0 * ((ARE + MRE) * ee - MRE * mp)) {
                 szr = s;
                 szi = 0.0;
@@ -430,22 +451,31 @@ struct Solver {
             omp = mp;
             kv = k[0];
             qk[0] = kv;
-            for (int i = 1; i < n; i++) {
-                kv = kv * s + k[i];
-                qk[i] = kv;
+#pragma unroll
+            for (int i = 1; i < 7; i++) {
+                if (i < n) {
+                    kv = kv * s + k[i];
+                    qk[i] = kv;
+                }
             }
-            if (dabs(kv) <= dabs(k[n - 1]) * 10.0 * ETA) {
+            if (dabs(kv) <= dabs(sel(k, n - 1)) * 10.0 * ETA) {
                 k[0] = 0.0;
-                for (int i = 1; i < n; i++) k[i] = qk[i - 1];
+#pragma unroll
+                for (int i = 1; i < 7; i++)
+                    if (i < n) k[i] = qk[i - 1];
             } else {
                 t = -pv / kv;
                 k[0] = qp[0];
-                for (int i = 1; i < n; i++) k[i] = t * qk[i - 1] + qp[i];
+#pragma unroll
+                for (int i = 1; i < 7; i++)
+                    if (i < n) k[i] = t * qk[i - 1] + qp[i];
             }
             kv = k[0];
-            for (int i = 1; i < n; i++) kv = kv * s + k[i];
+#pragma unroll
+            for (int i = 1; i < 7; i++)
+                if (i < n) kv = kv * s + k[i];
             t = 0.0;
-            if (dabs(kv) > dabs(k[n - 1] * 10.0 * ETA)) t = -pv / kv;
+            if (dabs(kv) > dabs(sel(k, n - 1) * 10.0 * ETA)) t = -pv / kv;
             s += t;
         }
     }
@@ -465,7 +495,8 @@ struct Solver {
             newest(type, ui, vi);
             vv = vi;
             ss = 0.0;
-            if (k[n - 1] != 0.0) ss = -p[n] / k[n - 1];
+            const double kn1 = sel(k, n - 1);
+            if (kn1 != 0.0) ss = -sel(p, n) / kn1;
             tv = 1.0;
             ts = 1.0;
             if (!(j == 0 || type == 3)) {
@@ -480,7 +511,9 @@ struct Solver {
                 if (spass || vpass) {
                     svu = u;
                     svv = v;
-                    for (int i = 0; i < n; i++) svk[i] = k[i];
+#pragma unroll
+                    for (int i = 0; i < 7; i++)
+                        if (i < n) svk[i] = k[i];
                     s = ss;
                     bool vtry = false, stry = false;
                     enum { QUAD, LIN, RESTORE } step = ((spass && !vpass) || tss < tvv) ? LIN : QUAD;
@@ -493,7 +526,9 @@ struct Solver {
                             if (stry || !spass) {
                                 step = RESTORE;
                             } else {
-                                for (int i = 0; i < n; i++) k[i] = svk[i];
+#pragma unroll
+                                for (int i = 0; i < 7; i++)
+                                    if (i < n) k[i] = svk[i];
                                 step = LIN;
                             }
                         } else if (step == LIN) {
@@ -511,7 +546,9 @@ struct Solver {
                         } else {
                             u = svu;
                             v = svv;
-                            for (int i = 0; i < n; i++) k[i] = svk[i];
+#pragma unroll
+                            for (int i = 0; i < 7; i++)
+                                if (i < n) k[i] = svk[i];
                             if (vpass && !vtry) {
                                 step = QUAD;
                                 continue;
@@ -531,43 +568,52 @@ struct Solver {
         return 0;
     }
 
-    // rpoly.h:49-261.  op[0..degree] descending, degree <= 6.  Returns the number of zeros found
-    // (degree, or fewer if 20 shifts fail), or -1 for a zero leading coefficient or a polynomial
-    // that is a monomial.
-    CCD_HD int rpoly(const double *op, int degree, double *zeror, double *zeroi)
+    // rpoly.h:49-261.  op[0..degree] descending, degree <= 6 (slots above degree are ignored).  Returns
+    // the number of zeros found (degree, or fewer if 20 shifts fail), or -1 for a zero leading
+    // coefficient or a polynomial that is a monomial.
+    CCD_HD int rpoly(const double (&op)[7], int degree, double (&zeror)[6], double (&zeroi)[6])
     {
         double xx = 0.70710678118654757;   // sqrt(0.5), correctly rounded
         double yy = -xx;
         const double lo = SMALNO / ETA;
         n = degree;
         if (op[0] == 0.0) return -1;
-        while (op[n] == 0.0) {
+        while (sel(op, n) == 0.0) {
             int j = degree - n;
-            zeror[j] = 0.0;
-            zeroi[j] = 0.0;
+            put(zeror, j, 0.0);
+            put(zeroi, j, 0.0);
             n--;
         }
         if (n < 1) return -1;
         double temp[7], pt[7];
-        for (int i = 0; i <= n; i++) p[i] = op[i];
+#pragma unroll
+        for (int i = 0; i < 7; i++) p[i] = i <= n ? op[i] : 0.0;
         while (true) {
             if (n == 1) {
-                zeror[degree - 1] = -p[1] / p[0];
-                zeroi[degree - 1] = 0.0;
+                put(zeror, degree - 1, -p[1] / p[0]);
+                put(zeroi, degree - 1, 0.0);
                 n -= 1;
                 break;
             }
             if (n == 2) {
-                quad(p[0], p[1], p[2], zeror[degree - 2], zeroi[degree - 2], zeror[degree - 1], zeroi[degree - 1]);
+                double r0, i0, r1, i1;
+                quad(p[0], p[1], p[2], r0, i0, r1, i1);
+                put(zeror, degree - 2, r0);
+                put(zeroi, degree - 2, i0);
+                put(zeror, degree - 1, r1);
+                put(zeroi, degree - 1, i1);
                 n -= 2;
                 break;
             }
             // Scale by a power of two if there are very large or very small coefficients.
             double mx = 0.0, mn = INFIN;
-            for (int i = 0; i <= n; i++) {
-                double x = dabs(p[i]);
-                if (x > mx) mx = x;
-                if (x != 0.0 && x < mn) mn = x;
+#pragma unroll
+            for (int i = 0; i < 7; i++) {
+                if (i <= n) {
+                    double x = dabs(p[i]);
+                    if (x > mx) mx = x;
+                    if (x != 0.0 && x < mn) mn = x;
+                }
             }
             double sc = lo / mn;
             bool scale = true;
@@ -582,15 +628,21 @@ struct Solver {
             if (scale) {
                 int l = (int)(log_cr(sc) / log_cr(2.0) + 0.5);
                 double factor = bits_ldexp(1.0, l);
-                if (factor != 1.0)
-                    for (int i = 0; i <= n; i++) p[i] = factor * p[i];
+                if (factor != 1.0) {
+#pragma unroll
+                    for (int i = 0; i < 7; i++)
+                        if (i <= n) p[i] = factor * p[i];
+                }
             }
             // Lower bound on the moduli of the zeros.
-            for (int i = 0; i <= n; i++) pt[i] = dabs(p[i]);
-            pt[n] = -pt[n];
-            double x = exp_cr((log_cr(-pt[n]) - log_cr(pt[0])) / (double)n);
-            if (pt[n - 1] != 0.0) {
-                double xm = -pt[n] / pt[n - 1];
+#pragma unroll
+            for (int i = 0; i < 7; i++) pt[i] = i <= n ? dabs(p[i]) : 0.0;
+            const double ptn = -sel(pt, n);
+            put(pt, n, ptn);
+            double x = exp_cr((log_cr(-ptn) - log_cr(pt[0])) / (double)n);
+            const double ptn1 = sel(pt, n - 1);
+            if (ptn1 != 0.0) {
+                double xm = -ptn / ptn1;
                 if (xm < x) x = xm;
             }
             // Chop (0, x) until the bound polynomial is <= 0 (the cap only guards non-finite input,
@@ -598,7 +650,9 @@ struct Solver {
             for (int guard = 0; guard < 1200; guard++) {
                 double xm = x * 0.1;
                 double ff = pt[0];
-                for (int i = 1; i <= n; i++) ff = ff * xm + pt[i];
+#pragma unroll
+                for (int i = 1; i < 7; i++)
+                    if (i <= n) ff = ff * xm + pt[i];
                 if (ff <= 0.0) break;
                 x = xm;
             }
@@ -606,41 +660,46 @@ struct Solver {
             for (int guard = 0; guard < 1000 && dabs(dx / x) > 0.005; guard++) {
                 double ff = pt[0];
                 double df = ff;
-                for (int i = 1; i < n; i++) {
-                    ff = ff * x + pt[i];
-                    df = df * x + ff;
+#pragma unroll
+                for (int i = 1; i < 7; i++) {
+                    if (i < n) {
+                        ff = ff * x + pt[i];
+                        df = df * x + ff;
+                    }
                 }
-                ff = ff * x + pt[n];
+                ff = ff * x + ptn;
                 dx = ff / df;
                 x -= dx;
             }
             double bnd = x;
             // The derivative as the first k polynomial, then 5 unshifted steps.
-            int nm1 = n - 1;
-            for (int i = 1; i < n; i++) k[i] = (double)(n - i) * p[i] / (double)n;
+#pragma unroll
+            for (int i = 1; i < 7; i++)
+                if (i < n) k[i] = (double)(n - i) * p[i] / (double)n;
             k[0] = p[0];
-            double aa = p[n], bb = p[n - 1];
-            bool zerok = k[n - 1] == 0;
+            double aa = sel(p, n), bb = sel(p, n - 1);
+            bool zerok = sel(k, n - 1) == 0;
             for (int jj = 0; jj < 5; jj++) {
-                double cc = k[n - 1];
+                double cc = sel(k, n - 1);
                 if (!zerok) {
                     double t = -aa / cc;
-                    for (int i = 0; i < nm1; i++) {
-                        int j = n - i - 1;
-                        k[j] = t * k[j - 1] + p[j];
-                    }
+                    // j = n-1 down to 1, each reading the not yet updated k[j-1]
+#pragma unroll
+                    for (int j = 6; j >= 1; j--)
+                        if (j <= n - 1) k[j] = t * k[j - 1] + p[j];
                     k[0] = p[0];
-                    zerok = dabs(k[n - 1]) <= dabs(bb) * ETA * 10.0;
+                    zerok = dabs(sel(k, n - 1)) <= dabs(bb) * ETA * 10.0;
                 } else {
-                    for (int i = 0; i < nm1; i++) {
-                        int j = n - i - 1;
-                        k[j] = k[j - 1];
-                    }
+#pragma unroll
+                    for (int j = 6; j >= 1; j--)
+                        if (j <= n - 1) k[j] = k[j - 1];
                     k[0] = 0.0;
-                    zerok = k[n - 1] == 0.0;
+                    zerok = sel(k, n - 1) == 0.0;
                 }
             }
-            for (int i = 0; i < n; i++) temp[i] = k[i];
+#pragma unroll
+            for (int i = 0; i < 7; i++)
+                if (i < n) temp[i] = k[i];
             // Up to 20 shifts, each rotated by 94 degrees from the last.
             bool found = false;
             for (int cnt = 0; cnt < 20; cnt++) {
@@ -654,18 +713,22 @@ struct Solver {
                 int nz = fxshfr(20 * (cnt + 1));
                 if (nz != 0) {
                     int j = degree - n;
-                    zeror[j] = szr;
-                    zeroi[j] = szi;
+                    put(zeror, j, szr);
+                    put(zeroi, j, szi);
                     n -= nz;
-                    for (int i = 0; i <= n; i++) p[i] = qp[i];
+#pragma unroll
+                    for (int i = 0; i < 7; i++)
+                        if (i <= n) p[i] = qp[i];
                     if (nz != 1) {
-                        zeror[j + 1] = lzr;
-                        zeroi[j + 1] = lzi;
+                        put(zeror, j + 1, lzr);
+                        put(zeroi, j + 1, lzi);
                     }
                     found = true;
                     break;
                 }
-                for (int i = 0; i < n; i++) k[i] = temp[i];
+#pragma unroll
+                for (int i = 0; i < 7; i++)
+                    if (i < n) k[i] = temp[i];
             }
             if (!found) break;   // no convergence with 20 shifts
         }
@@ -674,21 +737,33 @@ struct Solver {
 };
 
 // std::sort of at most 6 doubles as libstdc++ does it for short ranges (insertion sort whose
-// element smaller than the first goes straight to the front).
-CCD_HD void sort_small(double *x, int n)
+// element smaller than the first goes straight to the front), with compile-time indices.
+CCD_HD void sort_small(double (&x)[6], int n)
 {
-    for (int i = 1; i < n; i++) {
-        double val = x[i];
-        if (val < x[0]) {
-            for (int j = i; j > 0; j--) x[j] = x[j - 1];
-            x[0] = val;
-        } else {
-            int j = i;
-            while (j > 0 && val < x[j - 1]) {
-                x[j] = x[j - 1];
-                j--;
+#pragma unroll
+    for (int i = 1; i < 6; i++) {
+        if (i < n) {
+            double val = x[i];
+            if (val < x[0]) {
+#pragma unroll
+                for (int j = i; j >= 1; j--) x[j] = x[j - 1];
+                x[0] = val;
+            } else {
+                // unguarded linear insert: shift while val < x[j-1]; x[0] <= val stops it
+                bool moving = true;
+#pragma unroll
+                for (int j = i; j >= 1; j--) {
+                    if (moving) {
+                        if (val < x[j - 1]) {
+                            x[j] = x[j - 1];
+                        } else {
+                            x[j] = val;
+                            moving = false;
+                        }
+                    }
+                }
+                if (moving) x[0] = val;
             }
-            x[j] = val;
         }
     }
 }

@@ -86,13 +86,16 @@ extern "C" int core_find_intervals(const double *op, int n, int pos, double *out
 // The core's real roots of c[0..n] (n = 3..6) in [0,1].
 extern "C" int core_roots01(const double *c, int n, double *r)
 {
+    double buf[6] = {0, 0, 0, 0, 0, 0};
+    int k = -1;
     switch (n) {
-    case 3: return ccd::RootIsolator<3>::roots01(c, r);
-    case 4: return ccd::RootIsolator<4>::roots01(c, r);
-    case 5: return ccd::RootIsolator<5>::roots01(c, r);
-    case 6: return ccd::RootIsolator<6>::roots01(c, r);
+    case 3: k = ccd::isolate_into<3>(c, buf); break;
+    case 4: k = ccd::isolate_into<4>(c, buf); break;
+    case 5: k = ccd::isolate_into<5>(c, buf); break;
+    case 6: k = ccd::isolate_into<6>(c, buf); break;
     }
-    return -1;
+    for (int i = 0; i < n && i < 6; i++) r[i] = buf[i];
+    return k;
 }
 
 // Proximity statics (kind as CCD_DIST_*): returns the flag for kinds 0/1, else writes vec[3], bary[4].
@@ -112,8 +115,15 @@ extern "C" int core_distance(int kind, const double *q, double eta, double *vec,
 // The reference-root mode's solver on its own: op[0..degree] descending -> count, zeror, zeroi.
 extern "C" int core_rpoly(const double *op, int degree, double *zr, double *zi)
 {
+    double opc[7] = {0, 0, 0, 0, 0, 0, 0}, r[6] = {0, 0, 0, 0, 0, 0}, im[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i <= degree && i < 7; i++) opc[i] = op[i];
     ccd::jt::Solver js;
-    return js.rpoly(op, degree, zr, zi);
+    int k = js.rpoly(opc, degree, r, im);
+    for (int i = 0; i < 6; i++) {
+        zr[i] = r[i];
+        zi[i] = im[i];
+    }
+    return k;
 }
 
 extern "C" double core_log_cr(double x) { return ccd::jt::log_cr(x); }

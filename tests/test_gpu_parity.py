@@ -125,8 +125,9 @@ def test_gpu_primitives_vs_oracle(api, mode, kind):
           "vv": oracle.vertex_vertex_ctcd}[kind]
     recs = [fn(q[i], eta) for i in range(len(q))]
     if mode == "reference":
-        rec = {"hit": np.array([r[0] for r in recs], dtype=np.int32), "t": np.array([r[1] for r in recs]),
-               "margin": np.array([r[2] for r in recs])}
+        rec = {"hit": np.array([r[0] for r in recs], dtype=np.int32),
+               "t": np.array([r[1] if r[0] else 0.0 for r in recs], dtype=np.float64),
+               "margin": np.array([r[2] for r in recs], dtype=np.float64)}
         rep = compare_reference(kind, hit.astype(bool), t, rec)
         say(rep)
         assert not rep["hard"] and not rep["t_bad"], rep
