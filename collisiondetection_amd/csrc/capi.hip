@@ -361,11 +361,10 @@ struct DetectArgs {
 // launches (capturable).  n_max bounds the grids: the stencil counts are read on the device.
 static int narrow_launches(ccd_ctx *ctx, const DetectArgs &a, int64_t nvf_max, int64_t nee_max)
 {
-    unsigned long long *work = (unsigned long long *)buffer(ctx, "queue_work", 16);
+    unsigned long long *work = (unsigned long long *)buffer(ctx, "queue_work", 48);
     if (!work) return CCD_E_OOM;
     int rc = fill_u64(ctx, (uint64_t *)(ctx->d_counts + C_VF_HITS), 2, 0);
-    if (rc) return rc;
-    rc = fill_u64(ctx, (uint64_t *)work, 2, 0);
+    if (!rc) rc = fill_u64(ctx, (uint64_t *)work, 2, 0);
     if (rc) return rc;
     const int32_t *vfk = (const int32_t *)ctx->bufs["vf_keys"].ptr;
     const int32_t *eek = (const int32_t *)ctx->bufs["ee_keys"].ptr;

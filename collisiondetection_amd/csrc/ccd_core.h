@@ -35,10 +35,12 @@
 #include <hip/hip_runtime.h>
 #define CCD_HD __host__ __device__ inline
 #define CCD_FHD __host__ __device__ __forceinline__
+#define CCD_HD_NI __host__ __device__   // with static __attribute__((noinline)): one out-of-line copy per file
 #else
 #include <cmath>
 #define CCD_HD inline
 #define CCD_FHD inline
+#define CCD_HD_NI inline
 #endif
 
 namespace ccd {
@@ -142,9 +144,11 @@ CCD_FHD long long double_to_bits(double d)
 // Interval lists (CTCD.h:7-26).  CAP is the most pieces a degree-D polynomial can produce.
 // ------------------------------------------------------------------------------------------------
 template <int CAP>
+// Every slot starts defined: the lists are read through unrolled loops whose guards the compiler may
+// evaluate together with the read, and a read of an uninitialised slot would make that undefined.
 struct Ivls {
-    double l[CAP], u[CAP];
-    int n;
+    double l[CAP] = {}, u[CAP] = {};
+    int n = 0;
 };
 
 // Static-index access.  On the GPU an array indexed by a runtime value lives in scratch memory (per-lane
@@ -322,7 +326,7 @@ struct RootIsolator {
         double d[D];
 #pragma unroll
         for (int i = 0; i < D; i++) d[i] = c[i] * (double)(D - i);
-        double cr[D - 1];
+        double cr[D - 1] = {};
         int nc = RootIsolator<D - 1>::roots01(d, cr);
         int nr = 0;
         double a = 0.0;
@@ -412,11 +416,23 @@ namespace ccd {
 template <int D, int M>
 CCD_HD int isolate_into(const double *op, double (&time)[M])
 {
-    double r[D];
+    double r[D] = {};
     int n = RootIsolator<D>::roots01(op, r);
 #pragma unroll
     for (int k = 0; k < D && k < M; k++) time[k] = r[k];
     return n;
+}
+
+// Real roots in [0,1] of op[0..degree] (3 <= degree <= N) by isolation.
+template <int N, int M>
+CCD_FHD int isolate_roots(const double *op, int degree, double (&time)[M])
+{
+    int roots = 0;
+    if (degree == 3) roots = isolate_into<3>(op, time);
+    if (N >= 4 && degree == 4) roots = isolate_into<(N >= 4 ? 4 : 3)>(op, time);
+    if (N >= 5 && degree == 5) roots = isolate_into<(N >= 5 ? 5 : 3)>(op, time);
+    if (N >= 6 && degree == 6) roots = isolate_into<(N >= 6 ? 6 : 3)>(op, time);
+    return roots;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -425,6 +441,7 @@ CCD_HD int isolate_into(const double *op, double (&time)[M])
 template <int N, int CAP, class C = NoCount>
 CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos, C &&cnt = C())
 {
+    using CT = std::decay_t<C>;
     out.n = 0;
     double maxval = 0;
 #pragma unroll
@@ -459,14 +476,13 @@ CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos, C &&cnt = C())
     if (reduced > 2) {
         if (!could_have_roots<N>(op, reduced, pos)) return;
         cnt(S_ISOLATE);
-        if constexpr (uses_ref_roots<std::decay_t<C>>::value) {
+        if constexpr (uses_ref_roots<CT>::value) {
             // CTCD.cpp:139-141 and :160-176: all roots the solver returns, sorted, real parts of
             // complex ones included.
-            double zr[6], zi[6], opc[7];
+            double zr[6] = {}, zi[6] = {}, opc[7];
 #pragma unroll
             for (int i = 0; i < 7; i++) opc[i] = i <= N ? op[i] : 0.0;
-            jt::Solver js;
-            roots = js.rpoly(opc, reduced, zr, zi);
+            roots = jt::find_roots(opc, reduced, zr, zi);
             if (roots > 0) {
                 jt::sort_small(zr, roots);
                 if (zr[0] >= 0) check_interval<N>(0, zr[0], op, reduced, out, pos);
@@ -482,10 +498,7 @@ CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos, C &&cnt = C())
             }
             return;
         }
-        if (reduced == 3) roots = isolate_into<3>(op, time);
-        if (N >= 4 && reduced == 4) roots = isolate_into<(N >= 4 ? 4 : 3)>(op, time);
-        if (N >= 5 && reduced == 5) roots = isolate_into<(N >= 5 ? 5 : 3)>(op, time);
-        if (N >= 6 && reduced == 6) roots = isolate_into<(N >= 6 ? 6 : 3)>(op, time);
+        roots = isolate_roots<N>(op, reduced, time);
         // Pieces between consecutive roots (all in [0,1]) and the ends, as CTCD.cpp:160-176.
         if (roots == 0) {
             check_interval<N>(0.0, 1.0, op, reduced, out, pos);

@@ -206,35 +206,36 @@ constexpr double INFIN = 3.4e38, SMALNO = 1.2e-38;
 constexpr double COSR = -0x1.1db902ec147ep-4;   // cos(94.0 * 0.017453293), rpoly.h:67-70
 constexpr double SINR = 0x1.fec0b6fc05f2p-1;    // sin(94.0 * 0.017453293)
 
-// Every array access below uses a compile-time index (loops are unrolled over the 7 slots with a guard
-// on the live degree n) or goes through ccd::sel / ccd::put, so on the GPU the solver's arrays stay in
-// registers instead of per-lane scratch memory.  The arithmetic and its order are unchanged.
+// The solver works on the live degree NN as a template parameter: between deflations the degree is
+// fixed, so every routine below is instantiated per degree (3..6) and every array index is a
+// compile-time constant -- on the GPU the arrays stay in registers with no index arithmetic.  Only the
+// outer loop, which deflates, switches between the instantiations.  The arithmetic and its order are
+// the reference's, unchanged.
 struct Solver {
-    double p[7], qp[7], k[7], qk[7], svk[7];
-    double sr, si, u, v, a, b, c, d, a1, a3, a7, e, f, g, h, szr, szi, lzr, lzi;
-    int n;
+    double p[7] = {}, qp[7] = {}, k[7] = {}, qk[7] = {}, svk[7] = {};
+    double sr = 0, si = 0, u = 0, v = 0, a = 0, b = 0, c = 0, d = 0, a1 = 0, a3 = 0, a7 = 0, e = 0, f = 0, g = 0,
+           h = 0, szr = 0, szi = 0, lzr = 0, lzi = 0;
+    int n = 0;
 
     // rpoly.h:665-680: divide pp by the quadratic 1,u,v; quotient to q, remainder to (a, b).
-    CCD_HD static void quadsd(int nn, double uu, double vv, const double (&pp)[7], double (&q)[7], double &aa,
-                              double &bb)
+    template <int NN>
+    CCD_FHD static void quadsd(double uu, double vv, const double (&pp)[7], double (&q)[7], double &aa, double &bb)
     {
         bb = pp[0];
         q[0] = bb;
         aa = pp[1] - bb * uu;
         q[1] = aa;
 #pragma unroll
-        for (int i = 2; i < 7; i++) {
-            if (i <= nn) {
-                double cc = pp[i] - aa * uu - bb * vv;
-                q[i] = cc;
-                bb = aa;
-                aa = cc;
-            }
+        for (int i = 2; i <= NN; i++) {
+            double cc = pp[i] - aa * uu - bb * vv;
+            q[i] = cc;
+            bb = aa;
+            aa = cc;
         }
     }
 
     // rpoly.h:687-733: zeros of a z^2 + b1 z + c.
-    CCD_HD static void quad(double a_, double b1, double c_, double &sr_, double &si_, double &lr, double &li)
+    CCD_FHD static void quad(double a_, double b1, double c_, double &sr_, double &si_, double &lr, double &li)
     {
         double b_, d_, e_;
         if (a_ == 0.0) {
@@ -276,10 +277,11 @@ struct Solver {
     }
 
     // rpoly.h:557-596: scalars for the next k polynomial; returns the normalisation type.
-    CCD_HD int calcsc()
+    template <int NN>
+    CCD_FHD int calcsc()
     {
-        quadsd(n - 1, u, v, k, qk, c, d);
-        if (!(dabs(c) > dabs(sel(k, n - 1) * 100.0 * ETA)) && !(dabs(d) > dabs(sel(k, n - 2) * 100.0 * ETA))) return 3;
+        quadsd<NN - 1>(u, v, k, qk, c, d);
+        if (!(dabs(c) > dabs(k[NN - 1] * 100.0 * ETA)) && !(dabs(d) > dabs(k[NN - 2] * 100.0 * ETA))) return 3;
         if (dabs(d) < dabs(c)) {
             e = a / c;
             f = d / c;
@@ -301,14 +303,14 @@ struct Solver {
     }
 
     // rpoly.h:600-634
-    CCD_HD void nextk(int type)
+    template <int NN>
+    CCD_FHD void nextk(int type)
     {
         if (type == 3) {
             k[0] = 0.0;
             k[1] = 0.0;
 #pragma unroll
-            for (int i = 2; i < 7; i++)
-                if (i < n) k[i] = qk[i - 2];
+            for (int i = 2; i < NN; i++) k[i] = qk[i - 2];
             return;
         }
         double temp = type == 1 ? b : a;
@@ -316,8 +318,7 @@ struct Solver {
             k[0] = 0.0;
             k[1] = -a7 * qp[0];
 #pragma unroll
-            for (int i = 2; i < 7; i++)
-                if (i < n) k[i] = a3 * qk[i - 2] - a7 * qp[i - 1];
+            for (int i = 2; i < NN; i++) k[i] = a3 * qk[i - 2] - a7 * qp[i - 1];
             return;
         }
         a7 /= a1;
@@ -325,12 +326,12 @@ struct Solver {
         k[0] = qp[0];
         k[1] = qp[1] - a7 * qp[0];
 #pragma unroll
-        for (int i = 2; i < 7; i++)
-            if (i < n) k[i] = a3 * qk[i - 2] - a7 * qp[i - 1] + qp[i];
+        for (int i = 2; i < NN; i++) k[i] = a3 * qk[i - 2] - a7 * qp[i - 1] + qp[i];
     }
 
     // rpoly.h:638-661
-    CCD_HD void newest(int type, double &uu, double &vv)
+    template <int NN>
+    CCD_FHD void newest(int type, double &uu, double &vv)
     {
         if (type == 3) {
             uu = 0.0;
@@ -345,9 +346,8 @@ struct Solver {
             a4 = a + u * b + h * f;
             a5 = c + (u + v * f) * d;
         }
-        const double pn = sel(p, n);
-        double b1 = -sel(k, n - 1) / pn;
-        double b2 = -(sel(k, n - 2) + b1 * sel(p, n - 1)) / pn;
+        double b1 = -k[NN - 1] / p[NN];
+        double b2 = -(k[NN - 2] + b1 * p[NN - 1]) / p[NN];
         double c1 = v * b2 * a1;
         double c2 = b1 * a7;
         double c3 = b1 * b1 * a3;
@@ -363,7 +363,8 @@ struct Solver {
     }
 
     // rpoly.h:401-479: variable-shift iteration for a quadratic factor; returns the zeros found (0 or 2).
-    CCD_HD int quadit(double uu, double vv)
+    template <int NN>
+    CCD_FHD int quadit(double uu, double vv)
     {
         double ui, vi, mp, omp = 0, ee, relstp = 0, t, zm;
         bool tried = false;
@@ -373,14 +374,13 @@ struct Solver {
         while (true) {
             quad(1.0, u, v, szr, szi, lzr, lzi);
             if (dabs(dabs(szr) - dabs(lzr)) > 0.01 * dabs(lzr)) return 0;
-            quadsd(n, u, v, p, qp, a, b);
+            quadsd<NN>(u, v, p, qp, a, b);
             mp = dabs(a - szr * b) + dabs(szi * b);
             zm = dsqrt(dabs(v));
             ee = 2.0 * dabs(qp[0]);
             t = -szr * b;
 #pragma unroll
-            for (int i = 1; i < 7; i++)
-                if (i < n) ee = ee * zm + dabs(qp[i]);
+            for (int i = 1; i < NN; i++) ee = ee * zm + dabs(qp[i]);
             ee = ee * zm + dabs(a + t);
             ee *= (5.0 * MRE + 4.0 * ARE);
             ee = ee - (5.0 * MRE + 2.0 * ARE) * (dabs(a + t) + dabs(b) * zm) + 2.0 * ARE * dabs(t);
@@ -393,19 +393,19 @@ struct Solver {
                 relstp = dsqrt(relstp);
                 u = u - u * relstp;
                 v = v + v * relstp;
-                quadsd(n, u, v, p, qp, a, b);
+                quadsd<NN>(u, v, p, qp, a, b);
                 for (int i = 0; i < 5; i++) {
-                    int type = calcsc();
-                    nextk(type);
+                    int type = calcsc<NN>();
+                    nextk<NN>(type);
                 }
                 tried = true;
                 j = 0;
             }
             omp = mp;
-            int type = calcsc();
-            nextk(type);
-            type = calcsc();
-            newest(type, ui, vi);
+            int type = calcsc<NN>();
+            nextk<NN>(type);
+            type = calcsc<NN>();
+            newest<NN>(type, ui, vi);
             if (vi == 0.0) return 0;
             relstp = dabs((vi - v) / vi);
             u = ui;
@@ -415,7 +415,8 @@ struct Solver {
 
     // rpoly.h:485-550: variable-shift iteration for a real zero; returns the zeros found (0 or 1) and
     // sets iflag (with s updated) when a cluster near the real axis calls for the quadratic iteration.
-    CCD_HD int realit(double &sss, int &iflag)
+    template <int NN>
+    CCD_FHD int realit(double &sss, int &iflag)
     {
         double pv, kv, t = 0, s = sss, ms, mp, omp = 0, ee;
         iflag = 0;
@@ -424,18 +425,15 @@ struct Solver {
             pv = p[0];
             qp[0] = pv;
 #pragma unroll
-            for (int i = 1; i < 7; i++) {
-                if (i <= n) {
-                    pv = pv * s + p[i];
-                    qp[i] = pv;
-                }
+            for (int i = 1; i <= NN; i++) {
+                pv = pv * s + p[i];
+                qp[i] = pv;
             }
             mp = dabs(pv);
             ms = dabs(s);
             ee = (MRE / (ARE + MRE)) * dabs(qp[0]);
 #pragma unroll
-            for (int i = 1; i < 7; i++)
-                if (i <= n) ee = ee * ms + dabs(qp[i]);
+            for (int i = 1; i <= NN; i++) ee = ee * ms + dabs(qp[i]);
             if (mp <= 20.0 * ((ARE + MRE) * ee - MRE * mp)) {
                 szr = s;
                 szi = 0.0;
@@ -452,51 +450,46 @@ struct Solver {
             kv = k[0];
             qk[0] = kv;
 #pragma unroll
-            for (int i = 1; i < 7; i++) {
-                if (i < n) {
-                    kv = kv * s + k[i];
-                    qk[i] = kv;
-                }
+            for (int i = 1; i < NN; i++) {
+                kv = kv * s + k[i];
+                qk[i] = kv;
             }
-            if (dabs(kv) <= dabs(sel(k, n - 1)) * 10.0 * ETA) {
+            if (dabs(kv) <= dabs(k[NN - 1]) * 10.0 * ETA) {
                 k[0] = 0.0;
 #pragma unroll
-                for (int i = 1; i < 7; i++)
-                    if (i < n) k[i] = qk[i - 1];
+                for (int i = 1; i < NN; i++) k[i] = qk[i - 1];
             } else {
                 t = -pv / kv;
                 k[0] = qp[0];
 #pragma unroll
-                for (int i = 1; i < 7; i++)
-                    if (i < n) k[i] = t * qk[i - 1] + qp[i];
+                for (int i = 1; i < NN; i++) k[i] = t * qk[i - 1] + qp[i];
             }
             kv = k[0];
 #pragma unroll
-            for (int i = 1; i < 7; i++)
-                if (i < n) kv = kv * s + k[i];
+            for (int i = 1; i < NN; i++) kv = kv * s + k[i];
             t = 0.0;
-            if (dabs(kv) > dabs(sel(k, n - 1) * 10.0 * ETA)) t = -pv / kv;
+            if (dabs(kv) > dabs(k[NN - 1] * 10.0 * ETA)) t = -pv / kv;
             s += t;
         }
     }
 
     // rpoly.h:278-395: up to l2 fixed-shift steps, then the variable-shift iteration of the faster
     // converging sequence; returns the number of zeros found.
-    CCD_HD int fxshfr(int l2)
+    template <int NN>
+    CCD_FHD int fxshfr(int l2)
     {
         double svu = 0, svv = 0, ui, vi, s = 0;
         double betas = 0.25, betav = 0.25, oss = sr, ovv = v, ss, vv, ts, tv, ots = 0, otv = 0, tvv, tss;
         int iflag;
-        quadsd(n, u, v, p, qp, a, b);
-        int type = calcsc();
+        quadsd<NN>(u, v, p, qp, a, b);
+        int type = calcsc<NN>();
         for (int j = 0; j < l2; j++) {
-            nextk(type);
-            type = calcsc();
-            newest(type, ui, vi);
+            nextk<NN>(type);
+            type = calcsc<NN>();
+            newest<NN>(type, ui, vi);
             vv = vi;
             ss = 0.0;
-            const double kn1 = sel(k, n - 1);
-            if (kn1 != 0.0) ss = -sel(p, n) / kn1;
+            if (k[NN - 1] != 0.0) ss = -p[NN] / k[NN - 1];
             tv = 1.0;
             ts = 1.0;
             if (!(j == 0 || type == 3)) {
@@ -512,14 +505,13 @@ struct Solver {
                     svu = u;
                     svv = v;
 #pragma unroll
-                    for (int i = 0; i < 7; i++)
-                        if (i < n) svk[i] = k[i];
+                    for (int i = 0; i < NN; i++) svk[i] = k[i];
                     s = ss;
                     bool vtry = false, stry = false;
                     enum { QUAD, LIN, RESTORE } step = ((spass && !vpass) || tss < tvv) ? LIN : QUAD;
                     while (true) {
                         if (step == QUAD) {
-                            int nz = quadit(ui, vi);
+                            int nz = quadit<NN>(ui, vi);
                             if (nz > 0) return nz;
                             vtry = true;
                             betav *= 0.25;
@@ -527,12 +519,11 @@ struct Solver {
                                 step = RESTORE;
                             } else {
 #pragma unroll
-                                for (int i = 0; i < 7; i++)
-                                    if (i < n) k[i] = svk[i];
+                                for (int i = 0; i < NN; i++) k[i] = svk[i];
                                 step = LIN;
                             }
                         } else if (step == LIN) {
-                            int nz = realit(s, iflag);
+                            int nz = realit<NN>(s, iflag);
                             if (nz > 0) return nz;
                             stry = true;
                             betas *= 0.25;
@@ -547,14 +538,13 @@ struct Solver {
                             u = svu;
                             v = svv;
 #pragma unroll
-                            for (int i = 0; i < 7; i++)
-                                if (i < n) k[i] = svk[i];
+                            for (int i = 0; i < NN; i++) k[i] = svk[i];
                             if (vpass && !vtry) {
                                 step = QUAD;
                                 continue;
                             }
-                            quadsd(n, u, v, p, qp, a, b);
-                            type = calcsc();
+                            quadsd<NN>(u, v, p, qp, a, b);
+                            type = calcsc<NN>();
                             break;
                         }
                     }
@@ -568,14 +558,134 @@ struct Solver {
         return 0;
     }
 
+    // One pass of the outer loop at degree NN (rpoly.h:119-247): scaling, the root-modulus bound, the
+    // unshifted k steps, then up to 20 rotated shifts.  On success the zero(s) found are stored at
+    // zeror/zeroi[degree - NN ..], the polynomial is deflated (n = NN - nz) and true is returned.
+    template <int NN>
+    CCD_FHD bool stage(int degree, double &xx, double &yy, double (&zeror)[6], double (&zeroi)[6])
+    {
+        const double lo = SMALNO / ETA;
+        double temp[7] = {}, pt[7] = {};
+        // Scale by a power of two if there are very large or very small coefficients.
+        double mx = 0.0, mn = INFIN;
+#pragma unroll
+        for (int i = 0; i <= NN; i++) {
+            double x = dabs(p[i]);
+            if (x > mx) mx = x;
+            if (x != 0.0 && x < mn) mn = x;
+        }
+        double sc = lo / mn;
+        bool scale = true;
+        if (sc > 1.0 && INFIN / sc < mx)
+            scale = false;
+        else if (sc <= 1.0) {
+            if (mx < 10.0)
+                scale = false;
+            else if (sc == 0.0)
+                sc = SMALNO;
+        }
+        if (scale) {
+            int l = (int)(log_cr(sc) / log_cr(2.0) + 0.5);
+            double factor = bits_ldexp(1.0, l);
+            if (factor != 1.0) {
+#pragma unroll
+                for (int i = 0; i <= NN; i++) p[i] = factor * p[i];
+            }
+        }
+        // Lower bound on the moduli of the zeros.
+#pragma unroll
+        for (int i = 0; i <= NN; i++) pt[i] = dabs(p[i]);
+        pt[NN] = -pt[NN];
+        double x = exp_cr((log_cr(-pt[NN]) - log_cr(pt[0])) / (double)NN);
+        if (pt[NN - 1] != 0.0) {
+            double xm = -pt[NN] / pt[NN - 1];
+            if (xm < x) x = xm;
+        }
+        // Chop (0, x) until the bound polynomial is <= 0 (the cap only guards non-finite input, which
+        // would never terminate).
+        for (int guard = 0; guard < 1200; guard++) {
+            double xm = x * 0.1;
+            double ff = pt[0];
+#pragma unroll
+            for (int i = 1; i <= NN; i++) ff = ff * xm + pt[i];
+            if (ff <= 0.0) break;
+            x = xm;
+        }
+        double dx = x;
+        for (int guard = 0; guard < 1000 && dabs(dx / x) > 0.005; guard++) {
+            double ff = pt[0];
+            double df = ff;
+#pragma unroll
+            for (int i = 1; i < NN; i++) {
+                ff = ff * x + pt[i];
+                df = df * x + ff;
+            }
+            ff = ff * x + pt[NN];
+            dx = ff / df;
+            x -= dx;
+        }
+        double bnd = x;
+        // The derivative as the first k polynomial, then 5 unshifted steps.
+#pragma unroll
+        for (int i = 1; i < NN; i++) k[i] = (double)(NN - i) * p[i] / (double)NN;
+        k[0] = p[0];
+        double aa = p[NN], bb = p[NN - 1];
+        bool zerok = k[NN - 1] == 0;
+        for (int jj = 0; jj < 5; jj++) {
+            double cc = k[NN - 1];
+            if (!zerok) {
+                double t = -aa / cc;
+                // j = NN-1 down to 1, each reading the not yet updated k[j-1]
+#pragma unroll
+                for (int j = NN - 1; j >= 1; j--) k[j] = t * k[j - 1] + p[j];
+                k[0] = p[0];
+                zerok = dabs(k[NN - 1]) <= dabs(bb) * ETA * 10.0;
+            } else {
+#pragma unroll
+                for (int j = NN - 1; j >= 1; j--) k[j] = k[j - 1];
+                k[0] = 0.0;
+                zerok = k[NN - 1] == 0.0;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NN; i++) temp[i] = k[i];
+        // Up to 20 shifts, each rotated by 94 degrees from the last.
+        for (int cnt = 0; cnt < 20; cnt++) {
+            double xxx = COSR * xx - SINR * yy;
+            yy = SINR * xx + COSR * yy;
+            xx = xxx;
+            sr = bnd * xx;
+            si = bnd * yy;
+            u = -2.0 * sr;
+            v = bnd;
+            int nz = fxshfr<NN>(20 * (cnt + 1));
+            if (nz != 0) {
+                int j = degree - NN;
+                put(zeror, j, szr);
+                put(zeroi, j, szi);
+                n = NN - nz;
+#pragma unroll
+                for (int i = 0; i <= NN; i++)
+                    if (i <= NN - nz) p[i] = qp[i];
+                if (nz != 1) {
+                    put(zeror, j + 1, lzr);
+                    put(zeroi, j + 1, lzi);
+                }
+                return true;
+            }
+#pragma unroll
+            for (int i = 0; i < NN; i++) k[i] = temp[i];
+        }
+        return false;   // no convergence with 20 shifts
+    }
+
     // rpoly.h:49-261.  op[0..degree] descending, degree <= 6 (slots above degree are ignored).  Returns
     // the number of zeros found (degree, or fewer if 20 shifts fail), or -1 for a zero leading
     // coefficient or a polynomial that is a monomial.
-    CCD_HD int rpoly(const double (&op)[7], int degree, double (&zeror)[6], double (&zeroi)[6])
+    CCD_FHD int rpoly(const double (&op)[7], int degree, double (&zeror)[6], double (&zeroi)[6])
     {
         double xx = 0.70710678118654757;   // sqrt(0.5), correctly rounded
         double yy = -xx;
-        const double lo = SMALNO / ETA;
         n = degree;
         if (op[0] == 0.0) return -1;
         while (sel(op, n) == 0.0) {
@@ -585,7 +695,6 @@ struct Solver {
             n--;
         }
         if (n < 1) return -1;
-        double temp[7], pt[7];
 #pragma unroll
         for (int i = 0; i < 7; i++) p[i] = i <= n ? op[i] : 0.0;
         while (true) {
@@ -605,136 +714,25 @@ struct Solver {
                 n -= 2;
                 break;
             }
-            // Scale by a power of two if there are very large or very small coefficients.
-            double mx = 0.0, mn = INFIN;
-#pragma unroll
-            for (int i = 0; i < 7; i++) {
-                if (i <= n) {
-                    double x = dabs(p[i]);
-                    if (x > mx) mx = x;
-                    if (x != 0.0 && x < mn) mn = x;
-                }
-            }
-            double sc = lo / mn;
-            bool scale = true;
-            if (sc > 1.0 && INFIN / sc < mx)
-                scale = false;
-            else if (sc <= 1.0) {
-                if (mx < 10.0)
-                    scale = false;
-                else if (sc == 0.0)
-                    sc = SMALNO;
-            }
-            if (scale) {
-                int l = (int)(log_cr(sc) / log_cr(2.0) + 0.5);
-                double factor = bits_ldexp(1.0, l);
-                if (factor != 1.0) {
-#pragma unroll
-                    for (int i = 0; i < 7; i++)
-                        if (i <= n) p[i] = factor * p[i];
-                }
-            }
-            // Lower bound on the moduli of the zeros.
-#pragma unroll
-            for (int i = 0; i < 7; i++) pt[i] = i <= n ? dabs(p[i]) : 0.0;
-            const double ptn = -sel(pt, n);
-            put(pt, n, ptn);
-            double x = exp_cr((log_cr(-ptn) - log_cr(pt[0])) / (double)n);
-            const double ptn1 = sel(pt, n - 1);
-            if (ptn1 != 0.0) {
-                double xm = -ptn / ptn1;
-                if (xm < x) x = xm;
-            }
-            // Chop (0, x) until the bound polynomial is <= 0 (the cap only guards non-finite input,
-            // which would never terminate).
-            for (int guard = 0; guard < 1200; guard++) {
-                double xm = x * 0.1;
-                double ff = pt[0];
-#pragma unroll
-                for (int i = 1; i < 7; i++)
-                    if (i <= n) ff = ff * xm + pt[i];
-                if (ff <= 0.0) break;
-                x = xm;
-            }
-            double dx = x;
-            for (int guard = 0; guard < 1000 && dabs(dx / x) > 0.005; guard++) {
-                double ff = pt[0];
-                double df = ff;
-#pragma unroll
-                for (int i = 1; i < 7; i++) {
-                    if (i < n) {
-                        ff = ff * x + pt[i];
-                        df = df * x + ff;
-                    }
-                }
-                ff = ff * x + ptn;
-                dx = ff / df;
-                x -= dx;
-            }
-            double bnd = x;
-            // The derivative as the first k polynomial, then 5 unshifted steps.
-#pragma unroll
-            for (int i = 1; i < 7; i++)
-                if (i < n) k[i] = (double)(n - i) * p[i] / (double)n;
-            k[0] = p[0];
-            double aa = sel(p, n), bb = sel(p, n - 1);
-            bool zerok = sel(k, n - 1) == 0;
-            for (int jj = 0; jj < 5; jj++) {
-                double cc = sel(k, n - 1);
-                if (!zerok) {
-                    double t = -aa / cc;
-                    // j = n-1 down to 1, each reading the not yet updated k[j-1]
-#pragma unroll
-                    for (int j = 6; j >= 1; j--)
-                        if (j <= n - 1) k[j] = t * k[j - 1] + p[j];
-                    k[0] = p[0];
-                    zerok = dabs(sel(k, n - 1)) <= dabs(bb) * ETA * 10.0;
-                } else {
-#pragma unroll
-                    for (int j = 6; j >= 1; j--)
-                        if (j <= n - 1) k[j] = k[j - 1];
-                    k[0] = 0.0;
-                    zerok = sel(k, n - 1) == 0.0;
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < 7; i++)
-                if (i < n) temp[i] = k[i];
-            // Up to 20 shifts, each rotated by 94 degrees from the last.
             bool found = false;
-            for (int cnt = 0; cnt < 20; cnt++) {
-                double xxx = COSR * xx - SINR * yy;
-                yy = SINR * xx + COSR * yy;
-                xx = xxx;
-                sr = bnd * xx;
-                si = bnd * yy;
-                u = -2.0 * sr;
-                v = bnd;
-                int nz = fxshfr(20 * (cnt + 1));
-                if (nz != 0) {
-                    int j = degree - n;
-                    put(zeror, j, szr);
-                    put(zeroi, j, szi);
-                    n -= nz;
-#pragma unroll
-                    for (int i = 0; i < 7; i++)
-                        if (i <= n) p[i] = qp[i];
-                    if (nz != 1) {
-                        put(zeror, j + 1, lzr);
-                        put(zeroi, j + 1, lzi);
-                    }
-                    found = true;
-                    break;
-                }
-#pragma unroll
-                for (int i = 0; i < 7; i++)
-                    if (i < n) k[i] = temp[i];
-            }
-            if (!found) break;   // no convergence with 20 shifts
+            if (n == 3) found = stage<3>(degree, xx, yy, zeror, zeroi);
+            else if (n == 4) found = stage<4>(degree, xx, yy, zeror, zeroi);
+            else if (n == 5) found = stage<5>(degree, xx, yy, zeror, zeroi);
+            else if (n == 6) found = stage<6>(degree, xx, yy, zeror, zeroi);
+            if (!found) break;
         }
         return degree - n;
     }
 };
+
+// The solver as one out-of-line function: its state stays in this function's registers, and the
+// narrowphase kernels that call it keep their own register budget.
+static __attribute__((noinline)) CCD_HD_NI int find_roots(const double (&op)[7], int degree, double (&zeror)[6],
+                                                  double (&zeroi)[6])
+{
+    Solver js;
+    return js.rpoly(op, degree, zeror, zeroi);
+}
 
 // std::sort of at most 6 doubles as libstdc++ does it for short ranges (insertion sort whose
 // element smaller than the first goes straight to the front), with compile-time indices.

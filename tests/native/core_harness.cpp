@@ -128,3 +128,4 @@ extern "C" int core_rpoly(const double *op, int degree, double *zr, double *zi)
 
 extern "C" double core_log_cr(double x) { return ccd::jt::log_cr(x); }
 extern "C" double core_exp_cr(double x) { return ccd::jt::exp_cr(x); }
+

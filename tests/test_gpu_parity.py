@@ -321,3 +321,24 @@ def test_gpu_cloth_full_size(api, mode, n):
         say(f"cloth {pre}: comparing {len(idx)} stencils ({int(hit.sum())} GPU hits)")
         rec = oracle.narrowphase(h, keys[idx], s["eta"], is_ee)
         check_vs_oracle(mode, "cloth" + pre, hit[idx], r[pre + "_t"][idx], rec, h, keys[idx], s["eta"], is_ee)
+
+
+@pytest.mark.parametrize("scene", ["converging", "cloth64", "cloth128"])
+def test_gpu_detect_pipeline_equals_host_core(api, core_harness, mode, scene):
+    """The production pipeline (ccd_detect: broadphase + staged persistent narrowphase) against the
+    host-built core's single-pass cascade on every candidate: the same colliding stencils, TOI bits."""
+    say("case: detect pipeline vs host core", scene, mode)
+    s = SCENES[scene]() if scene in SCENES else scenes.cloth(128)
+    h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+    vf, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
+    d = api.detect(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"])
+    assert (d["vf_candidates"], d["ee_candidates"]) == (len(vf), len(ee))
+    for keys, pre, is_ee in ((vf, "vf", False), (ee, "ee", True)):
+        ch, ct, cs = core_narrowphase(core_harness, s["qstart"], s["qend"], keys, is_ee, s["eta"])
+        host = {tuple(k): t for k, t, hh in zip(keys.tolist(), ct, ch) if hh}
+        gpu = {tuple(k): t for k, t in zip(d[pre].tolist(), d[pre + "_toi"])}
+        only_gpu = sorted(set(gpu) - set(host))[:5]
+        only_host = sorted(set(host) - set(gpu))[:5]
+        tdiff = [k for k in set(gpu) & set(host) if gpu[k] != host[k]][:5]
+        say(pre, "gpu", len(gpu), "host", len(host), "only gpu", only_gpu, "only host", only_host, "t differs", tdiff)
+        assert not only_gpu and not only_host and not tdiff
