@@ -133,10 +133,12 @@ def bench_batched(steps, log):
             "candidates": {"vf": rg["vf_candidates"], "ee": rg["ee_candidates"]}}
 
 
-def bench_served(seconds, rate, log, sessions=256, budget_gib=32, max_inflight=16):
-    """The served workload: `sessions` device-resident sessions (log-uniform 1k-500k triangles, seed 4),
-    Poisson arrivals at `rate` requests/s for `seconds`, each to a uniform session, supplying new end
-    positions = previous end + N(0, 0.002^2).  Latency = request submission to result on the host."""
+def bench_served(seconds, rates, log, sessions=256, budget_gib=32, max_inflight=16):
+    """The served workload: `sessions` device-resident sessions (log-uniform 1k-500k triangles, seed 4);
+    for each rate, Poisson arrivals at that many requests/s for `seconds`, each to a uniform session,
+    supplying new end positions = previous end + N(0, 0.002^2) (the sessions carry on from run to run).
+    Latency = request submission to result on the host.  A run stops offering load once two seconds of
+    requests are waiting (overload), then drains."""
     import numpy as np
 
     from collisiondetection_amd import api, scenes
@@ -155,58 +157,63 @@ def bench_served(seconds, rate, log, sessions=256, budget_gib=32, max_inflight=1
     ends = [s["qend"].reshape(-1).copy() for s in sess]
     noise = scenes.NoiseBank(0.002, seed=4)
     rng = np.random.default_rng(4)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 1.2) + 16))
-    arrivals = arrivals[arrivals < seconds]
-    targets = rng.integers(0, len(sess), size=len(arrivals))
-    results = []
-    submitted = 0
-    backlog_cap = int(rate * 2)   # stop offering load once 2 s of requests wait (overload)
-    overloaded = False
-    start = time.perf_counter()
-    for i, (ta, k) in enumerate(zip(arrivals, targets)):
-        while True:
-            now = time.perf_counter() - start
-            if now >= ta:
-                break
-            results += srv.poll()
-            if ta - (time.perf_counter() - start) > 0.0005:
-                time.sleep(0.0002)
-        ends[k] = ends[k] + noise.draw(len(ends[k]))
-        srv.submit(ids[k], ends[k], i)
-        submitted += 1
-        if i % 256 == 0:
-            st = srv.stats()
-            if st["pending"] > backlog_cap:
-                overloaded = True
-                log(f"overloaded at request {i} ({ta:.2f} s): {st['pending']} pending")
-                break
-        if i % 4000 == 0:
-            log(f"served: {i} requests submitted at {ta:.1f} s, {len(results)} done")
-    offered_s = time.perf_counter() - start
-    srv.drain()
-    results += srv.poll(1 << 20)
-    end = time.perf_counter() - start
-    st = srv.stats()
+    runs = []
+    rid = 0
+    for rate in rates:
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 1.2) + 16))
+        arrivals = arrivals[arrivals < seconds]
+        targets = rng.integers(0, len(sess), size=len(arrivals))
+        results = []
+        submitted = 0
+        backlog_cap = max(16, int(rate * 2))
+        overloaded = False
+        st0 = srv.stats()
+        start = time.perf_counter()
+        for i, (ta, k) in enumerate(zip(arrivals, targets)):
+            while True:
+                now = time.perf_counter() - start
+                if now >= ta:
+                    break
+                results += srv.poll()
+                if ta - (time.perf_counter() - start) > 0.0005:
+                    time.sleep(0.0002)
+            ends[k] = ends[k] + noise.draw(len(ends[k]))
+            srv.submit(ids[k], ends[k], rid)
+            rid += 1
+            submitted += 1
+            if i % 64 == 0:
+                st = srv.stats()
+                if st["pending"] > backlog_cap:
+                    overloaded = True
+                    log(f"rate {rate}: overloaded at request {i} ({ta:.2f} s): {st['pending']} pending")
+                    break
+            if i % 2000 == 0:
+                log(f"rate {rate}: {i} requests submitted at {ta:.1f} s, {len(results)} done")
+        offered_s = time.perf_counter() - start
+        srv.drain()
+        results += srv.poll(1 << 20)
+        end = time.perf_counter() - start
+        st = srv.stats()
+        ok = [r for r in results if r["status"] == 0]
+        lat = np.array([r["t_done"] - r["t_submit"] for r in ok]) * 1e3
+        tris = np.array([nf[r["session"]] for r in ok])
+        small = lat[tris <= 100000]
+        t_first = min(r["t_submit"] for r in ok)
+        t_last = max(r["t_done"] for r in ok)
+        out = {"rate_offered": rate, "seconds": seconds, "submitted": submitted, "completed": len(ok),
+               "failed": len(results) - len(ok), "overloaded": overloaded, "offered_wall_s": offered_s,
+               "drain_wall_s": end - offered_s, "swept_tris_per_s": float(tris.sum() / (t_last - t_first)),
+               "requests_per_s": len(ok) / (t_last - t_first),
+               "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                              "p50_le100k": float(np.percentile(small, 50)) if len(small) else None,
+                              "p99_le100k": float(np.percentile(small, 99)) if len(small) else None},
+               "reruns": st["reruns"] - st0["reruns"], "mem_waits": st["mem_waits"] - st0["mem_waits"]}
+        log("served:", json.dumps(out))
+        runs.append(out)
     srv.close()
-    ok = [r for r in results if r["status"] == 0]
-    lat = np.array([r["t_done"] - r["t_submit"] for r in ok]) * 1e3
-    tris = np.array([nf[r["session"]] for r in ok])
-    small = lat[tris <= 100000]
-    t_first = min(r["t_submit"] for r in ok)
-    t_last = max(r["t_done"] for r in ok)
-    out = {"sessions": len(sess), "resident_triangles": int(nf.sum()), "rate_offered": rate, "seconds": seconds,
-           "submitted": submitted, "completed": len(ok), "failed": len(results) - len(ok), "overloaded": overloaded,
-           "offered_wall_s": offered_s, "drain_wall_s": end - offered_s,
-           "swept_tris_per_s": float(tris.sum() / (t_last - t_first)),
-           "requests_per_s": len(ok) / (t_last - t_first),
-           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                          "p50_le100k": float(np.percentile(small, 50)) if len(small) else None,
-                          "p99_le100k": float(np.percentile(small, 99)) if len(small) else None},
-           "reruns": st["reruns"], "pool_bytes_peak_note": "pool = admission budget", "budget_gib": budget_gib,
-           "max_inflight": max_inflight,
-           "client_noise": "one bank of 2^24 N(0, 0.002^2) draws, read at a random offset per request"}
-    log("served:", json.dumps(out))
-    return out
+    return {"sessions": len(sess), "resident_triangles": int(nf.sum()), "budget_gib": budget_gib,
+            "max_inflight": max_inflight, "runs": runs,
+            "client_noise": "one bank of 2^24 N(0, 0.002^2) draws, read at a random offset per request"}
 
 
 def main():
@@ -222,8 +229,9 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy 5 times (scale reference for HBM byte counters)")
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
-    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-rates", type=str, default="2000,20",
+                    help="Poisson request rates (req/s), run one after the other on the same sessions")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -320,6 +328,9 @@ def main():
     value = F / (wall / args.steps)
     nstencils = nv.value + ne.value
 
+    out_workload = (f"cloth self-CCD {args.grid}x{args.grid} ({V} V, {F} tris), eta 1e-6, seed 1"
+                    if args.workload == "cloth" else
+                    f"crumpled sheet self-CCD {args.grid}x{args.grid} ({V} V, {F} tris), eta 1e-7, seed 3")
     # Roofline of the dominant kernel.
     dom = max((k for k in stage_ms if k != "pipeline"), key=stage_ms.get)
     if dom in ("narrow_vf", "narrow_ee"):
@@ -336,6 +347,17 @@ def main():
         achieved = algo / (stage_ms[dom] * 1e-3) / 1e9
         roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": algo}
+    # HBM traffic of the dominant kernel per launch, from the committed rocprofv3 counter passes on this
+    # workload (FETCH_SIZE x 2 + WRITE_SIZE, the factors a plain device copy gives); null elsewhere.
+    try:
+        prof = json.load(open(os.path.join(ROOT, "profiles", "cloth256_traffic.json")))
+        key = roof["kernel"].replace("k_narrow2_queue<VF>", "k_narrow2_queue<true, 256, ccd::RefRoots>").replace(
+            "k_narrow2_queue<EE>", "k_narrow2_queue<false, 256, ccd::RefRoots>")
+        if prof["workload"] == out_workload and key in prof["kernels"]:
+            roof["traffic"] = prof["kernels"][key]["hbm_bytes"]
+            roof["traffic_source"] = "profiles/cloth256_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
+    except (OSError, ValueError, KeyError):
+        pass
     roof["stage_ms"] = {k: round(v, 5) for k, v in stage_ms.items()}
     roof["narrowphase_tflops"] = {
         "vf": flops_vf / max(stage_ms["narrow_vf"], 1e-9) / 1e9, "ee": flops_ee / max(stage_ms["narrow_ee"], 1e-9) / 1e9}
@@ -350,9 +372,7 @@ def main():
         "value": value, "unit": "swept tris/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": (f"cloth self-CCD {args.grid}x{args.grid} ({V} V, {F} tris), eta 1e-6, seed 1"
-                                if args.workload == "cloth" else
-                                f"crumpled sheet self-CCD {args.grid}x{args.grid} ({V} V, {F} tris), eta 1e-7, seed 3"),
+        "config": {"workload": out_workload,
                    "global_batch": 1, "parallelism": "single GPU"},
         "pair_tests_per_s": nstencils / ((stage_ms["narrow_vf"] + stage_ms["narrow_ee"]) * 1e-3),
         "candidates": {"vf": nv.value, "ee": ne.value}, "collisions": {"vf": res["hits"][0], "ee": res["hits"][1]},
@@ -370,7 +390,7 @@ def main():
     if not args.no_batched:
         out["batched"] = bench_batched(max(3, args.steps // 2), log)
     if not args.no_served:
-        out["served"] = bench_served(args.serve_seconds, args.serve_rate, log)
+        out["served"] = bench_served(args.serve_seconds, [float(r) for r in args.serve_rates.split(",")], log)
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, 1 thread) ...")
         out["cpu_baseline"] = cpu_baseline(scene)

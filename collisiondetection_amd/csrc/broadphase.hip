@@ -580,14 +580,60 @@ struct MeshView {
     const int32_t *vf_list;
     const double *dop64;   // by original face id
     const uint8_t *fixed;
+    const int32_t *adj_off = nullptr;   // [F + 1], or null
+    const int32_t *adj = nullptr;       // [2 * pairs]
+    bool use_adj = false;               // set per launch from the device pair count
 };
+
+// The related-face table is used while the pairs average at most this many per face.
+constexpr int64_t ADJ_MAX_AVG = 40;
+__device__ __forceinline__ bool adj_wanted(int64_t np, int64_t F) { return 2 * np <= ADJ_MAX_AVG * F; }
+
+__global__ __launch_bounds__(256) void k_adj_count(const int2 *__restrict__ pairs,
+                                                   const unsigned long long *__restrict__ np_dev, int64_t cap,
+                                                   int64_t F, int32_t *__restrict__ cnt)
+{
+    int64_t np = (int64_t)*np_dev;
+    if (np > cap) np = cap;
+    if (!adj_wanted(np, F)) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+        int2 p = pairs[i];
+        atomicAdd(&cnt[p.x], 1);
+        atomicAdd(&cnt[p.y], 1);
+    }
+}
+
+// cnt holds the row lengths on entry and is counted back down to zero.
+__global__ __launch_bounds__(256) void k_adj_fill(const int2 *__restrict__ pairs,
+                                                  const unsigned long long *__restrict__ np_dev, int64_t cap, int64_t F,
+                                                  const int32_t *__restrict__ off, int32_t *__restrict__ cnt,
+                                                  int32_t *__restrict__ adj)
+{
+    int64_t np = (int64_t)*np_dev;
+    if (np > cap) np = cap;
+    if (!adj_wanted(np, F)) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+        int2 p = pairs[i];
+        adj[off[p.x] + atomicSub(&cnt[p.x], 1) - 1] = p.y;
+        adj[off[p.y] + atomicSub(&cnt[p.y], 1) - 1] = p.x;
+    }
+}
 
 __device__ __forceinline__ int4 face3(const MeshView &m, int f)
 {
     return make_int4(m.faces[3 * f], m.faces[3 * f + 1], m.faces[3 * f + 2], f);
 }
-__device__ __forceinline__ bool related(const MeshView &m, int a, int b)   // overlapping, not neighbours
+// Related faces: overlapping swept k-DOPs and no shared vertex, i.e. exactly the face pairs the
+// traversal found.  When the mesh's pairs are few per face, the traversal's output is turned into a
+// face -> related faces table (adj_off / adj, rows unsorted) and `related` scans a row instead of
+// re-reading two 208-byte FP64 k-DOPs; otherwise (use_adj false) it repeats the FP64 test.
+__device__ __forceinline__ bool related(const MeshView &m, int a, int b)
 {
+    if (m.use_adj) {
+        for (int k = m.adj_off[a]; k < m.adj_off[a + 1]; k++)
+            if (m.adj[k] == b) return true;
+        return false;
+    }
     return !neighbours(face3(m, a), face3(m, b)) && overlap64(m.dop64 + NB * (int64_t)a, m.dop64 + NB * (int64_t)b);
 }
 __device__ __forceinline__ bool has_vertex(const MeshView &m, int f, int v)
@@ -660,7 +706,7 @@ __device__ __forceinline__ void append(bool want, int4 key, int4 *out, int64_t c
 
 // raw_dev: the mesh check's count of degenerate or duplicate faces; nonzero switches every lane to raw
 // emission (all 15 stencils of a pair), deduplicated afterwards by sorting.
-__global__ __launch_bounds__(256) void k_expand(MeshView m, const int2 *__restrict__ pairs,
+__global__ __launch_bounds__(256) void k_expand(MeshView m, int64_t F, const int2 *__restrict__ pairs,
                                                 const unsigned long long *__restrict__ npairs_dev, int64_t pairs_cap,
                                                 const unsigned long long *__restrict__ raw_dev,
                                                 int4 *__restrict__ vf_out, int64_t vf_cap,
@@ -670,6 +716,7 @@ __global__ __launch_bounds__(256) void k_expand(MeshView m, const int2 *__restri
     const bool OWNER = *raw_dev == 0;
     int64_t np = (int64_t)*npairs_dev;
     if (np > pairs_cap) np = pairs_cap;
+    m.use_adj = m.adj && adj_wanted(np, F);
     for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < np; base += (int64_t)gridDim.x * blockDim.x) {
         int64_t i = base + threadIdx.x;
         bool valid = i < np;
@@ -815,6 +862,8 @@ struct BPBufs {
     NodeRec *nodes;
     int4 *sfaces;
     void *sort_tmp;
+    int32_t *adj_cnt = nullptr, *adj_off = nullptr, *adj = nullptr;   // related-face table (null if too large)
+    void *adj_tmp = nullptr;
     int2 *pairs;
     int4 *vfk, *eek;
 };
@@ -843,10 +892,39 @@ static int bp_buffers(ccd_ctx *ctx, int64_t F, int64_t V, bool batched, BPBufs &
     b.pairs = (int2 *)buffer(ctx, "pairs", (size_t)ctx->cap_pairs * 8);
     b.vfk = (int4 *)buffer(ctx, "vf_keys", (size_t)ctx->cap_vf * 16);
     b.eek = (int4 *)buffer(ctx, "ee_keys", (size_t)ctx->cap_ee * 16);
+    if (2 * ctx->cap_pairs < ((int64_t)1 << 31) - 1) {   // int32 row offsets
+        b.adj_cnt = (int32_t *)buffer(ctx, "adj_cnt", (size_t)(F + 1) * 4);
+        b.adj_off = (int32_t *)buffer(ctx, "adj_off", (size_t)(F + 1) * 4);
+        b.adj = (int32_t *)buffer(ctx, "adj", (size_t)(2 * ctx->cap_pairs + 1) * 4);
+        b.adj_tmp = buffer(ctx, "adj_scan_tmp", scan_temp_bytes(F + 1));
+        if (!b.adj_cnt || !b.adj_off || !b.adj || !b.adj_tmp) return CCD_E_OOM;
+    }
     if (!b.dop64 || !b.centre || !b.partial || !b.keys || !b.keys2 || !b.vals || !b.order ||
         (batched && !b.face_inst) || !b.left || !b.rf || !b.rl || !b.parent || !b.escape || !b.arrivals ||
         !b.nodes || !b.sdop64 || !b.sfaces || !b.sort_tmp || !b.pairs || !b.vfk || !b.eek)
         return CCD_E_OOM;
+    return CCD_OK;
+}
+
+// Related-face table from the traversal's pairs (device-side count; skipped inside the kernels when
+// the pairs average more than ADJ_MAX_AVG per face).
+static int build_related_table(ccd_ctx *ctx, const BPBufs &b, int64_t F, MeshView &mv)
+{
+    if (!b.adj) return CCD_OK;
+    hipStream_t st = ctx->stream;
+    unsigned long long *c = (unsigned long long *)ctx->d_counts;
+    int rc = fill_u32(ctx, (uint32_t *)b.adj_cnt, F + 1, 0);
+    if (rc) return rc;
+    unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(ctx->cap_pairs, 256), 8192));
+    hipLaunchKernelGGL(k_adj_count, dim3(grid), dim3(256), 0, st, b.pairs, c + C_PAIRS, ctx->cap_pairs, F, b.adj_cnt);
+    CCD_LAUNCH_CHECK(ctx);
+    rc = exclusive_scan_i32(ctx, b.adj_cnt, b.adj_off, F + 1, b.adj_tmp, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_adj_fill, dim3(grid), dim3(256), 0, st, b.pairs, c + C_PAIRS, ctx->cap_pairs, F, b.adj_off,
+                       b.adj_cnt, b.adj);
+    CCD_LAUNCH_CHECK(ctx);
+    mv.adj_off = b.adj_off;
+    mv.adj = b.adj;
     return CCD_OK;
 }
 
@@ -946,8 +1024,10 @@ int broadphase_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const doubl
 
     prof_begin(ctx, CCD_T_EXPAND);
     MeshView mv{faces, voff, vlist, b.dop64, fixed};
+    rc = build_related_table(ctx, b, F, mv);
+    if (rc) return rc;
     unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(ctx->cap_pairs, 256), 16384));
-    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(256), 0, st, mv, b.pairs, c + C_PAIRS, ctx->cap_pairs, c + C_BAD,
+    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(256), 0, st, mv, F, b.pairs, c + C_PAIRS, ctx->cap_pairs, c + C_BAD,
                        b.vfk, ctx->cap_vf, c + C_VF, b.eek, ctx->cap_ee, c + C_EE);
     CCD_LAUNCH_CHECK(ctx);
     prof_end(ctx, CCD_T_EXPAND);
@@ -997,8 +1077,10 @@ int broadphase_refit_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const
     prof_end(ctx, CCD_T_TRAVERSE);
     prof_begin(ctx, CCD_T_EXPAND);
     MeshView mv{faces, voff, vlist, b.dop64, fixed};
+    rc = build_related_table(ctx, b, F, mv);
+    if (rc) return rc;
     unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(ctx->cap_pairs, 256), 16384));
-    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(256), 0, st, mv, b.pairs, c + C_PAIRS, ctx->cap_pairs, c + C_BAD,
+    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(256), 0, st, mv, F, b.pairs, c + C_PAIRS, ctx->cap_pairs, c + C_BAD,
                        b.vfk, ctx->cap_vf, c + C_VF, b.eek, ctx->cap_ee, c + C_EE);
     CCD_LAUNCH_CHECK(ctx);
     prof_end(ctx, CCD_T_EXPAND);
