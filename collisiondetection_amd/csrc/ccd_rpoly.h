@@ -199,6 +199,17 @@ CCD_HD double exp_cr(double y)
     return bits_ldexp(v, k);
 }
 
+// Test-only host build (CCD_HOST_LIBM): the solver calls the C library's log/exp instead.  The tests use
+// it to show that a reference-mode result which differs from the reference's own solver differs only
+// through the C library's rounding of log/exp (with them, the restatement is bit-identical).
+#if defined(CCD_HOST_LIBM) && !defined(__HIP_DEVICE_COMPILE__)
+CCD_HD double solver_log(double x) { return std::log(x); }
+CCD_HD double solver_exp(double x) { return std::exp(x); }
+#else
+CCD_HD double solver_log(double x) { return log_cr(x); }
+CCD_HD double solver_exp(double x) { return exp_cr(x); }
+#endif
+
 // ---- the three-stage algorithm (rpoly.h:49-759) -------------------------------------------------
 constexpr double ETA = 2.22e-16;   // rpoly.h:57-60: machine constants as the reference sets them
 constexpr double ARE = ETA, MRE = ETA;
@@ -585,7 +596,7 @@ struct Solver {
                 sc = SMALNO;
         }
         if (scale) {
-            int l = (int)(log_cr(sc) / log_cr(2.0) + 0.5);
+            int l = (int)(solver_log(sc) / solver_log(2.0) + 0.5);
             double factor = bits_ldexp(1.0, l);
             if (factor != 1.0) {
 #pragma unroll
@@ -596,7 +607,7 @@ struct Solver {
 #pragma unroll
         for (int i = 0; i <= NN; i++) pt[i] = dabs(p[i]);
         pt[NN] = -pt[NN];
-        double x = exp_cr((log_cr(-pt[NN]) - log_cr(pt[0])) / (double)NN);
+        double x = solver_exp((solver_log(-pt[NN]) - solver_log(pt[0])) / (double)NN);
         if (pt[NN - 1] != 0.0) {
             double xm = -pt[NN] / pt[NN - 1];
             if (xm < x) x = xm;

@@ -14,7 +14,7 @@ import pytest
 
 import oracle
 from collisiondetection_amd import scenes
-from tests.helpers import compare_reference
+from tests.helpers import compare_reference, libm_core
 
 pytestmark = pytest.mark.gpu
 
@@ -92,7 +92,7 @@ def test_gpu_sheet_window_completeness(sheet):
     torch.cuda.empty_cache()
 
 
-def test_gpu_sheet_detect_sample(sheet):
+def test_gpu_sheet_detect_sample(sheet, core_harness_libm):
     from collisiondetection_amd import api
 
     s = sheet
@@ -110,7 +110,9 @@ def test_gpu_sheet_detect_sample(sheet):
             assert len(keys) > 0
             idx = rng.choice(len(keys), size=min(20000, len(keys)), replace=False)
             rec = oracle.narrowphase(h, keys[idx], s["eta"], is_ee)
-            rep = compare_reference("sheet hits " + pre, np.ones(len(idx), bool), toi[idx], rec)
+            rep = compare_reference("sheet hits " + pre, np.ones(len(idx), bool), toi[idx], rec,
+                                    libm=libm_core(core_harness_libm, s["qstart"], s["qend"], keys[idx], is_ee,
+                                                   s["eta"]))
             print(rep, flush=True)
             assert not rep["hard"] and not rep["t_bad"], rep
     finally:
@@ -131,6 +133,9 @@ def test_gpu_sheet_detect_sample(sheet):
             ghit = hits[pos] == q
             oh = rec["hit"].astype(bool)
             bad = np.where((ghit != oh) & (rec["margin"] >= 1e-12))[0]
+            if len(bad):   # a flag that differs only through the C library's log/exp rounding is listed
+                lh, _ = libm_core(core_harness_libm, s["qstart"], s["qend"], keys, is_ee, s["eta"])(bad)
+                bad = bad[lh.astype(bool) != oh[bad]]
             print(f"sheet sample {pre}: {int(oh.sum())} oracle hits of {len(keys)}, {len(bad)} disagreements", flush=True)
             assert len(bad) == 0, keys[bad[:5]]
     finally:

@@ -16,7 +16,8 @@ import pytest
 
 import oracle
 from collisiondetection_amd import scenes
-from tests.helpers import compare_reference, compare_results, core_narrowphase, core_primitive, exact_arbiters
+from tests.helpers import (compare_reference, compare_results, core_narrowphase, core_primitive, exact_arbiters,
+                           libm_core)
 from tests.test_oracle import ETA, TESTCTCD, crossing_cases, expected_testctcd
 
 pytestmark = pytest.mark.gpu
@@ -55,9 +56,9 @@ def mode(request, api, core_harness):
     core_harness.core_set_ref_roots(0)
 
 
-def check_vs_oracle(mode, name, hit, t, rec, hist, keys, eta, is_ee):
+def check_vs_oracle(mode, name, hit, t, rec, hist, keys, eta, is_ee, libm=None):
     if mode == "reference":
-        rep = compare_reference(name, hit, t, rec)
+        rep = compare_reference(name, hit, t, rec, libm=libm)
         say(rep)
         assert not rep["hard"] and not rep["t_bad"], rep
     else:
@@ -193,7 +194,7 @@ def test_gpu_candidates_small_meshes(api):
 
 
 @pytest.mark.parametrize("scene", list(SCENES))
-def test_gpu_narrowphase_vs_oracle_and_core(api, core_harness, mode, scene):
+def test_gpu_narrowphase_vs_oracle_and_core(api, core_harness, core_harness_libm, mode, scene):
     say("case: narrowphase", scene, mode)
     s = SCENES[scene]()
     h = oracle.Hist.two_entry(s["qstart"], s["qend"])
@@ -201,7 +202,8 @@ def test_gpu_narrowphase_vs_oracle_and_core(api, core_harness, mode, scene):
     r = api.narrowphase(s["qstart"], s["qend"], vf, ee, s["eta"])
     for keys, is_ee, pre in ((vf, False, "vf"), (ee, True, "ee")):
         rec = oracle.narrowphase(h, keys, s["eta"], is_ee)
-        check_vs_oracle(mode, scene + pre, r[pre + "_hit"], r[pre + "_t"], rec, h, keys, s["eta"], is_ee)
+        check_vs_oracle(mode, scene + pre, r[pre + "_hit"], r[pre + "_t"], rec, h, keys, s["eta"], is_ee,
+                        libm=libm_core(core_harness_libm, s["qstart"], s["qend"], keys, is_ee, s["eta"]))
         ch, ct, cs = core_narrowphase(core_harness, s["qstart"], s["qend"], keys, is_ee, s["eta"])
         assert np.array_equal(ch, r[pre + "_hit"])
         assert np.array_equal(ct.view(np.int64), r[pre + "_t"].view(np.int64))
@@ -301,7 +303,7 @@ def test_gpu_work_counts_match_oracle(api):
 
 
 @pytest.mark.parametrize("n", [256])
-def test_gpu_cloth_full_size(api, mode, n):
+def test_gpu_cloth_full_size(api, core_harness_libm, mode, n):
     """The 256^2 cloth: candidate sets equal the oracle's; flags and times of impact on a sample and on
     every GPU hit."""
     say("case: cloth", n, mode)
@@ -320,7 +322,8 @@ def test_gpu_cloth_full_size(api, mode, n):
                                         np.where(hit)[0]]))
         say(f"cloth {pre}: comparing {len(idx)} stencils ({int(hit.sum())} GPU hits)")
         rec = oracle.narrowphase(h, keys[idx], s["eta"], is_ee)
-        check_vs_oracle(mode, "cloth" + pre, hit[idx], r[pre + "_t"][idx], rec, h, keys[idx], s["eta"], is_ee)
+        check_vs_oracle(mode, "cloth" + pre, hit[idx], r[pre + "_t"][idx], rec, h, keys[idx], s["eta"], is_ee,
+                        libm=libm_core(core_harness_libm, s["qstart"], s["qend"], keys[idx], is_ee, s["eta"]))
 
 
 @pytest.mark.parametrize("scene", ["converging", "cloth64", "cloth128"])
