@@ -148,8 +148,9 @@ CCD_FHD double bits_ldexp(double x, int e)   // x * 2^e for results in the norma
     return x * bits_to_double((long long)(1023 + e) << 52);
 }
 
-// Correctly rounded natural logarithm of a positive finite double (double-double evaluation).
-CCD_HD double log_cr(double x)
+// Correctly rounded natural logarithm of a positive finite double (full double-double evaluation: the
+// fallback of log_cr when its fast path cannot settle the rounding).
+static __attribute__((noinline)) CCD_HD_NI double log_cr_full(double x)
 {
     if (!(x > 0.0) || x != x) return x == 0.0 ? -bits_to_double(0x7ff0000000000000ll) : (x - x) / 0.0;
     if (x == bits_to_double(0x7ff0000000000000ll)) return x;
@@ -180,8 +181,9 @@ CCD_HD double log_cr(double x)
     return r.hi + r.lo;
 }
 
-// Correctly rounded exp of a double whose result is a normal double (double-double evaluation).
-CCD_HD double exp_cr(double y)
+// Correctly rounded exp of a double whose result is a normal double (full double-double evaluation: the
+// fallback of exp_cr when its fast path cannot settle the rounding).
+static __attribute__((noinline)) CCD_HD_NI double exp_cr_full(double y)
 {
     if (y != y) return y;
     if (y > 709.78) return bits_to_double(0x7ff0000000000000ll);
@@ -197,6 +199,103 @@ CCD_HD double exp_cr(double y)
     for (int j = 26; j >= 0; j--) acc = dd_add(dd_mul(acc, r), DD{INV_FACT[j][0], INV_FACT[j][1]});
     double v = acc.hi + acc.lo;
     return bits_ldexp(v, k);
+}
+
+// ---- fast paths with a rounding test (Ziv's strategy) ------------------------------------------
+// Each evaluates its function to a relative error below FAST_EPS (about 2^-69 by construction: the
+// leading series terms in double-double, the tail in double) and returns the rounded result only if
+// both ends of [v - FAST_EPS |v|, v + FAST_EPS |v|] round to the same double; otherwise ok is false
+// and the caller falls back to the full evaluation.  About one call in several thousand falls back.
+constexpr double FAST_EPS = 0x1p-64;
+
+// Double-double sum without the cancellation guard: its error is ~2^-104 of |a| + |b|, which is fine
+// wherever the two operands do not nearly cancel (every use below).
+CCD_FHD DD dd_add_fast(DD a, DD b)
+{
+    DD s = two_sum(a.hi, b.hi);
+    s.lo += a.lo + b.lo;
+    return quick_two_sum(s.hi, s.lo);
+}
+CCD_FHD double settle(DD v, bool &ok)
+{
+    double e = FAST_EPS * dabs(v.hi);
+    double a = v.hi + (v.lo - e);
+    double b = v.hi + (v.lo + e);
+    ok = a == b;
+    return a;
+}
+
+// log x = e ln2 + 2 atanh(s), s = (m - 1)/(m + 1), |s| <= 0.1716; x positive, normal and finite.
+CCD_FHD DD log_fast_dd(double x)
+{
+    long long bits = double_to_bits(x);
+    int e = (int)((bits >> 52) & 0x7ff) - 1023;
+    double m = bits_to_double((bits & 0x000fffffffffffffll) | 0x3ff0000000000000ll);
+    if (m > 1.4142135623730951) {
+        m *= 0.5;
+        e += 1;
+    }
+    double num = m - 1.0;            // exact (Sterbenz)
+    DD den = two_sum(m, 1.0);        // m + 1 exactly
+    double sh = num / den.hi;
+    double rem = dfma(-sh, den.hi, num) - sh * den.lo;
+    DD s = quick_two_sum(sh, rem / den.hi);
+    DD s2 = two_prod(s.hi, s.hi);
+    s2.lo += 2.0 * s.hi * s.lo;
+    s2 = quick_two_sum(s2.hi, s2.lo);
+    // sum_{k>=0} s^{2k}/(2k+1): k = 14..3 in double (the tail is below 2^-18), k = 2..0 in double-double.
+    double t = INV_ODD[14][0];
+#pragma unroll
+    for (int k = 13; k >= 3; k--) t = dfma(t, s2.hi, INV_ODD[k][0]);
+    DD acc = dd_add_fast(DD{INV_ODD[2][0], INV_ODD[2][1]}, DD{t * s2.hi, 0.0});
+    acc = dd_add_fast(dd_mul(acc, s2), DD{INV_ODD[1][0], INV_ODD[1][1]});
+    acc = dd_add_fast(dd_mul(acc, s2), DD{1.0, 0.0});
+    DD lm = dd_mul(DD{2.0 * s.hi, 2.0 * s.lo}, acc);
+    DD le = two_prod((double)e, LN2_HI);
+    le.lo += (double)e * LN2_LO;
+    return dd_add_fast(le, lm);
+}
+
+// exp y = 2^k exp(r), r = y - k ln2, |r| <= 0.35; y within [-708, 709.78].  Returns exp(r).
+CCD_FHD DD exp_fast_dd(double y, int &k)
+{
+    double kd = y / LN2_HI;
+    kd = kd < 0 ? -(double)(long long)(-kd + 0.5) : (double)(long long)(kd + 0.5);
+    k = (int)kd;
+    DD p = two_prod(kd, LN2_HI);
+    DD r = two_sum(y - p.hi, -p.lo);   // y - p.hi is exact (Sterbenz)
+    r.lo -= kd * LN2_LO;
+    r = quick_two_sum(r.hi, r.lo);
+    // sum_j r^j/j!: j = 17..6 in double (the tail is below 2^-18), j = 5..0 in double-double, at r.hi;
+    // then exp(r) = exp(r.hi) (1 + r.lo) to the precision needed.
+    double t = INV_FACT[17][0];
+#pragma unroll
+    for (int j = 16; j >= 6; j--) t = dfma(t, r.hi, INV_FACT[j][0]);
+    DD acc = dd_add_fast(DD{INV_FACT[5][0], INV_FACT[5][1]}, DD{t * r.hi, 0.0});
+#pragma unroll
+    for (int j = 4; j >= 0; j--) acc = dd_add_fast(dd_mul_d(acc, r.hi), DD{INV_FACT[j][0], INV_FACT[j][1]});
+    return dd_add_fast(acc, DD{acc.hi * r.lo, 0.0});
+}
+
+CCD_HD double log_cr(double x)
+{
+    if (x >= 0x1p-1000 && x < bits_to_double(0x7ff0000000000000ll)) {
+        bool ok;
+        double v = settle(log_fast_dd(x), ok);
+        if (ok) return v;
+    }
+    return log_cr_full(x);
+}
+
+CCD_HD double exp_cr(double y)
+{
+    if (y >= -708.0 && y <= 709.78) {
+        bool ok;
+        int k;
+        double v = settle(exp_fast_dd(y, k), ok);
+        if (ok) return bits_ldexp(v, k);
+    }
+    return exp_cr_full(y);
 }
 
 // Test-only host build (CCD_HOST_LIBM): the solver calls the C library's log/exp instead.  The tests use

@@ -14,6 +14,14 @@ namespace {
 
 using ccd::V3;
 
+// Every kernel here that reaches the root finder asks for two wavefronts per SIMD.  The register
+// allocator otherwise gives the reference-root kernels (and the out-of-line solver they share) up to
+// 512 registers and one wavefront per SIMD, which leaves the FP64 pipeline idle on dependent
+// instructions; at two wavefronts the extra spills cost less than the latency hiding gains (about 12%
+// on the 256x256 cloth).  The attribute must be on every caller: the shared solver gets the tightest
+// budget of its callers only when all of them carry it.
+#define CCD_TWO_WAVES __attribute__((amdgpu_waves_per_eu(2)))
+
 __device__ __forceinline__ V3 load3(const double *__restrict__ p, int v)
 {
     const double *q = p + 3 * (int64_t)v;
@@ -21,7 +29,7 @@ __device__ __forceinline__ V3 load3(const double *__restrict__ p, int v)
 }
 
 template <class M>
-__global__ __launch_bounds__(256) void k_primitives(int kind, const double *__restrict__ q,
+__global__ __launch_bounds__(256) CCD_TWO_WAVES void k_primitives(int kind, const double *__restrict__ q,
                                                     const double *__restrict__ eta, double eta_s, int64_t n,
                                                     uint8_t *__restrict__ hit, double *__restrict__ tout)
 {
@@ -74,7 +82,7 @@ __device__ __forceinline__ int stencil2(const double *__restrict__ qs, const dou
 }
 
 template <bool VF, class M>
-__global__ __launch_bounds__(256) void k_narrow2(const double *__restrict__ qs, const double *__restrict__ qe,
+__global__ __launch_bounds__(256) CCD_TWO_WAVES void k_narrow2(const double *__restrict__ qs, const double *__restrict__ qe,
                                                  const int4 *__restrict__ keys, const double *__restrict__ eta,
                                                  double eta_s, int64_t n, uint8_t *__restrict__ hit,
                                                  double *__restrict__ tout, uint8_t *__restrict__ sub)
@@ -91,7 +99,7 @@ __global__ __launch_bounds__(256) void k_narrow2(const double *__restrict__ qs, 
 // Narrowphase + compaction of the colliding stencils.  The stencil count is read from the device
 // (n_dev) so the launch needs no host round trip; the grid covers n_max.
 template <bool VF, class M>
-__global__ __launch_bounds__(256) void k_narrow2_compact(const double *__restrict__ qs, const double *__restrict__ qe,
+__global__ __launch_bounds__(256) CCD_TWO_WAVES void k_narrow2_compact(const double *__restrict__ qs, const double *__restrict__ qe,
                                                          const int4 *__restrict__ keys,
                                                          const int64_t *__restrict__ n_dev, int64_t n_max,
                                                          double eta, int4 *__restrict__ out_keys,
@@ -132,7 +140,7 @@ __global__ __launch_bounds__(256) void k_narrow2_compact(const double *__restric
 // time.  Per-stencil cost varies by orders of magnitude (early exits vs. full root isolation), so
 // dynamic tiles keep every SIMD busy to the end; tile and block size are tuning parameters.
 template <bool VF, int BS, class M>
-__global__ __launch_bounds__(BS) void k_narrow2_queue(const double *__restrict__ qs, const double *__restrict__ qe,
+__global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue(const double *__restrict__ qs, const double *__restrict__ qe,
                                                       const int4 *__restrict__ keys, const int64_t *__restrict__ n_dev,
                                                       int64_t n_max, double eta, int4 *__restrict__ out_keys,
                                                       double *__restrict__ out_toi, int64_t cap,
@@ -182,7 +190,7 @@ struct LaneCount {
 };
 
 template <bool VF, class LC>
-__global__ __launch_bounds__(256) void k_narrow2_count(const double *__restrict__ qs, const double *__restrict__ qe,
+__global__ __launch_bounds__(256) CCD_TWO_WAVES void k_narrow2_count(const double *__restrict__ qs, const double *__restrict__ qe,
                                                        const int4 *__restrict__ keys, int64_t n, double eta,
                                                        unsigned long long *counts)
 {
@@ -217,7 +225,7 @@ __global__ __launch_bounds__(256) void k_narrow2_count(const double *__restrict_
 
 // Multi-entry histories: stitch on the fly (History.cpp:98-140) and test window by window.
 template <bool VF, class M>
-__global__ __launch_bounds__(256) void k_narrow_hist(const int64_t *__restrict__ off, const double *__restrict__ ht,
+__global__ __launch_bounds__(256) CCD_TWO_WAVES void k_narrow_hist(const int64_t *__restrict__ off, const double *__restrict__ ht,
                                                      const double *__restrict__ hp, const int4 *__restrict__ keys,
                                                      const double *__restrict__ eta, double eta_s, int64_t n,
                                                      uint8_t *__restrict__ hit, double *__restrict__ tout,

@@ -129,3 +129,21 @@ extern "C" int core_rpoly(const double *op, int degree, double *zr, double *zi)
 extern "C" double core_log_cr(double x) { return ccd::jt::log_cr(x); }
 extern "C" double core_exp_cr(double x) { return ccd::jt::exp_cr(x); }
 
+extern "C" double core_log_cr_full(double x) { return ccd::jt::log_cr_full(x); }
+extern "C" double core_exp_cr_full(double x) { return ccd::jt::exp_cr_full(x); }
+// The fast paths before their rounding test: out[0..1] = the double-double value; exp also gives k.
+extern "C" void core_log_fast_dd(double x, double *out)
+{
+    ccd::jt::DD v = ccd::jt::log_fast_dd(x);
+    out[0] = v.hi;
+    out[1] = v.lo;
+}
+extern "C" int core_exp_fast_dd(double y, double *out)
+{
+    int k = 0;
+    ccd::jt::DD v = ccd::jt::exp_fast_dd(y, k);
+    out[0] = v.hi;
+    out[1] = v.lo;
+    return k;
+}
+extern "C" double core_fast_eps() { return ccd::jt::FAST_EPS; }

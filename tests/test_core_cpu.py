@@ -147,6 +147,43 @@ def test_core_log_exp_correctly_rounded(core_harness):
     assert agree > 0.99
 
 
+def test_core_log_exp_fast_paths(core_harness):
+    """The fast paths of log/exp (series tail in double, rounding settled by an error bound) stay well
+    inside the bound their rounding test assumes (measured against 200-bit mpmath), and with the
+    fallback the results equal the full double-double evaluation on every input."""
+    import mpmath
+
+    mpmath.mp.prec = 200
+    H = core_harness
+    for n in ("core_log_cr", "core_exp_cr", "core_log_cr_full", "core_exp_cr_full"):
+        _core_fn(H, n)
+    H.core_fast_eps.restype = ctypes.c_double
+    eps = H.core_fast_eps()
+    H.core_log_fast_dd.argtypes = [ctypes.c_double, dp]
+    H.core_exp_fast_dd.argtypes = [ctypes.c_double, dp]
+    H.core_exp_fast_dd.restype = ctypes.c_int
+    out = np.zeros(2)
+    rng = np.random.default_rng(12)
+    worst = 0.0
+    xs = np.concatenate([np.exp(rng.uniform(-690, 700, 3000)), rng.uniform(0.5, 2.0, 3000),
+                         1 + rng.uniform(-1e-3, 1e-3, 1000), 1 + rng.uniform(-1e-12, 1e-12, 1000)])
+    for x in xs:
+        H.core_log_fast_dd(x, out.ctypes.data_as(dp))
+        v = mpmath.log(mpmath.mpf(x))
+        if v != 0:
+            worst = max(worst, float(abs((mpmath.mpf(out[0]) + mpmath.mpf(out[1]) - v) / v)))
+    ys = np.concatenate([rng.uniform(-700, 700, 3000), rng.uniform(-1, 1, 3000), rng.uniform(-1e-6, 1e-6, 1000)])
+    for y in ys:
+        k = H.core_exp_fast_dd(y, out.ctypes.data_as(dp))
+        v = mpmath.exp(mpmath.mpf(y)) * mpmath.mpf(2) ** (-k)
+        worst = max(worst, float(abs((mpmath.mpf(out[0]) + mpmath.mpf(out[1]) - v) / v)))
+    assert worst < eps / 16, (worst, eps)
+    xs = np.exp(rng.uniform(-700, 700, 100000))
+    assert all(H.core_log_cr(x) == H.core_log_cr_full(x) for x in xs)
+    ys = rng.uniform(-700, 700, 100000)
+    assert all(H.core_exp_cr(y) == H.core_exp_cr_full(y) for y in ys)
+
+
 def _core_rpoly(H, op):
     op = np.ascontiguousarray(op, dtype=np.float64)
     zr, zi = np.zeros(8), np.zeros(8)
