@@ -370,11 +370,12 @@ __global__ __launch_bounds__(256) void k_leaf_records(const int32_t *__restrict_
                                                       const int32_t *__restrict__ faces, int64_t F,
                                                       const uint32_t *__restrict__ escape,
                                                       NodeRec *__restrict__ nodes, double *__restrict__ sdop64,
-                                                      int4 *__restrict__ sfaces)
+                                                      int4 *__restrict__ sfaces, int32_t *__restrict__ rank)
 {
     int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (j >= F) return;
     int64_t f = order[j];
+    rank[f] = (int32_t)j;
     const double *src = dop64 + NB * f;
     NodeRec r;
 #pragma unroll
@@ -571,6 +572,30 @@ __global__ void k_mesh_check(const int32_t *__restrict__ faces, int64_t F, const
     if (bad) atomicAdd(flag, 1ull);
 }
 
+// Edge -> opposite face: opp[3f + s] is the other face on edge (v[s], v[s+1]) of face f, -1 on a
+// boundary edge, -2 on an edge shared by more than two faces (the expansion then scans the vertex ->
+// face table for that pair, as it does for every pair without this table).
+__global__ __launch_bounds__(256) void k_edge_opp(const int32_t *__restrict__ faces, int64_t F,
+                                                  const int32_t *__restrict__ off, const int32_t *__restrict__ list,
+                                                  int32_t *__restrict__ opp)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= 3 * F) return;
+    int64_t f = i / 3;
+    int s = (int)(i - 3 * f);
+    int a = faces[i], b = faces[3 * f + (s + 1) % 3];
+    int found = -1, n = 0;
+    for (int k = off[a]; k < off[a + 1]; k++) {
+        int h = list[k];
+        if (h == (int)f) continue;
+        if (faces[3 * h] == b || faces[3 * h + 1] == b || faces[3 * h + 2] == b) {
+            found = h;
+            n++;
+        }
+    }
+    opp[i] = n > 1 ? -2 : found;
+}
+
 // ------------------------------------------------------------------------------------------------
 // 7. Stencil expansion with owner-computes deduplication
 // ------------------------------------------------------------------------------------------------
@@ -583,6 +608,10 @@ struct MeshView {
     const int32_t *adj_off = nullptr;   // [F + 1], or null
     const int32_t *adj = nullptr;       // [2 * pairs]
     bool use_adj = false;               // set per launch from the device pair count
+    const int32_t *opp = nullptr;       // [3F] edge -> opposite face (k_edge_opp), or null
+    const int32_t *rank = nullptr;      // [F] face id -> Morton-sorted position
+    const double *sdop64 = nullptr;     // FP64 bounds in Morton order
+    const int4 *sfaces = nullptr;       // vertex triples and face id in Morton order
 };
 
 // The related-face table is used while the pairs average at most this many per face.
@@ -634,6 +663,11 @@ __device__ __forceinline__ bool related(const MeshView &m, int a, int b)
             if (m.adj[k] == b) return true;
         return false;
     }
+    if (m.rank) {   // the Morton-ordered copies: spatially close faces share cache lines
+        int ra = m.rank[a], rb = m.rank[b];
+        return !neighbours(m.sfaces[ra], m.sfaces[rb]) &&
+               overlap64(m.sdop64 + NB * (int64_t)ra, m.sdop64 + NB * (int64_t)rb);
+    }
     return !neighbours(face3(m, a), face3(m, b)) && overlap64(m.dop64 + NB * (int64_t)a, m.dop64 + NB * (int64_t)b);
 }
 __device__ __forceinline__ bool has_vertex(const MeshView &m, int f, int v)
@@ -672,6 +706,21 @@ __device__ __forceinline__ bool ee_owner(const MeshView &m, int u0, int u1, int 
     return true;
 }
 
+// The same rule from the edge table: the faces on edge u are f and oa, those on edge w are g and ob
+// (-1: none), so the other generating pairs are (f, ob), (oa, g) and (oa, ob).
+__device__ __forceinline__ bool precedes(int a, int b, int f, int g)
+{
+    int lo = a < b ? a : b, hi = a < b ? b : a;
+    return lo < f || (lo == f && hi < g);
+}
+__device__ __forceinline__ bool ee_owner_opp(const MeshView &m, int oa, int ob, int f, int g)
+{
+    if (ob >= 0 && precedes(f, ob, f, g) && related(m, f, ob)) return false;
+    if (oa >= 0 && precedes(oa, g, f, g) && related(m, oa, g)) return false;
+    if (oa >= 0 && ob >= 0 && precedes(oa, ob, f, g) && related(m, oa, ob)) return false;
+    return true;
+}
+
 __device__ __forceinline__ int4 vf_key(int p, int a, int b, int c)   // Stencils.h:13-57
 {
     int lo = a < b ? (a < c ? a : c) : (b < c ? b : c);
@@ -687,37 +736,37 @@ __device__ __forceinline__ int4 ee_key(int a, int b, int c, int d)   // Stencils
     return make_int4(p0, p1, q0, q1);
 }
 
-// Wave-aggregated append of up to one key per lane.
-__device__ __forceinline__ void append(bool want, int4 key, int4 *out, int64_t cap, unsigned long long *counter)
-{
-    uint64_t m = __ballot(want);
-    if (!m) return;
-    int lane = __lane_id();
-    int leader = __ffsll((unsigned long long)m) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
-    base = __shfl(base, leader);
-    if (want) {
-        uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
-        int64_t pos = (int64_t)base + __popcll(below);
-        if (pos < cap) out[pos] = key;
-    }
-}
-
 // raw_dev: the mesh check's count of degenerate or duplicate faces; nonzero switches every lane to raw
 // emission (all 15 stencils of a pair), deduplicated afterwards by sorting.
-__global__ __launch_bounds__(256) void k_expand(MeshView m, int64_t F, const int2 *__restrict__ pairs,
-                                                const unsigned long long *__restrict__ npairs_dev, int64_t pairs_cap,
-                                                const unsigned long long *__restrict__ raw_dev,
-                                                int4 *__restrict__ vf_out, int64_t vf_cap,
-                                                unsigned long long *vf_counter, int4 *__restrict__ ee_out,
-                                                int64_t ee_cap, unsigned long long *ee_counter)
+//
+// Output slots are claimed per block, not per wavefront: each lane first decides which of its 15
+// stencils it emits (two bit masks), the block scans the counts through LDS and one lane claims the
+// block's range with a single atomic per stencil kind.  The counters are single words that every block
+// hits, so per-wavefront claims (15 per wavefront and pass) made their serialisation the bottleneck.
+// The grid-stride loop bound is uniform across a block, so every lane reaches the barriers.
+//
+// ADJ: the instance for the related-face table (true) or for the FP64 re-test (false); both are
+// launched and the one that does not match the device-side choice returns at once, so each carries
+// only its own test.
+constexpr int EXPAND_BLOCK = 256;
+template <bool ADJ>
+__global__ __launch_bounds__(EXPAND_BLOCK) void k_expand(MeshView m, int64_t F, const int2 *__restrict__ pairs,
+                                                         const unsigned long long *__restrict__ npairs_dev,
+                                                         int64_t pairs_cap, const unsigned long long *__restrict__ raw_dev,
+                                                         int4 *__restrict__ vf_out, int64_t vf_cap,
+                                                         unsigned long long *vf_counter, int4 *__restrict__ ee_out,
+                                                         int64_t ee_cap, unsigned long long *ee_counter)
 {
+    constexpr int NW = EXPAND_BLOCK / 64;
+    __shared__ unsigned s_off[2][NW];
+    __shared__ unsigned long long s_base[2];
     const bool OWNER = *raw_dev == 0;
     int64_t np = (int64_t)*npairs_dev;
     if (np > pairs_cap) np = pairs_cap;
-    m.use_adj = m.adj && adj_wanted(np, F);
-    for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < np; base += (int64_t)gridDim.x * blockDim.x) {
+    if ((m.adj != nullptr && adj_wanted(np, F)) != ADJ) return;
+    m.use_adj = ADJ;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    for (int64_t base = blockIdx.x * (int64_t)EXPAND_BLOCK; base < np; base += (int64_t)gridDim.x * EXPAND_BLOCK) {
         int64_t i = base + threadIdx.x;
         bool valid = i < np;
         int2 pr = valid ? pairs[i] : make_int2(0, 0);
@@ -728,19 +777,28 @@ __global__ __launch_bounds__(256) void k_expand(MeshView m, int64_t F, const int
         int gv[3] = {G0.x, G0.y, G0.z};
         bool ffix = valid && is_fixed(m, fv[0]) && is_fixed(m, fv[1]) && is_fixed(m, fv[2]);
         bool gfix = valid && is_fixed(m, gv[0]) && is_fixed(m, gv[1]) && is_fixed(m, gv[2]);
-        // 6 vertex-face stencils (KDOPBroadPhase.cpp:157-175)
+        int fo[3] = {-2, -2, -2}, go[3] = {-2, -2, -2};
+        if (valid && m.opp) {
+#pragma unroll
+            for (int s = 0; s < 3; s++) {
+                fo[s] = m.opp[3 * (int64_t)f + s];
+                go[s] = m.opp[3 * (int64_t)g + s];
+            }
+        }
+        // 6 vertex-face stencils (KDOPBroadPhase.cpp:157-175): bit 2s for (fv[s], g), 2s+1 for (gv[s], f)
+        unsigned vmask = 0, emask = 0;
 #pragma unroll
         for (int s = 0; s < 3; s++) {
             int p = fv[s];
             bool want = valid && !(is_fixed(m, p) && gfix);
             if (OWNER && want) want = vf_owner(m, p, g, f);
-            append(want, vf_key(p, gv[0], gv[1], gv[2]), vf_out, vf_cap, vf_counter);
+            if (want) vmask |= 1u << (2 * s);
             int q = gv[s];
             want = valid && !(is_fixed(m, q) && ffix);
             if (OWNER && want) want = vf_owner(m, q, f, g);
-            append(want, vf_key(q, fv[0], fv[1], fv[2]), vf_out, vf_cap, vf_counter);
+            if (want) vmask |= 1u << (2 * s + 1);
         }
-        // 9 edge-edge stencils (KDOPBroadPhase.cpp:176-183)
+        // 9 edge-edge stencils (KDOPBroadPhase.cpp:176-183): bit 3s+r for edges (s, r)
 #pragma unroll
         for (int s = 0; s < 3; s++) {
             int u0 = fv[s], u1 = fv[(s + 1) % 3];
@@ -748,10 +806,69 @@ __global__ __launch_bounds__(256) void k_expand(MeshView m, int64_t F, const int
             for (int r = 0; r < 3; r++) {
                 int w0 = gv[r], w1 = gv[(r + 1) % 3];
                 bool want = valid && !(is_fixed(m, u0) && is_fixed(m, u1) && is_fixed(m, w0) && is_fixed(m, w1));
-                if (OWNER && want) want = ee_owner(m, u0, u1, w0, w1, f, g);
-                append(want, ee_key(u0, u1, w0, w1), ee_out, ee_cap, ee_counter);
+                if (OWNER && want) {
+                    if (fo[s] != -2 && go[r] != -2)
+                        want = ee_owner_opp(m, fo[s], go[r], f, g);
+                    else
+                        want = ee_owner(m, u0, u1, w0, w1, f, g);
+                }
+                if (want) emask |= 1u << (3 * s + r);
             }
         }
+        // Block-wide exclusive offsets: wavefront inclusive scans, then the wavefront totals in LDS.
+        const unsigned nv = __popc(vmask), ne = __popc(emask);
+        unsigned pv = nv, pe = ne;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            unsigned tv = __shfl_up(pv, d), te = __shfl_up(pe, d);
+            if (lane >= d) {
+                pv += tv;
+                pe += te;
+            }
+        }
+        if (lane == 63) {
+            s_off[0][wave] = pv;
+            s_off[1][wave] = pe;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned tv = 0, te = 0;
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                unsigned a = s_off[0][w], b = s_off[1][w];
+                s_off[0][w] = tv;
+                s_off[1][w] = te;
+                tv += a;
+                te += b;
+            }
+            s_base[0] = tv ? atomicAdd(vf_counter, (unsigned long long)tv) : 0ull;
+            s_base[1] = te ? atomicAdd(ee_counter, (unsigned long long)te) : 0ull;
+        }
+        __syncthreads();
+        int64_t pos_v = (int64_t)(s_base[0] + s_off[0][wave] + (pv - nv));
+        int64_t pos_e = (int64_t)(s_base[1] + s_off[1][wave] + (pe - ne));
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+            if (vmask & (1u << (2 * s))) {
+                if (pos_v < vf_cap) vf_out[pos_v] = vf_key(fv[s], gv[0], gv[1], gv[2]);
+                pos_v++;
+            }
+            if (vmask & (1u << (2 * s + 1))) {
+                if (pos_v < vf_cap) vf_out[pos_v] = vf_key(gv[s], fv[0], fv[1], fv[2]);
+                pos_v++;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                if (emask & (1u << (3 * s + r))) {
+                    if (pos_e < ee_cap) ee_out[pos_e] = ee_key(fv[s], fv[(s + 1) % 3], gv[r], gv[(r + 1) % 3]);
+                    pos_e++;
+                }
+            }
+        }
+        __syncthreads();   // s_off / s_base are rewritten by the next pass
     }
 }
 
@@ -866,6 +983,8 @@ struct BPBufs {
     void *adj_tmp = nullptr;
     int2 *pairs;
     int4 *vfk, *eek;
+    int32_t *opp;    // edge -> opposite face, 3F (kept with the vertex -> face table for refits)
+    int32_t *rank;   // face id -> Morton-sorted position
 };
 constexpr int NPARTS = 512;
 
@@ -892,6 +1011,8 @@ static int bp_buffers(ccd_ctx *ctx, int64_t F, int64_t V, bool batched, BPBufs &
     b.pairs = (int2 *)buffer(ctx, "pairs", (size_t)ctx->cap_pairs * 8);
     b.vfk = (int4 *)buffer(ctx, "vf_keys", (size_t)ctx->cap_vf * 16);
     b.eek = (int4 *)buffer(ctx, "ee_keys", (size_t)ctx->cap_ee * 16);
+    b.opp = (int32_t *)buffer(ctx, "edge_opp", (size_t)(3 * F + 1) * 4);
+    b.rank = (int32_t *)buffer(ctx, "face_rank", (size_t)F * 4);
     if (2 * ctx->cap_pairs < ((int64_t)1 << 31) - 1) {   // int32 row offsets
         b.adj_cnt = (int32_t *)buffer(ctx, "adj_cnt", (size_t)(F + 1) * 4);
         b.adj_off = (int32_t *)buffer(ctx, "adj_off", (size_t)(F + 1) * 4);
@@ -901,7 +1022,7 @@ static int bp_buffers(ccd_ctx *ctx, int64_t F, int64_t V, bool batched, BPBufs &
     }
     if (!b.dop64 || !b.centre || !b.partial || !b.keys || !b.keys2 || !b.vals || !b.order ||
         (batched && !b.face_inst) || !b.left || !b.rf || !b.rl || !b.parent || !b.escape || !b.arrivals ||
-        !b.nodes || !b.sdop64 || !b.sfaces || !b.sort_tmp || !b.pairs || !b.vfk || !b.eek)
+        !b.nodes || !b.sdop64 || !b.sfaces || !b.sort_tmp || !b.pairs || !b.vfk || !b.eek || !b.opp || !b.rank)
         return CCD_E_OOM;
     return CCD_OK;
 }
@@ -925,6 +1046,20 @@ static int build_related_table(ccd_ctx *ctx, const BPBufs &b, int64_t F, MeshVie
     CCD_LAUNCH_CHECK(ctx);
     mv.adj_off = b.adj_off;
     mv.adj = b.adj;
+    return CCD_OK;
+}
+
+static int launch_expand(ccd_ctx *ctx, const MeshView &mv, int64_t F, const BPBufs &b, unsigned grid)
+{
+    unsigned long long *c = (unsigned long long *)ctx->d_counts;
+    if (mv.adj) {
+        hipLaunchKernelGGL(k_expand<true>, dim3(grid), dim3(EXPAND_BLOCK), 0, ctx->stream, mv, F, b.pairs, c + C_PAIRS,
+                           ctx->cap_pairs, c + C_BAD, b.vfk, ctx->cap_vf, c + C_VF, b.eek, ctx->cap_ee, c + C_EE);
+        CCD_LAUNCH_CHECK(ctx);
+    }
+    hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(EXPAND_BLOCK), 0, ctx->stream, mv, F, b.pairs, c + C_PAIRS,
+                       ctx->cap_pairs, c + C_BAD, b.vfk, ctx->cap_vf, c + C_VF, b.eek, ctx->cap_ee, c + C_EE);
+    CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
 }
 
@@ -997,7 +1132,7 @@ int broadphase_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const doubl
     hipLaunchKernelGGL(k_escape, dim3(blocks_for(2 * F - 1, 256)), dim3(256), 0, st, b.rf, b.rl, F, b.escape);
     CCD_LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_leaf_records, dim3(blocks_for(F, 256)), dim3(256), 0, st, order, b.dop64, faces, F,
-                       b.escape, b.nodes, b.sdop64, b.sfaces);
+                       b.escape, b.nodes, b.sdop64, b.sfaces, b.rank);
     CCD_LAUNCH_CHECK(ctx);
     rc = fill_u32(ctx, b.arrivals, F, 0);
     if (rc) return rc;
@@ -1013,6 +1148,8 @@ int broadphase_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const doubl
     vertex_face_table(ctx, &voff, &vlist);
     hipLaunchKernelGGL(k_mesh_check, dim3(blocks_for(F, 256)), dim3(256), 0, st, faces, F, voff, vlist, c + C_BAD);
     CCD_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_edge_opp, dim3(blocks_for(3 * F, 256)), dim3(256), 0, st, faces, F, voff, vlist, b.opp);
+    CCD_LAUNCH_CHECK(ctx);
     prof_end(ctx, CCD_T_TABLES);
 
     prof_begin(ctx, CCD_T_TRAVERSE);
@@ -1024,12 +1161,15 @@ int broadphase_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const doubl
 
     prof_begin(ctx, CCD_T_EXPAND);
     MeshView mv{faces, voff, vlist, b.dop64, fixed};
+    mv.opp = b.opp;
+    mv.rank = b.rank;
+    mv.sdop64 = b.sdop64;
+    mv.sfaces = b.sfaces;
     rc = build_related_table(ctx, b, F, mv);
     if (rc) return rc;
     unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(ctx->cap_pairs, 256), 16384));
-    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(256), 0, st, mv, F, b.pairs, c + C_PAIRS, ctx->cap_pairs, c + C_BAD,
-                       b.vfk, ctx->cap_vf, c + C_VF, b.eek, ctx->cap_ee, c + C_EE);
-    CCD_LAUNCH_CHECK(ctx);
+    rc = launch_expand(ctx, mv, F, b, grid);
+    if (rc) return rc;
     prof_end(ctx, CCD_T_EXPAND);
     return CCD_OK;
 }
@@ -1059,7 +1199,7 @@ int broadphase_refit_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const
     prof_end(ctx, CCD_T_BOUNDS);
     prof_begin(ctx, CCD_T_BUILD);
     hipLaunchKernelGGL(k_leaf_records, dim3(blocks_for(F, 256)), dim3(256), 0, st, ctx->tree_order, b.dop64, faces,
-                       F, b.escape, b.nodes, b.sdop64, b.sfaces);
+                       F, b.escape, b.nodes, b.sdop64, b.sfaces, b.rank);
     CCD_LAUNCH_CHECK(ctx);
     rc = fill_u32(ctx, b.arrivals, F, 0);
     if (rc) return rc;
@@ -1077,12 +1217,15 @@ int broadphase_refit_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const
     prof_end(ctx, CCD_T_TRAVERSE);
     prof_begin(ctx, CCD_T_EXPAND);
     MeshView mv{faces, voff, vlist, b.dop64, fixed};
+    mv.opp = b.opp;
+    mv.rank = b.rank;
+    mv.sdop64 = b.sdop64;
+    mv.sfaces = b.sfaces;
     rc = build_related_table(ctx, b, F, mv);
     if (rc) return rc;
     unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(ctx->cap_pairs, 256), 16384));
-    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(256), 0, st, mv, F, b.pairs, c + C_PAIRS, ctx->cap_pairs, c + C_BAD,
-                       b.vfk, ctx->cap_vf, c + C_VF, b.eek, ctx->cap_ee, c + C_EE);
-    CCD_LAUNCH_CHECK(ctx);
+    rc = launch_expand(ctx, mv, F, b, grid);
+    if (rc) return rc;
     prof_end(ctx, CCD_T_EXPAND);
     return CCD_OK;
 }
