@@ -13,7 +13,7 @@
 // Launch shape of the persistent work-queue narrowphase (tuned by ccd_autotune).
 struct QueueShape {
     int block = 256;       // threads per block: 64, 128 or 256
-    int tile = 256;        // stencils a wavefront takes per atomic
+    int tile = 64;         // stencils a wavefront takes per atomic (64 measured best in both root modes)
     int waves_per_cu = 8;  // resident wavefronts per compute unit in the persistent grid
 };
 

@@ -312,10 +312,12 @@ void worker_loop(ccd_server *S)
                 continue;
             }
             InFlight f;
-            f.cap_pairs = s.last_pairs + s.last_pairs / 4 + 1024;
-            f.cap_vf = s.last_vf + s.last_vf / 4 + 1024;
-            f.cap_ee = s.last_ee + s.last_ee / 4 + 1024;
-            f.cap_hits = s.last_hits + s.last_hits / 2 + 1024;
+            // Margins over the session's last counts: consecutive requests of a moving mesh vary by
+            // tens of percent, and every overflow costs a rerun of the whole request.
+            f.cap_pairs = s.last_pairs + s.last_pairs / 2 + 4096;
+            f.cap_vf = s.last_vf + s.last_vf / 2 + 4096;
+            f.cap_ee = s.last_ee + s.last_ee / 2 + 4096;
+            f.cap_hits = 2 * s.last_hits + 4096;
             f.pool_bytes = workspace_bytes(f.cap_pairs, f.cap_vf, f.cap_ee, f.cap_hits);
             if (f.pool_bytes > S->pool.size) {   // can never be admitted: fail it rather than wait forever
                 ccd_request_result r{};

@@ -184,6 +184,36 @@ def test_gpu_candidates_non_manifold_fallback(api):
     assert np.array_equal(ovf, gvf) and np.array_equal(oee, gee)
 
 
+def test_gpu_candidates_shared_edges_and_dense_pairs(api):
+    """The expansion's table paths: edges shared by three faces (the edge table marks them and the
+    owner rule scans the vertex -> face table there), and a mesh whose face pairs average more than
+    40 per face (no related-face rows: the FP64 re-test on Morton-ordered bounds), with and without
+    fixed vertices."""
+    say("case: candidates with shared edges and dense pairs")
+    s = scenes.cloth(12, drop=0.05, noise=0.01)
+    q0, q1 = s["qstart"], s["qend"]
+    V = len(q0)
+    # a fin on every 7th interior edge: a new vertex off the sheet, one face on that edge
+    fins, extra0, extra1 = [], [], []
+    for k, (a, b, c) in enumerate(s["faces"][::7]):
+        v = V + len(extra0)
+        extra0.append((q0[a] + q0[b]) / 2 + np.array([0.0, 0.0, 0.03]))
+        extra1.append((q1[a] + q1[b]) / 2 + np.array([0.0, 0.0, 0.03]))
+        fins.append((a, b, v))
+    faces = np.concatenate([s["faces"], np.array(fins, np.int32)]).astype(np.int32)
+    qs = np.concatenate([q0, np.array(extra0)])
+    qe = np.concatenate([q1, np.array(extra1)])
+    h = oracle.Hist.two_entry(qs, qe)
+    for eta in (0.02, 0.12):   # 0.02: sparse pairs (related-face rows); 0.12: ~57 pairs per face
+        for fixed in (None, set(range(0, len(qs), 3))):
+            ovf, oee, pairs = oracle.candidates(faces, h, eta, fixed=fixed, method="brute", want_pairs=True)
+            gvf, gee = api.kdop_candidates(faces, qs, qe, eta, fixed=fixed, sort=True)
+            say("eta", eta, "pairs per face", round(2 * len(pairs) / len(faces), 1), "stencils", len(ovf), len(oee))
+            assert np.array_equal(ovf, gvf), (eta, len(ovf), len(gvf))
+            assert np.array_equal(oee, gee), (eta, len(oee), len(gee))
+    assert 2 * len(pairs) / len(faces) > 40
+
+
 def test_gpu_candidates_small_meshes(api):
     say("case: candidates on 0, 1 and 2 faces")
     q = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 0.1], [1, 0, 0.1], [0, 1, 0.1]], float)
