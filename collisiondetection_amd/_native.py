@@ -50,6 +50,8 @@ SIGNATURES = {
     "ccd_device_bytes": ([_V], _I64),
     "ccd_ctcd_primitives": ([_V, _INT, _V, _V, _D, _I64, _V, _V], _INT),
     "ccd_ctcd_narrowphase": ([_V, _V, _V, _I64, _V, _V, _I64, _V, _V, _I64, _D, _V, _V, _V, _V, _V, _V], _INT),
+    "ccd_separating_plane_narrowphase": ([_V, _V, _V, _V, _I64, _I64, _V, _V, _I64, _V, _V, _I64, _D, _V, _V, _P64],
+                                         _INT),
     "ccd_ctcd_narrowphase_history": ([_V, _V, _V, _V, _I64, _I64, _V, _V, _I64, _V, _V, _I64, _D, _V, _V, _V, _V,
                                       _V, _V, _V, _V], _INT),
     "ccd_kdop_candidates": ([_V, _V, _I64, _V, _V, _I64, _D, _V, _V, _I64, _P64, _V, _I64, _P64, _INT], _INT),

@@ -214,6 +214,34 @@ def narrowphase_history(hist_off, hist_t, hist_pos, vf_keys, ee_keys, eta, ctx: 
     return out
 
 
+def separating_plane_narrowphase(hist_off, hist_t, hist_pos, vf_keys, ee_keys, eta, vf_eta=None, ee_eta=None,
+                                  ctx: Context | None = None):
+    """SeparatingPlaneNarrowPhase over stencil lists for a CSR history (SeparatingPlaneNarrowPhase.cpp:9-278).
+
+    Returns dict(vf_status, ee_status, undecided): per-stencil status 1 (collision), 0 (none) or 2
+    (undecided: the reference's recursion would not end there)."""
+    torch = _torch()
+    ctx = ctx or default_context()
+    ctx.bind_stream()
+    off = to_device(hist_off, torch.int64, ctx)
+    ht = to_device(hist_t, torch.float64, ctx)
+    hp = to_device(hist_pos, torch.float64, ctx).reshape(-1)
+    V, E = off.numel() - 1, ht.numel()
+    vk = to_device(vf_keys, torch.int32, ctx).reshape(-1)
+    ek = to_device(ee_keys, torch.int32, ctx).reshape(-1)
+    nvf, nee = vk.numel() // 4, ek.numel() // 4
+    ve = to_device(vf_eta, torch.float64, ctx) if vf_eta is not None else None
+    ee = to_device(ee_eta, torch.float64, ctx) if ee_eta is not None else None
+    dev = ctx.torch_device
+    vs = torch.empty(nvf, dtype=torch.uint8, device=dev)
+    es = torch.empty(nee, dtype=torch.uint8, device=dev)
+    und = ctypes.c_int64(0)
+    ctx.check(N.lib().ccd_separating_plane_narrowphase(
+        ctx.ptr, _ptr(off), _ptr(ht), _ptr(hp), V, E, _ptr(vk), _ptr(ve), nvf, _ptr(ek), _ptr(ee), nee, float(eta),
+        _ptr(vs), _ptr(es), ctypes.byref(und)))
+    return {"vf_status": vs.cpu().numpy(), "ee_status": es.cpu().numpy(), "undecided": und.value}
+
+
 def _fixed_mask(fixed, V, ctx):
     torch = _torch()
     if fixed is None:
@@ -420,6 +448,38 @@ class CTCDNarrowPhase:
                 vfs.add(tuple(s))
         for (s, _), hit in zip(ce, r["ee_hit"]):
             if hit:
+                ees.add(tuple(s))
+
+
+class SeparatingPlaneNarrowPhase:
+    """SeparatingPlaneNarrowPhase (src/SeparatingPlaneNarrowPhase.h:7-22): findCollisions with the same
+    arguments as CTCDNarrowPhase's; colliding stencils are inserted into vfs / ees (not cleared).  A
+    stencil on which the reference's recursion would not end raises RuntimeError (the reference
+    overflows its stack there)."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx
+
+    def findCollisions(self, h: History, candidateVFS, candidateEES, vfs: set, ees: set):
+        cv = sorted(candidateVFS)
+        ce = sorted(candidateEES)
+        vk = np.array([s for s, _ in cv], dtype=np.int32).reshape(-1, 4)
+        ve = np.array([e for _, e in cv], dtype=np.float64)
+        ek = np.array([s for s, _ in ce], dtype=np.int32).reshape(-1, 4)
+        eet = np.array([e for _, e in ce], dtype=np.float64)
+        off, ht, hp = h.csr()
+        r = separating_plane_narrowphase(off, ht, hp, vk, ek, 0.0, vf_eta=ve if len(ve) else None,
+                                         ee_eta=eet if len(eet) else None, ctx=self.ctx)
+        if r["undecided"]:
+            bad = [tuple(s) for (s, _), st in zip(cv, r["vf_status"]) if st == 2] + \
+                  [tuple(s) for (s, _), st in zip(ce, r["ee_status"]) if st == 2]
+            raise RuntimeError(f"SeparatingPlaneNarrowPhase: the recursion does not terminate for {len(bad)} "
+                               f"stencil(s), e.g. {bad[:3]}")
+        for (s, _), st in zip(cv, r["vf_status"]):
+            if st == 1:
+                vfs.add(tuple(s))
+        for (s, _), st in zip(ce, r["ee_status"]):
+            if st == 1:
                 ees.add(tuple(s))
 
 

@@ -272,6 +272,45 @@ int ccd_ctcd_narrowphase_history(ccd_ctx *ctx, const int64_t *hist_off, const do
                               ee_sub, ee_win);
 }
 
+int ccd_separating_plane_narrowphase(ccd_ctx *ctx, const int64_t *hist_off, const double *hist_t,
+                                     const double *hist_pos, int64_t nverts, int64_t nentries, const int32_t *vf_keys,
+                                     const double *vf_eta, int64_t nvf, const int32_t *ee_keys, const double *ee_eta,
+                                     int64_t nee, double eta_scalar, uint8_t *vf_status, uint8_t *ee_status,
+                                     int64_t *undecided)
+{
+    if (!ctx) return CCD_E_INVALID;
+    CCD_REQUIRE(ctx, nverts >= 0 && nentries >= 0 && nvf >= 0 && nee >= 0, "negative size");
+    CCD_REQUIRE(ctx, (nvf == 0 && nee == 0) || (hist_off && hist_t && hist_pos), "null history");
+    CCD_REQUIRE(ctx, nvf == 0 || (vf_keys && vf_status), "null vertex-face keys or outputs");
+    CCD_REQUIRE(ctx, nee == 0 || (ee_keys && ee_status), "null edge-edge keys or outputs");
+    CCD_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (undecided) *undecided = 0;
+    if (nvf == 0 && nee == 0) return CCD_OK;
+    int rc = validate_history(ctx, hist_off, hist_t, nverts, nentries);
+    if (rc) return rc;
+    rc = validate_keys(ctx, vf_keys, nvf, nverts, "vertex-face keys reference vertices outside [0, nverts)");
+    if (rc) return rc;
+    rc = validate_keys(ctx, ee_keys, nee, nverts, "edge-edge keys reference vertices outside [0, nverts)");
+    if (rc) return rc;
+    uint64_t *dev = (uint64_t *)buffer(ctx, "sp_scalars", 16);   // minimum gap bits, undecided count
+    if (!dev) return CCD_E_OOM;
+    rc = fill_u64(ctx, dev + 1, 1, 0);
+    if (rc) return rc;
+    rc = launch_min_gap(ctx, hist_off, hist_t, nverts, dev);
+    if (rc) return rc;
+    rc = launch_separating_plane(ctx, true, hist_off, hist_t, hist_pos, vf_keys, vf_eta, eta_scalar, nvf, dev,
+                                 vf_status, (int64_t *)(dev + 1));
+    if (rc) return rc;
+    rc = launch_separating_plane(ctx, false, hist_off, hist_t, hist_pos, ee_keys, ee_eta, eta_scalar, nee, dev,
+                                 ee_status, (int64_t *)(dev + 1));
+    if (rc) return rc;
+    int64_t und = 0;
+    CCD_HIP_CHECK(ctx, hipMemcpyAsync(&und, dev + 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+    CCD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (undecided) *undecided = und;
+    return CCD_OK;
+}
+
 static int candidates_common(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces, const double *qs, const double *qe,
                              const int64_t *hoff, const double *hpos, int64_t nverts, double outer_eta,
                              const uint8_t *fixed, int32_t *vf_out, int64_t vf_cap, int64_t *vf_count,

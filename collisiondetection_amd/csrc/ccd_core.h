@@ -628,10 +628,13 @@ CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3
     Ivls<7> raw;
     distance_poly(p0s - p1s, p0s - q0s, p1s - q1s, vp0 - vp1, vp0 - vq0, vp1 - vq1, minD, raw, cnt);
 
-    // Parallel-edge classification of each raw interval at its midpoint (CTCD.cpp:294-308).
-    Ivls<7> coplane, parallel;
-    coplane.n = 0;
-    parallel.n = 0;
+    // Parallel-edge classification of each raw interval at its midpoint (CTCD.cpp:294-308).  The
+    // reference copies the raw intervals into a coplanar and a parallel list; here they stay in `raw`
+    // and bit i of `par` marks interval i parallel.  Both lists keep the raw order, which is all the
+    // combination step below depends on, and two fewer 7-interval lists stay live across the four
+    // barycentric solves.
+    unsigned par = 0;
+    int ncop = 0;
 #pragma unroll
     for (int i = 0; i < 7; i++) {
         if (i < raw.n) {
@@ -640,12 +643,12 @@ CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3
             V3 a = (q0s - p0s) + midt * (vq0 - vp0);
             V3 b = (q1s - p1s) + midt * (vq1 - vp1);
             if (norm(cross(a, b)) < 1e-8)
-                append_ti(parallel, raw.l[i], raw.u[i]);
+                par |= 1u << i;
             else
-                append_ti(coplane, raw.l[i], raw.u[i]);
+                ncop++;
         }
     }
-    if (coplane.n == 0) return false;
+    if (ncop == 0) return false;
 
     Ivls<5> a0, a1, b0, b1;
     V3 x10 = p1s - q1s, v10 = vp1 - vq1;
@@ -668,13 +671,14 @@ CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3
     // pairwise and their intersection touches no parallel interval; t is the smallest lower end.
     bool col = false;
     double mint = 1.0;
-    for (int i = 0; i < coplane.n; i++)
+    for (int i = 0; i < raw.n; i++) {
+        if ((par >> i) & 1u) continue;   // the coplanar intervals, in order
         for (int j = 0; j < a0.n; j++)
             for (int k = 0; k < a1.n; k++)
                 for (int l = 0; l < b0.n; l++)
                     for (int m = 0; m < b1.n; m++) {
-                        double L[5] = {sel(coplane.l, i), sel(a0.l, j), sel(a1.l, k), sel(b0.l, l), sel(b1.l, m)};
-                        double U[5] = {sel(coplane.u, i), sel(a0.u, j), sel(a1.u, k), sel(b0.u, l), sel(b1.u, m)};
+                        double L[5] = {sel(raw.l, i), sel(a0.l, j), sel(a1.l, k), sel(b0.l, l), sel(b1.l, m)};
+                        double U[5] = {sel(raw.u, i), sel(a0.u, j), sel(a1.u, k), sel(b0.u, l), sel(b1.u, m)};
                         bool ov = true;
 #pragma unroll
                         for (int x = 0; x < 5; x++)
@@ -691,12 +695,13 @@ CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3
                         bool skip = false;
 #pragma unroll
                         for (int p = 0; p < 7; p++)
-                            if (p < parallel.n && !(il > parallel.u[p] || parallel.l[p] > iu)) skip = true;
+                            if (((par >> p) & 1u) && !(il > raw.u[p] || raw.l[p] > iu)) skip = true;
                         if (!skip) {
                             mint = smin(mint, il);
                             col = true;
                         }
                     }
+    }
     if (col) t = mint;
     return col;
 }
@@ -1052,5 +1057,201 @@ CCD_HD V3 edge_edge_distance(V3 p0, V3 p1, V3 q0, V3 q1, double &p0b, double &p1
 // (History.cpp:127-133 with dt = 1, a = 0, b = 1): (1*start + 0*end) / 1, kept literally so that
 // signed zeros come out as in the reference.
 CCD_FHD double stitch0(double s, double e) { return (1.0 * s + 0.0 * e) / 1.0; }
+
+// ------------------------------------------------------------------------------------------------
+// Separating-plane narrowphase (SeparatingPlaneNarrowPhase.cpp:9-278) over a CSR history: vertex v
+// owns entries off[v] .. off[v+1]-1 with times t[] (first 0, last 1, nondecreasing) and positions
+// p[3*e ..].  Every expression follows the reference's order (Eigen dot as dot() above).
+// ------------------------------------------------------------------------------------------------
+struct HistCSR {
+    const int64_t *off;
+    const double *t;
+    const double *p;
+};
+CCD_FHD V3 hist_pos(const HistCSR &h, int64_t e) { return mk(h.p[3 * e], h.p[3 * e + 1], h.p[3 * e + 2]); }
+
+// History::getPosAtTime (History.cpp:47-76).  Its bisection only narrows the start of a forward scan
+// that stops at the first entry later than `time`, so the result is that of an upper-bound search:
+// next = first entry with time > `time`; idx = next - 1 (relative to the vertex's first entry), or the
+// last entry when none is later.  Times passed here lie in [0, 1].
+CCD_HD V3 pos_at_time(const HistCSR &h, int v, double time, int &idx)
+{
+    const int64_t b = h.off[v], e = h.off[v + 1];
+    int64_t lo = b, hi = e;
+    while (lo < hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (h.t[mid] <= time)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo == e) {
+        idx = (int)(e - 1 - b);
+        return hist_pos(h, e - 1);
+    }
+    const int64_t prev = lo > b ? lo - 1 : b;
+    idx = (int)(prev - b);
+    double dt = h.t[lo] - h.t[prev];
+    double alpha = (time - h.t[prev]) / dt;
+    V3 a = hist_pos(h, prev), c = hist_pos(h, lo);
+    return (1.0 - alpha) * a + alpha * c;
+}
+
+// planeIntersect (SeparatingPlaneNarrowPhase.cpp:267-278)
+CCD_FHD double plane_intersect(V3 ppos, V3 pvel, V3 nrm, V3 old_p, V3 new_p, double dt, double eta)
+{
+    double numerator = 0.5 * eta - dot(old_p - ppos, nrm);
+    double denom = -dot(pvel, nrm);
+    if (dt != 0.0) denom += dot(new_p - old_p, nrm) / dt;
+    return numerator / denom;
+}
+
+// planeTrajectoryIntersect (SeparatingPlaneNarrowPhase.cpp:216-265); start is relative to the vertex's
+// first entry.  Returns the time from tstart until the vertex trajectory meets the moving plane, or
+// +infinity.
+CCD_HD double plane_trajectory_intersect(const HistCSR &h, int v, int start, bool forward, V3 ppos, V3 pvel, V3 nrm,
+                                         V3 ptstart, double tstart, double eta)
+{
+    const double inf = bits_to_double(0x7ff0000000000000ll);
+    const int64_t b = h.off[v];
+    const int n = (int)(h.off[v + 1] - b);
+    double dt = forward ? h.t[b + start] - tstart : tstart - h.t[b + start];
+    double t = plane_intersect(ppos, (forward ? 1.0 : -1.0) * pvel, nrm, ptstart, hist_pos(h, b + start), dt, eta);
+    if (t >= 0 && t <= dt) return t;
+    if (forward) {
+        for (int i = start; i < n; i++) {
+            if (i == n - 1) return inf;
+            V3 npp = ppos + (h.t[b + i] - tstart) * pvel;
+            double d = h.t[b + i + 1] - h.t[b + i];
+            double u = plane_intersect(npp, pvel, nrm, hist_pos(h, b + i), hist_pos(h, b + i + 1), d, eta);
+            if (u >= 0 && u <= d) return h.t[b + i] + u - tstart;
+        }
+    } else {
+        for (int i = start; i >= 0; i--) {
+            if (i == 0) return inf;
+            V3 npp = ppos - (tstart - h.t[b + i]) * pvel;
+            double d = h.t[b + i] - h.t[b + i - 1];
+            double u = plane_intersect(npp, -1.0 * pvel, nrm, hist_pos(h, b + i), hist_pos(h, b + i - 1), d, eta);
+            if (u >= 0 && u <= d) return tstart - (h.t[b + i] - u);
+        }
+    }
+    return inf;
+}
+
+// checkInterval (SeparatingPlaneNarrowPhase.cpp:63-214) for one stencil (v = the key's vertices), from
+// [0, 1].  The reference recurses into the lower sub-interval, then the upper; here a stack of pending
+// intervals visits them in the same depth-first order (each node's upper bound is computed before its
+// lower subtree is explored; it depends only on the node).  Returns 1 on collision, 0 if none, and 2
+// (undecided) where the reference's recursion would not end: a sub-interval equal to its parent (the
+// recursion is a pure function of the interval) or one more than SP_MAX_DEPTH levels down (each level
+// at most halves the interval; only an interval closing in on a history entry time gets that deep).
+constexpr int SP_MAX_DEPTH = 160;
+template <bool VF, class C = NoCount>
+CCD_HD int separating_plane_check(const HistCSR &h, const int (&v)[4], double eta, double eps, C &&cnt = C())
+{
+    const double inf = bits_to_double(0x7ff0000000000000ll);
+    // pending intervals and their depths; a node at depth d leaves at most d pending uppers below it
+    double st_lo[SP_MAX_DEPTH + 2], st_hi[SP_MAX_DEPTH + 2];
+    int st_d[SP_MAX_DEPTH + 2];
+    st_lo[0] = 0.0;
+    st_hi[0] = 1.0;
+    st_d[0] = 0;
+    int top = 1;
+    while (top > 0) {
+        top--;
+        const double mint = st_lo[top], maxt = st_hi[top];
+        const int depth = st_d[top];
+        if (depth > SP_MAX_DEPTH) return 2;
+        if (maxt < mint) continue;
+        if (maxt - mint < 3 * eps) {
+            V3 s[4], e[4];
+            bool ok = true;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                int io = 0, in = 0;
+                s[j] = pos_at_time(h, v[j], mint, io);
+                e[j] = pos_at_time(h, v[j], maxt, in);
+                if (io != in) ok = false;
+            }
+            if (ok) {
+                double t = 0.0;
+                int r = VF ? check_vf_window(s, e, eta, t, cnt) : check_ee_window(s, e, eta, t, cnt);
+                if (r >= 0) return 1;
+                continue;
+            }
+        }
+        const double midt = 0.5 * (mint + maxt);
+        V3 mp[4];
+        int mi[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) mp[j] = pos_at_time(h, v[j], midt, mi[j]);
+        double b1 = 0, b2 = 0, b3 = 0, b4 = 0;
+        V3 closest = VF ? vertex_face_distance(mp[0], mp[1], mp[2], mp[3], b1, b2, b3)
+                        : edge_edge_distance(mp[0], mp[1], mp[2], mp[3], b1, b2, b3, b4);
+        if (dot(closest, closest) < eta * eta) return 1;
+        // The separating plane: through the midpoint of the closest points, moving with their mean
+        // velocity over the current history windows (:162-185).
+        V3 dv[4];
+        double dts[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int64_t k = h.off[v[j]] + mi[j];
+            V3 d = hist_pos(h, k + 1) - hist_pos(h, k);
+            dts[j] = h.t[k + 1] - h.t[k];
+            dv[j] = d;
+        }
+        V3 ppos, pvel;
+        if (VF) {
+            ppos = 0.5 * (((mp[0] + b1 * mp[1]) + b2 * mp[2]) + b3 * mp[3]);
+            V3 w0 = mk(dv[0].x / dts[0], dv[0].y / dts[0], dv[0].z / dts[0]);
+            V3 w1 = b1 * dv[1], w2 = b2 * dv[2], w3 = b3 * dv[3];
+            w1 = mk(w1.x / dts[1], w1.y / dts[1], w1.z / dts[1]);
+            w2 = mk(w2.x / dts[2], w2.y / dts[2], w2.z / dts[2]);
+            w3 = mk(w3.x / dts[3], w3.y / dts[3], w3.z / dts[3]);
+            pvel = 0.5 * (((w0 + w1) + w2) + w3);
+        } else {
+            ppos = 0.5 * (((b1 * mp[0] + b2 * mp[1]) + b3 * mp[2]) + b4 * mp[3]);
+            V3 w0 = b1 * dv[0], w1 = b2 * dv[1], w2 = b3 * dv[2], w3 = b4 * dv[3];
+            w0 = mk(w0.x / dts[0], w0.y / dts[0], w0.z / dts[0]);
+            w1 = mk(w1.x / dts[1], w1.y / dts[1], w1.z / dts[1]);
+            w2 = mk(w2.x / dts[2], w2.y / dts[2], w2.z / dts[2]);
+            w3 = mk(w3.x / dts[3], w3.y / dts[3], w3.z / dts[3]);
+            pvel = 0.5 * (((w0 + w1) + w2) + w3);
+        }
+        const double cn = norm(closest);
+        closest = mk(closest.x / cn, closest.y / cn, closest.z / cn);
+        double t_lo = inf, t_hi = inf;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            double sign = (j == 0 || (j == 1 && !VF)) ? -1.0 : 1.0;
+            V3 nrm = sign * closest;
+            t_lo = smin(t_lo, plane_trajectory_intersect(h, v[j], mi[j], false, ppos, pvel, nrm, mp[j], midt, eta));
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            double sign = (j == 0 || (j == 1 && !VF)) ? -1.0 : 1.0;
+            V3 nrm = sign * closest;
+            t_hi = smin(t_hi, plane_trajectory_intersect(h, v[j], mi[j] + 1, true, ppos, pvel, nrm, mp[j], midt, eta));
+        }
+        // Depth first, lower sub-interval first: push the upper one underneath.  A child equal to its
+        // node is undecided when it is reached (the upper one only if the lower subtree finds nothing).
+        if (t_hi < 1.0) {
+            const double lo = midt + smax(0.0, t_hi - eps);
+            st_lo[top] = lo;
+            st_hi[top] = maxt;
+            st_d[top] = lo == mint ? SP_MAX_DEPTH + 1 : depth + 1;   // equal to this node: undecided if reached
+            top++;
+        }
+        if (t_lo < 1.0) {
+            const double hi = midt - smax(0.0, t_lo - eps);
+            if (hi == maxt) return 2;
+            st_lo[top] = mint;
+            st_hi[top] = hi;
+            st_d[top] = depth + 1;
+            top++;
+        }
+    }
+    return 0;
+}
 
 }   // namespace ccd

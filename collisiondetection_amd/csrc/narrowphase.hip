@@ -322,6 +322,43 @@ __global__ void k_validate_history(const int64_t *__restrict__ off, const double
     if (m && __lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(bad, (unsigned long long)__popcll(m));
 }
 
+// History::computeMinimumGap (History.cpp:78-93): min(1, every gap between consecutive entry times),
+// reduced with an atomic min on the bit pattern (the gaps of a validated history are non-negative, so
+// their bits order as the values do).  *gap_bits must hold the bits of 1.0 on entry.
+__global__ __launch_bounds__(256) void k_min_gap(const int64_t *__restrict__ off, const double *__restrict__ ht,
+                                                 int64_t nverts, unsigned long long *gap_bits)
+{
+    double g = 1.0;
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nverts;
+         v += (int64_t)gridDim.x * blockDim.x)
+        for (int64_t j = off[v] + 1; j < off[v + 1]; j++) g = ccd::smin(ht[j] - ht[j - 1], g);
+    for (int o = 32; o > 0; o >>= 1) g = ccd::smin(__shfl_xor(g, o), g);
+    if (__lane_id() == 0) atomicMin(gap_bits, (unsigned long long)ccd::double_to_bits(g));
+}
+__global__ void k_set_bits(unsigned long long *p, unsigned long long v) { *p = v; }
+
+// SeparatingPlaneNarrowPhase (SeparatingPlaneNarrowPhase.cpp:9-278): one lane per stencil, status 1
+// (collision), 0 (none) or 2 (undecided: the reference's recursion would not end; counted in *undecided).
+template <bool VF, class M>
+__global__ __launch_bounds__(256) CCD_TWO_WAVES void k_separating_plane(
+    const int64_t *__restrict__ off, const double *__restrict__ ht, const double *__restrict__ hp,
+    const int4 *__restrict__ keys, const double *__restrict__ eta, double eta_s, int64_t n,
+    const unsigned long long *__restrict__ gap_bits, uint8_t *__restrict__ status, unsigned long long *undecided)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int r = 0;
+    if (i < n) {
+        const double eps = ccd::bits_to_double((long long)*gap_bits) / 4.0;
+        int4 k = keys[i];
+        int v[4] = {k.x, k.y, k.z, k.w};
+        ccd::HistCSR h{off, ht, hp};
+        r = ccd::separating_plane_check<VF>(h, v, eta ? eta[i] : eta_s, eps, M());
+        status[i] = (uint8_t)r;
+    }
+    uint64_t m = __ballot(r == 2);
+    if (m && __lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(undecided, (unsigned long long)__popcll(m));
+}
+
 inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }   // namespace
@@ -454,6 +491,38 @@ int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *
         CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow_hist<false, M>), dim3(blocks_for(n, 256)), dim3(256), 0,
                                                   ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, hit, t,
                                                   sub, win));
+    CCD_LAUNCH_CHECK(ctx);
+    return CCD_OK;
+}
+
+int launch_min_gap(ccd_ctx *ctx, const int64_t *off, const double *ht, int64_t nverts, uint64_t *gap_bits)
+{
+    hipLaunchKernelGGL(k_set_bits, dim3(1), dim3(1), 0, ctx->stream, (unsigned long long *)gap_bits,
+                       (unsigned long long)ccd::double_to_bits(1.0));
+    CCD_LAUNCH_CHECK(ctx);
+    if (nverts == 0) return CCD_OK;
+    unsigned grid = (unsigned)std::min<int64_t>(blocks_for(nverts, 256), 4096);
+    hipLaunchKernelGGL(k_min_gap, dim3(grid), dim3(256), 0, ctx->stream, off, ht, nverts,
+                       (unsigned long long *)gap_bits);
+    CCD_LAUNCH_CHECK(ctx);
+    return CCD_OK;
+}
+
+int launch_separating_plane(ccd_ctx *ctx, bool vf, const int64_t *off, const double *ht, const double *hp,
+                            const int32_t *keys, const double *eta, double eta_s, int64_t n, const uint64_t *gap_bits,
+                            uint8_t *status, int64_t *undecided)
+{
+    if (n == 0) return CCD_OK;
+    auto gb = (const unsigned long long *)gap_bits;
+    auto und = (unsigned long long *)undecided;
+    if (vf)
+        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_separating_plane<true, M>), dim3(blocks_for(n, 256)), dim3(256),
+                                                  0, ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, gb,
+                                                  status, und));
+    else
+        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_separating_plane<false, M>), dim3(blocks_for(n, 256)), dim3(256),
+                                                  0, ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, gb,
+                                                  status, und));
     CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
 }

@@ -136,6 +136,23 @@ int ccd_kdop_candidates(ccd_ctx *ctx, const int32_t *faces, int64_t nfaces, cons
                         int64_t *vf_count, int32_t *ee_out, int64_t ee_cap, int64_t *ee_count, int flags);
 
 /*
+ * SeparatingPlaneNarrowPhase::findCollisions (SeparatingPlaneNarrowPhase.h:7-22,
+ * SeparatingPlaneNarrowPhase.cpp:9-278): the narrowphase that VelocityFilter uses.  Per stencil,
+ * recursive time bisection with moving separating planes over the history (layout as in
+ * ccd_ctcd_narrowphase_history), eps = History::computeMinimumGap / 4, and the CTCD primitives on
+ * windows narrower than 3 eps.  Writes a status per stencil (device, uint8): 1 collision, 0 none, 2
+ * undecided -- the reference's recursion would not end on this stencil (a sub-interval equal to its
+ * parent; the reference recurses until its stack overflows) or would go more than 160 levels deep.
+ * *undecided (host, may be NULL) receives the number of undecided stencils.  Root mode as for the
+ * other narrowphases.  Synchronous.
+ */
+int ccd_separating_plane_narrowphase(ccd_ctx *ctx, const int64_t *hist_off, const double *hist_t,
+                                     const double *hist_pos, int64_t nverts, int64_t nentries, const int32_t *vf_keys,
+                                     const double *vf_eta, int64_t nvf, const int32_t *ee_keys, const double *ee_eta,
+                                     int64_t nee, double eta_scalar, uint8_t *vf_status, uint8_t *ee_status,
+                                     int64_t *undecided);
+
+/*
  * Broadphase over multi-entry histories: as ccd_kdop_candidates, the leaf bounds covering every history
  * entry of the face's vertices (KDOPBroadPhase.cpp:60-70).  History layout as in
  * ccd_ctcd_narrowphase_history.

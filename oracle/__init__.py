@@ -82,6 +82,10 @@ def lib():
         L.orc_distance.restype = ctypes.c_int
         L.orc_mesh_self_distance.argtypes = [_dp, ctypes.c_int64, _ip, ctypes.c_int64, _up, _lp, _lp]
         L.orc_mesh_self_distance.restype = ctypes.c_double
+        L.orc_minimum_gap.argtypes = [_lp, _dp, _dp, ctypes.c_int64]
+        L.orc_minimum_gap.restype = ctypes.c_double
+        L.orc_separating_plane.argtypes = [_lp, _dp, _dp, ctypes.c_int64, _ip, _dp, ctypes.c_int64, ctypes.c_int,
+                                           _ip, _dp, _ip]
         assert L.orc_stage_count() == len(STAGES)
         _lib = L
     return _lib
@@ -335,6 +339,27 @@ def narrowphase(hist: Hist, keys, eta, is_ee: bool, with_counts=False):
     if with_counts:
         return rec, dict(zip(STAGES, counts.tolist()))
     return rec
+
+
+def minimum_gap(hist: Hist) -> float:
+    """History::computeMinimumGap (History.cpp:78-93)."""
+    return lib().orc_minimum_gap(_l(hist.off), _d(hist.ht), _d(hist.hp), hist.nverts)
+
+
+def separating_plane(hist: Hist, keys, eta, is_ee: bool):
+    """SeparatingPlaneNarrowPhase per stencil (SeparatingPlaneNarrowPhase.cpp:9-278), recursive as the
+    reference is -> (status array: 1 collision, 0 none, 2 undecided -- the reference's recursion would not
+    terminate; decision margin of the primitive tests; deepest recursion level)."""
+    keys = np.ascontiguousarray(keys, dtype=np.int32).reshape(-1, 4)
+    n = keys.shape[0]
+    eta_arr = np.ascontiguousarray(np.broadcast_to(np.asarray(eta, dtype=np.float64), (n,)))
+    hit = np.zeros(n, dtype=np.int32)
+    margin = np.zeros(n, dtype=np.float64)
+    depth = np.zeros(n, dtype=np.int32)
+    if n:
+        lib().orc_separating_plane(_l(hist.off), _d(hist.ht), _d(hist.hp), hist.nverts, _i(keys), _d(eta_arr), n,
+                                   int(is_ee), _i(hit), _d(margin), _i(depth))
+    return hit, margin, depth
 
 
 # ------------------------------------------------------------------------------------------------

@@ -1329,6 +1329,210 @@ double mesh_self_distance(const double *verts, int64_t V, const int32_t *faces, 
 }
 
 // Result holder handed across the C boundary.
+// ----------------------------------------------------------------------------------------------
+// SeparatingPlaneNarrowPhase: src/SeparatingPlaneNarrowPhase.cpp:9-278, History.cpp:42-93.
+// Written in the reference's own shape (recursive bisection, the History lookups as the reference
+// performs them); the device version is an explicit-stack restatement, which these tests check.
+// ----------------------------------------------------------------------------------------------
+// History::getPosAtTime (History.cpp:42-76) on vertex v: idx is relative to the vertex's first entry.
+void get_pos_at_time(const Hist &h, int v, double time, V3 &pos, int &idx, int64_t start, int64_t end)
+{
+    const int64_t b = h.off[v], size = h.off[v + 1] - b;
+    if (end - start < 10) {
+        int64_t next = start;
+        while (next < size && h.ht[b + next] <= time) next++;
+        int64_t prev = next - 1;
+        if (next == size) {
+            pos = load3(h.hp + 3 * (b + prev));
+            idx = (int)prev;
+            return;
+        }
+        double dt = h.ht[b + next] - h.ht[b + prev];
+        double alpha = (time - h.ht[b + prev]) / dt;
+        pos = (1.0 - alpha) * load3(h.hp + 3 * (b + prev)) + alpha * load3(h.hp + 3 * (b + next));
+        idx = (int)prev;
+        return;
+    }
+    int64_t mid = (start + end) / 2;
+    if (h.ht[b + mid] < time)
+        return get_pos_at_time(h, v, time, pos, idx, mid, end);
+    return get_pos_at_time(h, v, time, pos, idx, start, mid);
+}
+void get_pos_at_time(const Hist &h, int v, double time, V3 &pos, int &idx)
+{
+    get_pos_at_time(h, v, time, pos, idx, 0, h.off[v + 1] - h.off[v] - 1);
+}
+
+// History::computeMinimumGap (History.cpp:78-93)
+double compute_minimum_gap(const Hist &h, int64_t nverts)
+{
+    double result = 1.0;
+    for (int64_t i = 0; i < nverts; i++)
+        for (int64_t j = h.off[i] + 1; j < h.off[i + 1]; j++) result = std::min(h.ht[j] - h.ht[j - 1], result);
+    return result;
+}
+
+double plane_intersect(V3 planePos, V3 planeVel, V3 planeNormal, V3 ptOld, V3 ptNew, double ptdt, double eta)
+{   // :267-278
+    double numerator = 0.5 * eta - dot(ptOld - planePos, planeNormal);
+    double denom = -dot(planeVel, planeNormal);
+    if (ptdt != 0.0) denom += dot(ptNew - ptOld, planeNormal) / ptdt;
+    return numerator / denom;
+}
+
+double plane_trajectory_intersect(const Hist &h, int v, int startidx, bool forward, V3 planePos, V3 planeVel,
+                                  V3 planeNormal, V3 ptstart, double timestart, double eta)
+{   // :216-265
+    const int64_t b = h.off[v];
+    const int size = (int)(h.off[v + 1] - b);
+    auto T = [&](int i) { return h.ht[b + i]; };
+    auto P = [&](int i) { return load3(h.hp + 3 * (b + i)); };
+    double dt = (forward ? T(startidx) - timestart : timestart - T(startidx));
+    V3 newpos = P(startidx);
+    double t = plane_intersect(planePos, (forward ? 1.0 : -1.0) * planeVel, planeNormal, ptstart, newpos, dt, eta);
+    if (t >= 0 && t <= dt) return t;
+    if (forward) {
+        for (int i = startidx; i < size; i++) {
+            if (i == size - 1) return std::numeric_limits<double>::infinity();
+            int next = i + 1;
+            V3 newplanepos = planePos + (T(i) - timestart) * planeVel;
+            double d = T(next) - T(i);
+            double u = plane_intersect(newplanepos, planeVel, planeNormal, P(i), P(next), d, eta);
+            if (u >= 0 && u <= d) return T(i) + u - timestart;
+        }
+    } else {
+        for (int i = startidx; i >= 0; i--) {
+            if (i == 0) return std::numeric_limits<double>::infinity();
+            int prev = i - 1;
+            V3 newplanepos = planePos - (timestart - T(i)) * planeVel;
+            double d = T(i) - T(prev);
+            double u = plane_intersect(newplanepos, -1.0 * planeVel, planeNormal, P(i), P(prev), d, eta);
+            if (u >= 0 && u <= d) return timestart - (T(i) - u);
+        }
+    }
+    return std::numeric_limits<double>::infinity();
+}
+
+bool sp_leaf(bool vf, const V3 *o, const V3 *n, double eta, Ctx *ctx)
+{   // :76-140: the primitive tests in the reference's order on one window
+    double t;
+    if (vf) {
+        if (vertex_face_ctcd(o[0], o[1], o[2], o[3], n[0], n[1], n[2], n[3], eta, t, ctx)) return true;
+        for (int edge = 0; edge < 3; edge++) {
+            int a = 1 + (edge % 3), c = 1 + ((edge + 1) % 3);
+            if (vertex_edge_ctcd(o[0], o[a], o[c], n[0], n[a], n[c], eta, t, ctx)) return true;
+        }
+        for (int vert = 0; vert < 3; vert++)
+            if (vertex_vertex_ctcd(o[0], o[1 + vert], n[0], n[1 + vert], eta, t, ctx)) return true;
+        return false;
+    }
+    if (edge_edge_ctcd(o[0], o[1], o[2], o[3], n[0], n[1], n[2], n[3], eta, t, ctx)) return true;
+    static const int ve_idx[4][3] = {{0, 2, 3}, {1, 2, 3}, {2, 0, 1}, {3, 0, 1}};
+    static const int vv_idx[4][2] = {{0, 2}, {0, 3}, {1, 2}, {1, 3}};
+    for (int k = 0; k < 4; k++) {
+        const int *q = ve_idx[k];
+        if (vertex_edge_ctcd(o[q[0]], o[q[1]], o[q[2]], n[q[0]], n[q[1]], n[q[2]], eta, t, ctx)) return true;
+    }
+    for (int k = 0; k < 4; k++) {
+        const int *q = vv_idx[k];
+        if (vertex_vertex_ctcd(o[q[0]], o[q[1]], n[q[0]], n[q[1]], eta, t, ctx)) return true;
+    }
+    return false;
+}
+
+// A call whose interval equals its caller's never returns in the reference (checkInterval is a pure
+// function of the interval): such a stencil is reported undecided (*undecided = 1) rather than
+// crashing the process as the reference would.  So is one whose recursion goes deeper than
+// SP_MAX_DEPTH levels (the product's bound; each level at most halves the interval, so only an
+// interval that closes in on a history entry time gets that deep).
+constexpr int SP_MAX_DEPTH = 160;
+bool sp_check_interval(bool vf, const Hist &h, const int *verts, double eta, double mint, double maxt, double eps,
+                       Ctx *ctx, int depth, int &maxdepth, int *undecided)
+{   // :63-214
+    if (depth > maxdepth) maxdepth = depth;
+    if (depth > SP_MAX_DEPTH) {
+        *undecided = 1;
+        return false;
+    }
+    if (maxt < mint) return false;
+    if (maxt - mint < 3 * eps) {
+        bool ok = true;
+        V3 oldpos[4], newpos[4];
+        for (int i = 0; i < 4; i++) {
+            int oldidx, newidx;
+            get_pos_at_time(h, verts[i], mint, oldpos[i], oldidx);
+            get_pos_at_time(h, verts[i], maxt, newpos[i], newidx);
+            if (oldidx != newidx) {
+                ok = false;
+                break;
+            }
+        }
+        if (ok) return sp_leaf(vf, oldpos, newpos, eta, ctx);
+    }
+    double midt = 0.5 * (mint + maxt);
+    V3 midpos[4];
+    int mididx[4];
+    for (int i = 0; i < 4; i++) get_pos_at_time(h, verts[i], midt, midpos[i], mididx[i]);
+    double bary[4] = {0, 0, 0, 0};
+    V3 closest = vf ? vertex_face_distance(midpos[0], midpos[1], midpos[2], midpos[3], bary)
+                    : edge_edge_distance(midpos[0], midpos[1], midpos[2], midpos[3], bary);
+    double distsq = dot(closest, closest);
+    if (distsq < eta * eta) return true;
+    V3 prevpos[4], nextpos[4];
+    double dts[4];
+    for (int i = 0; i < 4; i++) {
+        const int64_t k = h.off[verts[i]] + mididx[i];
+        prevpos[i] = load3(h.hp + 3 * k);
+        nextpos[i] = load3(h.hp + 3 * (k + 1));
+        dts[i] = h.ht[k + 1] - h.ht[k];
+    }
+    auto div = [](V3 a, double s) { return V3{a.x / s, a.y / s, a.z / s}; };
+    V3 planepos, planevel;
+    if (vf) {
+        planepos = 0.5 * (((midpos[0] + bary[0] * midpos[1]) + bary[1] * midpos[2]) + bary[2] * midpos[3]);
+        planevel = 0.5 * (((div(nextpos[0] - prevpos[0], dts[0]) + div(bary[0] * (nextpos[1] - prevpos[1]), dts[1])) +
+                           div(bary[1] * (nextpos[2] - prevpos[2]), dts[2])) +
+                          div(bary[2] * (nextpos[3] - prevpos[3]), dts[3]));
+    } else {
+        planepos = 0.5 * (((bary[0] * midpos[0] + bary[1] * midpos[1]) + bary[2] * midpos[2]) + bary[3] * midpos[3]);
+        planevel = 0.5 * (((div(bary[0] * (nextpos[0] - prevpos[0]), dts[0]) + div(bary[1] * (nextpos[1] - prevpos[1]), dts[1])) +
+                           div(bary[2] * (nextpos[2] - prevpos[2]), dts[2])) +
+                          div(bary[3] * (nextpos[3] - prevpos[3]), dts[3]));
+    }
+    closest = div(closest, norm(closest));
+    double t = std::numeric_limits<double>::infinity();
+    for (int vert = 0; vert < 4; vert++) {
+        double sign = (vert == 0 || (vert == 1 && !vf)) ? -1.0 : 1.0;
+        t = std::min(t, plane_trajectory_intersect(h, verts[vert], mididx[vert], false, planepos, planevel,
+                                                   sign * closest, midpos[vert], midt, eta));
+    }
+    if (t < 1.0) {
+        double newmaxt = midt - std::max(0.0, t - eps);
+        if (newmaxt == maxt) {
+            *undecided = 1;
+            return false;
+        }
+        if (sp_check_interval(vf, h, verts, eta, mint, newmaxt, eps, ctx, depth + 1, maxdepth, undecided)) return true;
+        if (*undecided) return false;
+    }
+    t = std::numeric_limits<double>::infinity();
+    for (int vert = 0; vert < 4; vert++) {
+        double sign = (vert == 0 || (vert == 1 && !vf)) ? -1.0 : 1.0;
+        t = std::min(t, plane_trajectory_intersect(h, verts[vert], mididx[vert] + 1, true, planepos, planevel,
+                                                   sign * closest, midpos[vert], midt, eta));
+    }
+    if (t < 1.0) {
+        double newmint = midt + std::max(0.0, t - eps);
+        if (newmint == mint) {
+            *undecided = 1;
+            return false;
+        }
+        if (sp_check_interval(vf, h, verts, eta, newmint, maxt, eps, ctx, depth + 1, maxdepth, undecided)) return true;
+        if (*undecided) return false;
+    }
+    return false;
+}
+
 struct Result {
     CandSets cs;
     std::vector<int64_t> pairs;
@@ -1503,6 +1707,30 @@ void orc_narrowphase(const int64_t *off, const double *ht, const double *hp, con
         R[i] = res;
         if (counts)
             for (int k = 0; k < ST_COUNT; k++) counts[k] += ctx.counts[k];
+    }
+}
+
+// SeparatingPlaneNarrowPhase::findCollisions (SeparatingPlaneNarrowPhase.cpp:9-24) per stencil: hit
+// 1 / 0, or 2 where the reference's recursion would not terminate; the decision margin of the
+// primitive tests run; the deepest recursion level reached.
+double orc_minimum_gap(const int64_t *off, const double *ht, const double *hp, int64_t nverts)
+{
+    return compute_minimum_gap(Hist{off, ht, hp}, nverts);
+}
+
+void orc_separating_plane(const int64_t *off, const double *ht, const double *hp, int64_t nverts, const int32_t *keys,
+                          const double *eta, int64_t n, int is_ee, int32_t *hit, double *margin, int32_t *depth)
+{
+    Hist h{off, ht, hp};
+    double eps = compute_minimum_gap(h, nverts) / 4.0;
+    for (int64_t i = 0; i < n; i++) {
+        Ctx ctx;
+        int verts[4] = {keys[4 * i], keys[4 * i + 1], keys[4 * i + 2], keys[4 * i + 3]};
+        int md = 0, und = 0;
+        bool r = sp_check_interval(!is_ee, h, verts, eta[i], 0.0, 1.0, eps, &ctx, 0, md, &und);
+        hit[i] = und ? 2 : (r ? 1 : 0);
+        if (margin) margin[i] = ctx.margin;
+        if (depth) depth[i] = md;
     }
 }
 

@@ -21,6 +21,31 @@ def core_narrowphase(H, qs, qe, keys, is_ee, eta):
     return hit.astype(bool), t, sub
 
 
+def core_separating_plane(H, hist, keys, is_ee, eta):
+    """The host-built core's separating-plane narrowphase: statuses 1 / 0 / 2 (undecided)."""
+    keys = np.ascontiguousarray(keys, np.int32).reshape(-1, 4)
+    n = len(keys)
+    st = np.zeros(n, np.int32)
+    e = np.ascontiguousarray(np.broadcast_to(np.asarray(eta, dtype=np.float64), (n,)))
+    lp = ctypes.POINTER(ctypes.c_int64)
+    H.core_separating_plane(hist.off.ctypes.data_as(lp), hist.ht.ctypes.data_as(dp), hist.hp.ctypes.data_as(dp),
+                            ctypes.c_int64(hist.nverts), keys.ctypes.data_as(ip), e.ctypes.data_as(dp),
+                            ctypes.c_int64(n), int(is_ee), st.ctypes.data_as(ip))
+    return st
+
+
+def three_entry_history(s, seed=9, wobble=0.02):
+    """A bent path through a per-vertex midpoint: entries at 0, tmid ~ U(0.2, 0.8) and 1."""
+    rng = np.random.default_rng(seed)
+    V = len(s["qstart"])
+    qm = 0.5 * (s["qstart"] + s["qend"]) + rng.normal(0, wobble, size=(V, 3))
+    tmid = rng.uniform(0.2, 0.8, size=V)
+    off = np.arange(0, 3 * V + 1, 3, dtype=np.int64)
+    ht = np.stack([np.zeros(V), tmid, np.ones(V)], axis=1).reshape(-1)
+    hp = np.stack([s["qstart"], qm, s["qend"]], axis=1).reshape(-1)
+    return off, ht, hp
+
+
 def core_primitive(H, kind, q, eta):
     q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
     t = ctypes.c_double(0.0)

@@ -147,3 +147,24 @@ extern "C" int core_exp_fast_dd(double y, double *out)
     return k;
 }
 extern "C" double core_fast_eps() { return ccd::jt::FAST_EPS; }
+
+// The separating-plane narrowphase of the core over a CSR history; eps = the minimum entry gap / 4
+// (History::computeMinimumGap).  hit[i] = 1, 0, or 2 (undecided: the reference's recursion would not end).
+extern "C" void core_separating_plane(const int64_t *off, const double *ht, const double *hp, int64_t nverts,
+                                      const int32_t *keys, const double *eta, int64_t n, int is_ee, int32_t *hit)
+{
+    double gap = 1.0;
+    for (int64_t v = 0; v < nverts; v++)
+        for (int64_t j = off[v] + 1; j < off[v + 1]; j++) gap = ccd::smin(ht[j] - ht[j - 1], gap);
+    const double eps = gap / 4.0;
+    ccd::HistCSR h{off, ht, hp};
+    for (int64_t i = 0; i < n; i++) {
+        int v[4] = {keys[4 * i], keys[4 * i + 1], keys[4 * i + 2], keys[4 * i + 3]};
+        if (g_ref_roots)
+            hit[i] = is_ee ? ccd::separating_plane_check<false>(h, v, eta[i], eps, ccd::RefRoots())
+                           : ccd::separating_plane_check<true>(h, v, eta[i], eps, ccd::RefRoots());
+        else
+            hit[i] = is_ee ? ccd::separating_plane_check<false>(h, v, eta[i], eps)
+                           : ccd::separating_plane_check<true>(h, v, eta[i], eps);
+    }
+}

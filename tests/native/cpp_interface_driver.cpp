@@ -1,6 +1,6 @@
 // Test driver for the C++ interface (include/ccdamd/), written the way the reference's AlecTest driver
 // uses the library (example/AlecTest.cpp:82-115): History + finishHistory, KDOPBroadPhase, per-stencil
-// eta, CTCDNarrowPhase, plus the CTCD statics on testCTCD's four cases.
+// eta, CTCDNarrowPhase and SeparatingPlaneNarrowPhase, plus the CTCD statics on testCTCD's four cases.
 // Usage: cpp_interface_driver <faces.i32> <qstart.f64> <qend.f64> <eta> <out.txt>
 #include <cstdio>
 #include <fstream>
@@ -12,6 +12,7 @@
 #include "History.h"
 #include "KDOPBroadPhase.h"
 #include "Mesh.h"
+#include "SeparatingPlaneNarrowPhase.h"
 
 template <class T>
 static std::vector<T> read_all(const char *path)
@@ -60,6 +61,12 @@ int main(int argc, char **argv)
     out << "collisions " << vfs.size() << " " << ees.size() << "\n";
     for (const auto &s : vfs) out << "vf " << s.p << " " << s.q0 << " " << s.q1 << " " << s.q2 << "\n";
     for (const auto &s : ees) out << "ee " << s.p0 << " " << s.p1 << " " << s.q0 << " " << s.q1 << "\n";
+    std::set<VertexFaceStencil> spvfs;
+    std::set<EdgeEdgeStencil> spees;
+    SeparatingPlaneNarrowPhase().findCollisions(h, vfs_eta, ees_eta, spvfs, spees);
+    out << "sp_collisions " << spvfs.size() << " " << spees.size() << "\n";
+    for (const auto &s : spvfs) out << "spvf " << s.p << " " << s.q0 << " " << s.q1 << " " << s.q2 << "\n";
+    for (const auto &s : spees) out << "spee " << s.p0 << " " << s.p1 << " " << s.q0 << " " << s.q1 << "\n";
 
     // example/testCTCD.cpp's four cases (eta = 1e-6)
     double t = -1;

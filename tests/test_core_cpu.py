@@ -253,3 +253,66 @@ def test_core_reference_mode_vs_oracle(core_harness, core_harness_libm, scene):
     finally:
         core_harness.core_set_ref_roots(0)
         oracle.use_default_rootfinder()
+
+
+# ------------------------------------------------------------------------------------------------
+# Separating-plane narrowphase (SeparatingPlaneNarrowPhase.cpp:9-278)
+# ------------------------------------------------------------------------------------------------
+def _sampled_min_distance(q0, q1, key, is_ee, samples):
+    """Smallest distance between the stencil's primitives over `samples` times of straight motion."""
+    best = np.inf
+    for t in np.linspace(0.0, 1.0, samples):
+        P = (1 - t) * q0[key] + t * q1[key]
+        vec, _ = oracle.distance(3 if is_ee else 2, P.reshape(-1))
+        best = min(best, float(np.sqrt(vec @ vec)))
+    return best
+
+
+def test_oracle_separating_plane_pinned_by_sampled_distance():
+    """The oracle's separating-plane restatement against what the method guarantees: a stencil it clears
+    stays at least eta apart (each side keeps 0.5 eta from a moving plane), and one it reports comes
+    within eta.  Checked on sampled minimum distances of straight-line motion (a flipped plane normal,
+    a wrong bisection bound or a dropped sub-interval breaks one side)."""
+    s = scenes.cloth(24, drop=0.05, noise=0.01)
+    eta = 0.01
+    h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+    vf, ee = oracle.candidates(s["faces"], h, eta, method="tree")
+    rng = np.random.default_rng(3)
+    for keys, is_ee in ((vf, False), (ee, True)):
+        st, _, _ = oracle.separating_plane(h, keys, eta, is_ee)
+        assert not (st == 2).any() and (st == 1).sum() >= 10
+        for i in np.where(st == 1)[0][:40]:
+            assert _sampled_min_distance(s["qstart"], s["qend"], keys[i], is_ee, 801) < 1.02 * eta, (is_ee, i)
+        for i in rng.choice(np.where(st == 0)[0], 60, replace=False):
+            assert _sampled_min_distance(s["qstart"], s["qend"], keys[i], is_ee, 201) > 0.98 * eta, (is_ee, i)
+
+
+@pytest.mark.parametrize("hist_kind", ["two_entry", "three_entry"])
+def test_core_separating_plane_equals_oracle(core_harness, hist_kind):
+    """The core's explicit-stack restatement equals the oracle's recursive one, status for status
+    (undecided included), in both root modes."""
+    from tests.helpers import core_separating_plane, three_entry_history
+
+    if hist_kind == "two_entry":
+        s = scenes.cloth(48, drop=0.05, noise=0.01)
+        h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+    else:
+        s = scenes.converging_spheres()
+        h = oracle.Hist(*three_entry_history(s))
+    vf, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
+    for mode in ((0, "aberth"), (1, "rpoly")) if oracle.ref_rpoly_available() else ((0, "aberth"),):
+        core_harness.core_set_ref_roots(mode[0])
+        oracle.use_rootfinder(mode[1])
+        try:
+            for keys, is_ee in ((vf, False), (ee, True)):
+                st, margin, depth = oracle.separating_plane(h, keys, s["eta"], is_ee)
+                got = core_separating_plane(core_harness, h, keys, is_ee, s["eta"])
+                diff = np.where(st != got)[0]
+                # isolating roots vs the oracle's exact (Aberth) roots: only borderline decisions may differ
+                assert all(margin[i] < 1e-12 for i in diff) if mode[0] == 0 else len(diff) == 0, (mode, is_ee, diff[:5])
+                print(hist_kind, mode, is_ee, np.bincount(st, minlength=3), "max depth", depth.max())
+                assert (st == 1).sum() > 0
+        finally:
+            core_harness.core_set_ref_roots(0)
+            oracle.use_default_rootfinder()
+

@@ -46,6 +46,14 @@ def test_cpp_interface_matches_oracle(tmp_path):
         o = {tuple(k) for k, hit in zip(keys.tolist(), rec["hit"]) if hit}
         border = {tuple(k) for k, m in zip(keys.tolist(), rec["margin"]) if m < 1e-12}
         assert got ^ o <= border
+    # SeparatingPlaneNarrowPhase through the C++ class: the oracle's statuses (no undecided stencil here)
+    for pre, keys, is_ee in (("spvf ", vf, False), ("spee ", ee, True)):
+        got = {tuple(int(x) for x in l.split()[1:]) for l in lines if l.startswith(pre)}
+        st, margin, _ = oracle.separating_plane(h, keys, s["eta"], is_ee)
+        assert not (st == 2).any()
+        o = {tuple(k) for k, x in zip(keys.tolist(), st) if x == 1}
+        border = {tuple(k) for k, m in zip(keys.tolist(), margin) if m < 1e-12}
+        assert got ^ o <= border
     prims = {l.split()[0]: (int(l.split()[1]), float(l.split()[2])) for l in lines if "_prim" in l}
     assert prims["vv_prim"] == (1, 0.1666665000090396)
     assert prims["ve_prim"] == (1, 0.24999975000276525)
