@@ -59,6 +59,71 @@ def cpu_baseline(scene, sample_stencils=60000, seed=0):
                        f"{total:.1f} s per step")}
 
 
+def bench_separating_plane(ctx, scene, qs, qe, steps, log, cpu_sample=30000, seed=1):
+    """SeparatingPlaneNarrowPhase (the narrowphase VelocityFilter uses) on the workload's candidate
+    stencils with a two-entry history built on the device, reference-root mode: stencils per second;
+    the oracle's recursive restatement timed on a random sample of the same stencils (1 thread)."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import oracle
+    from collisiondetection_amd import _native as N
+    from collisiondetection_amd import api
+
+    L = N.lib()
+    dev = qs.device
+    V = qs.numel() // 3
+    off = torch.arange(0, 2 * V + 1, 2, dtype=torch.int64, device=dev)
+    ht = torch.tensor([0.0, 1.0], dtype=torch.float64, device=dev).repeat(V)
+    hp = torch.stack([qs.reshape(-1, 3), qe.reshape(-1, 3)], dim=1).reshape(-1).contiguous()
+    vk, ek = api.kdop_candidates(scene["faces"], scene["qstart"], scene["qend"], scene["eta"], ctx=ctx,
+                                 as_numpy=False)
+    vk, ek = vk.reshape(-1).contiguous(), ek.reshape(-1).contiguous()
+    nv, ne = vk.numel() // 4, ek.numel() // 4
+    vs = torch.empty(max(nv, 1), dtype=torch.uint8, device=dev)
+    es = torch.empty(max(ne, 1), dtype=torch.uint8, device=dev)
+    und = ctypes.c_int64(0)
+    eta = float(scene["eta"])
+
+    def run():
+        ctx.check(L.ccd_separating_plane_narrowphase(ctx.ptr, off.data_ptr(), ht.data_ptr(), hp.data_ptr(), V, 2 * V,
+                                                     vk.data_ptr(), None, nv, ek.data_ptr(), None, ne, eta,
+                                                     vs.data_ptr(), es.data_ptr(), ctypes.byref(und)))
+
+    run()
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize(dev)
+    wall = (time.perf_counter() - t0) / steps
+    n = nv + ne
+    log(f"separating plane: {n} stencils in {wall * 1e3:.2f} ms")
+    out = {"stencils": n, "ms": wall * 1e3, "stencils_per_s": n / wall, "root_mode": "reference",
+           "status": {"vf": np.bincount(vs[:nv].cpu().numpy(), minlength=3).tolist(),
+                      "ee": np.bincount(es[:ne].cpu().numpy(), minlength=3).tolist()},
+           "undecided": und.value}
+    # CPU: the oracle (recursive, as the reference) on a random sample of the same stencils
+    rf = oracle.use_default_rootfinder()
+    h = oracle.Hist.two_entry(scene["qstart"], scene["qend"])
+    rng = np.random.default_rng(seed)
+    kv = vk.reshape(-1, 4).cpu().numpy()
+    ke = ek.reshape(-1, 4).cpu().numpy()
+    sv = kv[rng.choice(nv, size=min(nv, cpu_sample * nv // max(n, 1)), replace=False)] if nv else kv
+    se = ke[rng.choice(ne, size=min(ne, cpu_sample - len(sv)), replace=False)] if ne else ke
+    t0 = time.perf_counter()
+    oracle.separating_plane(h, sv, eta, False)
+    oracle.separating_plane(h, se, eta, True)
+    tc = time.perf_counter() - t0
+    per = tc / max(len(sv) + len(se), 1)
+    out["cpu_baseline"] = {"value": 1.0 / per, "unit": "stencils/s", "cores": 1, "kind": "port",
+                           "sample": f"oracle (root finder '{rf}') on {len(sv)} VF + {len(se)} EE random stencils "
+                                     f"of the same set, {tc:.2f} s"}
+    return out
+
+
 def bench_batched(steps, log):
     """The 512-instance mixed workload: plain (ccd_detect per instance, the same kernels launched per
     instance) vs batched (one ccd_batch_detect pipeline) vs batched replayed from a captured HIP graph."""
@@ -226,6 +291,7 @@ def main():
                     help="cloth: the draped cloth (default); sheet: the crumpled sheet (--grid 1024 = 2,093,058 tris)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-separating-plane", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy 5 times (scale reference for HBM byte counters)")
@@ -389,6 +455,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize(dev)
         out["copy_probe_bytes"] = {"read": 256 << 20, "written": 256 << 20, "dispatches": 5}
+    if not args.no_separating_plane:
+        ctx.set_root_mode(N.ROOTS_REFERENCE)
+        out["separating_plane"] = bench_separating_plane(ctx, scene, qs, qe, max(3, args.steps // 2), log)
     if not args.no_batched:
         out["batched"] = bench_batched(max(3, args.steps // 2), log)
     if not args.no_served:
