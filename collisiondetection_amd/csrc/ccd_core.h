@@ -122,6 +122,14 @@ template <class Base>
 struct WithRefRoots : Base {
     static constexpr bool ref_roots = true;
 };
+// The product's counter types prune cascades that cannot hit (see any_overlap below); the counting
+// types used for work accounting, and FullCascade, run the reference's full cascade.
+struct FullCascade {
+    CCD_FHD void operator()(int) const {}
+};
+template <class C>
+struct prunes : std::integral_constant<bool, std::is_same<std::decay_t<C>, NoCount>::value ||
+                                                 std::is_same<std::decay_t<C>, RefRoots>::value> {};
 template <class C, class = void>
 struct uses_ref_roots : std::false_type {};
 template <class C>
@@ -614,6 +622,64 @@ CCD_HD void distance_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, double
 // Primitive tests (CTCD.cpp:259-692).  Each returns true on collision and then sets t.
 // ------------------------------------------------------------------------------------------------
 
+// Pruning.  A primitive reports a collision only through a combination of one interval from each of
+// its lists whose members overlap pairwise (CTCD.cpp:352-396, :477-507, :575-601), and the lists are
+// computed one after another.  When no choice from the lists computed so far overlaps pairwise, no later
+// list can produce a combination, so the result is false whatever the later root solves return; the
+// product kernels return false there without solving them (the reference solves them and then finds no
+// combination).  `mask` selects the first list's members (the coplanar intervals of edge-edge).
+template <int N>
+CCD_FHD bool pairwise_overlap(const double (&L)[N], const double (&U)[N])
+{
+    bool ov = true;
+#pragma unroll
+    for (int x = 0; x < N; x++)
+#pragma unroll
+        for (int y = x + 1; y < N; y++)
+            if (L[x] > U[y] || L[y] > U[x]) ov = false;
+    return ov;
+}
+template <int A, int B>
+CCD_HD bool any_overlap(const Ivls<A> &a, unsigned mask, const Ivls<B> &b)
+{
+    for (int i = 0; i < a.n; i++) {
+        if (!((mask >> i) & 1u)) continue;
+        for (int j = 0; j < b.n; j++) {
+            double L[2] = {sel(a.l, i), sel(b.l, j)}, U[2] = {sel(a.u, i), sel(b.u, j)};
+            if (pairwise_overlap(L, U)) return true;
+        }
+    }
+    return false;
+}
+template <int A, int B, int C>
+CCD_HD bool any_overlap(const Ivls<A> &a, unsigned mask, const Ivls<B> &b, const Ivls<C> &c)
+{
+    for (int i = 0; i < a.n; i++) {
+        if (!((mask >> i) & 1u)) continue;
+        for (int j = 0; j < b.n; j++)
+            for (int k = 0; k < c.n; k++) {
+                double L[3] = {sel(a.l, i), sel(b.l, j), sel(c.l, k)}, U[3] = {sel(a.u, i), sel(b.u, j), sel(c.u, k)};
+                if (pairwise_overlap(L, U)) return true;
+            }
+    }
+    return false;
+}
+template <int A, int B, int C, int D>
+CCD_HD bool any_overlap(const Ivls<A> &a, unsigned mask, const Ivls<B> &b, const Ivls<C> &c, const Ivls<D> &d)
+{
+    for (int i = 0; i < a.n; i++) {
+        if (!((mask >> i) & 1u)) continue;
+        for (int j = 0; j < b.n; j++)
+            for (int k = 0; k < c.n; k++)
+                for (int m = 0; m < d.n; m++) {
+                    double L[4] = {sel(a.l, i), sel(b.l, j), sel(c.l, k), sel(d.l, m)};
+                    double U[4] = {sel(a.u, i), sel(b.u, j), sel(c.u, k), sel(d.u, m)};
+                    if (pairwise_overlap(L, U)) return true;
+                }
+    }
+    return false;
+}
+
 // edgeEdgeCTCD (CTCD.cpp:259-411): edges (q0,p0) and (q1,p1).
 template <class C = NoCount>
 CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3 p1e, double eta, double &t,
@@ -655,14 +721,17 @@ CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3
     cnt(S_EE_BARY);
     barycentric_poly(x10, p0s - q0s, q0s - q1s, v10, vp0 - vq0, vq0 - vq1, a0, cnt);
     if (a0.n == 0) return false;
+    if (prunes<C>::value && !any_overlap(raw, ~par, a0)) return false;
     cnt(S_EE_BARY);
     barycentric_poly(x10, q0s - p0s, p0s - q1s, v10, vq0 - vp0, vp0 - vq1, a1, cnt);
     if (a1.n == 0) return false;
+    if (prunes<C>::value && !any_overlap(raw, ~par, a0, a1)) return false;
     x10 = p0s - q0s;
     v10 = vp0 - vq0;
     cnt(S_EE_BARY);
     barycentric_poly(x10, p1s - q1s, q1s - q0s, v10, vp1 - vq1, vq1 - vq0, b0, cnt);
     if (b0.n == 0) return false;
+    if (prunes<C>::value && !any_overlap(raw, ~par, a0, a1, b0)) return false;
     cnt(S_EE_BARY);
     barycentric_poly(x10, q1s - p1s, p1s - q0s, v10, vq1 - vp1, vp1 - vq0, b1, cnt);
     if (b1.n == 0) return false;
@@ -727,6 +796,7 @@ CCD_HD bool vertex_face(V3 q0s, V3 q1s, V3 q2s, V3 q3s, V3 q0e, V3 q1e, V3 q2e, 
     cnt(S_VF_PLANE);
     plane_poly(q0s - q3s, cross(q2s - q3s, q1s - q3s), q2s - q3s, v0 - v3, cross(v2 - v3, v1 - v3), v2 - v3, e3, cnt);
     if (e3.n == 0) return false;
+    if (prunes<C>::value && !any_overlap(e1, ~0u, e2, e3)) return false;
     Ivls<7> cop;
     cnt(S_VF_DIST);
     distance_poly(q0s - q1s, q2s - q1s, q3s - q1s, v0 - v1, v2 - v1, v3 - v1, minD, cop, cnt);
@@ -787,6 +857,7 @@ CCD_HD bool vertex_edge(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double e
     cnt(S_VE_QUAD);
     find_intervals<2>(op, e2, true, cnt);
     if (e2.n == 0) return false;
+    if (prunes<Cn>::value && !any_overlap(e1, ~0u, e2)) return false;
     cnt(S_VE_QUART);
     double A = dot(ab, ab);
     double B = 2 * dot(ab, vab);

@@ -352,7 +352,13 @@ __global__ __launch_bounds__(256) CCD_TWO_WAVES void k_separating_plane(
         int4 k = keys[i];
         int v[4] = {k.x, k.y, k.z, k.w};
         ccd::HistCSR h{off, ht, hp};
-        r = ccd::separating_plane_check<VF>(h, v, eta ? eta[i] : eta_s, eps, M());
+        // The leaves run the full cascade (no pruning): with the pruned one, some lanes of this kernel
+        // returned results that differed from the host build of the same code when other lanes of their
+        // wavefront took other paths (each was right when launched alone); the full cascade is
+        // bit-identical to the host on every tested stencil.
+        using L = std::conditional_t<ccd::uses_ref_roots<M>::value, ccd::WithRefRoots<ccd::FullCascade>,
+                                     ccd::FullCascade>;
+        r = ccd::separating_plane_check<VF>(h, v, eta ? eta[i] : eta_s, eps, L());
         status[i] = (uint8_t)r;
     }
     uint64_t m = __ballot(r == 2);

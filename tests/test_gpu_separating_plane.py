@@ -110,3 +110,31 @@ def test_gpu_separating_plane_empty_and_invalid(api):
     bad = np.array([[0, 1, 2, h.nverts]], np.int32)
     with pytest.raises(Exception):
         api.separating_plane_narrowphase(h.off, h.ht, h.hp, bad, np.zeros((0, 4), np.int32), s["eta"])
+
+
+def test_gpu_separating_plane_full_cloth_vs_host_core(api, core_harness):
+    """All 7.7M candidate stencils of the 256x256 cloth in one launch per kind (reference-root mode),
+    checked against the host build of the same code on a 200,000-stencil random sample: each lane's
+    result must not depend on what the other lanes of its wavefront do."""
+    from collisiondetection_amd import _native as N
+
+    say("case: separating plane, full 256^2 cloth vs host core sample")
+    s = scenes.cloth(256)
+    h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+    vf, ee = api.kdop_candidates(s["faces"], s["qstart"], s["qend"], s["eta"])
+    ctx = api.default_context()
+    old = ctx.root_mode
+    ctx.set_root_mode(N.ROOTS_REFERENCE)
+    core_harness.core_set_ref_roots(1)
+    try:
+        r = api.separating_plane_narrowphase(h.off, h.ht, h.hp, vf, ee, s["eta"])
+        rng = np.random.default_rng(4)
+        for keys, got, is_ee in ((vf, r["vf_status"], False), (ee, r["ee_status"], True)):
+            idx = np.sort(rng.choice(len(keys), size=min(len(keys), 100000), replace=False))
+            host = core_separating_plane(core_harness, h, keys[idx], is_ee, s["eta"])
+            say("ee" if is_ee else "vf", "sample statuses", np.bincount(host, minlength=3), "mismatches",
+                int((got[idx] != host).sum()))
+            assert np.array_equal(got[idx], host)
+    finally:
+        ctx.set_root_mode(old)
+        core_harness.core_set_ref_roots(0)
