@@ -422,8 +422,14 @@ def main():
         key = roof["kernel"].replace("k_narrow2_queue<VF>", "k_narrow2_queue<true, 256, ccd::RefRoots>").replace(
             "k_narrow2_queue<EE>", "k_narrow2_queue<false, 256, ccd::RefRoots>")
         if prof["workload"] == out_workload and key in prof["kernels"]:
-            roof["traffic"] = prof["kernels"][key]["hbm_bytes"]
+            kp = prof["kernels"][key]
+            roof["traffic"] = kp["hbm_bytes"]
             roof["traffic_source"] = "profiles/cloth256_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
+            if "fp64_wave_instructions" in kp and roof.get("unit") == "TFLOP/s":
+                # Executed FP64 wavefront instructions (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 of the same
+                # profile) against the issue peak: 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction.
+                issue_peak = 1024 * 2.4e9 / 4
+                roof["fp64_issue_frac"] = kp["fp64_wave_instructions"] / (kp["avg_ms"] * 1e-3) / issue_peak
     except (OSError, ValueError, KeyError):
         pass
     roof["stage_ms"] = {k: round(v, 5) for k, v in stage_ms.items()}
