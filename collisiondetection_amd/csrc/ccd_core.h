@@ -793,6 +793,7 @@ CCD_HD bool vertex_face(V3 q0s, V3 q1s, V3 q2s, V3 q3s, V3 q0e, V3 q1e, V3 q2e, 
     cnt(S_VF_PLANE);
     plane_poly(q0s - q2s, cross(q1s - q2s, q3s - q2s), q1s - q2s, v0 - v2, cross(v1 - v2, v3 - v2), v1 - v2, e2, cnt);
     if (e2.n == 0) return false;
+    if (prunes<C>::value && !any_overlap(e1, ~0u, e2)) return false;
     cnt(S_VF_PLANE);
     plane_poly(q0s - q3s, cross(q2s - q3s, q1s - q3s), q2s - q3s, v0 - v3, cross(v2 - v3, v1 - v3), v2 - v3, e3, cnt);
     if (e3.n == 0) return false;
