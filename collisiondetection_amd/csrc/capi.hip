@@ -398,7 +398,9 @@ struct DetectArgs {
 
 // The narrowphase half: counters reset by fill kernels, then the two persistent queue kernels.  Pure
 // launches (capturable).  n_max bounds the grids: the stencil counts are read on the device.
-static int narrow_launches(ccd_ctx *ctx, const DetectArgs &a, int64_t nvf_max, int64_t nee_max)
+// exact: nvf_max / nee_max are this call's stencil counts (the synchronous path), so the stencils can
+// be grouped by their vertex-edge signature first; the captured path passes capacities instead.
+static int narrow_launches(ccd_ctx *ctx, const DetectArgs &a, int64_t nvf_max, int64_t nee_max, bool exact)
 {
     unsigned long long *work = (unsigned long long *)buffer(ctx, "queue_work", 48);
     if (!work) return CCD_E_OOM;
@@ -407,12 +409,21 @@ static int narrow_launches(ccd_ctx *ctx, const DetectArgs &a, int64_t nvf_max, i
     if (rc) return rc;
     const int32_t *vfk = (const int32_t *)ctx->bufs["vf_keys"].ptr;
     const int32_t *eek = (const int32_t *)ctx->bufs["ee_keys"].ptr;
+    const bool group = exact && ctx->group_stencils;   // the stage timers include the grouping
     prof_begin(ctx, CCD_T_NARROW_VF);
+    if (group) {
+        rc = group_by_signature(ctx, true, a.qs, a.qe, vfk, nvf_max, &vfk);
+        if (rc) return rc;
+    }
     rc = launch_narrow2_queue(ctx, true, a.qs, a.qe, vfk, ctx->d_counts + C_VF, nvf_max, a.eta, a.vf_hits, a.vf_toi,
                               a.vf_cap, ctx->d_counts + C_VF_HITS, work, ctx->vf_shape);
     if (rc) return rc;
     prof_end(ctx, CCD_T_NARROW_VF);
     prof_begin(ctx, CCD_T_NARROW_EE);
+    if (group) {
+        rc = group_by_signature(ctx, false, a.qs, a.qe, eek, nee_max, &eek);
+        if (rc) return rc;
+    }
     rc = launch_narrow2_queue(ctx, false, a.qs, a.qe, eek, ctx->d_counts + C_EE, nee_max, a.eta, a.ee_hits, a.ee_toi,
                               a.ee_cap, ctx->d_counts + C_EE_HITS, work + 1, ctx->ee_shape);
     if (rc) return rc;
@@ -431,7 +442,7 @@ static int detect_sync(ccd_ctx *ctx, const DetectArgs &a)
     ctx->last_ee = c.ee;
     ctx->last_nvf = c.nvf;
     ctx->last_nee = c.nee;
-    rc = narrow_launches(ctx, a, c.nvf, c.nee);
+    rc = narrow_launches(ctx, a, c.nvf, c.nee, true);
     if (rc) return rc;
     CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts, ctx->d_counts, sizeof(int64_t) * NCOUNTERS, hipMemcpyDeviceToHost,
                                       ctx->stream));
@@ -497,7 +508,7 @@ static int detect_graph(ccd_ctx *ctx, const DetectArgs &a)
     }
     rc = broadphase_launch(ctx, a.faces, a.F, a.qs, a.qe, nullptr, nullptr, a.V, a.outer_eta, a.fixed, false,
                            a.inst_off, (int)a.ninst);
-    if (!rc) rc = narrow_launches(ctx, a, ctx->cap_vf, ctx->cap_ee);
+    if (!rc) rc = narrow_launches(ctx, a, ctx->cap_vf, ctx->cap_ee, false);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(ctx->stream, &g);
     ctx->capturing = false;

@@ -828,12 +828,12 @@ CCD_HD bool vertex_face(V3 q0s, V3 q1s, V3 q2s, V3 q3s, V3 q0e, V3 q1e, V3 q2e, 
     return col;
 }
 
-// vertexEdgeCTCD (CTCD.cpp:511-602): vertex q0 and edge (q1,q2).
-template <class Cn = NoCount>
-CCD_HD bool vertex_edge(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double eta, double &t, Cn &&cnt = Cn())
+// vertexEdgeCTCD's two side quadratics (CTCD.cpp:520-555): when the vertex projects between the edge's
+// ends.  False when either interval list is empty (the reference returns there); e2 is computed only
+// if e1 is not empty, as in the reference.
+template <class Cn>
+CCD_HD bool ve_sides(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, Ivls<3> &e1, Ivls<3> &e2, Cn &cnt)
 {
-    cnt(S_VE_CALL);
-    double minD = eta * eta;
     V3 v0 = q0e - q0s;
     V3 v1 = q1e - q1s;
     V3 v2 = q2e - q2s;
@@ -843,9 +843,7 @@ CCD_HD bool vertex_edge(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double e
     V3 vab = v2 - v1;
     V3 vac = v0 - v1;
     V3 vcb = v2 - v0;
-    Ivls<3> e1, e2;
-    Ivls<5> colin;
-    double op[5];
+    double op[3];
     op[0] = dot(vab, vac);
     op[1] = dot(ac, vab) + dot(ab, vac);
     op[2] = dot(ab, ac);
@@ -857,8 +855,52 @@ CCD_HD bool vertex_edge(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double e
     op[2] = dot(ab, cb);
     cnt(S_VE_QUAD);
     find_intervals<2>(op, e2, true, cnt);
-    if (e2.n == 0) return false;
+    return e2.n != 0;
+}
+
+// Which vertex-edge sub-tests of a stencil's window would reach their quartic (both side quadratics
+// non-empty and overlapping): bit j for the j-th vertex-edge test in the reference's order.  Cheap
+// (closed-form quadratics only); the narrowphase groups stencils by it so that a wavefront's lanes
+// take the same quartic solves.
+CCD_HD unsigned ve_signature(bool vf, const V3 *s, const V3 *e)
+{
+    unsigned sig = 0;
+    NoCount nc;
+    auto bit = [&](int j, int a, int b, int c) {
+        Ivls<3> e1, e2;
+        if (ve_sides(s[a], s[b], s[c], e[a], e[b], e[c], e1, e2, nc) && any_overlap(e1, ~0u, e2)) sig |= 1u << j;
+    };
+    if (vf) {   // check_vf_window's order
+        bit(0, 0, 1, 2);
+        bit(1, 0, 2, 3);
+        bit(2, 0, 3, 1);
+    } else {    // check_ee_window's order
+        bit(0, 0, 2, 3);
+        bit(1, 1, 2, 3);
+        bit(2, 2, 0, 1);
+        bit(3, 3, 0, 1);
+    }
+    return sig;
+}
+
+// vertexEdgeCTCD (CTCD.cpp:511-602): vertex q0 and edge (q1,q2).
+template <class Cn = NoCount>
+CCD_HD bool vertex_edge(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double eta, double &t, Cn &&cnt = Cn())
+{
+    cnt(S_VE_CALL);
+    double minD = eta * eta;
+    V3 v0 = q0e - q0s;
+    V3 v1 = q1e - q1s;
+    V3 v2 = q2e - q2s;
+    V3 ab = q2s - q1s;
+    V3 ac = q0s - q1s;
+    V3 vab = v2 - v1;
+    V3 vac = v0 - v1;
+    Ivls<3> e1, e2;
+    Ivls<5> colin;
+    if (!ve_sides(q0s, q1s, q2s, q0e, q1e, q2e, e1, e2, cnt)) return false;
     if (prunes<Cn>::value && !any_overlap(e1, ~0u, e2)) return false;
+    double op[5];
     cnt(S_VE_QUART);
     double A = dot(ab, ab);
     double B = 2 * dot(ab, vab);

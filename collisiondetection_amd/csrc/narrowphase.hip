@@ -322,6 +322,36 @@ __global__ void k_validate_history(const int64_t *__restrict__ off, const double
     if (m && __lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(bad, (unsigned long long)__popcll(m));
 }
 
+// Grouping of stencils by the vertex-edge quartics their windows will solve (ccd::ve_signature): the
+// narrowphase then runs on the keys in signature order (stable, so each group keeps the broadphase's
+// spatial order), and a wavefront's lanes mostly take the same solver calls.
+template <bool VF>
+__global__ __launch_bounds__(256) void k_stencil_signature(const double *__restrict__ qs, const double *__restrict__ qe,
+                                                           const int4 *__restrict__ keys, int64_t n,
+                                                           uint64_t *__restrict__ sig, int32_t *__restrict__ idx)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int4 k = keys[i];
+    int v[4] = {k.x, k.y, k.z, k.w};
+    V3 s[4], e[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        V3 a = load3(qs, v[j]);
+        V3 b = load3(qe, v[j]);
+        s[j] = ccd::mk(ccd::stitch0(a.x, b.x), ccd::stitch0(a.y, b.y), ccd::stitch0(a.z, b.z));
+        e[j] = b;
+    }
+    sig[i] = ccd::ve_signature(VF, s, e);
+    idx[i] = (int32_t)i;
+}
+__global__ __launch_bounds__(256) void k_gather_keys(const int4 *__restrict__ keys, const int32_t *__restrict__ perm,
+                                                     int64_t n, int4 *__restrict__ out)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = keys[perm[i]];
+}
+
 // History::computeMinimumGap (History.cpp:78-93): min(1, every gap between consecutive entry times),
 // reduced with an atomic min on the bit pattern (the gaps of a validated history are non-negative, so
 // their bits order as the values do).  *gap_bits must hold the bits of 1.0 on entry.
@@ -498,6 +528,35 @@ int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *
                                                   ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, hit, t,
                                                   sub, win));
     CCD_LAUNCH_CHECK(ctx);
+    return CCD_OK;
+}
+
+int group_by_signature(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys, int64_t n,
+                       const int32_t **grouped)
+{
+    *grouped = keys;
+    if (n < 2) return CCD_OK;
+    uint64_t *sig = (uint64_t *)buffer(ctx, "grp_sig", (size_t)n * 8);
+    uint64_t *sig2 = (uint64_t *)buffer(ctx, "grp_sig2", (size_t)n * 8);
+    int32_t *idx = (int32_t *)buffer(ctx, "grp_idx", (size_t)n * 4);
+    int32_t *idx2 = (int32_t *)buffer(ctx, "grp_idx2", (size_t)n * 4);
+    int4 *out = (int4 *)buffer(ctx, vf ? "grp_keys_vf" : "grp_keys_ee", (size_t)n * 16);
+    void *tmp = buffer(ctx, "grp_tmp", radix_sort_temp_bytes(n));
+    if (!sig || !sig2 || !idx || !idx2 || !out || !tmp) return CCD_E_OOM;
+    if (vf)
+        hipLaunchKernelGGL(k_stencil_signature<true>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
+                           (const int4 *)keys, n, sig, idx);
+    else
+        hipLaunchKernelGGL(k_stencil_signature<false>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
+                           (const int4 *)keys, n, sig, idx);
+    CCD_LAUNCH_CHECK(ctx);
+    bool in_alt = false;
+    int rc = radix_sort_pairs(ctx, sig, idx, sig2, idx2, n, 4, tmp, in_alt);   // one stable 8-bit pass
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_gather_keys, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, (const int4 *)keys,
+                       in_alt ? idx2 : idx, n, out);
+    CCD_LAUNCH_CHECK(ctx);
+    *grouped = (const int32_t *)out;
     return CCD_OK;
 }
 
