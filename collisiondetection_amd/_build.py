@@ -10,7 +10,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libccd_amd.so")
 OBJDIR = os.path.join(PKG, "_build")
-SOURCES = ["narrowphase.hip", "broadphase.hip", "distance.hip", "devprims.hip", "server.hip", "capi.hip"]
+SOURCES = ["narrowphase.hip", "broadphase.hip", "distance.hip", "devprims.hip", "server.hip", "capi.hip",
+           "velocity_filter.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off: the reference's arithmetic has no fused multiply-add; every a*b+c must round twice.

@@ -90,7 +90,18 @@ class RequestResult(ctypes.Structure):
                 ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double)]
 
 
+class VFStats(ctypes.Structure):
+    """ccd_vf_stats (include/ccd_amd.h)."""
+    _fields_ = [("iterations", ctypes.c_int64), ("layers", ctypes.c_int64), ("firings", ctypes.c_int64),
+                ("history_entries", ctypes.c_int64), ("vf_candidates", ctypes.c_int64),
+                ("ee_candidates", ctypes.c_int64), ("vf_collisions", ctypes.c_int64),
+                ("ee_collisions", ctypes.c_int64), ("closest", ctypes.c_double), ("host_seconds", ctypes.c_double),
+                ("device_seconds", ctypes.c_double)]
+
+
 SIGNATURES.update({
+    "ccd_velocity_filter": ([_V, _V, _V, _I64, _V, _I64, _V, _D, _D, _D, _D, _D, _INT, _INT, ctypes.POINTER(_INT),
+                             ctypes.POINTER(VFStats)], _INT),
     "ccd_set_root_mode": ([_V, _INT], _INT),
     "ccd_root_mode": ([_V, ctypes.POINTER(_INT)], _INT),
     "ccd_server_create": ([_INT, _I64, _INT, ctypes.POINTER(_V)], _INT),

@@ -547,6 +547,56 @@ def mesh_self_distance(verts, faces, fixed=None, ctx: Context | None = None):
 
 
 # ------------------------------------------------------------------------------------------------
+# Velocity filter
+# ------------------------------------------------------------------------------------------------
+def velocity_filter(qstart, qend, faces, inv_masses, outer_eta, inner_eta, base_stiffness=1e-4, base_substep=1e-4,
+                    cor=0.1, max_rollbacks=1000, verbose=False, ctx: Context | None = None):
+    """VelocityFilter::velocityFilter on host arrays -> (result, qend_out, stats).
+
+    result is the reference's return value (outer iterations taken, or -1 / -2 / -3); qend_out is a new
+    float64 array, the filtered end positions when result >= 0 and a copy of qend otherwise; stats is a
+    dict of the ccd_vf_stats counters."""
+    ctx = ctx or default_context()
+    ctx.bind_stream()
+    qs = np.ascontiguousarray(qstart, dtype=np.float64).reshape(-1)
+    qe = np.array(qend, dtype=np.float64).reshape(-1)
+    f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+    im = np.ascontiguousarray(inv_masses, dtype=np.float64).reshape(-1)
+    V, F = len(qs) // 3, len(f) // 3
+    if len(qs) != 3 * V or len(qe) != 3 * V or len(im) != 3 * V or len(f) != 3 * F:
+        raise ValueError("qstart, qend and inv_masses need 3*V entries and faces 3*F")
+    res = ctypes.c_int(0)
+    st = N.VFStats()
+    dp = ctypes.POINTER(ctypes.c_double)
+    ctx.check(N.lib().ccd_velocity_filter(ctx.ptr, qs.ctypes.data_as(dp), qe.ctypes.data_as(dp), V,
+                                          f.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), F, im.ctypes.data_as(dp),
+                                          float(outer_eta), float(inner_eta), float(base_stiffness),
+                                          float(base_substep), float(cor), int(max_rollbacks), int(bool(verbose)),
+                                          ctypes.byref(res), ctypes.byref(st)))
+    return res.value, qe, {name: getattr(st, name) for name, _ in N.VFStats._fields_}
+
+
+class VelocityFilter:
+    """VelocityFilter (include/VelocityFilter.h:7-45): the return codes and the static velocityFilter."""
+
+    DANGLING_VERTICES = -1
+    STARTING_STATE_INTERSECTS = -2
+    MAXROLLBACKS_EXCEEDED = -3
+
+    @staticmethod
+    def velocityFilter(qstart, qend, faces, masses, outerEta, innerEta, baseStiffness=1e-4, baseSubstepSize=1e-4,
+                       CoR=0.1, maxRollbacks=1000):
+        """As the reference: qend (a float64 numpy array) is overwritten with the filtered end positions on
+        success; returns the number of outer iterations or a negative code.  masses are inverse masses
+        per DOF.  Prints the reference's progress lines."""
+        r, qe, _ = velocity_filter(qstart, qend, faces, masses, outerEta, innerEta, baseStiffness, baseSubstepSize,
+                                   CoR, maxRollbacks, verbose=True)
+        if r >= 0:
+            np.copyto(np.asarray(qend).reshape(-1), qe)
+        return r
+
+
+# ------------------------------------------------------------------------------------------------
 # Batched instances
 # ------------------------------------------------------------------------------------------------
 class Batch:

@@ -159,6 +159,31 @@ def crumpled_sheet(n: int = 1024, seed: int = 3, eta: float = 1e-7):
     return _scene(f"crumpled_{n}_s{seed}", f, v, qe, eta)
 
 
+def falling_sheets(n: int = 4, seed: int = 0, gap: float = 0.1, drop: float = 0.2, drift: float = 0.05,
+                   jitter: float = 0.02, outer_eta: float = 0.05, inner_eta: float = 0.01):
+    """A velocity-filter contact problem: two jittered n x n grid sheets, the upper one (gap above,
+    shifted sideways) falling by `drop` through the lower one while drifting by `drift` in x.  The
+    start state is separated (closest pair ~gap apart), the end state interpenetrates.  Adds the
+    filter's layer thicknesses outer_eta / inner_eta to the scene dict (eta = outer_eta)."""
+    rng = np.random.default_rng(seed)
+    sheets = []
+    for z, off in ((0.0, (0.0, 0.0)), (gap, (0.13, 0.07))):
+        v, f = grid(n)
+        v[:, 0] += off[0] + rng.uniform(-jitter, jitter, n * n)
+        v[:, 1] += off[1] + rng.uniform(-jitter, jitter, n * n)
+        v[:, 2] = z + 0.1 * rng.uniform(-jitter, jitter, n * n)
+        sheets.append((v, f))
+    (va, fa), (vb, fb) = sheets
+    qstart = np.concatenate([va, vb])
+    qend = qstart.copy()
+    qend[len(va):, 2] -= drop
+    qend[len(va):, 0] += drift
+    s = _scene(f"falling_sheets{n}_s{seed}", np.concatenate([fa, fb + len(va)]), qstart, qend, outer_eta)
+    s["outer_eta"] = float(outer_eta)
+    s["inner_eta"] = float(inner_eta)
+    return s
+
+
 def _rotation(rng, max_angle):
     axis = rng.normal(size=3)
     axis /= np.linalg.norm(axis)

@@ -196,6 +196,38 @@ int ccd_distance_primitives(ccd_ctx *ctx, int kind, const double *q, double eta,
 int ccd_mesh_self_distance(ccd_ctx *ctx, const double *verts, int64_t nverts, const int32_t *faces, int64_t nfaces,
                            const uint8_t *fixed, double *distance, int64_t *nvf, int64_t *nee);
 
+/* Counters of one ccd_velocity_filter call (the reference prints some of them, ActiveLayers.cpp:219-246). */
+typedef struct ccd_vf_stats {
+    int64_t iterations;      /* outer iterations run */
+    int64_t layers;          /* penalty layers at the end */
+    int64_t firings;         /* penalty group firings, all iterations */
+    int64_t history_entries; /* history entries of the last iteration */
+    int64_t vf_candidates;   /* broadphase candidates, summed over iterations */
+    int64_t ee_candidates;
+    int64_t vf_collisions;   /* separating-plane collisions, summed over iterations */
+    int64_t ee_collisions;
+    double closest;          /* meshSelfDistance of the start state */
+    double host_seconds;     /* penalty integration (host event loop) */
+    double device_seconds;   /* collision queries on the device, transfers included */
+} ccd_vf_stats;
+
+/*
+ * VelocityFilter::velocityFilter (include/VelocityFilter.h:7-45, src/VelocityFilter.cpp:16-87) with the
+ * layered penalty response of src/ActiveLayers.cpp, src/PenaltyGroup.cpp and src/PenaltyPotential.cpp.
+ * Host arrays: qstart and qend (3*nverts; qend is overwritten when the filter succeeds), faces (3*nfaces),
+ * inv_masses (3*nverts, per DOF; 0 = infinite mass).  The event-driven penalty integration runs on the
+ * host; meshSelfDistance, the k-DOP broadphase over the multi-entry history and the separating-plane
+ * narrowphase run on the device.  *result receives the reference's return value: the number of outer
+ * iterations taken, or -1 (DANGLING_VERTICES), -2 (STARTING_STATE_INTERSECTS), -3 (MAXROLLBACKS_EXCEEDED).
+ * verbose = 1 prints the reference's progress lines to stdout.  stats (may be NULL) receives the counters.
+ * Where the separating-plane recursion never terminates for a stencil (the reference overflows its
+ * stack there) the call fails with CCD_E_INVALID.  Synchronous.
+ */
+int ccd_velocity_filter(ccd_ctx *ctx, const double *qstart, double *qend, int64_t nverts, const int32_t *faces,
+                        int64_t nfaces, const double *inv_masses, double outer_eta, double inner_eta,
+                        double base_stiffness, double base_substep, double cor, int max_rollbacks, int verbose,
+                        int *result, ccd_vf_stats *stats);
+
 /*
  * Device memory helpers for host callers that do not use the HIP API themselves (the C++ classes in
  * include/ccdamd/ use them to stage host data).  Copies are synchronous on the context's stream.

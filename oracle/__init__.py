@@ -390,3 +390,56 @@ def mesh_self_distance(verts, faces, fixed=None):
     ne = ctypes.c_int64(0)
     d = lib().orc_mesh_self_distance(_d(v), V, _i(f), F, fx, ctypes.byref(nv), ctypes.byref(ne))
     return d, nv.value, ne.value
+
+
+# ------------------------------------------------------------------------------------------------
+# Velocity filter (include/VelocityFilter.h, src/VelocityFilter.cpp, src/ActiveLayers.cpp,
+# src/PenaltyGroup.cpp, src/PenaltyPotential.cpp)
+# ------------------------------------------------------------------------------------------------
+VF_STATS = ("iterations", "layers", "firings", "history_entries", "vf_candidates", "ee_candidates",
+            "vf_collisions", "ee_collisions", "undecided")
+
+
+def velocity_filter(qstart, qend, faces, inv_masses, outer_eta, inner_eta, base_stiffness=1e-4,
+                    base_substep=1e-4, cor=0.1, max_rollbacks=1000, want_history=False):
+    """VelocityFilter::velocityFilter -> (result, qend_out, stats, closest[, hist]).
+
+    result is the reference's return value (iterations, -1 dangling vertices, -2 starting state
+    intersects, -3 rollbacks exceeded), or -100 where the separating-plane recursion never ends.
+    hist (want_history, on success) is the final iteration's history as a Hist."""
+    L = lib()
+    if not hasattr(L, "_vf_bound"):
+        L.orc_velocity_filter.argtypes = [_dp, _dp, ctypes.c_int64, _ip, ctypes.c_int64, _dp, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_int, _lp, _dp, ctypes.POINTER(ctypes.c_void_p)]
+        L.orc_velocity_filter.restype = ctypes.c_int
+        L.orc_vhist_entries.argtypes = [ctypes.c_void_p]
+        L.orc_vhist_entries.restype = ctypes.c_int64
+        L.orc_vhist_copy.argtypes = [ctypes.c_void_p, _lp, _dp, _dp]
+        L.orc_vhist_free.argtypes = [ctypes.c_void_p]
+        L._vf_bound = True
+    qs = np.ascontiguousarray(qstart, dtype=np.float64).reshape(-1)
+    qe = np.array(qend, dtype=np.float64).reshape(-1)
+    f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+    im = np.ascontiguousarray(inv_masses, dtype=np.float64).reshape(-1)
+    V, F = len(qs) // 3, len(f) // 3
+    assert len(qe) == 3 * V and len(im) == 3 * V
+    stats = np.zeros(len(VF_STATS), dtype=np.int64)
+    closest = ctypes.c_double(0.0)
+    h = ctypes.c_void_p(None)
+    r = L.orc_velocity_filter(_d(qs), _d(qe), V, _i(f), F, _d(im), float(outer_eta), float(inner_eta),
+                              float(base_stiffness), float(base_substep), float(cor), int(max_rollbacks), _l(stats),
+                              ctypes.byref(closest), ctypes.byref(h) if want_history else None)
+    st = dict(zip(VF_STATS, (int(x) for x in stats)))
+    if not want_history:
+        return r, qe, st, closest.value
+    hist = None
+    if h.value:
+        E = L.orc_vhist_entries(h)
+        off = np.zeros(V + 1, dtype=np.int64)
+        ht = np.zeros(E)
+        hp = np.zeros(3 * E)
+        L.orc_vhist_copy(h, _l(off), _d(ht), _d(hp))
+        L.orc_vhist_free(h)
+        hist = Hist(off, ht, hp)
+    return r, qe, st, closest.value, hist

@@ -29,6 +29,8 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <queue>
 #include <set>
 #include <vector>
 
@@ -1538,6 +1540,363 @@ struct Result {
     std::vector<int64_t> pairs;
 };
 
+// ----------------------------------------------------------------------------------------------
+// Velocity filter: include/VelocityFilter.h:7-45, src/VelocityFilter.cpp:16-87, src/ActiveLayers.cpp,
+// src/PenaltyGroup.cpp, src/PenaltyPotential.cpp, src/SimulationState.h.  Written in the reference's
+// own shape: full-length state and force vectors per firing, std::set group stencils, std::map layer
+// depths and a std::priority_queue of groups.  The collision queries are this file's brute-force k-DOP
+// broadphase and recursive separating-plane narrowphase.
+// ----------------------------------------------------------------------------------------------
+struct VHist {   // src/History.h:7-42: per-vertex (time, pos) entries
+    std::vector<std::vector<std::pair<double, V3>>> e;
+    explicit VHist(const std::vector<double> &q) : e(q.size() / 3)
+    {   // History.cpp:8-16
+        for (size_t v = 0; v < e.size(); v++) e[v].push_back({0.0, load3(&q[3 * v])});
+    }
+    void finish(const std::vector<double> &q)
+    {   // History.cpp:31-39
+        for (size_t v = 0; v < e.size(); v++) e[v].push_back({1.0, load3(&q[3 * v])});
+    }
+    int64_t entries() const
+    {
+        int64_t n = 0;
+        for (const auto &x : e) n += (int64_t)x.size();
+        return n;
+    }
+    void csr(std::vector<int64_t> &off, std::vector<double> &t, std::vector<double> &p) const
+    {
+        off.assign(e.size() + 1, 0);
+        t.clear();
+        p.clear();
+        for (size_t v = 0; v < e.size(); v++) {
+            for (const auto &x : e[v]) {
+                t.push_back(x.first);
+                p.push_back(x.second.x);
+                p.push_back(x.second.y);
+                p.push_back(x.second.z);
+            }
+            off[v + 1] = (int64_t)t.size();
+        }
+    }
+};
+
+struct PGroup {   // PenaltyGroup.h:12-42, PenaltyGroup.cpp:7-71
+    std::vector<Key> vfs, ees;
+    std::set<int> verts;
+    int nextstep = 1;
+    double dt, outer, inner, k, cor;
+    double next_fire() const { return nextstep * dt; }
+};
+struct PGroupCmp {   // ActiveLayers.cpp:18-21
+    bool operator()(const PGroup *a, const PGroup *b) const { return b->next_fire() < a->next_fire(); }
+};
+
+inline void add_seg(std::vector<double> &F, int v, double s, V3 f)
+{   // F.segment<3>(3v) += s * f
+    F[3 * v] = F[3 * v] + s * f.x;
+    F[3 * v + 1] = F[3 * v + 1] + s * f.y;
+    F[3 * v + 2] = F[3 * v + 2] + s * f.z;
+}
+inline void sub_seg(std::vector<double> &F, int v, double s, V3 f)
+{   // F.segment<3>(3v) -= s * f  (s = 1 for the unscaled form, exact)
+    F[3 * v] = F[3 * v] - s * f.x;
+    F[3 * v + 1] = F[3 * v + 1] - s * f.y;
+    F[3 * v + 2] = F[3 * v + 2] - s * f.z;
+}
+
+// VertexFacePenaltyPotential::addForce (PenaltyPotential.cpp:9-31)
+bool vf_penalty(const std::vector<double> &q, const std::vector<double> &v, std::vector<double> &F, const Key &s,
+                double outer, double inner, double k, double cor)
+{
+    V3 p = load3(&q[3 * (int64_t)s.k[0]]), a = load3(&q[3 * (int64_t)s.k[1]]), b = load3(&q[3 * (int64_t)s.k[2]]),
+       c = load3(&q[3 * (int64_t)s.k[3]]);
+    if (!vertex_plane_lt(p, a, b, c, outer)) return false;
+    double bary[4];
+    V3 cv = vertex_face_distance(p, a, b, c, bary);
+    double dist = norm(cv);
+    if (dist >= outer || dist < inner) return false;
+    V3 vp = load3(&v[3 * (int64_t)s.k[0]]), va = load3(&v[3 * (int64_t)s.k[1]]), vb = load3(&v[3 * (int64_t)s.k[2]]),
+       vc = load3(&v[3 * (int64_t)s.k[3]]);
+    V3 rel = ((V3{-vp.x, -vp.y, -vp.z} + bary[0] * va) + bary[1] * vb) + bary[2] * vc;
+    if (dot(rel, cv) > 0) k *= cor;
+    double sc = k * (outer - dist) / (outer - inner);
+    V3 lf{(sc * cv.x) / dist, (sc * cv.y) / dist, (sc * cv.z) / dist};
+    F[3 * (int64_t)s.k[0]] -= lf.x;
+    F[3 * (int64_t)s.k[0] + 1] -= lf.y;
+    F[3 * (int64_t)s.k[0] + 2] -= lf.z;
+    add_seg(F, s.k[1], bary[0], lf);
+    add_seg(F, s.k[2], bary[1], lf);
+    add_seg(F, s.k[3], bary[2], lf);
+    return true;
+}
+
+// EdgeEdgePenaltyPotential::addForce (PenaltyPotential.cpp:34-61)
+bool ee_penalty(const std::vector<double> &q, const std::vector<double> &v, std::vector<double> &F, const Key &s,
+                double outer, double inner, double k, double cor)
+{
+    V3 p0 = load3(&q[3 * (int64_t)s.k[0]]), p1 = load3(&q[3 * (int64_t)s.k[1]]), q0 = load3(&q[3 * (int64_t)s.k[2]]),
+       q1 = load3(&q[3 * (int64_t)s.k[3]]);
+    if (!line_line_lt(p0, p1, q0, q1, outer)) return false;
+    double bary[4];
+    V3 cv = edge_edge_distance(p0, p1, q0, q1, bary);
+    double dist = norm(cv);
+    if (dist >= outer || dist < inner) return false;
+    V3 v0 = load3(&v[3 * (int64_t)s.k[0]]), v1 = load3(&v[3 * (int64_t)s.k[1]]), v2 = load3(&v[3 * (int64_t)s.k[2]]),
+       v3_ = load3(&v[3 * (int64_t)s.k[3]]);
+    V3 rel = (((-bary[0]) * v0 - bary[1] * v1) + bary[2] * v2) + bary[3] * v3_;
+    if (dot(rel, cv) > 0) k *= cor;
+    double sc = k * (outer - dist) / (outer - inner);
+    V3 lf{(sc * cv.x) / dist, (sc * cv.y) / dist, (sc * cv.z) / dist};
+    sub_seg(F, s.k[0], bary[0], lf);
+    sub_seg(F, s.k[1], bary[1], lf);
+    add_seg(F, s.k[2], bary[2], lf);
+    add_seg(F, s.k[3], bary[3], lf);
+    return true;
+}
+
+struct SimState {   // SimulationState.h
+    std::vector<double> q, v, minv, last;
+};
+
+struct VFStats {
+    int64_t iterations = 0, layers = 0, firings = 0, history_entries = 0, vf_candidates = 0, ee_candidates = 0,
+            vf_collisions = 0, ee_collisions = 0, undecided = 0;
+    double closest = 0;
+};
+
+class Layers {   // ActiveLayers.h:23-75, ActiveLayers.cpp:23-256
+public:
+    Layers(double outer, double inner, double dt, double k, double term, double cor, const int32_t *faces,
+           int64_t F)
+        : outer_(outer), inner_(inner), dt_(dt), k_(k), term_(term), cor_(cor), faces_(faces), F_(F)
+    {
+    }
+    ~Layers()
+    {
+        for (PGroup *g : groups_) delete g;
+    }
+    // ActiveLayers.cpp:217-256.  Returns 1 when no collision was found, 0 otherwise, -1 when the
+    // separating-plane recursion does not terminate for some stencil (the reference overflows its stack).
+    int run_one_iteration(SimState &s, VFStats &st, VHist **final_hist)
+    {
+        delete hist_;
+        hist_ = new VHist(s.q);
+        while (!step(s, st)) {
+        }
+        st.history_entries = hist_->entries();
+        std::set<int> fixed;
+        for (size_t i = 0; i < s.minv.size(); i++)
+            if (s.minv[i] == 0.0) fixed.insert((int)(i / 3));
+        std::vector<Key> vfs, ees;
+        int rc = collision_detection(fixed, vfs, ees, st);
+        if (rc < 0) return -1;
+        rollback();
+        for (const Key &k : vfs) add_stencil(k, false);
+        for (const Key &k : ees) add_stencil(k, true);
+        if (final_hist) *final_hist = hist_;
+        return (vfs.empty() && ees.empty()) ? 1 : 0;
+    }
+    int64_t layers() const { return deepest_; }
+
+private:
+    double layer_depth(int layer) const { return inner_ + (outer_ - inner_) / double(layer); }   // :167-170
+
+    void add_groups(int maxdepth)
+    {   // :67-83
+        while (deepest_ < maxdepth) {
+            int depth = deepest_ + 1;
+            double fudge = 1e-4;
+            PGroup *g = new PGroup;
+            g->dt = dt_ / double(depth) / std::sqrt(double(depth) + fudge);
+            g->outer = layer_depth(depth);
+            g->inner = inner_;
+            g->k = k_ * depth * depth * depth;
+            g->cor = cor_;
+            groups_.push_back(g);
+            queue_.push(g);
+            deepest_++;
+        }
+    }
+
+    void add_stencil(const Key &k, bool ee)
+    {   // :41-65
+        int &d = (ee ? eedepth_ : vfdepth_)[k];
+        int olddepth = d;
+        for (int i = 0; i < 5; i++) {
+            add_groups(olddepth + i + 1);
+            PGroup *g = groups_[olddepth + i];
+            (ee ? g->ees : g->vfs).push_back(k);
+            for (int j = 0; j < 4; j++) g->verts.insert(k.k[j]);
+            d++;
+        }
+    }
+
+    bool step(SimState &s, VFStats &st)
+    {   // :85-150
+        size_t n = s.q.size();
+        std::vector<double> F(n), newq(n, 0.0), newv(n, 0.0);
+        if (!queue_.empty()) {
+            PGroup *g = queue_.top();
+            double newtime = g->next_fire();
+            if (newtime < term_) {
+                queue_.pop();
+                st.firings++;
+                for (int vert : g->verts)
+                    for (int j = 0; j < 3; j++) {
+                        size_t i = 3 * (size_t)vert + j;
+                        double dt = newtime - s.last[i];
+                        newq[i] = s.q[i] + (newtime - s.last[i]) * s.v[i];
+                        newv[i] = (newq[i] - s.q[i]) / dt;
+                    }
+                std::fill(F.begin(), F.end(), 0.0);
+                std::vector<double> gf(n, 0.0);   // PenaltyGroup::addForce (PenaltyGroup.cpp:35-53)
+                for (const Key &k : g->vfs) vf_penalty(newq, newv, gf, k, g->outer, g->inner, g->k, g->cor);
+                for (const Key &k : g->ees) ee_penalty(newq, newv, gf, k, g->outer, g->inner, g->k, g->cor);
+                for (size_t i = 0; i < n; i++) F[i] = F[i] + gf[i] * g->dt;
+                for (int vert : g->verts) {
+                    bool touched = false;
+                    for (int j = 0; j < 3; j++)
+                        if (F[3 * (size_t)vert + j] != 0.0) touched = true;
+                    if (!touched) continue;
+                    for (int j = 0; j < 3; j++) {
+                        size_t i = 3 * (size_t)vert + j;
+                        s.v[i] = s.v[i] + s.minv[i] * F[i];
+                        s.q[i] = newq[i];
+                        s.last[i] = newtime;
+                    }
+                    hist_->e[vert].push_back({newtime, load3(&s.q[3 * (size_t)vert])});
+                }
+                g->nextstep++;
+                queue_.push(g);
+                return false;
+            }
+        }
+        for (size_t i = 0; i < n; i++) {
+            s.q[i] = s.q[i] + (term_ - s.last[i]) * s.v[i];
+            s.last[i] = term_;
+        }
+        hist_->finish(s.q);
+        return true;
+    }
+
+    void rollback()
+    {   // :152-165
+        for (PGroup *g : groups_) g->nextstep = 1;
+        std::priority_queue<PGroup *, std::vector<PGroup *>, PGroupCmp> nq;
+        while (!queue_.empty()) {
+            nq.push(queue_.top());
+            queue_.pop();
+        }
+        queue_ = nq;
+    }
+
+    double thickness(const Key &k, bool ee) const
+    {   // :172-186
+        const std::map<Key, int> &m = ee ? eedepth_ : vfdepth_;
+        auto it = m.find(k);
+        return it != m.end() ? layer_depth(it->second + 1) : layer_depth(1);
+    }
+
+    int collision_detection(const std::set<int> &fixedset, std::vector<Key> &vfs, std::vector<Key> &ees,
+                            VFStats &st)
+    {   // :188-215: KDOPBroadPhase over the history, then SeparatingPlaneNarrowPhase with per-stencil eta
+        std::vector<int64_t> off;
+        std::vector<double> ht, hp;
+        hist_->csr(off, ht, hp);
+        Hist h{off.data(), ht.data(), hp.data()};
+        int64_t V = (int64_t)hist_->e.size();
+        std::vector<uint8_t> fixed(V, 0);
+        for (int v : fixedset) fixed[v] = 1;
+        std::vector<Dop> d;
+        leaf_dops(faces_, F_, h, outer_, false, d);
+        CandSets cs;
+        candidates_bruteforce(faces_, F_, d, KAX, fixedset.empty() ? nullptr : fixed.data(), cs, nullptr);
+        st.vf_candidates += (int64_t)cs.vf.size();
+        st.ee_candidates += (int64_t)cs.ee.size();
+        double eps = compute_minimum_gap(h, V) / 4.0;   // SeparatingPlaneNarrowPhase.cpp:9-24
+        int und = 0;
+        for (int pass = 0; pass < 2; pass++) {
+            bool ee = pass == 1;
+            for (const Key &k : ee ? cs.ee : cs.vf) {
+                Ctx ctx;
+                int md = 0, u = 0;
+                int verts[4] = {k.k[0], k.k[1], k.k[2], k.k[3]};
+                if (sp_check_interval(!ee, h, verts, thickness(k, ee), 0.0, 1.0, eps, &ctx, 0, md, &u))
+                    (ee ? ees : vfs).push_back(k);
+                und += u;
+            }
+        }
+        st.vf_collisions += (int64_t)vfs.size();
+        st.ee_collisions += (int64_t)ees.size();
+        st.undecided += und;
+        return und ? -1 : 0;
+    }
+
+    double outer_, inner_, dt_, k_, term_, cor_;
+    const int32_t *faces_;
+    int64_t F_;
+    int deepest_ = 0;
+    std::vector<PGroup *> groups_;
+    std::priority_queue<PGroup *, std::vector<PGroup *>, PGroupCmp> queue_;
+    std::map<Key, int> vfdepth_, eedepth_;
+    VHist *hist_ = nullptr;
+
+public:
+    Layers(const Layers &) = delete;
+    Layers &operator=(const Layers &) = delete;
+    void release_history(VHist **keep)
+    {
+        *keep = hist_;
+        hist_ = nullptr;
+    }
+};
+
+// VelocityFilter::velocityFilter (VelocityFilter.cpp:16-87).  Returns the reference's value (iterations,
+// or -1 / -2 / -3), or -100 when the separating-plane recursion does not terminate for some stencil.
+int velocity_filter(const double *qstart, double *qend, int64_t V, const int32_t *faces, int64_t F,
+                    const double *invm, double outer, double inner, double stiff, double substep, double cor,
+                    int max_rollbacks, VFStats &st, VHist **final_hist)
+{
+    for (int64_t i = 0; i < V; i++) {   // :24-37
+        bool found = false;
+        for (int64_t j = 0; j < F; j++)
+            for (int k = 0; k < 3; k++)
+                if (faces[3 * j + k] == i) found = true;
+        if (!found) return -1;
+    }
+    double maxmass = 0;
+    for (int64_t i = 0; i < 3 * V; i++) maxmass = smax(maxmass, invm[i]);
+    double dt = substep;
+    double k = stiff / dt * std::sqrt(maxmass);
+    Layers al(outer, inner, dt, k, 1.0, cor, faces, F);
+    std::vector<uint8_t> fixed(V, 0);
+    bool any_fixed = false;
+    for (int64_t i = 0; i < 3 * V; i++)
+        if (invm[i] == 0.0) fixed[i / 3] = 1, any_fixed = true;
+    int64_t nvf = 0, nee = 0;
+    double closest = mesh_self_distance(qstart, V, faces, F, any_fixed ? fixed.data() : nullptr, &nvf, &nee);
+    st.closest = closest;
+    if (closest < inner) return -2;
+    for (int iter = 0; iter < max_rollbacks; iter++) {
+        SimState s;
+        s.q.assign(qstart, qstart + 3 * V);
+        s.v.resize(3 * V);
+        for (int64_t i = 0; i < 3 * V; i++) s.v[i] = qend[i] - qstart[i];
+        s.minv.assign(invm, invm + 3 * V);
+        s.last.assign(3 * V, 0.0);
+        st.iterations = iter + 1;
+        int r = al.run_one_iteration(s, st, nullptr);
+        st.layers = al.layers();
+        if (r < 0) return -100;
+        if (r == 1) {
+            for (int64_t i = 0; i < 3 * V; i++) qend[i] = s.q[i];
+            if (final_hist) al.release_history(final_hist);
+            return iter;
+        }
+    }
+    return -3;
+}
+
 }   // namespace
 
 // ==============================================================================================
@@ -1754,5 +2113,38 @@ double orc_mesh_self_distance(const double *verts, int64_t V, const int32_t *fac
 {
     return mesh_self_distance(verts, V, faces, F, fixed, nvf, nee);
 }
+
+// VelocityFilter::velocityFilter on host arrays; qend is overwritten on success.  stats receives
+// iterations, layers, firings, history entries of the last iteration, VF / EE candidates and collisions
+// summed over iterations, undecided stencils (9 int64).  *hist (optional) receives the final iteration's
+// history on success (free with orc_vhist_free).
+int orc_velocity_filter(const double *qstart, double *qend, int64_t V, const int32_t *faces, int64_t F,
+                        const double *invm, double outer, double inner, double stiff, double substep, double cor,
+                        int max_rollbacks, int64_t *stats, double *closest, void **hist)
+{
+    VFStats st;
+    VHist *h = nullptr;
+    int r = velocity_filter(qstart, qend, V, faces, F, invm, outer, inner, stiff, substep, cor, max_rollbacks, st,
+                            hist ? &h : nullptr);
+    if (stats) {
+        int64_t v[9] = {st.iterations, st.layers, st.firings, st.history_entries, st.vf_candidates,
+                        st.ee_candidates, st.vf_collisions, st.ee_collisions, st.undecided};
+        std::memcpy(stats, v, sizeof v);
+    }
+    if (closest) *closest = st.closest;
+    if (hist) *hist = h;
+    return r;
+}
+int64_t orc_vhist_entries(void *h) { return ((VHist *)h)->entries(); }
+void orc_vhist_copy(void *h, int64_t *off, double *t, double *p)
+{
+    std::vector<int64_t> o;
+    std::vector<double> tt, pp;
+    ((VHist *)h)->csr(o, tt, pp);
+    std::memcpy(off, o.data(), o.size() * sizeof(int64_t));
+    std::memcpy(t, tt.data(), tt.size() * sizeof(double));
+    std::memcpy(p, pp.data(), pp.size() * sizeof(double));
+}
+void orc_vhist_free(void *h) { delete (VHist *)h; }
 
 }   // extern "C"

@@ -1,0 +1,247 @@
+"""Velocity filter (include/VelocityFilter.h:7-45, src/VelocityFilter.cpp, src/ActiveLayers.cpp,
+src/PenaltyGroup.cpp, src/PenaltyPotential.cpp) and the drivers' OBJ input/output.
+
+CPU: the oracle's restatement is pinned by what the filter guarantees and what the physics fixes --
+the reference's return codes, free flight when nothing comes close, conservation of linear momentum
+(penalty forces are equal and opposite), fixed vertices that do not move, and a filtered motion that
+the CTCD narrowphase (a different algorithm from the separating-plane test the filter uses) finds
+collision-free at innerEta while the unfiltered motion is not.
+GPU: ccd_velocity_filter (host event loop, device collision queries) equals the oracle bit for bit --
+return value, filtered positions and every counter -- on seeded contact scenes, and the reference's own
+example (example/mesh1.obj -> mesh2.obj with testVelocityFilter's parameters) equals the committed
+golden result, through Python and through the C++ header."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from collisiondetection_amd import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+BUILD = os.path.join(ROOT, "tests", "_build")
+COUNTERS = ("iterations", "layers", "firings", "history_entries", "vf_candidates", "ee_candidates",
+            "vf_collisions", "ee_collisions")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _load_obj(path):
+    v, f = [], []
+    with open(path) as fh:
+        for line in fh:
+            t = line.split()
+            if t and t[0] == "v":
+                v.extend(float(x) for x in t[1:4])
+            elif t and t[0] == "f":
+                f.extend(int(x) - 1 for x in t[1:4])
+    return np.array(v, dtype=np.float64), np.array(f, dtype=np.int32)
+
+
+def _sheet_case(n, seed, fixed):
+    s = scenes.falling_sheets(n, seed)
+    im = np.ones(s["qstart"].size)
+    if fixed:
+        im[:3 * n * n] = 0.0   # the lower sheet has infinite mass
+    return s, im
+
+
+def _need_rpoly():
+    if not oracle.ref_rpoly_available():
+        pytest.skip("the reference's root finder (oracle/_ref) is not built")
+    oracle.use_rootfinder("rpoly")
+
+
+# ------------------------------------------------------------------------------------------------
+# CPU: the oracle's restatement
+# ------------------------------------------------------------------------------------------------
+def test_oracle_velocity_filter_return_codes():
+    s, im = _sheet_case(3, 0, False)
+    qs, qe, f = s["qstart"].reshape(-1), s["qend"].reshape(-1), s["faces"].reshape(-1)
+    # VelocityFilter.cpp:24-37: a vertex in no face
+    r, qo, _, _ = oracle.velocity_filter(np.append(qs, [5.0, 5, 5]), np.append(qe, [5.0, 5, 5]), f,
+                                         np.ones(qs.size + 3), s["outer_eta"], s["inner_eta"])
+    assert r == -1
+    # :58-64: closest pair (~0.097 apart) nearer than innerEta
+    r, qo, st, closest = oracle.velocity_filter(qs, qe, f, im, 0.5, 0.2)
+    assert r == -2 and 0.09 < closest < 0.1 and (qo == qe).all()
+    # :69-86: no iteration allowed
+    r, qo, st, _ = oracle.velocity_filter(qs, qe, f, im, s["outer_eta"], s["inner_eta"], max_rollbacks=0)
+    assert r == -3 and (qo == qe).all() and st["iterations"] == 0
+    # one iteration is not enough here (the first one only finds the contacts)
+    r, qo, st, _ = oracle.velocity_filter(qs, qe, f, im, s["outer_eta"], s["inner_eta"], max_rollbacks=1)
+    assert r == -3 and (qo == qe).all() and st["iterations"] == 1 and st["vf_collisions"] + st["ee_collisions"] > 0
+
+
+def test_oracle_velocity_filter_free_flight():
+    """Sheets that never come within outerEta: one iteration, no firing, q = qstart + (1 - 0) * v."""
+    s = scenes.falling_sheets(3, 1, gap=1.0, drop=0.2)
+    qs, qe = s["qstart"].reshape(-1), s["qend"].reshape(-1)
+    r, qo, st, _ = oracle.velocity_filter(qs, qe, s["faces"], np.ones(qs.size), s["outer_eta"], s["inner_eta"])
+    assert r == 0 and st["iterations"] == 1 and st["firings"] == 0 and st["layers"] == 0
+    assert (qo == qs + 1.0 * (qe - qs)).all()
+
+
+def test_oracle_velocity_filter_momentum_and_separation():
+    """Unit masses: total momentum, hence the sum of end positions, is that of the unfiltered motion; the
+    filtered trajectory keeps every stencil innerEta apart where the unfiltered one does not."""
+    s, im = _sheet_case(3, 0, False)
+    qs, qe, f = s["qstart"].reshape(-1), s["qend"].reshape(-1), s["faces"]
+    r, qo, st, _, h = oracle.velocity_filter(qs, qe, f, im, s["outer_eta"], s["inner_eta"], want_history=True)
+    say("sheets3 unit masses", r, st)
+    assert r >= 1 and st["vf_collisions"] + st["ee_collisions"] > 0 and np.abs(qo - qe).max() > 0.05
+    drift = np.abs(qo.reshape(-1, 3).sum(0) - qe.reshape(-1, 3).sum(0)).max()
+    assert drift < 1e-12, drift
+    vf, ee = oracle.candidates(f, h, s["outer_eta"])
+    assert not oracle.narrowphase(h, vf, s["inner_eta"], False)["hit"].any()
+    assert not oracle.narrowphase(h, ee, s["inner_eta"], True)["hit"].any()
+    h0 = oracle.Hist.two_entry(qs, qe)
+    vf0, ee0 = oracle.candidates(f, h0, s["outer_eta"])
+    assert oracle.narrowphase(h0, vf0, s["inner_eta"], False)["hit"].any()
+    assert oracle.narrowphase(h0, ee0, s["inner_eta"], True)["hit"].any()
+    # the straight-line motion to the filtered end positions is collision-free as well
+    h1 = oracle.Hist.two_entry(qs, qo)
+    vf1, ee1 = oracle.candidates(f, h1, s["outer_eta"])
+    assert not oracle.narrowphase(h1, vf1, s["inner_eta"], False)["hit"].any()
+    assert not oracle.narrowphase(h1, ee1, s["inner_eta"], True)["hit"].any()
+
+
+def test_oracle_velocity_filter_fixed_vertices():
+    """Infinite-mass vertices keep their (zero) velocity, so they end exactly where they started; the
+    free sheet is pushed back."""
+    s, im = _sheet_case(3, 2, True)
+    qs, qe = s["qstart"].reshape(-1), s["qend"].reshape(-1)
+    r, qo, st, _ = oracle.velocity_filter(qs, qe, s["faces"], im, s["outer_eta"], s["inner_eta"])
+    assert r >= 1
+    assert (qo[:27] == qe[:27]).all()
+    assert np.abs(qo[27:] - qe[27:]).max() > 0.05
+    assert (qo[27:].reshape(-1, 3)[:, 2] > qe[27:].reshape(-1, 3)[:, 2]).all()   # pushed up
+
+
+def test_obj_load_and_write():
+    """ccdamd::loadMesh / writeMesh (include/ccdamd/ObjIO.h) as the drivers' loadMesh / writeMesh:
+    the reference's example mesh loads to the file's values; the written file has 6 significant digits."""
+    subprocess.run(["bash", os.path.join(ROOT, "tests", "native", "build_cpp_driver.sh")], check=True)
+    exe = os.path.join(BUILD, "objio_check")
+    src = os.path.join(GOLDEN, "velocity_filter_start.obj")
+    tmp = os.path.join(BUILD, "objio_tmp")
+    subprocess.run([exe, src, tmp + ".f64", tmp + ".i32", tmp + ".obj"], check=True, timeout=60)
+    v = np.fromfile(tmp + ".f64", dtype="<f8")
+    f = np.fromfile(tmp + ".i32", dtype="<i4")
+    v0, f0 = _load_obj(src)
+    assert v.size == 3 * 197 and f.size == 3 * 386
+    assert (v == v0).all() and (f == f0).all()
+    lines = open(tmp + ".obj").read().splitlines()
+    assert len(lines) == 197 + 386
+    assert lines[0] == "v " + " ".join("%g" % x for x in v0[:3]) + " "
+    assert lines[197] == "f " + " ".join(str(x + 1) for x in f0[:3]) + " "
+
+
+# ------------------------------------------------------------------------------------------------
+# GPU: ccd_velocity_filter
+# ------------------------------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def api():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    from collisiondetection_amd import api as A
+
+    return A
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,seed,fixed", [(3, 0, False), (4, 1, False), (4, 2, True), (5, 3, True)])
+def test_gpu_velocity_filter_vs_oracle(api, n, seed, fixed):
+    _need_rpoly()
+    s, im = _sheet_case(n, seed, fixed)
+    say("case: velocity filter, falling sheets", n, "seed", seed, "fixed", fixed)
+    qs, qe, f = s["qstart"].reshape(-1), s["qend"].reshape(-1), s["faces"].reshape(-1)
+    r, qo, st = api.velocity_filter(qs, qe, f, im, s["outer_eta"], s["inner_eta"])
+    say("gpu", r, {k: st[k] for k in COUNTERS}, "host s %.3f device s %.3f" % (st["host_seconds"], st["device_seconds"]))
+    r0, q0, st0, c0 = oracle.velocity_filter(qs, qe, f, im, s["outer_eta"], s["inner_eta"])
+    say("oracle", r0, st0)
+    assert r == r0 and r >= 1
+    assert st["closest"] == c0
+    assert {k: st[k] for k in COUNTERS} == {k: st0[k] for k in COUNTERS}
+    assert np.array_equal(qo.view(np.int64), q0.view(np.int64))
+
+
+@pytest.mark.gpu
+def test_gpu_velocity_filter_return_codes(api):
+    from collisiondetection_amd import _native as N
+
+    s, im = _sheet_case(3, 0, False)
+    qs, qe, f = s["qstart"].reshape(-1), s["qend"].reshape(-1), s["faces"].reshape(-1)
+    say("case: velocity filter return codes")
+    r, qo, _ = api.velocity_filter(np.append(qs, [5.0, 5, 5]), np.append(qe, [5.0, 5, 5]), f, np.ones(qs.size + 3),
+                                   s["outer_eta"], s["inner_eta"])
+    assert r == -1
+    r, qo, st = api.velocity_filter(qs, qe, f, im, 0.5, 0.2)
+    assert r == -2 and (qo == qe).all() and st["closest"] == oracle.mesh_self_distance(qs, f)[0]
+    r, qo, st = api.velocity_filter(qs, qe, f, im, s["outer_eta"], s["inner_eta"], max_rollbacks=1)
+    assert r == -3 and (qo == qe).all() and st["iterations"] == 1
+    bad = f.copy()
+    bad[4] = qs.size // 3
+    with pytest.raises(N.CCDError):
+        api.velocity_filter(qs, qe, bad, im, s["outer_eta"], s["inner_eta"])
+
+
+def _golden():
+    g = np.load(os.path.join(GOLDEN, "velocity_filter_example.npz"))
+    q1, f = _load_obj(os.path.join(GOLDEN, "velocity_filter_start.obj"))
+    q2, _ = _load_obj(os.path.join(GOLDEN, "velocity_filter_end.obj"))
+    return g, q1, q2, f
+
+
+@pytest.mark.gpu
+def test_gpu_velocity_filter_reference_example(api):
+    """example/testVelocityFilter.cpp on the reference's mesh1.obj -> mesh2.obj: the oracle's result with
+    the reference's root finder (tests/golden/make_velocity_filter_golden.py), bit for bit."""
+    g, q1, q2, f = _golden()
+    say("case: velocity filter, reference example (197 vertices, 386 faces, eta 2e-8 / 1e-8)")
+    r, qo, st = api.velocity_filter(q1, q2, f, np.ones_like(q1), 2e-8, 1e-8)
+    say("gpu", r, {k: st[k] for k in COUNTERS}, "host s %.3f device s %.3f" % (st["host_seconds"], st["device_seconds"]))
+    assert r == int(g["result"]) == 7
+    assert st["closest"] == float(g["closest"])
+    assert {k: st[k] for k in COUNTERS} == {k: int(g[k]) for k in COUNTERS}
+    assert np.array_equal(qo.view(np.int64), g["qend"].view(np.int64))
+
+
+@pytest.mark.gpu
+def test_gpu_velocity_filter_cpp_driver(tmp_path):
+    """The C++ header (include/ccdamd/VelocityFilter.h, Distance.h, ObjIO.h) driven as testVelocityFilter
+    drives the reference: same return value and end positions as the golden, and the Distance statics
+    equal the oracle's."""
+    exe = os.path.join(BUILD, "velocity_filter_driver")
+    if not os.path.exists(exe):
+        subprocess.run(["bash", os.path.join(ROOT, "tests", "native", "build_cpp_driver.sh")], check=True)
+    g, q1, q2, f = _golden()
+    out = tmp_path / "out"
+    say("case: velocity filter C++ driver on the reference example")
+    p = subprocess.run([exe, os.path.join(GOLDEN, "velocity_filter_start.obj"),
+                        os.path.join(GOLDEN, "velocity_filter_end.obj"), "0", "2e-8", "1e-8", "0.0001",
+                        str(out) + ".f64", str(out) + ".obj", str(out) + ".txt"], capture_output=True, text=True,
+                       timeout=300)
+    say(p.stdout[-2000:], p.stderr[-2000:])
+    assert p.returncode == 0
+    assert "Closest primitive pair is distance" in p.stdout and "Taking an outer iteration" in p.stdout
+    res = dict((l.split()[0], l.split()[1:]) for l in open(str(out) + ".txt").read().splitlines())
+    assert int(res["result"][0]) == 7
+    assert float(res["closest"][0]) == float(g["closest"])
+    assert np.array_equal(np.fromfile(str(out) + ".f64", dtype="<f8").view(np.int64), g["qend"].view(np.int64))
+    a, b, c, p_, r_, s_ = ([0, 0, 0], [1, 0, 0], [0, 1, 0], [0.25, 0.25, 0.5], [2, 2, 1], [2, 3, 1])
+    vec, bary = oracle.distance(2, np.array(p_ + a + b + c, dtype=np.float64))
+    assert [float(x) for x in res["vf"]] == list(vec) + list(bary[:3])
+    vec, bary = oracle.distance(3, np.array(a + b + r_ + s_, dtype=np.float64))
+    assert [float(x) for x in res["ee"]] == list(vec) + list(bary)
+    q = np.array(p_ + a + b + c, dtype=np.float64)
+    assert res["plane"] == [str(int(oracle.distance(0, q, 0.6))), str(int(oracle.distance(0, q, 0.4)))]
+    q = np.array(a + b + r_ + s_, dtype=np.float64)
+    assert res["line"] == [str(int(oracle.distance(1, q, 1.5))), str(int(oracle.distance(1, q, 0.5)))]
