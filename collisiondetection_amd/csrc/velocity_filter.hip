@@ -17,6 +17,15 @@
 // (ActiveLayers.cpp:88-90, :111; PenaltyGroup.cpp:36-37), whose entries outside the group are never
 // read.  A firing in which no penalty fires leaves the state as it was (every force entry is +0, so no
 // vertex counts as touched, ActiveLayers.cpp:116-123) and is cut short after the penalty tests.
+//
+// Quiet groups.  Most firings apply no force: a penalty needs its stencil closer than the layer's
+// thickness.  After a firing in which nothing fired, the group records a time before which none of its
+// stencils can get that close while its vertices keep their velocities (the distance between a vertex
+// and a triangle, or between two segments, changes no faster than the sum of the largest speeds on
+// either side), with an absolute margin far above the rounding error of the distance evaluation.  Until
+// then its firings only advance its step counter -- exactly what an evaluation would have done -- and
+// any touch of one of its vertices by any group cancels the record.  The firing sequence, the priority
+// queue's pushes and pops and every result stay those of the full evaluation.
 #include <algorithm>
 #include <chrono>
 #include <climits>
@@ -53,10 +62,17 @@ struct KeyLess {   // Stencils.h:59-77, :119-137: lexicographic on the four indi
 struct Layer {
     double dt, outer, inner, k, cor;
     int next = 1;
+    double t_next = 0;                 // next * dt (PenaltyGroup.cpp:59-62), kept with next
+    double quiet_until = -INFINITY;    // no penalty of this group can fire before this time
     std::vector<Key4> vf, ee;
     std::vector<int32_t> verts, vf_slot, ee_slot;
     bool dirty = false;
-    double fire_time() const { return next * dt; }   // PenaltyGroup.cpp:59-62
+    double fire_time() const { return t_next; }
+    void set_next(int n)
+    {
+        next = n;
+        t_next = next * dt;
+    }
 
     void rebuild()
     {
@@ -76,7 +92,7 @@ struct Layer {
     }
 };
 struct LayerCmp {   // ActiveLayers.cpp:18-21: the group with the earliest next firing on top
-    bool operator()(const Layer *a, const Layer *b) const { return b->fire_time() < a->fire_time(); }
+    bool operator()(const Layer *a, const Layer *b) const { return b->t_next < a->t_next; }
 };
 
 inline V3 ld(const double *x, int32_t slot) { return ccd::mk(x[3 * slot], x[3 * slot + 1], x[3 * slot + 2]); }
@@ -173,6 +189,7 @@ public:
             hist_[v].clear();
             hist_[v].push_back(HEntry{0.0, s.q[3 * v], s.q[3 * v + 1], s.q[3 * v + 2]});
         }
+        index_groups();
         Clock::time_point t0 = Clock::now();
         while (!step(s)) {
         }
@@ -210,6 +227,7 @@ private:
             g->inner = inner_;
             g->dt = dt_ / double(depth) / std::sqrt(double(depth) + fudge);
             g->cor = cor_;
+            g->set_next(1);
             queue_.push(g.get());
             layers_.push_back(std::move(g));
             deepest_++;
@@ -231,7 +249,7 @@ private:
 
     void rollback()
     {   // ActiveLayers.cpp:152-165
-        for (auto &g : layers_) g->next = 1;
+        for (auto &g : layers_) g->set_next(1);
         std::priority_queue<Layer *, std::vector<Layer *>, LayerCmp> nq;
         while (!queue_.empty()) {
             nq.push(queue_.top());
@@ -255,7 +273,7 @@ private:
             if (newtime < 1.0) {
                 queue_.pop();
                 fire(*g, s, newtime);
-                g->next++;
+                g->set_next(g->next + 1);
                 queue_.push(g);
                 return false;
             }
@@ -269,9 +287,66 @@ private:
         return true;
     }
 
+    // Vertex -> groups containing it (CSR over layer indices), and every group's quiet record cleared.
+    void index_groups()
+    {
+        vg_off_.assign(V_ + 1, 0);
+        for (auto &g : layers_) {
+            if (g->dirty) g->rebuild();
+            g->quiet_until = -INFINITY;
+            for (int32_t v : g->verts) vg_off_[v + 1]++;
+        }
+        for (int64_t v = 0; v < V_; v++) vg_off_[v + 1] += vg_off_[v];
+        vg_idx_.resize(vg_off_[V_]);
+        std::vector<int64_t> pos(vg_off_.begin(), vg_off_.end() - 1);
+        for (size_t i = 0; i < layers_.size(); i++)
+            for (int32_t v : layers_[i]->verts) vg_idx_[pos[v]++] = (int32_t)i;
+    }
+
+    // A time before which no stencil of g can come within g.outer, from the group's positions nq_ at t0
+    // and its vertices' velocities; -inf when some stencil is already that close (or not a number).
+    double quiet_bound(const Layer &g, const State &s, double t0) const
+    {
+        auto speed = [&](int32_t slot) {
+            const double *v = &s.v[3 * (size_t)g.verts[slot]];
+            return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        };
+        auto mag = [&](int32_t slot) {
+            const double *x = &nq_[3 * (size_t)slot];
+            return std::max(std::fabs(x[0]), std::max(std::fabs(x[1]), std::fabs(x[2])));
+        };
+        double lim = INFINITY;
+        for (int pass = 0; pass < 2; pass++) {
+            const bool ee = pass == 1;
+            const std::vector<int32_t> &slots = ee ? g.ee_slot : g.vf_slot;
+            for (size_t i = 0; i < slots.size(); i += 4) {
+                const int32_t *sl = &slots[i];
+                V3 a = ld(nq_.data(), sl[0]), b = ld(nq_.data(), sl[1]), c = ld(nq_.data(), sl[2]),
+                   d = ld(nq_.data(), sl[3]);
+                double w0, w1, w2, w3, S;
+                double dist;
+                if (ee) {
+                    dist = ccd::norm(ccd::edge_edge_distance(a, b, c, d, w0, w1, w2, w3));
+                    S = std::max(speed(sl[0]), speed(sl[1])) + std::max(speed(sl[2]), speed(sl[3]));
+                } else {
+                    dist = ccd::norm(ccd::vertex_face_distance(a, b, c, d, w0, w1, w2));
+                    S = speed(sl[0]) + std::max(speed(sl[1]), std::max(speed(sl[2]), speed(sl[3])));
+                }
+                // Rounding of positions and distances is a few ulps of the coordinates' magnitude over
+                // the rest of [0, 1]; the margin is 10^6 times that.
+                double scale = std::max(std::max(mag(sl[0]), mag(sl[1])), std::max(mag(sl[2]), mag(sl[3]))) + S;
+                double gap = dist - g.outer - 1e-10 * (1.0 + scale);
+                if (!(gap > 0)) return -INFINITY;
+                if (S > 0) lim = std::min(lim, gap / (S * (1.0 + 1e-9)));
+            }
+        }
+        return t0 + 0.999999 * lim;
+    }
+
     void fire(Layer &g, State &s, double newtime)
     {   // ActiveLayers.cpp:94-137 with PenaltyGroup::addForce (PenaltyGroup.cpp:35-53)
         st_.firings++;
+        if (newtime < g.quiet_until) return;
         if (g.dirty) g.rebuild();
         const size_t m = g.verts.size();
         if (m == 0) return;
@@ -291,7 +366,10 @@ private:
             fired |= vf_penalty(nq_.data(), nv_.data(), gf_.data(), &g.vf_slot[4 * i], g.outer, g.inner, g.k, g.cor);
         for (size_t i = 0; i < g.ee.size(); i++)
             fired |= ee_penalty(nq_.data(), nv_.data(), gf_.data(), &g.ee_slot[4 * i], g.outer, g.inner, g.k, g.cor);
-        if (!fired) return;
+        if (!fired) {
+            g.quiet_until = quiet_bound(g, s, newtime);
+            return;
+        }
         for (size_t i = 0; i < m; i++) {
             double F[3];
             bool touched = false;
@@ -307,6 +385,8 @@ private:
                 s.last[b + j] = newtime;
             }
             hist_[g.verts[i]].push_back(HEntry{newtime, s.q[b], s.q[b + 1], s.q[b + 2]});
+            for (int64_t j = vg_off_[g.verts[i]]; j < vg_off_[g.verts[i] + 1]; j++)
+                layers_[vg_idx_[j]]->quiet_until = -INFINITY;   // its velocity changed
         }
     }
 
@@ -407,6 +487,8 @@ private:
     std::map<Key4, int, KeyLess> vfdepth_, eedepth_;
     std::vector<std::vector<HEntry>> hist_;
     std::vector<double> nq_, nv_, gf_;
+    std::vector<int64_t> vg_off_;
+    std::vector<int32_t> vg_idx_;
     std::vector<int64_t> off_;
     std::vector<double> ht_, hp_, eta_;
     std::vector<int32_t> keys_;
