@@ -124,6 +124,55 @@ def bench_separating_plane(ctx, scene, qs, qe, steps, log, cpu_sample=30000, see
     return out
 
 
+def bench_velocity_filter(ctx, log):
+    """VelocityFilter::velocityFilter: the reference's own example (example/mesh1.obj -> mesh2.obj, kept as
+    tests/golden/velocity_filter_{start,end}.obj, testVelocityFilter's parameters) and a falling-sheets
+    contact problem (5x5 sheets, lower one fixed), seconds per call; the oracle (the reference's shape,
+    1 thread) timed on the sheets."""
+    import numpy as np
+
+    import oracle
+    from collisiondetection_amd import api, scenes
+
+    def load_obj(path):
+        v, f = [], []
+        with open(path) as fh:
+            for line in fh:
+                t = line.split()
+                if t and t[0] == "v":
+                    v.extend(float(x) for x in t[1:4])
+                elif t and t[0] == "f":
+                    f.extend(int(x) - 1 for x in t[1:4])
+        return np.array(v), np.array(f, dtype=np.int32)
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden")
+    q1, f = load_obj(os.path.join(here, "velocity_filter_start.obj"))
+    q2, _ = load_obj(os.path.join(here, "velocity_filter_end.obj"))
+    s = scenes.falling_sheets(5, 3)
+    im = np.ones(s["qstart"].size)
+    im[:75] = 0.0
+    cases = {"reference_example": (q1, q2, f, np.ones_like(q1), 2e-8, 1e-8),
+             "falling_sheets5": (s["qstart"], s["qend"], s["faces"], im, s["outer_eta"], s["inner_eta"])}
+    out = {}
+    for name, (a, b, faces, masses, outer, inner) in cases.items():
+        t0 = time.perf_counter()
+        r, _, st = api.velocity_filter(a, b, faces, masses, outer, inner, ctx=ctx)
+        wall = time.perf_counter() - t0
+        log(f"velocity filter {name}: result {r}, {st['firings']} firings, {wall:.2f} s "
+            f"(host loop {st['host_seconds']:.2f} s, device {st['device_seconds']:.3f} s)")
+        out[name] = {"result": r, "seconds": wall, "host_seconds": st["host_seconds"],
+                     "device_seconds": st["device_seconds"], "iterations": st["iterations"],
+                     "layers": st["layers"], "firings": st["firings"]}
+    rf = oracle.use_default_rootfinder()
+    a, b, faces, masses, outer, inner = cases["falling_sheets5"]
+    t0 = time.perf_counter()
+    oracle.velocity_filter(a, b, faces, masses, outer, inner)
+    tc = time.perf_counter() - t0
+    out["falling_sheets5"]["cpu_baseline"] = {"value": tc, "unit": "s per call", "cores": 1, "kind": "port",
+                                              "sample": f"oracle (root finder '{rf}'), the whole call"}
+    return out
+
+
 def bench_batched(steps, log):
     """The 512-instance mixed workload: plain (ccd_detect per instance, the same kernels launched per
     instance) vs batched (one ccd_batch_detect pipeline) vs batched replayed from a captured HIP graph."""
@@ -292,6 +341,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-separating-plane", action="store_true")
+    ap.add_argument("--no-velocity-filter", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy 5 times (scale reference for HBM byte counters)")
@@ -464,6 +514,9 @@ def main():
     if not args.no_separating_plane:
         ctx.set_root_mode(N.ROOTS_REFERENCE)
         out["separating_plane"] = bench_separating_plane(ctx, scene, qs, qe, max(3, args.steps // 2), log)
+    if not args.no_velocity_filter:
+        ctx.set_root_mode(N.ROOTS_REFERENCE)
+        out["velocity_filter"] = bench_velocity_filter(ctx, log)
     if not args.no_batched:
         out["batched"] = bench_batched(max(3, args.steps // 2), log)
     if not args.no_served:
