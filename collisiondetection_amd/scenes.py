@@ -184,6 +184,20 @@ def falling_sheets(n: int = 4, seed: int = 0, gap: float = 0.1, drop: float = 0.
     return s
 
 
+def vertex_bounce(height: float = 0.1, drop: float = 0.2, outer_eta: float = 0.01, inner_eta: float = 0.005):
+    """A single contact: a large triangle lying in z = 0 (vertices 0-2) and a small upright triangle whose
+    lowest vertex (3) starts `height` above it and falls by `drop` with the rest of its triangle, whose
+    other vertices stay far above.  Only the vertex-face stencil (3; 0, 1, 2) comes within outer_eta."""
+    qstart = np.array([[-10.0, -10.0, 0.0], [10.0, -10.0, 0.0], [0.0, 10.0, 0.0],
+                       [0.1, 0.2, height], [0.1, 0.5, height + 0.5], [0.4, 0.2, height + 0.5]])
+    qend = qstart.copy()
+    qend[3:, 2] -= drop
+    s = _scene("vertex_bounce", np.array([[0, 1, 2], [3, 4, 5]]), qstart, qend, outer_eta)
+    s["outer_eta"] = float(outer_eta)
+    s["inner_eta"] = float(inner_eta)
+    return s
+
+
 def _rotation(rng, max_angle):
     axis = rng.normal(size=3)
     axis /= np.linalg.norm(axis)

@@ -3,9 +3,10 @@ src/PenaltyGroup.cpp, src/PenaltyPotential.cpp) and the drivers' OBJ input/outpu
 
 CPU: the oracle's restatement is pinned by what the filter guarantees and what the physics fixes --
 the reference's return codes, free flight when nothing comes close, conservation of linear momentum
-(penalty forces are equal and opposite), fixed vertices that do not move, and a filtered motion that
+(penalty forces are equal and opposite), fixed vertices that do not move, a filtered motion that
 the CTCD narrowphase (a different algorithm from the separating-plane test the filter uses) finds
-collision-free at innerEta while the unfiltered motion is not.
+collision-free at innerEta while the unfiltered motion is not, and for a single contact the turning
+point given by energy conservation over the layered springs and the exit speed sqrt(CoR) * v_in.
 GPU: ccd_velocity_filter (host event loop, device collision queries) equals the oracle bit for bit --
 return value, filtered positions and every counter -- on seeded contact scenes, and the reference's own
 example (example/mesh1.obj -> mesh2.obj with testVelocityFilter's parameters) equals the committed
@@ -124,6 +125,51 @@ def test_oracle_velocity_filter_fixed_vertices():
     assert (qo[27:].reshape(-1, 3)[:, 2] > qe[27:].reshape(-1, 3)[:, 2]).all()   # pushed up
 
 
+def _bounce_case(cor):
+    s = scenes.vertex_bounce()
+    im = np.ones(18)
+    im[:9] = 0.0   # the lower triangle has infinite mass
+    return s, im
+
+
+def _zmin_from_energy(v, outer, inner, k, layers):
+    """Lowest height of a vertex meeting a fixed plane at speed v under the layered penalty: layer d pushes
+    with k d^3 (eta_d - z) / (eta_d - inner) below eta_d = inner + (outer - inner) / d, a linear spring, so
+    the turning point is where the springs' stored energy equals the kinetic energy (bisection)."""
+    def excess(z):
+        e = 0.0
+        for d in range(1, layers + 1):
+            eta = inner + (outer - inner) / d
+            if z < eta:
+                e += 0.5 * k * d ** 3 / (eta - inner) * (eta - z) ** 2
+        return e - 0.5 * v * v
+    lo, hi = inner, outer
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if excess(mid) > 0 else (lo, mid)
+    return 0.5 * (lo + hi)
+
+
+@pytest.mark.parametrize("cor", [1.0, 0.25, 0.1])
+def test_oracle_velocity_filter_bounce_energy(cor):
+    """One vertex falling onto a fixed triangle: the layers act as linear springs, so the lowest point
+    is fixed by energy conservation, and the penalty is scaled by CoR while the vertex separates, so it
+    leaves with sqrt(CoR) times its incoming speed (stored energy returned in proportion CoR)."""
+    s, im = _bounce_case(cor)
+    qs, qe = s["qstart"].reshape(-1), s["qend"].reshape(-1)
+    outer, inner = s["outer_eta"], s["inner_eta"]
+    r, qo, st, _, h = oracle.velocity_filter(qs, qe, s["faces"], im, outer, inner, cor=cor, want_history=True)
+    assert r >= 1 and st["vf_collisions"] >= 1
+    b, e = h.off[3], h.off[4]
+    z, t = h.hp[3 * b:3 * e].reshape(-1, 3)[:, 2], h.ht[b:e]
+    k = 1e-4 / 1e-4 * 1.0   # baseStiffness / baseSubstepSize * sqrt(max inverse mass)
+    zmin = _zmin_from_energy(0.2, outer, inner, k, st["layers"])
+    assert abs(z.min() - zmin) < 1e-4 * (outer - zmin), (z.min(), zmin)
+    v_out = (z[-1] - z[-2]) / (t[-1] - t[-2])
+    assert abs(v_out / (np.sqrt(cor) * 0.2) - 1) < 1e-4, v_out
+    assert (qo[:9] == qe[:9]).all()
+
+
 def test_obj_load_and_write():
     """ccdamd::loadMesh / writeMesh (include/ccdamd/ObjIO.h) as the drivers' loadMesh / writeMesh:
     the reference's example mesh loads to the file's values; the written file has 6 significant digits."""
@@ -169,6 +215,21 @@ def test_gpu_velocity_filter_vs_oracle(api, n, seed, fixed):
     say("oracle", r0, st0)
     assert r == r0 and r >= 1
     assert st["closest"] == c0
+    assert {k: st[k] for k in COUNTERS} == {k: st0[k] for k in COUNTERS}
+    assert np.array_equal(qo.view(np.int64), q0.view(np.int64))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cor", [1.0, 0.1])
+def test_gpu_velocity_filter_bounce_vs_oracle(api, cor):
+    _need_rpoly()
+    s, im = _bounce_case(cor)
+    say("case: velocity filter, vertex bounce, CoR", cor)
+    qs, qe, f = s["qstart"].reshape(-1), s["qend"].reshape(-1), s["faces"].reshape(-1)
+    r, qo, st = api.velocity_filter(qs, qe, f, im, s["outer_eta"], s["inner_eta"], cor=cor)
+    r0, q0, st0, c0 = oracle.velocity_filter(qs, qe, f, im, s["outer_eta"], s["inner_eta"], cor=cor)
+    say("gpu", r, {k: st[k] for k in COUNTERS}, "oracle", r0, st0)
+    assert r == r0 and r >= 1 and st["closest"] == c0
     assert {k: st[k] for k in COUNTERS} == {k: st0[k] for k in COUNTERS}
     assert np.array_equal(qo.view(np.int64), q0.view(np.int64))
 
