@@ -275,6 +275,25 @@ def test_gpu_velocity_filter_reference_example(api):
     assert np.array_equal(qo.view(np.int64), g["qend"].view(np.int64))
 
 
+def test_oracle_velocity_filter_nonterminating_recursion():
+    """The reference's example with a 100x longer substep: in the third pass the separating-plane
+    recursion meets a sub-interval equal to its parent, where the reference recurses until its stack
+    overflows; the oracle reports it (-100) instead."""
+    g, q1, q2, f = _golden()
+    r, qo, st, _ = oracle.velocity_filter(q1, q2, f, np.ones_like(q1), 2e-8, 1e-8, base_substep=1e-2)
+    assert r == -100 and st["undecided"] >= 1 and st["iterations"] == 3 and (qo == q2).all()
+
+
+@pytest.mark.gpu
+def test_gpu_velocity_filter_nonterminating_recursion_raises(api):
+    from collisiondetection_amd import _native as N
+
+    g, q1, q2, f = _golden()
+    say("case: velocity filter, reference example with substep 1e-2 (non-terminating recursion)")
+    with pytest.raises(N.CCDError, match="does not terminate"):
+        api.velocity_filter(q1, q2, f, np.ones_like(q1), 2e-8, 1e-8, base_substep=1e-2)
+
+
 @pytest.mark.gpu
 def test_gpu_velocity_filter_cpp_driver(tmp_path):
     """The C++ header (include/ccdamd/VelocityFilter.h, Distance.h, ObjIO.h) driven as testVelocityFilter
