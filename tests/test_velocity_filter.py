@@ -170,13 +170,13 @@ def test_oracle_velocity_filter_bounce_energy(cor):
     assert (qo[:9] == qe[:9]).all()
 
 
-def test_obj_load_and_write():
+def test_obj_load_and_write(tmp_path):
     """ccdamd::loadMesh / writeMesh (include/ccdamd/ObjIO.h) as the drivers' loadMesh / writeMesh:
     the reference's example mesh loads to the file's values; the written file has 6 significant digits."""
     subprocess.run(["bash", os.path.join(ROOT, "tests", "native", "build_cpp_driver.sh")], check=True)
     exe = os.path.join(BUILD, "objio_check")
     src = os.path.join(GOLDEN, "velocity_filter_start.obj")
-    tmp = os.path.join(BUILD, "objio_tmp")
+    tmp = str(tmp_path / "objio")
     subprocess.run([exe, src, tmp + ".f64", tmp + ".i32", tmp + ".obj"], check=True, timeout=60)
     v = np.fromfile(tmp + ".f64", dtype="<f8")
     f = np.fromfile(tmp + ".i32", dtype="<i4")
