@@ -298,11 +298,14 @@ int ccd_separating_plane_narrowphase(ccd_ctx *ctx, const int64_t *hist_off, cons
     if (rc) return rc;
     rc = launch_min_gap(ctx, hist_off, hist_t, nverts, dev);
     if (rc) return rc;
+    // Long histories (16 or more entries per vertex on average, as the velocity filter builds): one
+    // wavefront per stencil shares each trajectory scan; short ones: one lane per stencil.
+    const bool wave = nentries >= 16 * nverts;
     rc = launch_separating_plane(ctx, true, hist_off, hist_t, hist_pos, vf_keys, vf_eta, eta_scalar, nvf, dev,
-                                 vf_status, (int64_t *)(dev + 1));
+                                 vf_status, (int64_t *)(dev + 1), wave);
     if (rc) return rc;
     rc = launch_separating_plane(ctx, false, hist_off, hist_t, hist_pos, ee_keys, ee_eta, eta_scalar, nee, dev,
-                                 ee_status, (int64_t *)(dev + 1));
+                                 ee_status, (int64_t *)(dev + 1), wave);
     if (rc) return rc;
     int64_t und = 0;
     CCD_HIP_CHECK(ctx, hipMemcpyAsync(&und, dev + 1, 8, hipMemcpyDeviceToHost, ctx->stream));

@@ -1260,8 +1260,20 @@ CCD_HD double plane_trajectory_intersect(const HistCSR &h, int v, int start, boo
 // recursion is a pure function of the interval) or one more than SP_MAX_DEPTH levels down (each level
 // at most halves the interval; only an interval closing in on a history entry time gets that deep).
 constexpr int SP_MAX_DEPTH = 160;
-template <bool VF, class C = NoCount>
-CCD_HD int separating_plane_check(const HistCSR &h, const int (&v)[4], double eta, double eps, C &&cnt = C())
+
+// The trajectory scan one lane runs by itself.  A scanner with the same call operator may split the
+// scan over the lanes of a wavefront (every lane then runs the same check on the same stencil).
+struct SeqScan {
+    CCD_HD double operator()(const HistCSR &h, int v, int start, bool forward, V3 ppos, V3 pvel, V3 nrm, V3 ptstart,
+                             double tstart, double eta) const
+    {
+        return plane_trajectory_intersect(h, v, start, forward, ppos, pvel, nrm, ptstart, tstart, eta);
+    }
+};
+
+template <bool VF, class C = NoCount, class Scan = SeqScan>
+CCD_HD int separating_plane_check(const HistCSR &h, const int (&v)[4], double eta, double eps, C &&cnt = C(),
+                                  Scan scan = Scan())
 {
     const double inf = bits_to_double(0x7ff0000000000000ll);
     // pending intervals and their depths; a node at depth d leaves at most d pending uppers below it
@@ -1339,13 +1351,13 @@ CCD_HD int separating_plane_check(const HistCSR &h, const int (&v)[4], double et
         for (int j = 0; j < 4; j++) {
             double sign = (j == 0 || (j == 1 && !VF)) ? -1.0 : 1.0;
             V3 nrm = sign * closest;
-            t_lo = smin(t_lo, plane_trajectory_intersect(h, v[j], mi[j], false, ppos, pvel, nrm, mp[j], midt, eta));
+            t_lo = smin(t_lo, scan(h, v[j], mi[j], false, ppos, pvel, nrm, mp[j], midt, eta));
         }
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             double sign = (j == 0 || (j == 1 && !VF)) ? -1.0 : 1.0;
             V3 nrm = sign * closest;
-            t_hi = smin(t_hi, plane_trajectory_intersect(h, v[j], mi[j] + 1, true, ppos, pvel, nrm, mp[j], midt, eta));
+            t_hi = smin(t_hi, scan(h, v[j], mi[j] + 1, true, ppos, pvel, nrm, mp[j], midt, eta));
         }
         // Depth first, lower sub-interval first: push the upper one underneath.  A child equal to its
         // node is undecided when it is reached (the upper one only if the lower subtree finds nothing).

@@ -122,9 +122,10 @@ int group_by_signature(ccd_ctx *ctx, bool vf, const double *qs, const double *qe
 // Separating-plane narrowphase over a CSR history: launch_min_gap writes History::computeMinimumGap's
 // value (as bits) to *gap_bits on the device; launch_separating_plane reads it (eps = gap / 4).
 int launch_min_gap(ccd_ctx *ctx, const int64_t *off, const double *ht, int64_t nverts, uint64_t *gap_bits);
+// wave: one wavefront per stencil sharing the trajectory scans (for long histories), else one lane.
 int launch_separating_plane(ccd_ctx *ctx, bool vf, const int64_t *off, const double *ht, const double *hp,
                             const int32_t *keys, const double *eta, double eta_s, int64_t n, const uint64_t *gap_bits,
-                            uint8_t *status, int64_t *undecided);
+                            uint8_t *status, int64_t *undecided, bool wave);
 int launch_validate_keys(ccd_ctx *ctx, const int32_t *keys, int64_t n4, int64_t nverts, int64_t *bad);
 int launch_validate_history(ccd_ctx *ctx, const int64_t *off, const double *ht, int64_t nverts, int64_t nentries,
                             int64_t *bad);

@@ -395,6 +395,86 @@ __global__ __launch_bounds__(256) CCD_TWO_WAVES void k_separating_plane(
     if (m && __lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(undecided, (unsigned long long)__popcll(m));
 }
 
+// The trajectory scan of planeTrajectoryIntersect (SeparatingPlaneNarrowPhase.cpp:216-265) split over
+// the 64 lanes of a wavefront that all run the same stencil: each lane tests one history segment of a
+// 64-segment chunk, and the first segment in scan order that meets the plane (the lowest lane whose
+// test passes, in the first chunk with any) gives the result, computed by that lane with the
+// sequential scan's own expressions.  Needs the whole wavefront active and on the same call.
+struct WaveScan {
+    __device__ double operator()(const ccd::HistCSR &h, int v, int start, bool forward, ccd::V3 ppos, ccd::V3 pvel,
+                                 ccd::V3 nrm, ccd::V3 ptstart, double tstart, double eta) const
+    {
+        using ccd::V3;
+        const double inf = ccd::bits_to_double(0x7ff0000000000000ll);
+        const int64_t b = h.off[v];
+        const int n = (int)(h.off[v + 1] - b);
+        const double dt = forward ? h.t[b + start] - tstart : tstart - h.t[b + start];
+        const double t = ccd::plane_intersect(ppos, (forward ? 1.0 : -1.0) * pvel, nrm, ptstart,
+                                              ccd::hist_pos(h, b + start), dt, eta);
+        if (t >= 0 && t <= dt) return t;
+        const int lane = (int)__lane_id();
+        if (forward) {
+            // segments (i, i + 1) for i = start .. n - 2, in increasing i
+            for (int base = start; base < n - 1; base += 64) {
+                const int i = base + lane;
+                bool hit = false;
+                double r = 0.0;
+                if (i < n - 1) {
+                    V3 npp = ppos + (h.t[b + i] - tstart) * pvel;
+                    double d = h.t[b + i + 1] - h.t[b + i];
+                    double u = ccd::plane_intersect(npp, pvel, nrm, ccd::hist_pos(h, b + i), ccd::hist_pos(h, b + i + 1),
+                                                    d, eta);
+                    hit = u >= 0 && u <= d;
+                    r = h.t[b + i] + u - tstart;
+                }
+                const unsigned long long m = __ballot(hit);
+                if (m) return __shfl(r, __ffsll(m) - 1);
+            }
+        } else {
+            // segments (i, i - 1) for i = start .. 1, in decreasing i
+            for (int base = start; base >= 1; base -= 64) {
+                const int i = base - lane;
+                bool hit = false;
+                double r = 0.0;
+                if (i >= 1) {
+                    V3 npp = ppos - (tstart - h.t[b + i]) * pvel;
+                    double d = h.t[b + i] - h.t[b + i - 1];
+                    double u = ccd::plane_intersect(npp, -1.0 * pvel, nrm, ccd::hist_pos(h, b + i),
+                                                    ccd::hist_pos(h, b + i - 1), d, eta);
+                    hit = u >= 0 && u <= d;
+                    r = tstart - (h.t[b + i] - u);
+                }
+                const unsigned long long m = __ballot(hit);
+                if (m) return __shfl(r, __ffsll(m) - 1);
+            }
+        }
+        return inf;
+    }
+};
+
+// SeparatingPlaneNarrowPhase with one wavefront per stencil, for long histories: the lanes share the
+// trajectory scans (WaveScan) and otherwise compute the same values, so control flow stays uniform.
+template <bool VF, class M>
+__global__ __launch_bounds__(64) CCD_TWO_WAVES void k_separating_plane_wave(
+    const int64_t *__restrict__ off, const double *__restrict__ ht, const double *__restrict__ hp,
+    const int4 *__restrict__ keys, const double *__restrict__ eta, double eta_s, int64_t n,
+    const unsigned long long *__restrict__ gap_bits, uint8_t *__restrict__ status, unsigned long long *undecided)
+{
+    const int64_t i = blockIdx.x;
+    if (i >= n) return;
+    const double eps = ccd::bits_to_double((long long)*gap_bits) / 4.0;
+    const int4 k = keys[i];
+    const int v[4] = {k.x, k.y, k.z, k.w};
+    ccd::HistCSR h{off, ht, hp};
+    using L = std::conditional_t<ccd::uses_ref_roots<M>::value, ccd::WithRefRoots<ccd::FullCascade>,
+                                 ccd::FullCascade>;
+    const int r = ccd::separating_plane_check<VF>(h, v, eta ? eta[i] : eta_s, eps, L(), WaveScan());
+    if (__lane_id() == 0) {
+        status[i] = (uint8_t)r;
+        if (r == 2) atomicAdd(undecided, 1ull);
+    }
+}
+
 inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // One wavefront per block for the kernels that give each lane one stencil of very uneven cost (the
@@ -581,11 +661,24 @@ int launch_min_gap(ccd_ctx *ctx, const int64_t *off, const double *ht, int64_t n
 
 int launch_separating_plane(ccd_ctx *ctx, bool vf, const int64_t *off, const double *ht, const double *hp,
                             const int32_t *keys, const double *eta, double eta_s, int64_t n, const uint64_t *gap_bits,
-                            uint8_t *status, int64_t *undecided)
+                            uint8_t *status, int64_t *undecided, bool wave)
 {
     if (n == 0) return CCD_OK;
     auto gb = (const unsigned long long *)gap_bits;
     auto und = (unsigned long long *)undecided;
+    if (wave) {
+        if (n > 0x7fffffffll) return set_error(ctx, CCD_E_INVALID, "too many stencils for one launch");
+        if (vf)
+            CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_separating_plane_wave<true, M>), dim3((unsigned)n), dim3(64),
+                                                      0, ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n,
+                                                      gb, status, und));
+        else
+            CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_separating_plane_wave<false, M>), dim3((unsigned)n),
+                                                      dim3(64), 0, ctx->stream, off, ht, hp, (const int4 *)keys, eta,
+                                                      eta_s, n, gb, status, und));
+        CCD_LAUNCH_CHECK(ctx);
+        return CCD_OK;
+    }
     if (vf)
         CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_separating_plane<true, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK),
                                                   0, ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, gb,

@@ -29,6 +29,14 @@ def api():
 
 
 def _case(kind):
+    if kind == "sheets_long_history":
+        # the history the velocity filter builds for two colliding sheets: ~9,500 entries per vertex,
+        # which the device runs with one wavefront per stencil sharing each trajectory scan
+        s = dict(scenes.falling_sheets(3, 0))
+        r, _, _, _, h = oracle.velocity_filter(s["qstart"], s["qend"], s["faces"], np.ones(s["qstart"].size),
+                                               s["outer_eta"], s["inner_eta"], want_history=True)
+        assert r >= 1 and len(h.ht) >= 16 * h.nverts
+        return s, h
     if kind == "cloth48_two_entry":
         s = scenes.cloth(48, drop=0.05, noise=0.01)
         return s, oracle.Hist.two_entry(s["qstart"], s["qend"])
@@ -36,7 +44,7 @@ def _case(kind):
     return s, oracle.Hist(*three_entry_history(s))
 
 
-@pytest.mark.parametrize("kind", ["cloth48_two_entry", "spheres_three_entry"])
+@pytest.mark.parametrize("kind", ["cloth48_two_entry", "spheres_three_entry", "sheets_long_history"])
 @pytest.mark.parametrize("mode", ["reference", "isolate"])
 def test_gpu_separating_plane_vs_oracle_and_core(api, core_harness, kind, mode):
     from collisiondetection_amd import _native as N
