@@ -265,11 +265,14 @@ int ccd_ctcd_narrowphase_history(ccd_ctx *ctx, const int64_t *hist_off, const do
     if (rc) return rc;
     rc = validate_keys(ctx, ee_keys, nee, nverts, "edge-edge keys reference vertices outside [0, nverts)");
     if (rc) return rc;
+    // Long histories (16 or more entries per vertex on average): one wavefront per stencil tests 64
+    // stitched windows at a time; short ones: one lane per stencil.
+    const bool wave = nentries >= 16 * nverts;
     rc = launch_narrow_hist(ctx, true, hist_off, hist_t, hist_pos, vf_keys, vf_eta, eta_scalar, nvf, vf_hit, vf_t,
-                            vf_sub, vf_win);
+                            vf_sub, vf_win, wave);
     if (rc) return rc;
     return launch_narrow_hist(ctx, false, hist_off, hist_t, hist_pos, ee_keys, ee_eta, eta_scalar, nee, ee_hit, ee_t,
-                              ee_sub, ee_win);
+                              ee_sub, ee_win, wave);
 }
 
 int ccd_separating_plane_narrowphase(ccd_ctx *ctx, const int64_t *hist_off, const double *hist_t,

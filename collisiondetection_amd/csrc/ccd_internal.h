@@ -99,9 +99,10 @@ int launch_primitives(ccd_ctx *ctx, int kind, const double *q, const double *eta
                       uint8_t *hit, double *t);
 int launch_narrow2(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
                    const double *eta, double eta_s, int64_t n, uint8_t *hit, double *t, uint8_t *sub);
+// wave: one wavefront per stencil sharing the stitched windows (for long histories), else one lane.
 int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *ht, const double *hp,
                        const int32_t *keys, const double *eta, double eta_s, int64_t n, uint8_t *hit, double *t,
-                       uint8_t *sub, int32_t *win);
+                       uint8_t *sub, int32_t *win, bool wave);
 // Narrowphase over a stencil list, compacting the colliding stencils (keys + toi) into out_keys /
 // out_toi with a device counter at counters[slot]; at most cap are written.
 int launch_narrow2_compact(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,

@@ -395,6 +395,105 @@ __global__ __launch_bounds__(256) CCD_TWO_WAVES void k_separating_plane(
     if (m && __lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(undecided, (unsigned long long)__popcll(m));
 }
 
+// The narrowphase over stitched history windows (k_narrow_hist) with one wavefront per stencil, for
+// long histories.  The stitching (History.cpp:98-140) is a merge of the four vertices' timelines; every
+// lane runs the same merge (uniform control flow, identical values) 65 steps at a time and keeps the
+// stitched positions of two consecutive steps, so lane l holds window W0 + l of a 64-window chunk and
+// tests it.  The first window in order that collides (the lowest set bit of a ballot, in the first
+// chunk with any) gives the answer, as the sequential loop's early exit does.
+template <bool VF, class M>
+__global__ __launch_bounds__(64) CCD_TWO_WAVES void k_narrow_hist_wave(
+    const int64_t *__restrict__ off, const double *__restrict__ ht, const double *__restrict__ hp,
+    const int4 *__restrict__ keys, const double *__restrict__ eta, double eta_s, int64_t n, uint8_t *__restrict__ hit,
+    double *__restrict__ tout, uint8_t *__restrict__ sub, int32_t *__restrict__ win)
+{
+    using ccd::V3;
+    const int64_t i = blockIdx.x;
+    if (i >= n) return;
+    const int lane = (int)__lane_id();
+    const int4 k = keys[i];
+    const int v[4] = {k.x, k.y, k.z, k.w};
+    const double e = eta ? eta[i] : eta_s;
+    int64_t its[4], ends[4];
+    int64_t total = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        its[j] = off[v[j]];
+        ends[j] = off[v[j] + 1];
+        total += ends[j] - its[j];
+    }
+    double curtime = 0.0;
+    int64_t g = 0;   // step index of the sequential loop (its guard); cur is evaluated at curtime
+    int r = -1, w = -1;
+    double t = 0.0;
+    bool alive = true;
+    while (alive) {
+        const int64_t w0 = g;   // this chunk: windows w0 .. w0 + 63, window w = (step w, step w + 1)
+        V3 pa[4], pb[4];
+        bool va = false, vb = false;
+        for (int s = 0; s <= 64; s++) {
+            alive = g <= total && curtime <= 1.0;
+            if (!alive) break;
+            V3 cur[4];
+            double newtime = __builtin_inf();
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                int64_t next = its[j] + 1;
+                while (next != ends[j] && ht[next] <= curtime) {
+                    ++next;
+                    ++its[j];
+                }
+                V3 oldp = ccd::mk(hp[3 * its[j]], hp[3 * its[j] + 1], hp[3 * its[j] + 2]);
+                if (next == ends[j]) {
+                    cur[j] = oldp;
+                } else {
+                    V3 newp = ccd::mk(hp[3 * next], hp[3 * next + 1], hp[3 * next + 2]);
+                    double dt = ht[next] - ht[its[j]];
+                    double a = curtime - ht[its[j]];
+                    double b = ht[next] - curtime;
+                    cur[j] = ccd::mk((b * oldp.x + a * newp.x) / dt, (b * oldp.y + a * newp.y) / dt,
+                                     (b * oldp.z + a * newp.z) / dt);
+                    newtime = ccd::smin(newtime, ht[next]);
+                }
+            }
+            if (s == lane) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) pa[j] = cur[j];
+                va = true;
+            }
+            if (s == lane + 1) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) pb[j] = cur[j];
+                vb = true;
+            }
+            if (s == 64) break;   // the next chunk starts by evaluating this step again (idempotent)
+            curtime = newtime;
+            g++;
+        }
+        bool h = false;
+        int rl = -1;
+        double tl = 0.0;
+        if (va && vb) {
+            rl = VF ? ccd::check_vf_window(pa, pb, e, tl, M()) : ccd::check_ee_window(pa, pb, e, tl, M());
+            h = rl >= 0;
+        }
+        const unsigned long long m = __ballot(h);
+        if (m) {
+            const int f = __ffsll(m) - 1;
+            r = __shfl(rl, f);
+            t = __shfl(tl, f);
+            w = (int)(w0 + f);
+            break;
+        }
+    }
+    if (lane == 0) {
+        hit[i] = r >= 0 ? 1 : 0;
+        if (tout) tout[i] = r >= 0 ? t : 0.0;
+        if (sub) sub[i] = r >= 0 ? (uint8_t)r : (uint8_t)255;
+        if (win) win[i] = r >= 0 ? w : -1;
+    }
+}
+
 // The trajectory scan of planeTrajectoryIntersect (SeparatingPlaneNarrowPhase.cpp:216-265) split over
 // the 64 lanes of a wavefront that all run the same stencil: each lane tests one history segment of a
 // 64-segment chunk, and the first segment in scan order that meets the plane (the lowest lane whose
@@ -602,9 +701,22 @@ int launch_narrow2_queue(ccd_ctx *ctx, bool vf, const double *qs, const double *
 
 int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *ht, const double *hp,
                        const int32_t *keys, const double *eta, double eta_s, int64_t n, uint8_t *hit, double *t,
-                       uint8_t *sub, int32_t *win)
+                       uint8_t *sub, int32_t *win, bool wave)
 {
     if (n == 0) return CCD_OK;
+    if (wave) {
+        if (n > 0x7fffffffll) return set_error(ctx, CCD_E_INVALID, "too many stencils for one launch");
+        if (vf)
+            CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow_hist_wave<true, M>), dim3((unsigned)n), dim3(64), 0,
+                                                      ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, hit,
+                                                      t, sub, win));
+        else
+            CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow_hist_wave<false, M>), dim3((unsigned)n), dim3(64), 0,
+                                                      ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, hit,
+                                                      t, sub, win));
+        CCD_LAUNCH_CHECK(ctx);
+        return CCD_OK;
+    }
     if (vf)
         CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow_hist<true, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK), 0,
                                                   ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, hit, t,

@@ -10,6 +10,7 @@ Bars: candidate sets bit-exact; the GPU core bit-identical to the same core comp
     finder misses (listed as ref_root_error).
 """
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -279,6 +280,32 @@ def test_gpu_narrowphase_history(api):
         hit = r[pre + "_hit"]
         assert np.array_equal(r[pre + "_win"][hit & rec["hit"].astype(bool)],
                               rec["window"][hit & rec["hit"].astype(bool)])
+    assert r["vf_hit"].sum() + r["ee_hit"].sum() > 0
+
+
+def test_gpu_narrowphase_long_history(api):
+    """A velocity filter's history (two colliding sheets, ~9,500 entries per vertex): the broadphase over
+    every entry and the stitched-window narrowphase equal the oracle's."""
+    say("case: long multi-entry history from the velocity filter")
+    s = scenes.falling_sheets(3, 0)
+    r0, _, _, _, h = oracle.velocity_filter(s["qstart"], s["qend"], s["faces"], np.ones(s["qstart"].size),
+                                            s["outer_eta"], s["inner_eta"], want_history=True)
+    assert r0 >= 1 and len(h.ht) > 100_000
+    eta = s["outer_eta"]   # the penalty layers keep the sheets inside outerEta for a while: hits
+    ovf, oee = oracle.candidates(s["faces"], h, eta, method="tree")
+    gvf, gee = api.kdop_candidates(s["faces"], None, None, eta, hist=(h.off, h.ht, h.hp))
+    assert np.array_equal(ovf, gvf) and np.array_equal(oee, gee)
+    api.narrowphase_history(h.off, h.ht, h.hp, ovf, oee, eta)
+    t0 = time.perf_counter()
+    r = api.narrowphase_history(h.off, h.ht, h.hp, ovf, oee, eta)
+    say("device narrowphase over %d entries: %.1f ms" % (len(h.ht), (time.perf_counter() - t0) * 1e3))
+    for keys, is_ee, pre in ((ovf, False, "vf"), (oee, True, "ee")):
+        rec = oracle.narrowphase(h, keys, eta, is_ee)
+        rep = compare_results("long" + pre, r[pre + "_hit"], r[pre + "_t"], rec)
+        assert not rep["hard"] and not rep["t_bad_hard"], rep
+        both = r[pre + "_hit"] & rec["hit"].astype(bool)
+        assert np.array_equal(r[pre + "_win"][both], rec["window"][both])
+        say(pre, "hits", int(r[pre + "_hit"].sum()), "of", len(keys), "max window", int(r[pre + "_win"].max()))
     assert r["vf_hit"].sum() + r["ee_hit"].sum() > 0
 
 
