@@ -235,6 +235,22 @@ def test_gpu_velocity_filter_bounce_vs_oracle(api, cor):
 
 
 @pytest.mark.gpu
+def test_gpu_velocity_filter_class_mirror(api):
+    """api.VelocityFilter.velocityFilter keeps the reference's in-place form: qend is overwritten with the
+    filtered positions on success and left alone on failure."""
+    s, im = _bounce_case(0.1)
+    say("case: velocity filter class mirror")
+    qs, f = s["qstart"].reshape(-1), s["faces"].reshape(-1)
+    qe = s["qend"].reshape(-1).copy()
+    r0, q0, _ = api.velocity_filter(qs, qe, f, im, s["outer_eta"], s["inner_eta"])
+    r = api.VelocityFilter.velocityFilter(qs, qe, f, im, s["outer_eta"], s["inner_eta"])
+    assert r == r0 >= 1 and np.array_equal(qe.view(np.int64), q0.view(np.int64))
+    qe2 = s["qend"].reshape(-1).copy()
+    assert api.VelocityFilter.velocityFilter(qs, qe2, f, im, 1.0, 0.5) == api.VelocityFilter.STARTING_STATE_INTERSECTS
+    assert (qe2 == s["qend"].reshape(-1)).all()
+
+
+@pytest.mark.gpu
 def test_gpu_velocity_filter_return_codes(api):
     from collisiondetection_amd import _native as N
 
