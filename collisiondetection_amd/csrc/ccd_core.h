@@ -1271,9 +1271,11 @@ struct SeqScan {
     }
 };
 
+// max_nodes bounds the intervals examined: when more remain the result is 3 ("not decided yet"), which
+// a caller uses to sort stencils by how much work they need before running them in full.
 template <bool VF, class C = NoCount, class Scan = SeqScan>
 CCD_HD int separating_plane_check(const HistCSR &h, const int (&v)[4], double eta, double eps, C &&cnt = C(),
-                                  Scan scan = Scan())
+                                  Scan scan = Scan(), int max_nodes = 0x7fffffff)
 {
     const double inf = bits_to_double(0x7ff0000000000000ll);
     // pending intervals and their depths; a node at depth d leaves at most d pending uppers below it
@@ -1284,6 +1286,7 @@ CCD_HD int separating_plane_check(const HistCSR &h, const int (&v)[4], double et
     st_d[0] = 0;
     int top = 1;
     while (top > 0) {
+        if (max_nodes-- <= 0) return 3;
         top--;
         const double mint = st_lo[top], maxt = st_hi[top];
         const int depth = st_d[top];

@@ -373,11 +373,13 @@ template <bool VF, class M>
 __global__ __launch_bounds__(256) CCD_TWO_WAVES void k_separating_plane(
     const int64_t *__restrict__ off, const double *__restrict__ ht, const double *__restrict__ hp,
     const int4 *__restrict__ keys, const double *__restrict__ eta, double eta_s, int64_t n,
-    const unsigned long long *__restrict__ gap_bits, uint8_t *__restrict__ status, unsigned long long *undecided)
+    const unsigned long long *__restrict__ gap_bits, uint8_t *__restrict__ status, unsigned long long *undecided,
+    const int32_t *__restrict__ order)
 {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int r = 0;
-    if (i < n) {
+    if (j < n) {
+        const int64_t i = order ? (int64_t)order[j] : j;   // lane j runs stencil order[j]
         const double eps = ccd::bits_to_double((long long)*gap_bits) / 4.0;
         int4 k = keys[i];
         int v[4] = {k.x, k.y, k.z, k.w};
@@ -393,6 +395,27 @@ __global__ __launch_bounds__(256) CCD_TWO_WAVES void k_separating_plane(
     }
     uint64_t m = __ballot(r == 2);
     if (m && __lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(undecided, (unsigned long long)__popcll(m));
+}
+
+// Sort key for the separating-plane narrowphase: 1 if a stencil's root interval [0, 1] does not decide
+// it (its recursion goes on; with two-entry histories that means one or two CTCD window tests), else 0.
+// Most stencils are decided at the root; grouping the others keeps a wavefront's lanes on the same path.
+template <bool VF>
+__global__ __launch_bounds__(256) void k_sp_order_key(const int64_t *__restrict__ off, const double *__restrict__ ht,
+                                                      const double *__restrict__ hp, const int4 *__restrict__ keys,
+                                                      const double *__restrict__ eta, double eta_s, int64_t n,
+                                                      const unsigned long long *__restrict__ gap_bits,
+                                                      uint64_t *__restrict__ key, int32_t *__restrict__ idx)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double eps = ccd::bits_to_double((long long)*gap_bits) / 4.0;
+    const int4 k = keys[i];
+    const int v[4] = {k.x, k.y, k.z, k.w};
+    ccd::HistCSR h{off, ht, hp};
+    const int r = ccd::separating_plane_check<VF>(h, v, eta ? eta[i] : eta_s, eps, ccd::FullCascade(), ccd::SeqScan(), 1);
+    key[i] = r == 3 ? 1u : 0u;
+    idx[i] = (int32_t)i;
 }
 
 // The narrowphase over stitched history windows (k_narrow_hist) with one wavefront per stencil, for
@@ -791,14 +814,36 @@ int launch_separating_plane(ccd_ctx *ctx, bool vf, const int64_t *off, const dou
         CCD_LAUNCH_CHECK(ctx);
         return CCD_OK;
     }
+    // Order the stencils so that those decided at the root run together (results are per stencil and
+    // land at the stencil's own index, so the order changes nothing else).
+    const int32_t *order = nullptr;
+    if (n >= 4096) {
+        uint64_t *key = (uint64_t *)buffer(ctx, "spo_key", (size_t)n * 8);
+        uint64_t *key2 = (uint64_t *)buffer(ctx, "spo_key2", (size_t)n * 8);
+        int32_t *idx = (int32_t *)buffer(ctx, "spo_idx", (size_t)n * 4);
+        int32_t *idx2 = (int32_t *)buffer(ctx, "spo_idx2", (size_t)n * 4);
+        void *tmp = buffer(ctx, "spo_tmp", radix_sort_temp_bytes(n));
+        if (!key || !key2 || !idx || !idx2 || !tmp) return CCD_E_OOM;
+        if (vf)
+            hipLaunchKernelGGL(k_sp_order_key<true>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, off, ht, hp,
+                               (const int4 *)keys, eta, eta_s, n, gb, key, idx);
+        else
+            hipLaunchKernelGGL(k_sp_order_key<false>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, off, ht, hp,
+                               (const int4 *)keys, eta, eta_s, n, gb, key, idx);
+        CCD_LAUNCH_CHECK(ctx);
+        bool in_alt = false;
+        int rc = radix_sort_pairs(ctx, key, idx, key2, idx2, n, 1, tmp, in_alt);   // one stable pass
+        if (rc) return rc;
+        order = in_alt ? idx2 : idx;
+    }
     if (vf)
         CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_separating_plane<true, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK),
                                                   0, ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, gb,
-                                                  status, und));
+                                                  status, und, order));
     else
         CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_separating_plane<false, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK),
                                                   0, ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, gb,
-                                                  status, und));
+                                                  status, und, order));
     CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
 }
