@@ -418,7 +418,7 @@ static int narrow_launches(ccd_ctx *ctx, const DetectArgs &a, int64_t nvf_max, i
     const bool group = exact && ctx->group_stencils;   // the stage timers include the grouping
     prof_begin(ctx, CCD_T_NARROW_VF);
     if (group) {
-        rc = group_by_signature(ctx, true, a.qs, a.qe, vfk, nvf_max, &vfk);
+        rc = group_by_signature(ctx, true, a.qs, a.qe, vfk, nvf_max, a.eta, &vfk);
         if (rc) return rc;
     }
     rc = launch_narrow2_queue(ctx, true, a.qs, a.qe, vfk, ctx->d_counts + C_VF, nvf_max, a.eta, a.vf_hits, a.vf_toi,
@@ -427,7 +427,7 @@ static int narrow_launches(ccd_ctx *ctx, const DetectArgs &a, int64_t nvf_max, i
     prof_end(ctx, CCD_T_NARROW_VF);
     prof_begin(ctx, CCD_T_NARROW_EE);
     if (group) {
-        rc = group_by_signature(ctx, false, a.qs, a.qe, eek, nee_max, &eek);
+        rc = group_by_signature(ctx, false, a.qs, a.qe, eek, nee_max, a.eta, &eek);
         if (rc) return rc;
     }
     rc = launch_narrow2_queue(ctx, false, a.qs, a.qe, eek, ctx->d_counts + C_EE, nee_max, a.eta, a.ee_hits, a.ee_toi,

@@ -130,6 +130,14 @@ struct FullCascade {
 template <class C>
 struct prunes : std::integral_constant<bool, std::is_same<std::decay_t<C>, NoCount>::value ||
                                                  std::is_same<std::decay_t<C>, RefRoots>::value> {};
+// A counter type whose `stops_at_solver` member is true makes find_intervals return, with no intervals,
+// where it would call the root solver.  SolverProbe records whether that point was reached; the
+// stencil-ordering pre-pass uses it to see which polynomials need the solver.
+template <class C, class = void>
+struct stops_at_solver : std::false_type {};
+template <class C>
+struct stops_at_solver<C, std::void_t<decltype(C::stops_at_solver)>> : std::integral_constant<bool, C::stops_at_solver> {};
+
 template <class C, class = void>
 struct uses_ref_roots : std::false_type {};
 template <class C>
@@ -484,6 +492,7 @@ CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos, C &&cnt = C())
     if (reduced > 2) {
         if (!could_have_roots<N>(op, reduced, pos)) return;
         cnt(S_ISOLATE);
+        if constexpr (stops_at_solver<CT>::value) return;
         if constexpr (uses_ref_roots<CT>::value) {
             // CTCD.cpp:139-141 and :160-176: all roots the solver returns, sorted, real parts of
             // complex ones included.
@@ -862,6 +871,38 @@ CCD_HD bool ve_sides(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, Ivls<3> &e1
 // non-empty and overlapping): bit j for the j-th vertex-edge test in the reference's order.  Cheap
 // (closed-form quadratics only); the narrowphase groups stencils by it so that a wavefront's lanes
 // take the same quartic solves.
+struct SolverProbe {
+    static constexpr bool stops_at_solver = true;
+    bool called = false;
+    CCD_FHD void operator()(int stage)
+    {
+        if (stage == S_ISOLATE) called = true;
+    }
+};
+
+// Which of a window's first polynomials reach the root solver: for vertex-face the three edge-side
+// cubics (bits 0-2, vertexFaceCTCD's order and arguments), for edge-edge the coplanar-distance sextic
+// (bit 0, edgeEdgeCTCD's arguments for check_ee_window's vertex order).  A sort key only.
+CCD_HD unsigned first_phase_signature(bool vf, const V3 *s, const V3 *e, double eta)
+{
+    const V3 v0 = e[0] - s[0], v1 = e[1] - s[1], v2 = e[2] - s[2], v3 = e[3] - s[3];
+    if (vf) {
+        Ivls<4> iv;
+        SolverProbe p0, p1, p2;
+        plane_poly(s[0] - s[1], cross(s[3] - s[1], s[2] - s[1]), s[3] - s[1], v0 - v1, cross(v3 - v1, v2 - v1), v3 - v1,
+                   iv, p0);
+        plane_poly(s[0] - s[2], cross(s[1] - s[2], s[3] - s[2]), s[1] - s[2], v0 - v2, cross(v1 - v2, v3 - v2), v1 - v2,
+                   iv, p1);
+        plane_poly(s[0] - s[3], cross(s[2] - s[3], s[1] - s[3]), s[2] - s[3], v0 - v3, cross(v2 - v3, v1 - v3), v2 - v3,
+                   iv, p2);
+        return (p0.called ? 1u : 0u) | (p1.called ? 2u : 0u) | (p2.called ? 4u : 0u);
+    }
+    Ivls<7> iv;
+    SolverProbe p;
+    distance_poly(s[1] - s[3], s[1] - s[0], s[3] - s[2], v1 - v3, v1 - v0, v3 - v2, eta * eta, iv, p);
+    return p.called ? 1u : 0u;
+}
+
 CCD_HD unsigned ve_signature(bool vf, const V3 *s, const V3 *e)
 {
     unsigned sig = 0;

@@ -119,7 +119,7 @@ int launch_narrow2_count(ccd_ctx *ctx, bool vf, const double *qs, const double *
 // The n stencils (device keys, n exact on the host) reordered by their vertex-edge signature
 // (ccd::ve_signature; stable) into a context buffer; *grouped receives it (or keys when n < 2).
 int group_by_signature(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys, int64_t n,
-                       const int32_t **grouped);
+                       double eta, const int32_t **grouped);
 // Separating-plane narrowphase over a CSR history: launch_min_gap writes History::computeMinimumGap's
 // value (as bits) to *gap_bits on the device; launch_separating_plane reads it (eps = gap / 4).
 int launch_min_gap(ccd_ctx *ctx, const int64_t *off, const double *ht, int64_t nverts, uint64_t *gap_bits);

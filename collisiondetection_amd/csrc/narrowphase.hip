@@ -322,12 +322,13 @@ __global__ void k_validate_history(const int64_t *__restrict__ off, const double
     if (m && __lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(bad, (unsigned long long)__popcll(m));
 }
 
-// Grouping of stencils by the vertex-edge quartics their windows will solve (ccd::ve_signature): the
-// narrowphase then runs on the keys in signature order (stable, so each group keeps the broadphase's
-// spatial order), and a wavefront's lanes mostly take the same solver calls.
+// Grouping of stencils by which of their first polynomials reach the root solver
+// (ccd::first_phase_signature, the major key) and by the vertex-edge quartics their windows will solve
+// (ccd::ve_signature): the narrowphase then runs on the keys in signature order (stable, so each group
+// keeps the broadphase's spatial order), and a wavefront's lanes mostly take the same solver calls.
 template <bool VF>
 __global__ __launch_bounds__(256) void k_stencil_signature(const double *__restrict__ qs, const double *__restrict__ qe,
-                                                           const int4 *__restrict__ keys, int64_t n,
+                                                           const int4 *__restrict__ keys, int64_t n, double eta,
                                                            uint64_t *__restrict__ sig, int32_t *__restrict__ idx)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -342,7 +343,7 @@ __global__ __launch_bounds__(256) void k_stencil_signature(const double *__restr
         s[j] = ccd::mk(ccd::stitch0(a.x, b.x), ccd::stitch0(a.y, b.y), ccd::stitch0(a.z, b.z));
         e[j] = b;
     }
-    sig[i] = ccd::ve_signature(VF, s, e);
+    sig[i] = ccd::ve_signature(VF, s, e) | (ccd::first_phase_signature(VF, s, e, eta) << 4);
     idx[i] = (int32_t)i;
 }
 __global__ __launch_bounds__(256) void k_gather_keys(const int4 *__restrict__ keys, const int32_t *__restrict__ perm,
@@ -753,7 +754,7 @@ int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *
 }
 
 int group_by_signature(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys, int64_t n,
-                       const int32_t **grouped)
+                       double eta, const int32_t **grouped)
 {
     *grouped = keys;
     if (n < 2) return CCD_OK;
@@ -766,13 +767,13 @@ int group_by_signature(ccd_ctx *ctx, bool vf, const double *qs, const double *qe
     if (!sig || !sig2 || !idx || !idx2 || !out || !tmp) return CCD_E_OOM;
     if (vf)
         hipLaunchKernelGGL(k_stencil_signature<true>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
-                           (const int4 *)keys, n, sig, idx);
+                           (const int4 *)keys, n, eta, sig, idx);
     else
         hipLaunchKernelGGL(k_stencil_signature<false>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
-                           (const int4 *)keys, n, sig, idx);
+                           (const int4 *)keys, n, eta, sig, idx);
     CCD_LAUNCH_CHECK(ctx);
     bool in_alt = false;
-    int rc = radix_sort_pairs(ctx, sig, idx, sig2, idx2, n, 4, tmp, in_alt);   // one stable 8-bit pass
+    int rc = radix_sort_pairs(ctx, sig, idx, sig2, idx2, n, 8, tmp, in_alt);   // one stable 8-bit pass
     if (rc) return rc;
     hipLaunchKernelGGL(k_gather_keys, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, (const int4 *)keys,
                        in_alt ? idx2 : idx, n, out);
