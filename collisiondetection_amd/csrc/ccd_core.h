@@ -880,27 +880,38 @@ struct SolverProbe {
     }
 };
 
-// Which of a window's first polynomials reach the root solver: for vertex-face the three edge-side
-// cubics (bits 0-2, vertexFaceCTCD's order and arguments), for edge-edge the coplanar-distance sextic
-// (bit 0, edgeEdgeCTCD's arguments for check_ee_window's vertex order).  A sort key only.
+// Which of the polynomials of a window's face or edge-edge test would reach the root solver, each
+// evaluated whether or not the test would get that far (vertexFaceCTCD's / edgeEdgeCTCD's arguments for
+// check_vf_window's / check_ee_window's vertex order).  Vertex-face: the three edge-side cubics (bits
+// 0-2) and the coplanar-distance sextic (bit 3).  Edge-edge: the coplanar-distance sextic (bit 0) and
+// how many of the four barycentric quartics (bits 1-3).  A sort key only.
 CCD_HD unsigned first_phase_signature(bool vf, const V3 *s, const V3 *e, double eta)
 {
     const V3 v0 = e[0] - s[0], v1 = e[1] - s[1], v2 = e[2] - s[2], v3 = e[3] - s[3];
     if (vf) {
         Ivls<4> iv;
-        SolverProbe p0, p1, p2;
+        Ivls<7> iv7;
+        SolverProbe p0, p1, p2, p3;
         plane_poly(s[0] - s[1], cross(s[3] - s[1], s[2] - s[1]), s[3] - s[1], v0 - v1, cross(v3 - v1, v2 - v1), v3 - v1,
                    iv, p0);
         plane_poly(s[0] - s[2], cross(s[1] - s[2], s[3] - s[2]), s[1] - s[2], v0 - v2, cross(v1 - v2, v3 - v2), v1 - v2,
                    iv, p1);
         plane_poly(s[0] - s[3], cross(s[2] - s[3], s[1] - s[3]), s[2] - s[3], v0 - v3, cross(v2 - v3, v1 - v3), v2 - v3,
                    iv, p2);
-        return (p0.called ? 1u : 0u) | (p1.called ? 2u : 0u) | (p2.called ? 4u : 0u);
+        distance_poly(s[0] - s[1], s[2] - s[1], s[3] - s[1], v0 - v1, v2 - v1, v3 - v1, eta * eta, iv7, p3);
+        return (p0.called ? 1u : 0u) | (p1.called ? 2u : 0u) | (p2.called ? 4u : 0u) | (p3.called ? 8u : 0u);
     }
-    Ivls<7> iv;
-    SolverProbe p;
-    distance_poly(s[1] - s[3], s[1] - s[0], s[3] - s[2], v1 - v3, v1 - v0, v3 - v2, eta * eta, iv, p);
-    return p.called ? 1u : 0u;
+    // edgeEdgeCTCD(q0 = s[0], p0 = s[1], q1 = s[2], p1 = s[3])
+    Ivls<7> iv7;
+    Ivls<5> iv5;
+    SolverProbe pd, pa0, pa1, pb0, pb1;
+    distance_poly(s[1] - s[3], s[1] - s[0], s[3] - s[2], v1 - v3, v1 - v0, v3 - v2, eta * eta, iv7, pd);
+    barycentric_poly(s[3] - s[2], s[1] - s[0], s[0] - s[2], v3 - v2, v1 - v0, v0 - v2, iv5, pa0);
+    barycentric_poly(s[3] - s[2], s[0] - s[1], s[1] - s[2], v3 - v2, v0 - v1, v1 - v2, iv5, pa1);
+    barycentric_poly(s[1] - s[0], s[3] - s[2], s[2] - s[0], v1 - v0, v3 - v2, v2 - v0, iv5, pb0);
+    barycentric_poly(s[1] - s[0], s[2] - s[3], s[3] - s[0], v1 - v0, v2 - v3, v3 - v0, iv5, pb1);
+    const unsigned nb = (pa0.called ? 1u : 0u) + (pa1.called ? 1u : 0u) + (pb0.called ? 1u : 0u) + (pb1.called ? 1u : 0u);
+    return (pd.called ? 1u : 0u) | (nb << 1);
 }
 
 CCD_HD unsigned ve_signature(bool vf, const V3 *s, const V3 *e)
