@@ -7,7 +7,10 @@ CTCDNarrowPhase on every candidate stencil, with positions already resident in H
 stencils and their times of impact are the output.
 
 Printed: one JSON line with the metric, the per-stage HIP-event timings, the roofline of the dominant
-kernel and a CPU baseline (the oracle's reference-structured CPU path on a bounded sample).
+kernel and a CPU baseline (the oracle's reference-structured CPU path on a bounded sample), plus
+secondary legs: the isolating root mode, the separating-plane narrowphase on the same candidates, the
+velocity filter (the reference's example input and a falling-sheets contact problem), batched
+instances and the serving loop (each can be switched off with its --no-* flag).
 """
 import argparse
 import json
