@@ -85,10 +85,12 @@ template <bool VF, class M>
 __global__ __launch_bounds__(256) CCD_TWO_WAVES void k_narrow2(const double *__restrict__ qs, const double *__restrict__ qe,
                                                  const int4 *__restrict__ keys, const double *__restrict__ eta,
                                                  double eta_s, int64_t n, uint8_t *__restrict__ hit,
-                                                 double *__restrict__ tout, uint8_t *__restrict__ sub)
+                                                 double *__restrict__ tout, uint8_t *__restrict__ sub,
+                                                 const int32_t *__restrict__ order)
 {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int64_t i = order ? (int64_t)order[j] : j;   // lane j runs stencil order[j]
     double t = 0.0;
     int r = stencil2<VF, M>(qs, qe, keys[i], eta ? eta[i] : eta_s, t);
     hit[i] = r >= 0 ? 1 : 0;
@@ -329,6 +331,7 @@ __global__ void k_validate_history(const int64_t *__restrict__ off, const double
 template <bool VF>
 __global__ __launch_bounds__(256) void k_stencil_signature(const double *__restrict__ qs, const double *__restrict__ qe,
                                                            const int4 *__restrict__ keys, int64_t n, double eta,
+                                                           const double *__restrict__ eta_arr,
                                                            uint64_t *__restrict__ sig, int32_t *__restrict__ idx)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -343,7 +346,7 @@ __global__ __launch_bounds__(256) void k_stencil_signature(const double *__restr
         s[j] = ccd::mk(ccd::stitch0(a.x, b.x), ccd::stitch0(a.y, b.y), ccd::stitch0(a.z, b.z));
         e[j] = b;
     }
-    sig[i] = ccd::ve_signature(VF, s, e) | (ccd::first_phase_signature(VF, s, e, eta) << 4);
+    sig[i] = ccd::ve_signature(VF, s, e) | (ccd::first_phase_signature(VF, s, e, eta_arr ? eta_arr[i] : eta) << 4);
     idx[i] = (int32_t)i;
 }
 __global__ __launch_bounds__(256) void k_gather_keys(const int4 *__restrict__ keys, const int32_t *__restrict__ perm,
@@ -637,12 +640,36 @@ int launch_narrow2(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, co
                    const double *eta, double eta_s, int64_t n, uint8_t *hit, double *t, uint8_t *sub)
 {
     if (n == 0) return CCD_OK;
+    // Run the stencils in the grouping order of the detection pipeline (k_stencil_signature); each
+    // result lands at its stencil's own index, so only the speed depends on the order.
+    const int32_t *order = nullptr;
+    if (n >= 4096 && ctx->group_stencils) {
+        uint64_t *sig = (uint64_t *)buffer(ctx, "n2o_sig", (size_t)n * 8);
+        uint64_t *sig2 = (uint64_t *)buffer(ctx, "n2o_sig2", (size_t)n * 8);
+        int32_t *idx = (int32_t *)buffer(ctx, "n2o_idx", (size_t)n * 4);
+        int32_t *idx2 = (int32_t *)buffer(ctx, "n2o_idx2", (size_t)n * 4);
+        void *tmp = buffer(ctx, "n2o_tmp", radix_sort_temp_bytes(n));
+        if (!sig || !sig2 || !idx || !idx2 || !tmp) return CCD_E_OOM;
+        if (vf)
+            hipLaunchKernelGGL(k_stencil_signature<true>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
+                               (const int4 *)keys, n, eta_s, eta, sig, idx);
+        else
+            hipLaunchKernelGGL(k_stencil_signature<false>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
+                               (const int4 *)keys, n, eta_s, eta, sig, idx);
+        CCD_LAUNCH_CHECK(ctx);
+        bool in_alt = false;
+        int rc = radix_sort_pairs(ctx, sig, idx, sig2, idx2, n, 8, tmp, in_alt);   // one stable 8-bit pass
+        if (rc) return rc;
+        order = in_alt ? idx2 : idx;
+    }
     if (vf)
         CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow2<true, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK), 0,
-                                                  ctx->stream, qs, qe, (const int4 *)keys, eta, eta_s, n, hit, t, sub));
+                                                  ctx->stream, qs, qe, (const int4 *)keys, eta, eta_s, n, hit, t, sub,
+                                                  order));
     else
         CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow2<false, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK), 0,
-                                                  ctx->stream, qs, qe, (const int4 *)keys, eta, eta_s, n, hit, t, sub));
+                                                  ctx->stream, qs, qe, (const int4 *)keys, eta, eta_s, n, hit, t, sub,
+                                                  order));
     CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
 }
@@ -767,10 +794,10 @@ int group_by_signature(ccd_ctx *ctx, bool vf, const double *qs, const double *qe
     if (!sig || !sig2 || !idx || !idx2 || !out || !tmp) return CCD_E_OOM;
     if (vf)
         hipLaunchKernelGGL(k_stencil_signature<true>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
-                           (const int4 *)keys, n, eta, sig, idx);
+                           (const int4 *)keys, n, eta, nullptr, sig, idx);
     else
         hipLaunchKernelGGL(k_stencil_signature<false>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
-                           (const int4 *)keys, n, eta, sig, idx);
+                           (const int4 *)keys, n, eta, nullptr, sig, idx);
     CCD_LAUNCH_CHECK(ctx);
     bool in_alt = false;
     int rc = radix_sort_pairs(ctx, sig, idx, sig2, idx2, n, 8, tmp, in_alt);   // one stable 8-bit pass
