@@ -415,10 +415,12 @@ static int narrow_launches(ccd_ctx *ctx, const DetectArgs &a, int64_t nvf_max, i
     if (rc) return rc;
     const int32_t *vfk = (const int32_t *)ctx->bufs["vf_keys"].ptr;
     const int32_t *eek = (const int32_t *)ctx->bufs["ee_keys"].ptr;
-    const bool group = exact && ctx->group_stencils;   // the stage timers include the grouping
+    // The stage timers include the grouping.  Without exact host counts (a captured pipeline) the
+    // signature pass reads the device counts and sorts the unused capacity last.
+    const bool group = ctx->group_stencils;
     prof_begin(ctx, CCD_T_NARROW_VF);
     if (group) {
-        rc = group_by_signature(ctx, true, a.qs, a.qe, vfk, nvf_max, a.eta, &vfk);
+        rc = group_by_signature(ctx, true, a.qs, a.qe, vfk, nvf_max, a.eta, exact ? nullptr : ctx->d_counts + C_VF, &vfk);
         if (rc) return rc;
     }
     rc = launch_narrow2_queue(ctx, true, a.qs, a.qe, vfk, ctx->d_counts + C_VF, nvf_max, a.eta, a.vf_hits, a.vf_toi,
@@ -427,7 +429,7 @@ static int narrow_launches(ccd_ctx *ctx, const DetectArgs &a, int64_t nvf_max, i
     prof_end(ctx, CCD_T_NARROW_VF);
     prof_begin(ctx, CCD_T_NARROW_EE);
     if (group) {
-        rc = group_by_signature(ctx, false, a.qs, a.qe, eek, nee_max, a.eta, &eek);
+        rc = group_by_signature(ctx, false, a.qs, a.qe, eek, nee_max, a.eta, exact ? nullptr : ctx->d_counts + C_EE, &eek);
         if (rc) return rc;
     }
     rc = launch_narrow2_queue(ctx, false, a.qs, a.qe, eek, ctx->d_counts + C_EE, nee_max, a.eta, a.ee_hits, a.ee_toi,
@@ -503,7 +505,12 @@ static int detect_graph(ccd_ctx *ctx, const DetectArgs &a)
     if (ctx->h_counts[C_BAD] != 0) return CCD_OK;   // raw emission needs a host-driven sort: no graph
     int64_t saved[NCOUNTERS];
     memcpy(saved, ctx->h_counts, sizeof(saved));
-    // Capture with the capacities the synchronous run settled on.
+    // Capture with the capacities the synchronous run settled on; buffers used inside the capture are
+    // sized for them first (nothing can be allocated while capturing).
+    if (ctx->group_stencils) {
+        rc = reserve_grouping(ctx, ctx->cap_vf, ctx->cap_ee);
+        if (rc) return rc;
+    }
     int64_t caps2[3] = {ctx->cap_pairs, ctx->cap_vf, ctx->cap_ee};
     memcpy(key + sizeof(DetectArgs), caps2, sizeof(caps2));
     ctx->capturing = true;

@@ -117,9 +117,12 @@ int launch_narrow2_count(ccd_ctx *ctx, bool vf, const double *qs, const double *
                          int64_t n, double eta, unsigned long long *counts);
 // Validation: counts keys outside [0, nverts) into *bad.
 // The n stencils (device keys, n exact on the host) reordered by their vertex-edge signature
-// (ccd::ve_signature; stable) into a context buffer; *grouped receives it (or keys when n < 2).
+// (ccd::ve_signature; stable) into a context buffer; *grouped receives it (or keys when n < 2).  With
+// n_dev (the stored count on the device, for captured pipelines) the n slots past it sort last.
+int reserve_grouping(ccd_ctx *ctx, int64_t nvf, int64_t nee);
 int group_by_signature(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys, int64_t n,
-                       double eta, const int32_t **grouped);
+                       double eta, const int64_t *n_dev,
+                       const int32_t **grouped);
 // Separating-plane narrowphase over a CSR history: launch_min_gap writes History::computeMinimumGap's
 // value (as bits) to *gap_bits on the device; launch_separating_plane reads it (eps = gap / 4).
 int launch_min_gap(ccd_ctx *ctx, const int64_t *off, const double *ht, int64_t nverts, uint64_t *gap_bits);

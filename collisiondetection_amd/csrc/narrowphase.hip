@@ -332,10 +332,16 @@ template <bool VF>
 __global__ __launch_bounds__(256) void k_stencil_signature(const double *__restrict__ qs, const double *__restrict__ qe,
                                                            const int4 *__restrict__ keys, int64_t n, double eta,
                                                            const double *__restrict__ eta_arr,
+                                                           const int64_t *__restrict__ n_dev,
                                                            uint64_t *__restrict__ sig, int32_t *__restrict__ idx)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (n_dev && i >= *n_dev) {   // beyond the stored stencils (count on the device): last, keys not read
+        sig[i] = 0xff;             // above every signature (vertex-face <= 0xf7, edge-edge <= 0x9f)
+        idx[i] = (int32_t)i;
+        return;
+    }
     int4 k = keys[i];
     int v[4] = {k.x, k.y, k.z, k.w};
     V3 s[4], e[4];
@@ -652,10 +658,10 @@ int launch_narrow2(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, co
         if (!sig || !sig2 || !idx || !idx2 || !tmp) return CCD_E_OOM;
         if (vf)
             hipLaunchKernelGGL(k_stencil_signature<true>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
-                               (const int4 *)keys, n, eta_s, eta, sig, idx);
+                               (const int4 *)keys, n, eta_s, eta, nullptr, sig, idx);
         else
             hipLaunchKernelGGL(k_stencil_signature<false>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
-                               (const int4 *)keys, n, eta_s, eta, sig, idx);
+                               (const int4 *)keys, n, eta_s, eta, nullptr, sig, idx);
         CCD_LAUNCH_CHECK(ctx);
         bool in_alt = false;
         int rc = radix_sort_pairs(ctx, sig, idx, sig2, idx2, n, 8, tmp, in_alt);   // one stable 8-bit pass
@@ -780,8 +786,22 @@ int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *
     return CCD_OK;
 }
 
+// Sizes the grouping buffers for up to nvf / nee stencils (before a capture, which cannot grow them).
+int reserve_grouping(ccd_ctx *ctx, int64_t nvf, int64_t nee)
+{
+    const int64_t n = std::max(nvf, nee);
+    if (n < 2) return CCD_OK;
+    if (!buffer(ctx, "grp_sig", (size_t)n * 8) || !buffer(ctx, "grp_sig2", (size_t)n * 8) ||
+        !buffer(ctx, "grp_idx", (size_t)n * 4) || !buffer(ctx, "grp_idx2", (size_t)n * 4) ||
+        !buffer(ctx, "grp_keys_vf", (size_t)std::max<int64_t>(nvf, 1) * 16) ||
+        !buffer(ctx, "grp_keys_ee", (size_t)std::max<int64_t>(nee, 1) * 16) ||
+        !buffer(ctx, "grp_tmp", radix_sort_temp_bytes(n)))
+        return CCD_E_OOM;
+    return CCD_OK;
+}
+
 int group_by_signature(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys, int64_t n,
-                       double eta, const int32_t **grouped)
+                       double eta, const int64_t *n_dev, const int32_t **grouped)
 {
     *grouped = keys;
     if (n < 2) return CCD_OK;
@@ -794,10 +814,10 @@ int group_by_signature(ccd_ctx *ctx, bool vf, const double *qs, const double *qe
     if (!sig || !sig2 || !idx || !idx2 || !out || !tmp) return CCD_E_OOM;
     if (vf)
         hipLaunchKernelGGL(k_stencil_signature<true>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
-                           (const int4 *)keys, n, eta, nullptr, sig, idx);
+                           (const int4 *)keys, n, eta, nullptr, n_dev, sig, idx);
     else
         hipLaunchKernelGGL(k_stencil_signature<false>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,
-                           (const int4 *)keys, n, eta, nullptr, sig, idx);
+                           (const int4 *)keys, n, eta, nullptr, n_dev, sig, idx);
     CCD_LAUNCH_CHECK(ctx);
     bool in_alt = false;
     int rc = radix_sort_pairs(ctx, sig, idx, sig2, idx2, n, 8, tmp, in_alt);   // one stable 8-bit pass
