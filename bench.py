@@ -104,7 +104,7 @@ def bench_separating_plane(ctx, scene, qs, qe, steps, log, cpu_sample=30000, see
     wall = (time.perf_counter() - t0) / steps
     n = nv + ne
     log(f"separating plane: {n} stencils in {wall * 1e3:.2f} ms")
-    out = {"stencils": n, "ms": wall * 1e3, "stencils_per_s": n / wall, "root_mode": "reference",
+    out = {"stencils": n, "ms": wall * 1e3, "stencils_per_s": n / wall,
            "status": {"vf": np.bincount(vs[:nv].cpu().numpy(), minlength=3).tolist(),
                       "ee": np.bincount(es[:ne].cpu().numpy(), minlength=3).tolist()},
            "undecided": und.value}
@@ -400,12 +400,11 @@ def main():
             return step()
         ctx.check(rc)
 
-    def run_mode(mode):
-        """Warmup, work accounting and the timed region for one root mode; returns its figures."""
-        ctx.set_root_mode(mode)
+    def run_detect():
+        """Warmup, work accounting and the timed region; returns the figures."""
         for i in range(args.warmup):
             step()
-            log(f"[{N.ROOT_MODE_NAMES[mode]}] warmup {i + 1}/{args.warmup}: {cnt[2].value} VF + {cnt[3].value} EE "
+            log(f"warmup {i + 1}/{args.warmup}: {cnt[2].value} VF + {cnt[3].value} EE "
                 f"candidates, {cnt[0].value} + {cnt[1].value} collisions")
         torch.cuda.synchronize(dev)
 
@@ -437,12 +436,11 @@ def main():
         L.ccd_stage_times(ctx.ptr, ms, calls, 1)
         L.ccd_set_profiling(ctx.ptr, 0)
         stage_ms = {N.TIMER_NAMES[i]: ms[i] / max(calls[i], 1) for i in range(N.NTIMERS)}
-        log(f"[{N.ROOT_MODE_NAMES[mode]}] stage ms/step:", json.dumps({k: round(v, 4) for k, v in stage_ms.items()}))
+        log("stage ms/step:", json.dumps({k: round(v, 4) for k, v in stage_ms.items()}))
         return {"wall": wall, "stage_ms": stage_ms, "nv": nv.value, "ne": ne.value, "flops_vf": flops_vf,
                 "flops_ee": flops_ee, "hits": (cnt[0].value, cnt[1].value)}
 
-    alt = run_mode(N.ROOTS_ISOLATE)
-    res = run_mode(N.ROOTS_REFERENCE)   # the default mode: the reference's results bit for bit
+    res = run_detect()
     wall, stage_ms, flops_vf, flops_ee = res["wall"], res["stage_ms"], res["flops_vf"], res["flops_ee"]
     nv, ne = ctypes.c_int64(res["nv"]), ctypes.c_int64(res["ne"])
     ms_per_step = 1000.0 * wall / args.steps
@@ -457,7 +455,7 @@ def main():
     if dom in ("narrow_vf", "narrow_ee"):
         flops = flops_vf if dom == "narrow_vf" else flops_ee
         achieved = flops / (stage_ms[dom] * 1e-3) / 1e12
-        roof = {"kernel": "k_narrow2_queue<%s>" % ("VF" if dom == "narrow_vf" else "EE"), "bound": "mfma",
+        roof = {"kernel": "k_narrow2_queue<%s>" % ("VF" if dom == "narrow_vf" else "EE"), "bound": "fp64",
                 "pipe": "FP64 VALU (no dense contraction, so no MFMA; peak is MI355X's FP64 figure)",
                 "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                 "traffic": None, "algorithmic_flops": flops}
@@ -472,8 +470,8 @@ def main():
     # workload (FETCH_SIZE x 2 + WRITE_SIZE, the factors a plain device copy gives); null elsewhere.
     try:
         prof = json.load(open(os.path.join(ROOT, "profiles", "cloth256_traffic.json")))
-        key = roof["kernel"].replace("k_narrow2_queue<VF>", "k_narrow2_queue<true, 256, ccd::RefRoots>").replace(
-            "k_narrow2_queue<EE>", "k_narrow2_queue<false, 256, ccd::RefRoots>")
+        key = roof["kernel"].replace("k_narrow2_queue<VF>", "k_narrow2_queue<true, 256, ccd::NoCount>").replace(
+            "k_narrow2_queue<EE>", "k_narrow2_queue<false, 256, ccd::NoCount>")
         if prof["workload"] == out_workload and key in prof["kernels"]:
             kp = prof["kernels"][key]
             roof["traffic"] = kp["hbm_bytes"]
@@ -488,11 +486,6 @@ def main():
     roof["stage_ms"] = {k: round(v, 5) for k, v in stage_ms.items()}
     roof["narrowphase_tflops"] = {
         "vf": flops_vf / max(stage_ms["narrow_vf"], 1e-9) / 1e9, "ee": flops_ee / max(stage_ms["narrow_ee"], 1e-9) / 1e9}
-    isolate = {"value": F / (alt["wall"] / args.steps), "ms_per_step": 1000.0 * alt["wall"] / args.steps,
-               "stage_ms": {k: round(v, 5) for k, v in alt["stage_ms"].items()},
-               "narrowphase_tflops": {"vf": alt["flops_vf"] / max(alt["stage_ms"]["narrow_vf"], 1e-9) / 1e9,
-                                      "ee": alt["flops_ee"] / max(alt["stage_ms"]["narrow_ee"], 1e-9) / 1e9},
-               "collisions": {"vf": alt["hits"][0], "ee": alt["hits"][1]}}
 
     out = {
         "metric": "swept tris/sec on 1 GPU, batched vs plain, exact sets; p50/p99 CCD query latency",
@@ -503,9 +496,8 @@ def main():
                    "global_batch": 1, "parallelism": "single GPU"},
         "pair_tests_per_s": nstencils / ((stage_ms["narrow_vf"] + stage_ms["narrow_ee"]) * 1e-3),
         "candidates": {"vf": nv.value, "ee": ne.value}, "collisions": {"vf": res["hits"][0], "ee": res["hits"][1]},
-        "root_mode": "reference (the reference's Jenkins-Traub root finder restated; its results bit for bit)",
+        "roots": "isolated in [0,1] (derivative bracketing + Newton), in place of the reference's rpoly",
         "roofline": roof,
-        "isolating_root_mode": isolate,
     }
     if args.copy_probe:
         src = torch.empty(256 << 20, dtype=torch.uint8, device=dev).fill_(1)
@@ -515,10 +507,8 @@ def main():
         torch.cuda.synchronize(dev)
         out["copy_probe_bytes"] = {"read": 256 << 20, "written": 256 << 20, "dispatches": 5}
     if not args.no_separating_plane:
-        ctx.set_root_mode(N.ROOTS_REFERENCE)
         out["separating_plane"] = bench_separating_plane(ctx, scene, qs, qe, max(3, args.steps // 2), log)
     if not args.no_velocity_filter:
-        ctx.set_root_mode(N.ROOTS_REFERENCE)
         out["velocity_filter"] = bench_velocity_filter(ctx, log)
     if not args.no_batched:
         out["batched"] = bench_batched(max(3, args.steps // 2), log)

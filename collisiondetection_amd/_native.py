@@ -26,9 +26,6 @@ PRIM_VERTEX_VERTEX = 3
 
 CCD_SORTED = 1
 CCD_GRAPH = 2
-ROOTS_REFERENCE = 0   # CCD_ROOTS_REFERENCE: the reference's Jenkins-Traub solver, restated (default)
-ROOTS_ISOLATE = 1     # CCD_ROOTS_ISOLATE: exact real-root isolation
-ROOT_MODE_NAMES = {ROOTS_REFERENCE: "reference roots", ROOTS_ISOLATE: "isolating roots"}
 
 DIST_VERTEX_PLANE_LT = 0
 DIST_LINE_LINE_LT = 1
@@ -102,8 +99,6 @@ class VFStats(ctypes.Structure):
 SIGNATURES.update({
     "ccd_velocity_filter": ([_V, _V, _V, _I64, _V, _I64, _V, _D, _D, _D, _D, _D, _INT, _INT, ctypes.POINTER(_INT),
                              ctypes.POINTER(VFStats)], _INT),
-    "ccd_set_root_mode": ([_V, _INT], _INT),
-    "ccd_root_mode": ([_V, ctypes.POINTER(_INT)], _INT),
     "ccd_server_create": ([_INT, _I64, _INT, ctypes.POINTER(_V)], _INT),
     "ccd_server_open_session": ([_V, _V, _I64, _V, _V, _I64, _D, _D, ctypes.POINTER(_INT)], _INT),
     "ccd_server_submit": ([_V, _INT, _V, _I64], _INT),
@@ -111,7 +106,6 @@ SIGNATURES.update({
     "ccd_server_drain": ([_V], _INT),
     "ccd_server_stats": ([_V, _P64], _INT),
     "ccd_server_keep_hits": ([_V, _INT], _INT),
-    "ccd_server_set_root_mode": ([_V, _INT], _INT),
     "ccd_server_fetch_hits": ([_V, _I64, _V, _I64, _P64], _INT),
     "ccd_server_destroy": ([_V], _INT),
 })

@@ -68,35 +68,6 @@ class Context:
     def device_bytes(self) -> int:
         return int(N.lib().ccd_device_bytes(self._p))
 
-    @property
-    def root_mode(self) -> int:
-        m = ctypes.c_int(0)
-        self.check(N.lib().ccd_root_mode(self._p, ctypes.byref(m)))
-        return m.value
-
-    def set_root_mode(self, mode: int):
-        """N.ROOTS_REFERENCE (default: the reference's root finder, bit for bit) or N.ROOTS_ISOLATE."""
-        self.check(N.lib().ccd_set_root_mode(self._p, int(mode)))
-
-
-class roots:
-    """Context manager: run the default context's narrowphase in another root mode, e.g.
-    `with api.roots(N.ROOTS_ISOLATE): api.detect(...)`."""
-
-    def __init__(self, mode: int, ctx: Context | None = None):
-        self.mode = mode
-        self.ctx = ctx
-
-    def __enter__(self):
-        self.ctx = self.ctx or default_context()
-        self.old = self.ctx.root_mode
-        self.ctx.set_root_mode(self.mode)
-        return self.ctx
-
-    def __exit__(self, *exc):
-        self.ctx.set_root_mode(self.old)
-        return False
-
 
 def default_context(device: int = 0) -> Context:
     if device not in _ctx_cache:
@@ -740,11 +711,6 @@ class Server:
 
     def keep_hits(self, on=True):
         N.lib().ccd_server_keep_hits(self._p, int(on))
-
-    def set_root_mode(self, mode: int):
-        """Root mode (N.ROOTS_REFERENCE / N.ROOTS_ISOLATE) of sessions opened afterwards."""
-        if N.lib().ccd_server_set_root_mode(self._p, int(mode)) != N.CCD_OK:
-            raise N.CCDError(N.CCD_E_INVALID, f"invalid root mode {mode}")
 
     def fetch_hits(self, request_id):
         """-> (vf keys (n,4), vf toi, ee keys (m,4), ee toi) of a request that completed while keep_hits

@@ -61,6 +61,8 @@ void *buffer(ccd_ctx *ctx, const char *name, size_t bytes)
         return nullptr;
     }
     if (b.ptr) {
+        // A captured pipeline holds the old address: it must not be replayed again.
+        ctx->graph_valid = false;
         // The old allocation may still be read by queued work on the stream.
         if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
             set_error(ctx, CCD_E_HIP, "stream synchronisation failed before growing %s", name);
@@ -619,25 +621,6 @@ int ccd_set_queue_shape(ccd_ctx *ctx, const int *shape)
     ctx->vf_shape = s[0];
     ctx->ee_shape = s[1];
     graph_drop(ctx);   // captured launches carry the old shape
-    return CCD_OK;
-}
-
-int ccd_set_root_mode(ccd_ctx *ctx, int mode)
-{
-    if (!ctx) return CCD_E_INVALID;
-    if (mode != CCD_ROOTS_REFERENCE && mode != CCD_ROOTS_ISOLATE)
-        return set_error(ctx, CCD_E_INVALID, "unknown root mode %d", mode);
-    if (mode != ctx->root_mode) {
-        ctx->root_mode = mode;
-        graph_drop(ctx);   // captured launches carry the old kernels
-    }
-    return CCD_OK;
-}
-
-int ccd_root_mode(ccd_ctx *ctx, int *mode)
-{
-    if (!ctx || !mode) return CCD_E_INVALID;
-    *mode = ctx->root_mode;
     return CCD_OK;
 }
 

@@ -14,17 +14,14 @@
 //   * the sign of each piece between roots is decided at its midpoint with plain Horner on the
 //     normalised coefficients (CTCD.cpp:59-79), and std::max / std::min keep their (a < b) ? b : a
 //     and (b < a) ? b : a argument order.
-// Roots of degree 3..6 polynomials come from one of two methods, chosen per call by the counter
-// type (see RefRoots):
-//   * default -- this file isolates the real roots in [0,1] by recursion on derivatives (critical
-//     points split [0,1] into monotone pieces; each piece with a sign change holds exactly one root,
-//     found by bracketed Newton iteration).  The real parts of complex roots, which the reference
-//     also uses as split points, never change the union of the sign intervals, so vertex-face and
-//     vertex-edge results agree except where a root lies within rounding distance of a decision.
-//     Times of impact are the exact roots of the double polynomials to ~1e-15.
-//   * reference roots -- the reference's own Jenkins-Traub solver restated operation for operation
-//     (ccd_rpoly.h), splitting at the real parts of all its roots as CTCD.cpp:135-176 does: the
-//     reference's outputs bit for bit, its root-finding error included.
+// Roots of degree 3..6 polynomials are isolated, not found by the reference's Jenkins-Traub solver
+// (rpoly.h): the real roots in [0,1] come from recursion on derivatives (critical points split [0,1]
+// into monotone pieces; each piece with a sign change holds exactly one root, found by bracketed
+// Newton iteration).  The real parts of complex roots, which the reference also uses as split points,
+// never change the union of the sign intervals, so vertex-face and vertex-edge results agree except
+// where a root lies within rounding distance of a decision.  Times of impact are the exact roots of
+// the double polynomials to ~1e-15, where the reference's solver can be off by far more (its error on
+// the near-double roots of the distance sextics reaches 1e-4).
 #pragma once
 
 #include <stdint.h>
@@ -41,6 +38,22 @@
 #define CCD_HD inline
 #define CCD_FHD inline
 #define CCD_HD_NI inline
+#endif
+
+// Test builds for the host (CCD_CHECK_INDEX) stop on any runtime index outside its array; the product
+// build compiles the checks away.
+#if defined(CCD_CHECK_INDEX) && !defined(__HIPCC__)
+#include <cstdio>
+#include <cstdlib>
+#define CCD_INDEX_CHECK(i, n)                                                                                      \
+    do {                                                                                                           \
+        if ((i) < 0 || (i) >= (n)) {                                                                               \
+            std::fprintf(stderr, "ccd_core: index %d outside [0, %d) at line %d\n", (int)(i), (int)(n), __LINE__); \
+            std::abort();                                                                                          \
+        }                                                                                                          \
+    } while (0)
+#else
+#define CCD_INDEX_CHECK(i, n) ((void)0)
 #endif
 
 namespace ccd {
@@ -113,23 +126,13 @@ struct NoCount {
     CCD_FHD void operator()(int) const {}
 };
 
-// A counter type whose `ref_roots` member is true selects the reference-root mode in find_intervals.
-struct RefRoots {
-    static constexpr bool ref_roots = true;
-    CCD_FHD void operator()(int) const {}
-};
-template <class Base>
-struct WithRefRoots : Base {
-    static constexpr bool ref_roots = true;
-};
 // The product's counter types prune cascades that cannot hit (see any_overlap below); the counting
 // types used for work accounting, and FullCascade, run the reference's full cascade.
 struct FullCascade {
     CCD_FHD void operator()(int) const {}
 };
 template <class C>
-struct prunes : std::integral_constant<bool, std::is_same<std::decay_t<C>, NoCount>::value ||
-                                                 std::is_same<std::decay_t<C>, RefRoots>::value> {};
+struct prunes : std::integral_constant<bool, std::is_same<std::decay_t<C>, NoCount>::value> {};
 // A counter type whose `stops_at_solver` member is true makes find_intervals return, with no intervals,
 // where it would call the root solver.  SolverProbe records whether that point was reached; the
 // stencil-ordering pre-pass uses it to see which polynomials need the solver.
@@ -137,11 +140,6 @@ template <class C, class = void>
 struct stops_at_solver : std::false_type {};
 template <class C>
 struct stops_at_solver<C, std::void_t<decltype(C::stops_at_solver)>> : std::integral_constant<bool, C::stops_at_solver> {};
-
-template <class C, class = void>
-struct uses_ref_roots : std::false_type {};
-template <class C>
-struct uses_ref_roots<C, std::void_t<decltype(C::ref_roots)>> : std::integral_constant<bool, C::ref_roots> {};
 
 CCD_FHD double bits_to_double(long long v)
 {
@@ -173,6 +171,7 @@ struct Ivls {
 template <int N>
 CCD_FHD double sel(const double (&a)[N], int i)
 {
+    CCD_INDEX_CHECK(i, N);
     double v = a[0];
 #pragma unroll
     for (int k = 1; k < N; k++)
@@ -182,6 +181,7 @@ CCD_FHD double sel(const double (&a)[N], int i)
 template <int N>
 CCD_FHD void put(double (&a)[N], int i, double v)
 {
+    CCD_INDEX_CHECK(i, N);
 #pragma unroll
     for (int k = 0; k < N; k++)
         if (k == i) a[k] = v;
@@ -424,10 +424,6 @@ struct RootIsolator<2> {
     }
 };
 
-}   // namespace ccd
-#include "ccd_rpoly.h"
-namespace ccd {
-
 // Roots of the degree-D polynomial op in [0,1] into time[0..), D <= M.
 template <int D, int M>
 CCD_HD int isolate_into(const double *op, double (&time)[M])
@@ -493,28 +489,6 @@ CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos, C &&cnt = C())
         if (!could_have_roots<N>(op, reduced, pos)) return;
         cnt(S_ISOLATE);
         if constexpr (stops_at_solver<CT>::value) return;
-        if constexpr (uses_ref_roots<CT>::value) {
-            // CTCD.cpp:139-141 and :160-176: all roots the solver returns, sorted, real parts of
-            // complex ones included.
-            double zr[6] = {}, zi[6] = {}, opc[7];
-#pragma unroll
-            for (int i = 0; i < 7; i++) opc[i] = i <= N ? op[i] : 0.0;
-            roots = jt::find_roots(opc, reduced, zr, zi);
-            if (roots > 0) {
-                jt::sort_small(zr, roots);
-                if (zr[0] >= 0) check_interval<N>(0, zr[0], op, reduced, out, pos);
-#pragma unroll
-                for (int i = 0; i < 5; i++) {
-                    if (i < roots - 1 && !((zr[i] < 0 && zr[i + 1] < 0) || (zr[i] > 1.0 && zr[i + 1] > 1.0)))
-                        check_interval<N>(zr[i], zr[i + 1], op, reduced, out, pos);
-                }
-                double zl = sel(zr, roots - 1);
-                if (zl <= 1.0) check_interval<N>(zl, 1.0, op, reduced, out, pos);
-            } else {
-                check_interval<N>(0.0, 1.0, op, reduced, out, pos);
-            }
-            return;
-        }
         roots = isolate_roots<N>(op, reduced, time);
         // Pieces between consecutive roots (all in [0,1]) and the ends, as CTCD.cpp:160-176.
         if (roots == 0) {
@@ -1340,6 +1314,7 @@ CCD_HD int separating_plane_check(const HistCSR &h, const int (&v)[4], double et
     while (top > 0) {
         if (max_nodes-- <= 0) return 3;
         top--;
+        CCD_INDEX_CHECK(top, SP_MAX_DEPTH + 2);
         const double mint = st_lo[top], maxt = st_hi[top];
         const int depth = st_d[top];
         if (depth > SP_MAX_DEPTH) return 2;
@@ -1418,6 +1393,7 @@ CCD_HD int separating_plane_check(const HistCSR &h, const int (&v)[4], double et
         // node is undecided when it is reached (the upper one only if the lower subtree finds nothing).
         if (t_hi < 1.0) {
             const double lo = midt + smax(0.0, t_hi - eps);
+            CCD_INDEX_CHECK(top, SP_MAX_DEPTH + 2);
             st_lo[top] = lo;
             st_hi[top] = maxt;
             st_d[top] = lo == mint ? SP_MAX_DEPTH + 1 : depth + 1;   // equal to this node: undecided if reached
@@ -1426,6 +1402,7 @@ CCD_HD int separating_plane_check(const HistCSR &h, const int (&v)[4], double et
         if (t_lo < 1.0) {
             const double hi = midt - smax(0.0, t_lo - eps);
             if (hi == maxt) return 2;
+            CCD_INDEX_CHECK(top, SP_MAX_DEPTH + 2);
             st_lo[top] = mint;
             st_hi[top] = hi;
             st_d[top] = depth + 1;

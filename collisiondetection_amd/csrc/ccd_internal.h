@@ -13,14 +13,13 @@
 // Launch shape of the persistent work-queue narrowphase (tuned by ccd_autotune).
 struct QueueShape {
     int block = 256;       // threads per block: 64, 128 or 256
-    int tile = 64;         // stencils a wavefront takes per atomic (64 measured best in both root modes)
+    int tile = 64;         // stencils a wavefront takes per atomic
     int waves_per_cu = 8;  // resident wavefronts per compute unit in the persistent grid
 };
 
 struct ccd_ctx {
     int device = 0;
     int num_cus = 256;
-    int root_mode = CCD_ROOTS_REFERENCE;
     bool group_stencils = true;   // group stencils by vertex-edge signature before the narrowphase
     QueueShape vf_shape, ee_shape;
     hipStream_t own_stream = nullptr;

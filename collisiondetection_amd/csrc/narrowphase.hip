@@ -394,13 +394,7 @@ __global__ __launch_bounds__(256) CCD_TWO_WAVES void k_separating_plane(
         int4 k = keys[i];
         int v[4] = {k.x, k.y, k.z, k.w};
         ccd::HistCSR h{off, ht, hp};
-        // The leaves run the full cascade (no pruning): with the pruned one, some lanes of this kernel
-        // returned results that differed from the host build of the same code when other lanes of their
-        // wavefront took other paths (each was right when launched alone); the full cascade is
-        // bit-identical to the host on every tested stencil.
-        using L = std::conditional_t<ccd::uses_ref_roots<M>::value, ccd::WithRefRoots<ccd::FullCascade>,
-                                     ccd::FullCascade>;
-        r = ccd::separating_plane_check<VF>(h, v, eta ? eta[i] : eta_s, eps, L());
+        r = ccd::separating_plane_check<VF>(h, v, eta ? eta[i] : eta_s, eps, M());
         status[i] = (uint8_t)r;
     }
     uint64_t m = __ballot(r == 2);
@@ -598,9 +592,7 @@ __global__ __launch_bounds__(64) CCD_TWO_WAVES void k_separating_plane_wave(
     const int4 k = keys[i];
     const int v[4] = {k.x, k.y, k.z, k.w};
     ccd::HistCSR h{off, ht, hp};
-    using L = std::conditional_t<ccd::uses_ref_roots<M>::value, ccd::WithRefRoots<ccd::FullCascade>,
-                                 ccd::FullCascade>;
-    const int r = ccd::separating_plane_check<VF>(h, v, eta ? eta[i] : eta_s, eps, L(), WaveScan());
+    const int r = ccd::separating_plane_check<VF>(h, v, eta ? eta[i] : eta_s, eps, M(), WaveScan());
     if (__lane_id() == 0) {
         status[i] = (uint8_t)r;
         if (r == 2) atomicAdd(undecided, 1ull);
@@ -619,24 +611,19 @@ constexpr int LANE_BLOCK = 64;
 
 namespace ccdi {
 
-// Every narrowphase launch instantiates its kernel for the context's root mode: ccd::NoCount is the
-// default root isolation, ccd::RefRoots the reference's Jenkins-Traub solver (CCD_ROOTS_REFERENCE).
-#define CCD_MODE_DISPATCH(ctx, LAUNCH)                                                                              \
+// Every narrowphase launch instantiates its kernel with the product's counter type: no work
+// accounting, cascades pruned once no interval combination can overlap.
+#define CCD_PRODUCT_LAUNCH(LAUNCH)                                                                                  \
     do {                                                                                                            \
-        if ((ctx)->root_mode == CCD_ROOTS_REFERENCE) {                                                              \
-            using M = ccd::RefRoots;                                                                                \
-            LAUNCH;                                                                                                 \
-        } else {                                                                                                    \
-            using M = ccd::NoCount;                                                                                 \
-            LAUNCH;                                                                                                 \
-        }                                                                                                           \
+        using M = ccd::NoCount;                                                                                     \
+        LAUNCH;                                                                                                     \
     } while (0)
 
 int launch_primitives(ccd_ctx *ctx, int kind, const double *q, const double *eta, double eta_s, int64_t n,
                       uint8_t *hit, double *t)
 {
     if (n == 0) return CCD_OK;
-    CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL(k_primitives<M>, dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK), 0, ctx->stream,
+    CCD_PRODUCT_LAUNCH(hipLaunchKernelGGL(k_primitives<M>, dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK), 0, ctx->stream,
                                               kind, q, eta, eta_s, n, hit, t));
     CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
@@ -649,7 +636,8 @@ int launch_narrow2(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, co
     // Run the stencils in the grouping order of the detection pipeline (k_stencil_signature); each
     // result lands at its stencil's own index, so only the speed depends on the order.
     const int32_t *order = nullptr;
-    if (n >= 4096 && ctx->group_stencils) {
+    // (the order holds int32 stencil indices: launches past 2^31 - 1 stencils run in the given order)
+    if (n >= 4096 && n <= 0x7fffffffll && ctx->group_stencils) {
         uint64_t *sig = (uint64_t *)buffer(ctx, "n2o_sig", (size_t)n * 8);
         uint64_t *sig2 = (uint64_t *)buffer(ctx, "n2o_sig2", (size_t)n * 8);
         int32_t *idx = (int32_t *)buffer(ctx, "n2o_idx", (size_t)n * 4);
@@ -669,11 +657,11 @@ int launch_narrow2(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, co
         order = in_alt ? idx2 : idx;
     }
     if (vf)
-        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow2<true, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK), 0,
+        CCD_PRODUCT_LAUNCH(hipLaunchKernelGGL((k_narrow2<true, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK), 0,
                                                   ctx->stream, qs, qe, (const int4 *)keys, eta, eta_s, n, hit, t, sub,
                                                   order));
     else
-        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow2<false, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK), 0,
+        CCD_PRODUCT_LAUNCH(hipLaunchKernelGGL((k_narrow2<false, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK), 0,
                                                   ctx->stream, qs, qe, (const int4 *)keys, eta, eta_s, n, hit, t, sub,
                                                   order));
     CCD_LAUNCH_CHECK(ctx);
@@ -686,11 +674,11 @@ int launch_narrow2_compact(ccd_ctx *ctx, bool vf, const double *qs, const double
 {
     if (n_max == 0) return CCD_OK;
     if (vf)
-        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow2_compact<true, M>), dim3(blocks_for(n_max, 256)),
+        CCD_PRODUCT_LAUNCH(hipLaunchKernelGGL((k_narrow2_compact<true, M>), dim3(blocks_for(n_max, 256)),
                                                   dim3(256), 0, ctx->stream, qs, qe, (const int4 *)keys, n_dev, n_max,
                                                   eta, (int4 *)out_keys, out_toi, cap, (unsigned long long *)counter));
     else
-        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow2_compact<false, M>), dim3(blocks_for(n_max, 256)),
+        CCD_PRODUCT_LAUNCH(hipLaunchKernelGGL((k_narrow2_compact<false, M>), dim3(blocks_for(n_max, 256)),
                                                   dim3(256), 0, ctx->stream, qs, qe, (const int4 *)keys, n_dev, n_max,
                                                   eta, (int4 *)out_keys, out_toi, cap, (unsigned long long *)counter));
     CCD_LAUNCH_CHECK(ctx);
@@ -701,16 +689,11 @@ int launch_narrow2_count(ccd_ctx *ctx, bool vf, const double *qs, const double *
                          int64_t n, double eta, unsigned long long *counts)
 {
     if (n == 0) return CCD_OK;
-    const bool ref = ctx->root_mode == CCD_ROOTS_REFERENCE;
 #define CCD_COUNT_LAUNCH(V, LCT)                                                                                    \
     hipLaunchKernelGGL((k_narrow2_count<V, LCT>), dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,     \
                        (const int4 *)keys, n, eta, counts)
-    if (vf && ref)
-        CCD_COUNT_LAUNCH(true, ccd::WithRefRoots<LaneCount>);
-    else if (vf)
+    if (vf)
         CCD_COUNT_LAUNCH(true, LaneCount);
-    else if (ref)
-        CCD_COUNT_LAUNCH(false, ccd::WithRefRoots<LaneCount>);
     else
         CCD_COUNT_LAUNCH(false, LaneCount);
 #undef CCD_COUNT_LAUNCH
@@ -747,10 +730,10 @@ int launch_narrow2_queue(ccd_ctx *ctx, bool vf, const double *qs, const double *
     waves = std::max<int64_t>(1, std::min(waves, need));
     unsigned grid = (unsigned)((waves * 64 + bs - 1) / bs);
     if (vf)
-        CCD_MODE_DISPATCH(ctx, (launch_queue_bs<true, M>(ctx, bs, grid, qs, qe, keys, n_dev, n_max, eta, out_keys,
+        CCD_PRODUCT_LAUNCH((launch_queue_bs<true, M>(ctx, bs, grid, qs, qe, keys, n_dev, n_max, eta, out_keys,
                                                          out_toi, cap, counter, work, shape.tile)));
     else
-        CCD_MODE_DISPATCH(ctx, (launch_queue_bs<false, M>(ctx, bs, grid, qs, qe, keys, n_dev, n_max, eta, out_keys,
+        CCD_PRODUCT_LAUNCH((launch_queue_bs<false, M>(ctx, bs, grid, qs, qe, keys, n_dev, n_max, eta, out_keys,
                                                           out_toi, cap, counter, work, shape.tile)));
     CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
@@ -764,22 +747,22 @@ int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *
     if (wave) {
         if (n > 0x7fffffffll) return set_error(ctx, CCD_E_INVALID, "too many stencils for one launch");
         if (vf)
-            CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow_hist_wave<true, M>), dim3((unsigned)n), dim3(64), 0,
+            CCD_PRODUCT_LAUNCH(hipLaunchKernelGGL((k_narrow_hist_wave<true, M>), dim3((unsigned)n), dim3(64), 0,
                                                       ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, hit,
                                                       t, sub, win));
         else
-            CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow_hist_wave<false, M>), dim3((unsigned)n), dim3(64), 0,
+            CCD_PRODUCT_LAUNCH(hipLaunchKernelGGL((k_narrow_hist_wave<false, M>), dim3((unsigned)n), dim3(64), 0,
                                                       ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, hit,
                                                       t, sub, win));
         CCD_LAUNCH_CHECK(ctx);
         return CCD_OK;
     }
     if (vf)
-        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow_hist<true, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK), 0,
+        CCD_PRODUCT_LAUNCH(hipLaunchKernelGGL((k_narrow_hist<true, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK), 0,
                                                   ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, hit, t,
                                                   sub, win));
     else
-        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_narrow_hist<false, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK), 0,
+        CCD_PRODUCT_LAUNCH(hipLaunchKernelGGL((k_narrow_hist<false, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK), 0,
                                                   ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, hit, t,
                                                   sub, win));
     CCD_LAUNCH_CHECK(ctx);
@@ -790,7 +773,7 @@ int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *
 int reserve_grouping(ccd_ctx *ctx, int64_t nvf, int64_t nee)
 {
     const int64_t n = std::max(nvf, nee);
-    if (n < 2) return CCD_OK;
+    if (n < 2 || n > 0x7fffffffll) return CCD_OK;
     if (!buffer(ctx, "grp_sig", (size_t)n * 8) || !buffer(ctx, "grp_sig2", (size_t)n * 8) ||
         !buffer(ctx, "grp_idx", (size_t)n * 4) || !buffer(ctx, "grp_idx2", (size_t)n * 4) ||
         !buffer(ctx, "grp_keys_vf", (size_t)std::max<int64_t>(nvf, 1) * 16) ||
@@ -804,7 +787,7 @@ int group_by_signature(ccd_ctx *ctx, bool vf, const double *qs, const double *qe
                        double eta, const int64_t *n_dev, const int32_t **grouped)
 {
     *grouped = keys;
-    if (n < 2) return CCD_OK;
+    if (n < 2 || n > 0x7fffffffll) return CCD_OK;   // int32 indices: larger launches keep the given order
     uint64_t *sig = (uint64_t *)buffer(ctx, "grp_sig", (size_t)n * 8);
     uint64_t *sig2 = (uint64_t *)buffer(ctx, "grp_sig2", (size_t)n * 8);
     int32_t *idx = (int32_t *)buffer(ctx, "grp_idx", (size_t)n * 4);
@@ -852,11 +835,11 @@ int launch_separating_plane(ccd_ctx *ctx, bool vf, const int64_t *off, const dou
     if (wave) {
         if (n > 0x7fffffffll) return set_error(ctx, CCD_E_INVALID, "too many stencils for one launch");
         if (vf)
-            CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_separating_plane_wave<true, M>), dim3((unsigned)n), dim3(64),
+            CCD_PRODUCT_LAUNCH(hipLaunchKernelGGL((k_separating_plane_wave<true, M>), dim3((unsigned)n), dim3(64),
                                                       0, ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n,
                                                       gb, status, und));
         else
-            CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_separating_plane_wave<false, M>), dim3((unsigned)n),
+            CCD_PRODUCT_LAUNCH(hipLaunchKernelGGL((k_separating_plane_wave<false, M>), dim3((unsigned)n),
                                                       dim3(64), 0, ctx->stream, off, ht, hp, (const int4 *)keys, eta,
                                                       eta_s, n, gb, status, und));
         CCD_LAUNCH_CHECK(ctx);
@@ -865,7 +848,7 @@ int launch_separating_plane(ccd_ctx *ctx, bool vf, const int64_t *off, const dou
     // Order the stencils so that those decided at the root run together (results are per stencil and
     // land at the stencil's own index, so the order changes nothing else).
     const int32_t *order = nullptr;
-    if (n >= 4096) {
+    if (n >= 4096 && n <= 0x7fffffffll) {   // int32 order indices
         uint64_t *key = (uint64_t *)buffer(ctx, "spo_key", (size_t)n * 8);
         uint64_t *key2 = (uint64_t *)buffer(ctx, "spo_key2", (size_t)n * 8);
         int32_t *idx = (int32_t *)buffer(ctx, "spo_idx", (size_t)n * 4);
@@ -885,11 +868,11 @@ int launch_separating_plane(ccd_ctx *ctx, bool vf, const int64_t *off, const dou
         order = in_alt ? idx2 : idx;
     }
     if (vf)
-        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_separating_plane<true, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK),
+        CCD_PRODUCT_LAUNCH(hipLaunchKernelGGL((k_separating_plane<true, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK),
                                                   0, ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, gb,
                                                   status, und, order));
     else
-        CCD_MODE_DISPATCH(ctx, hipLaunchKernelGGL((k_separating_plane<false, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK),
+        CCD_PRODUCT_LAUNCH(hipLaunchKernelGGL((k_separating_plane<false, M>), dim3(blocks_for(n, LANE_BLOCK)), dim3(LANE_BLOCK),
                                                   0, ctx->stream, off, ht, hp, (const int4 *)keys, eta, eta_s, n, gb,
                                                   status, und, order));
     CCD_LAUNCH_CHECK(ctx);

@@ -153,7 +153,6 @@ struct ccd_server {
     std::deque<ccd_request_result> done;
     std::map<int64_t, std::vector<int32_t>> kept;   // request id -> colliding stencils (when keeping)
     bool keep_hits = false;
-    int root_mode = CCD_ROOTS_REFERENCE;   // for sessions opened from now on
     int64_t resident_bytes = 0, budget = 0;
     std::mutex mu;
     std::condition_variable cv;
@@ -420,10 +419,6 @@ int ccd_server_open_session(ccd_server *S, const int32_t *faces, int64_t nfaces,
     Session s;
     int rc = ccd_create(&s.ctx, S->device);
     if (rc) return rc;
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        s.ctx->root_mode = S->root_mode;
-    }
     s.F = nfaces;
     s.V = nverts;
     s.outer_eta = outer_eta;
@@ -536,14 +531,6 @@ int ccd_server_stats(ccd_server *S, int64_t *stats)
     stats[7] = (int64_t)S->sessions.size();
     stats[8] = S->mem_waits;
     stats[9] = S->max_request_bytes;
-    return CCD_OK;
-}
-
-int ccd_server_set_root_mode(ccd_server *S, int mode)
-{
-    if (!S || (mode != CCD_ROOTS_REFERENCE && mode != CCD_ROOTS_ISOLATE)) return CCD_E_INVALID;
-    std::lock_guard<std::mutex> lk(S->mu);
-    S->root_mode = mode;
     return CCD_OK;
 }
 

@@ -47,12 +47,6 @@ extern "C" {
 #define CCD_GRAPH 2  /* ccd_batch_detect: capture the pipeline in a HIP graph and replay it while the
                         arguments (pointers, sizes, eta) stay the same; the data behind them may change */
 
-/* Root modes (ccd_set_root_mode) for polynomials of reduced degree 3..6 in the narrowphase. */
-#define CCD_ROOTS_REFERENCE 0 /* the reference's Jenkins-Traub solver (rpoly.h:49-759) restated operation for
-                                 operation: the reference's flags and times of impact bit for bit (default) */
-#define CCD_ROOTS_ISOLATE 1   /* real-root isolation on [0,1] (derivative bracketing + Newton): times of impact
-                                 are the exact roots of the double polynomials to ~1e-15 */
-
 /* Stage timers (ccd_stage_times), in pipeline order. */
 #define CCD_T_BOUNDS 0    /* leaf k-DOPs, scene box, Morton codes */
 #define CCD_T_SORT 1      /* Morton radix sort */
@@ -292,15 +286,6 @@ int ccd_autotune(ccd_ctx *ctx, const double *qstart, const double *qend, double 
 int ccd_set_queue_shape(ccd_ctx *ctx, const int *shape);
 
 /*
- * Root mode of every narrowphase on this context (CCD_ROOTS_REFERENCE or CCD_ROOTS_ISOLATE).  The
- * reference's findIntervals (CTCD.cpp:98-177) always calls its Jenkins-Traub solver; the isolating mode
- * trades that bit-for-bit agreement for exact roots (the reference's solver is off by up to ~1e-4 on
- * near-double roots).  ccd_root_mode reads the current mode.
- */
-int ccd_set_root_mode(ccd_ctx *ctx, int mode);
-int ccd_root_mode(ccd_ctx *ctx, int *mode);
-
-/*
  * Serving (no counterpart in the reference): device-resident sessions and a continuous-batching loop.
  *
  * ccd_server_create: a server on `device` with an HBM pool of hbm_budget_bytes for the transient
@@ -339,7 +324,6 @@ int ccd_server_poll(ccd_server *server, ccd_request_result *out, int max, int *n
 int ccd_server_drain(ccd_server *server);
 int ccd_server_stats(ccd_server *server, int64_t *stats);
 int ccd_server_keep_hits(ccd_server *server, int on);
-int ccd_server_set_root_mode(ccd_server *server, int mode);   /* for sessions opened afterwards */
 int ccd_server_fetch_hits(ccd_server *server, int64_t request_id, int32_t *out, int64_t cap, int64_t *n);
 int ccd_server_destroy(ccd_server *server);
 

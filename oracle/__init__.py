@@ -59,6 +59,9 @@ def lib():
         L.orc_find_roots.argtypes = [_dp, ctypes.c_int, _dp, _dp]
         L.orc_find_roots.restype = ctypes.c_int
         L.orc_kdop_axes.argtypes = [_dp]
+        L.orc_stitch.argtypes = [_lp, _dp, _dp, _ip, ctypes.c_int, ctypes.c_int64, _dp, _dp]
+        L.orc_stitch.restype = ctypes.c_int64
+        L.orc_sampled_min_distance.argtypes = [_dp, _dp, _ip, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_get_quad_roots.argtypes = [ctypes.c_double] * 3 + [_dp, _dp]
         L.orc_get_quad_roots.restype = ctypes.c_int
         L.orc_find_intervals.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
@@ -275,6 +278,30 @@ class Hist:
     @property
     def nverts(self):
         return len(self.off) - 1
+
+
+def stitch(hist: Hist, verts):
+    """History::stitchCommonHistory (History.cpp:98-140) for up to 4 vertices: (times (n,), positions
+    (n, len(verts), 3))."""
+    v = np.ascontiguousarray(verts, dtype=np.int32)
+    nv = len(v)
+    cap = int(sum(hist.off[x + 1] - hist.off[x] for x in v)) + 2
+    t = np.zeros(cap)
+    p = np.zeros((cap, nv, 3))
+    n = lib().orc_stitch(_l(hist.off), _d(hist.ht), _d(hist.hp), _i(v), nv, cap, _d(t), _d(p))
+    assert n <= cap
+    return t[:n], p[:n]
+
+
+def sampled_min_distance(qstart, qend, keys, is_ee: bool, samples: int = 401):
+    """Per stencil, the smallest vertex-face / edge-edge distance over `samples` equally spaced times of
+    the straight motion qstart -> qend (Distance.h's closest points)."""
+    keys = np.ascontiguousarray(keys, dtype=np.int32).reshape(-1, 4)
+    qstart = np.ascontiguousarray(qstart, dtype=np.float64)
+    qend = np.ascontiguousarray(qend, dtype=np.float64)
+    out = np.zeros(len(keys))
+    lib().orc_sampled_min_distance(_d(qstart), _d(qend), _i(keys), len(keys), int(is_ee), int(samples), _d(out))
+    return out
 
 
 def kdop_axes():

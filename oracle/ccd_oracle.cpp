@@ -1915,6 +1915,30 @@ int orc_aberth_roots(const double *op, int degree, double *zr, double *zi)
 
 int orc_find_roots(double *op, int degree, double *zr, double *zi) { return find_roots(op, degree, zr, zi); }
 
+// stitch() (History.cpp:98-140) for the nv <= 4 vertices verts[]: writes up to cap stitched times and
+// nv positions per time (xyz, vertex-major within a time); returns the number of stitched times.
+int64_t orc_stitch(const int64_t *off, const double *ht, const double *hp, const int32_t *verts, int nv, int64_t cap,
+                   double *times, double *pos)
+{
+    Hist h{off, ht, hp};
+    int v[4] = {0, 0, 0, 0};
+    for (int i = 0; i < nv && i < 4; i++) v[i] = verts[i];
+    std::vector<double> T;
+    std::vector<V3> P;
+    stitch(h, v, nv, T, P);
+    const int64_t n = (int64_t)T.size();
+    for (int64_t k = 0; k < n && k < cap; k++) {
+        times[k] = T[k];
+        for (int i = 0; i < nv; i++) {
+            const V3 &q = P[k * nv + i];
+            pos[3 * (k * nv + i)] = q.x;
+            pos[3 * (k * nv + i) + 1] = q.y;
+            pos[3 * (k * nv + i) + 2] = q.z;
+        }
+    }
+    return n;
+}
+
 void orc_kdop_axes(double *out39)
 {
     Axes ax = make_axes();
@@ -2096,6 +2120,30 @@ void orc_separating_plane(const int64_t *off, const double *ht, const double *hp
 int orc_stage_count() { return ST_COUNT; }
 
 // Proximity statics on one item: q = 4 points (12 doubles).  kind as CCD_DIST_* in include/ccd_amd.h.
+// Smallest distance between each stencil's primitives (vertex-face or edge-edge, Distance.h:32-174)
+// over `samples` equally spaced times of the straight motion qs -> qe: a check of the narrowphases
+// (which must report every stencil that comes within eta), independent of their polynomials.
+void orc_sampled_min_distance(const double *qs, const double *qe, const int32_t *keys, int64_t n, int is_ee,
+                              int samples, double *out)
+{
+    for (int64_t i = 0; i < n; i++) {
+        double best = std::numeric_limits<double>::infinity();
+        for (int k = 0; k < samples; k++) {
+            const double t = samples > 1 ? (double)k / (double)(samples - 1) : 0.0;
+            V3 P[4];
+            for (int j = 0; j < 4; j++) {
+                const V3 a = load3(qs + 3 * (int64_t)keys[4 * i + j]), b = load3(qe + 3 * (int64_t)keys[4 * i + j]);
+                P[j] = V3{(1 - t) * a.x + t * b.x, (1 - t) * a.y + t * b.y, (1 - t) * a.z + t * b.z};
+            }
+            double bary[4];
+            const V3 d = is_ee ? edge_edge_distance(P[0], P[1], P[2], P[3], bary)
+                               : vertex_face_distance(P[0], P[1], P[2], P[3], bary);
+            best = std::min(best, norm(d));
+        }
+        out[i] = best;
+    }
+}
+
 int orc_distance(int kind, const double *q, double eta, double *vec, double *bary)
 {
     V3 a = load3(q), b = load3(q + 3), c = load3(q + 6), d = load3(q + 9);

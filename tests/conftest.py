@@ -29,18 +29,9 @@ def core_harness():
     import subprocess
 
     so = os.path.join(ROOT, "tests", "_build", "core_harness.so")
-    so_libm = os.path.join(ROOT, "tests", "_build", "core_harness_libm.so")
     src = os.path.join(ROOT, "tests", "native", "core_harness.cpp")
-    hdrs = [os.path.join(ROOT, "collisiondetection_amd", "csrc", h) for h in ("ccd_core.h", "ccd_rpoly.h")]
+    hdrs = [os.path.join(ROOT, "collisiondetection_amd", "csrc", "ccd_core.h")]
     newest = max(os.path.getmtime(x) for x in [src] + hdrs)
-    if not all(os.path.exists(x) and os.path.getmtime(x) >= newest for x in (so, so_libm)):
+    if not (os.path.exists(so) and os.path.getmtime(so) >= newest):
         subprocess.run(["bash", os.path.join(ROOT, "tests", "native", "build.sh")], check=True)
     return ctypes.CDLL(so)
-
-
-@pytest.fixture(scope="session")
-def core_harness_libm(core_harness):
-    """The same host build with the C library's log/exp in the reference-root solver (test-only)."""
-    import ctypes
-
-    return ctypes.CDLL(os.path.join(ROOT, "tests", "_build", "core_harness_libm.so"))

@@ -119,52 +119,6 @@ def compare_results(name, hit_a, t_a, rec, tol_t=1e-9, margin_tol=1e-12, arbiter
             "t_bad_hard": t_bad_hard.tolist()[:20], "ref_root_error": ref_err[:20], "n_ref_root_error": len(ref_err)}
 
 
-def compare_reference(name, hit_a, t_a, rec, tol_t=1e-9, margin_tol=1e-12, libm=None):
-    """Reference-root mode against the oracle running the reference's own root finder: flags equal
-    (a disagreement is tolerated only with an oracle margin below margin_tol, and listed), times of
-    impact within tol_t, and the count of times of impact that are not bit-identical reported.
-
-    libm: idx -> (hit, t) of the host-built core whose solver uses the C library's log/exp.  A
-    disagreement that this variant does not have -- it equals the oracle bit for bit -- comes from the C
-    library's rounding of log/exp inside the solver alone; it is listed under libm_rounding."""
-    hit_o = rec["hit"].astype(bool)
-    mism = np.where(hit_a != hit_o)[0]
-    border = mism[rec["margin"][mism] < margin_tol]
-    hard = mism[rec["margin"][mism] >= margin_tol]
-    both = np.where(hit_a & hit_o)[0]
-    ta, to = t_a[both], rec["t"][both]
-    not_bits = both[ta.view(np.int64) != to.view(np.int64)]
-    t_bad = both[np.abs(ta - to) > tol_t]
-    libm_rounding = []
-    if libm is not None and (len(hard) or len(not_bits)):
-        disputed = np.unique(np.concatenate([hard, not_bits])).astype(np.int64)
-        lh, lt = libm(disputed)
-        explained = set()
-        for j, i in enumerate(disputed):
-            if bool(lh[j]) == bool(hit_o[i]) and (not hit_o[i] or np.float64(lt[j]).view(np.int64) ==
-                                                   np.float64(rec["t"][i]).view(np.int64)):
-                explained.add(int(i))
-                libm_rounding.append((int(i), float(t_a[i]) if hit_a[i] else None,
-                                      float(rec["t"][i]) if hit_o[i] else None))
-        hard = np.array([i for i in hard if int(i) not in explained], dtype=np.int64)
-        t_bad = np.array([i for i in t_bad if int(i) not in explained], dtype=np.int64)
-    return {"name": name, "n": len(hit_a), "hits": int(hit_o.sum()), "borderline": border.tolist()[:20],
-            "hard": hard.tolist()[:20], "t_bad": t_bad.tolist()[:20], "n_not_bit_identical": len(not_bits),
-            "not_bit_identical": [(int(i), float(t_a[i]), float(rec["t"][i])) for i in not_bits[:10]],
-            "libm_rounding": libm_rounding[:20], "n_libm_rounding": len(libm_rounding),
-            "max_tdiff": float(np.abs(ta - to).max()) if len(both) else 0.0}
-
-
-def libm_core(H, qs, qe, keys, is_ee, eta):
-    """idx -> (hit, t) of the C-library-log/exp variant of the host-built core, reference-root mode."""
-    def run(idx):
-        H.core_set_ref_roots(1)
-        h, t, _ = core_narrowphase(H, qs, qe, np.asarray(keys)[idx], is_ee, eta)
-        return h, t
-
-    return run
-
-
 def aberth_arbiter(hist, keys, eta, is_ee):
     """idx -> oracle records for keys[idx] computed with the oracle's long-double Aberth roots (accurate
     to ~1e-16 on these polynomials, and fast): a first arbiter, backed by mpmath_arbiter."""

@@ -1,7 +1,8 @@
 // Test-only host build of the product's FP64 core (collisiondetection_amd/csrc/ccd_core.h).  It lets
 // the CPU test suite compare the core's algorithm with the oracle without a GPU, and the GPU tests
 // separate "the algorithm differs from the reference" from "the device arithmetic differs from the
-// host".  Built by tests/native/build.sh into tests/_build/; never shipped.
+// host".  Built by tests/native/build.sh into tests/_build/ (OpenMP over stencils, so that whole
+// candidate sets of the large scenes can be checked); never shipped.
 #include <cstdint>
 
 #include "../../collisiondetection_amd/csrc/ccd_core.h"
@@ -10,9 +11,10 @@ using ccd::V3;
 
 static V3 ld(const double *p) { return ccd::mk(p[0], p[1], p[2]); }
 
-// 0 = the core's own root isolation, 1 = the reference-root mode (ccd_rpoly.h).
-static int g_ref_roots = 0;
-extern "C" void core_set_ref_roots(int on) { g_ref_roots = on; }
+// Separating-plane leaves: 1 = pruned cascades (ccd::NoCount), 0 = the full cascade (ccd::FullCascade, what
+// the device kernels run).  Both give the same statuses by construction.
+static int g_sp_prune = 1;
+extern "C" void core_set_sp_pruning(int on) { g_sp_prune = on; }
 
 template <class C>
 static bool primitive(int kind, const double *q, double eta, double &tt, C c)
@@ -28,7 +30,7 @@ static bool primitive(int kind, const double *q, double eta, double &tt, C c)
 extern "C" int core_primitive(int kind, const double *q, double eta, double *t)
 {
     double tt = 0.0;
-    bool r = g_ref_roots ? primitive(kind, q, eta, tt, ccd::RefRoots()) : primitive(kind, q, eta, tt, ccd::NoCount());
+    bool r = primitive(kind, q, eta, tt, ccd::NoCount());
     *t = r ? tt : 0.0;
     return r ? 1 : 0;
 }
@@ -36,6 +38,7 @@ extern "C" int core_primitive(int kind, const double *q, double eta, double *t)
 extern "C" void core_narrowphase(const double *qs, const double *qe, const int32_t *keys, int64_t n, int is_ee,
                                  double eta, int32_t *hit, double *t, int32_t *sub)
 {
+#pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < n; i++) {
         V3 s[4], e[4];
         for (int j = 0; j < 4; j++) {
@@ -45,11 +48,7 @@ extern "C" void core_narrowphase(const double *qs, const double *qe, const int32
             s[j] = ccd::mk(ccd::stitch0(a[0], b[0]), ccd::stitch0(a[1], b[1]), ccd::stitch0(a[2], b[2]));
         }
         double tt = 0.0;
-        int r;
-        if (g_ref_roots)
-            r = is_ee ? ccd::check_ee_window(s, e, eta, tt, ccd::RefRoots()) : ccd::check_vf_window(s, e, eta, tt, ccd::RefRoots());
-        else
-            r = is_ee ? ccd::check_ee_window(s, e, eta, tt) : ccd::check_vf_window(s, e, eta, tt);
+        const int r = is_ee ? ccd::check_ee_window(s, e, eta, tt) : ccd::check_vf_window(s, e, eta, tt);
         hit[i] = r >= 0;
         t[i] = r >= 0 ? tt : 0.0;
         sub[i] = r;
@@ -63,19 +62,11 @@ extern "C" int core_find_intervals(const double *op, int n, int pos, double *out
     for (int i = 0; i <= n; i++) c[i] = op[i];
     ccd::Ivls<8> L;
     L.n = 0;
-    if (g_ref_roots) {
-        if (n == 2) ccd::find_intervals<2>(c, L, pos != 0, ccd::RefRoots());
-        else if (n == 3) ccd::find_intervals<3>(c, L, pos != 0, ccd::RefRoots());
-        else if (n == 4) ccd::find_intervals<4>(c, L, pos != 0, ccd::RefRoots());
-        else if (n == 6) ccd::find_intervals<6>(c, L, pos != 0, ccd::RefRoots());
-        else return -1;
-    } else {
-        if (n == 2) ccd::find_intervals<2>(c, L, pos != 0);
-        else if (n == 3) ccd::find_intervals<3>(c, L, pos != 0);
-        else if (n == 4) ccd::find_intervals<4>(c, L, pos != 0);
-        else if (n == 6) ccd::find_intervals<6>(c, L, pos != 0);
-        else return -1;
-    }
+    if (n == 2) ccd::find_intervals<2>(c, L, pos != 0);
+    else if (n == 3) ccd::find_intervals<3>(c, L, pos != 0);
+    else if (n == 4) ccd::find_intervals<4>(c, L, pos != 0);
+    else if (n == 6) ccd::find_intervals<6>(c, L, pos != 0);
+    else return -1;
     for (int i = 0; i < L.n; i++) {
         out[2 * i] = L.l[i];
         out[2 * i + 1] = L.u[i];
@@ -112,42 +103,6 @@ extern "C" int core_distance(int kind, const double *q, double eta, double *vec,
     return 1;
 }
 
-// The reference-root mode's solver on its own: op[0..degree] descending -> count, zeror, zeroi.
-extern "C" int core_rpoly(const double *op, int degree, double *zr, double *zi)
-{
-    double opc[7] = {0, 0, 0, 0, 0, 0, 0}, r[6] = {0, 0, 0, 0, 0, 0}, im[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = 0; i <= degree && i < 7; i++) opc[i] = op[i];
-    ccd::jt::Solver js;
-    int k = js.rpoly(opc, degree, r, im);
-    for (int i = 0; i < 6; i++) {
-        zr[i] = r[i];
-        zi[i] = im[i];
-    }
-    return k;
-}
-
-extern "C" double core_log_cr(double x) { return ccd::jt::log_cr(x); }
-extern "C" double core_exp_cr(double x) { return ccd::jt::exp_cr(x); }
-
-extern "C" double core_log_cr_full(double x) { return ccd::jt::log_cr_full(x); }
-extern "C" double core_exp_cr_full(double x) { return ccd::jt::exp_cr_full(x); }
-// The fast paths before their rounding test: out[0..1] = the double-double value; exp also gives k.
-extern "C" void core_log_fast_dd(double x, double *out)
-{
-    ccd::jt::DD v = ccd::jt::log_fast_dd(x);
-    out[0] = v.hi;
-    out[1] = v.lo;
-}
-extern "C" int core_exp_fast_dd(double y, double *out)
-{
-    int k = 0;
-    ccd::jt::DD v = ccd::jt::exp_fast_dd(y, k);
-    out[0] = v.hi;
-    out[1] = v.lo;
-    return k;
-}
-extern "C" double core_fast_eps() { return ccd::jt::FAST_EPS; }
-
 // The separating-plane narrowphase of the core over a CSR history; eps = the minimum entry gap / 4
 // (History::computeMinimumGap).  hit[i] = 1, 0, or 2 (undecided: the reference's recursion would not end).
 extern "C" void core_separating_plane(const int64_t *off, const double *ht, const double *hp, int64_t nverts,
@@ -158,13 +113,14 @@ extern "C" void core_separating_plane(const int64_t *off, const double *ht, cons
         for (int64_t j = off[v] + 1; j < off[v + 1]; j++) gap = ccd::smin(ht[j] - ht[j - 1], gap);
     const double eps = gap / 4.0;
     ccd::HistCSR h{off, ht, hp};
+#pragma omp parallel for schedule(dynamic, 64)
     for (int64_t i = 0; i < n; i++) {
         int v[4] = {keys[4 * i], keys[4 * i + 1], keys[4 * i + 2], keys[4 * i + 3]};
-        if (g_ref_roots)
-            hit[i] = is_ee ? ccd::separating_plane_check<false>(h, v, eta[i], eps, ccd::RefRoots())
-                           : ccd::separating_plane_check<true>(h, v, eta[i], eps, ccd::RefRoots());
-        else
+        if (g_sp_prune)
             hit[i] = is_ee ? ccd::separating_plane_check<false>(h, v, eta[i], eps)
                            : ccd::separating_plane_check<true>(h, v, eta[i], eps);
+        else
+            hit[i] = is_ee ? ccd::separating_plane_check<false>(h, v, eta[i], eps, ccd::FullCascade())
+                           : ccd::separating_plane_check<true>(h, v, eta[i], eps, ccd::FullCascade());
     }
 }

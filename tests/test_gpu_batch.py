@@ -109,3 +109,34 @@ def test_autotune_keeps_results():
     d1 = api.detect(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"])
     assert _as_set(d0["vf"], d0["vf_toi"]) == _as_set(d1["vf"], d1["vf_toi"])
     assert _as_set(d0["ee"], d0["ee_toi"]) == _as_set(d1["ee"], d1["ee_toi"])
+
+
+def test_batch_graph_after_buffer_regrow():
+    """A captured pipeline holds raw buffer addresses.  Detections of other meshes on the same context
+    that grow buffers sized by vertex or face count (without changing the stencil capacities) must make
+    the next graph call recapture, not replay into freed memory: its result equals a synchronous run."""
+    from collisiondetection_amd import api
+
+    print("case: graph after buffer regrowth", flush=True)
+    inst = _instances(count=8, seed=17)
+    b = api.Batch(inst)
+    r0 = b.detect(1e-6, 1e-6, graph=False)
+    b.detect(1e-6, 1e-6, graph=True)   # synchronous run + capture
+    # many vertices, few faces: vertex-sized buffers grow, pair and stencil counts stay small
+    rng = np.random.default_rng(5)
+    V = 8 * b.V
+    q = rng.uniform(-50, 50, size=(V, 3))
+    faces = np.arange(30, dtype=np.int32).reshape(10, 3)
+    api.detect(faces, q, q, 1e-6, 1e-6)
+    rg = b.detect(1e-6, 1e-6, graph=True)
+    assert _as_set(rg["vf"], rg["vf_toi"]) == _as_set(r0["vf"], r0["vf_toi"])
+    assert _as_set(rg["ee"], rg["ee_toi"]) == _as_set(r0["ee"], r0["ee_toi"])
+    # more faces, spread out so that they pair with nothing: face-sized buffers grow
+    b.detect(1e-6, 1e-6, graph=True)
+    g = scenes.cloth(int(np.sqrt(b.F)) + 8)
+    flat = np.array(g["qstart"], dtype=np.float64)
+    flat[:, 2] = 0.0
+    api.detect(g["faces"], flat, flat, 1e-9, 1e-9)
+    rg = b.detect(1e-6, 1e-6, graph=True)
+    assert _as_set(rg["vf"], rg["vf_toi"]) == _as_set(r0["vf"], r0["vf_toi"])
+    assert _as_set(rg["ee"], rg["ee_toi"]) == _as_set(r0["ee"], r0["ee_toi"])

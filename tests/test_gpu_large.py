@@ -7,14 +7,15 @@ through properties that do not depend on size:
     full-mesh candidate iff it is a candidate of the submesh W; the oracle enumerates the submesh;
   * no duplicate stencils, every key a valid vertex index;
   * narrowphase: a random sample of the colliding stencils and a random sample of all candidates
-    against the oracle (reference root finder), flags and times of impact.
+    against the oracle (the reference's root finder where built, with exact-root arbitration of every
+    disagreement), flags and times of impact.
 """
 import numpy as np
 import pytest
 
 import oracle
 from collisiondetection_amd import scenes
-from tests.helpers import compare_reference, libm_core
+from tests.helpers import compare_results, exact_arbiters
 
 pytestmark = pytest.mark.gpu
 
@@ -92,7 +93,7 @@ def test_gpu_sheet_window_completeness(sheet):
     torch.cuda.empty_cache()
 
 
-def test_gpu_sheet_detect_sample(sheet, core_harness_libm):
+def test_gpu_sheet_detect_sample(sheet):
     from collisiondetection_amd import api
 
     s = sheet
@@ -101,42 +102,28 @@ def test_gpu_sheet_detect_sample(sheet, core_harness_libm):
     print("candidates", d["vf_candidates"], d["ee_candidates"], "hits", len(d["vf"]), len(d["ee"]), flush=True)
     h = oracle.Hist.two_entry(s["qstart"], s["qend"])
     rng = np.random.default_rng(3)
-    if not oracle.ref_rpoly_available():
-        pytest.skip("the reference's root finder (oracle/_ref) is not built")
-    oracle.use_rootfinder("rpoly")
-    try:
-        for pre, is_ee in (("vf", False), ("ee", True)):
-            keys, toi = d[pre], d[pre + "_toi"]
-            assert len(keys) > 0
-            idx = rng.choice(len(keys), size=min(20000, len(keys)), replace=False)
-            rec = oracle.narrowphase(h, keys[idx], s["eta"], is_ee)
-            rep = compare_reference("sheet hits " + pre, np.ones(len(idx), bool), toi[idx], rec,
-                                    libm=libm_core(core_harness_libm, s["qstart"], s["qend"], keys[idx], is_ee,
-                                                   s["eta"]))
-            print(rep, flush=True)
-            assert not rep["hard"] and not rep["t_bad"], rep
-    finally:
-        oracle.use_default_rootfinder()
+    for pre, is_ee in (("vf", False), ("ee", True)):
+        keys, toi = d[pre], d[pre + "_toi"]
+        assert len(keys) > 0
+        idx = rng.choice(len(keys), size=min(20000, len(keys)), replace=False)
+        rec = oracle.narrowphase(h, keys[idx], s["eta"], is_ee)
+        rep = compare_results("sheet hits " + pre, np.ones(len(idx), bool), toi[idx], rec,
+                              arbiter=exact_arbiters(h, keys[idx], s["eta"], is_ee))
+        print(rep, flush=True)
+        assert not rep["hard"] and not rep["t_bad_hard"], rep
     # candidates that the oracle finds colliding must be among the GPU's collisions
     gvf, gee = api.kdop_candidates(s["faces"], s["qstart"], s["qend"], s["eta"], sort=False, as_numpy=False)
-    oracle.use_rootfinder("rpoly")
-    try:
-        for g, pre, is_ee in ((gvf, "vf", False), (gee, "ee", True)):
-            import torch
+    for g, pre, is_ee in ((gvf, "vf", False), (gee, "ee", True)):
+        import torch
 
-            sel = torch.from_numpy(rng.choice(g.shape[0], size=60000, replace=False)).to(g.device)
-            keys = g[sel].cpu().numpy()
-            rec = oracle.narrowphase(h, keys, s["eta"], is_ee)
-            hits = np.sort(_void(d[pre]))
-            q = _void(keys)
-            pos = np.minimum(np.searchsorted(hits, q), len(hits) - 1)
-            ghit = hits[pos] == q
-            oh = rec["hit"].astype(bool)
-            bad = np.where((ghit != oh) & (rec["margin"] >= 1e-12))[0]
-            if len(bad):   # a flag that differs only through the C library's log/exp rounding is listed
-                lh, _ = libm_core(core_harness_libm, s["qstart"], s["qend"], keys, is_ee, s["eta"])(bad)
-                bad = bad[lh.astype(bool) != oh[bad]]
-            print(f"sheet sample {pre}: {int(oh.sum())} oracle hits of {len(keys)}, {len(bad)} disagreements", flush=True)
-            assert len(bad) == 0, keys[bad[:5]]
-    finally:
-        oracle.use_default_rootfinder()
+        sel = torch.from_numpy(rng.choice(g.shape[0], size=60000, replace=False)).to(g.device)
+        keys = g[sel].cpu().numpy()
+        rec = oracle.narrowphase(h, keys, s["eta"], is_ee)
+        hits = np.sort(_void(d[pre]))
+        q = _void(keys)
+        pos = np.minimum(np.searchsorted(hits, q), len(hits) - 1)
+        ghit = hits[pos] == q
+        rep = compare_results("sheet sample " + pre, ghit, np.zeros(len(keys)), rec, tol_t=np.inf,
+                              arbiter=exact_arbiters(h, keys, s["eta"], is_ee))
+        print(f"sheet sample {pre}: {int(rec['hit'].sum())} oracle hits of {len(keys)}", rep, flush=True)
+        assert not rep["hard"], keys[rep["hard"][:5]]

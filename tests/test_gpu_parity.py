@@ -1,13 +1,11 @@
 """GPU parity: libccd_amd.so (HIP, gfx950) against the CPU oracle and the host-built core.
 
 Bars: candidate sets bit-exact; the GPU core bit-identical to the same core compiled for the host
-(device FP64 arithmetic: no contraction, IEEE div/sqrt), in both root modes.  Against the oracle:
-  * reference-root mode (the default): the oracle runs the reference's own root finder; flags equal
-    (a disagreement only with an oracle margin below 1e-12, listed), times of impact within 1e-9, and
-    the count of times of impact that are not bit-identical reported;
-  * isolating mode: flags equal except margin < 1e-12 (listed); times of impact within 1e-9 of the
-    reference or, where they are not, within 1e-9 of the exact-root answer that the reference's root
-    finder misses (listed as ref_root_error).
+(device FP64 arithmetic: no contraction, IEEE div/sqrt).  Against the oracle, which runs the
+reference's own root finder where it is built (oracle/_ref) and its exact Aberth roots otherwise:
+flags equal except margin < 1e-12 (listed); times of impact within 1e-9 of the oracle or, where the
+reference's root finder misses the exact root, within 1e-9 of the exact-root answer (listed as
+ref_root_error, re-decided with the oracle's Aberth roots and then 60-digit mpmath).
 """
 import sys
 import time
@@ -17,8 +15,7 @@ import pytest
 
 import oracle
 from collisiondetection_amd import scenes
-from tests.helpers import (compare_reference, compare_results, core_narrowphase, core_primitive, exact_arbiters,
-                           libm_core)
+from tests.helpers import compare_results, core_narrowphase, core_primitive, exact_arbiters
 from tests.test_oracle import ETA, TESTCTCD, crossing_cases, expected_testctcd
 
 pytestmark = pytest.mark.gpu
@@ -39,33 +36,10 @@ def say(*a):
     sys.stdout.flush()
 
 
-@pytest.fixture(params=["reference", "isolate"])
-def mode(request, api, core_harness):
-    """Runs a test in each root mode, on the GPU's default context and the host-built core alike."""
-    from collisiondetection_amd import _native as N
-
-    if request.param == "reference" and not oracle.ref_rpoly_available():
-        pytest.skip("the reference's root finder (oracle/_ref) is not built")
-    m = N.ROOTS_REFERENCE if request.param == "reference" else N.ROOTS_ISOLATE
-    ctx = api.default_context()
-    old = ctx.root_mode
-    ctx.set_root_mode(m)
-    core_harness.core_set_ref_roots(1 if request.param == "reference" else 0)
-    oracle.use_default_rootfinder()
-    yield request.param
-    ctx.set_root_mode(old)
-    core_harness.core_set_ref_roots(0)
-
-
-def check_vs_oracle(mode, name, hit, t, rec, hist, keys, eta, is_ee, libm=None):
-    if mode == "reference":
-        rep = compare_reference(name, hit, t, rec, libm=libm)
-        say(rep)
-        assert not rep["hard"] and not rep["t_bad"], rep
-    else:
-        rep = compare_results(name, hit, t, rec, arbiter=exact_arbiters(hist, keys, eta, is_ee))
-        say(rep)
-        assert not rep["hard"] and not rep["t_bad_hard"], rep
+def check_vs_oracle(name, hit, t, rec, hist, keys, eta, is_ee):
+    rep = compare_results(name, hit, t, rec, arbiter=exact_arbiters(hist, keys, eta, is_ee))
+    say(rep)
+    assert not rep["hard"] and not rep["t_bad_hard"], rep
     return rep
 
 
@@ -100,8 +74,8 @@ def _random_primitives(kind, n, rng):
 
 
 @pytest.mark.parametrize("kind", ["vf", "ee", "ve", "vv"])
-def test_gpu_primitives_bit_identical_to_host_core(api, core_harness, mode, kind):
-    say("case: primitives bit-identical", kind, mode)
+def test_gpu_primitives_bit_identical_to_host_core(api, core_harness, kind):
+    say("case: primitives bit-identical", kind)
     rng = np.random.default_rng(100 + KIND[kind])
     q = _random_primitives(kind, 4000, rng)
     eta = 0.05
@@ -117,8 +91,8 @@ def test_gpu_primitives_bit_identical_to_host_core(api, core_harness, mode, kind
 
 
 @pytest.mark.parametrize("kind", ["vf", "ee", "ve", "vv"])
-def test_gpu_primitives_vs_oracle(api, mode, kind):
-    say("case: primitives vs oracle", kind, mode)
+def test_gpu_primitives_vs_oracle(api, kind):
+    say("case: primitives vs oracle", kind)
     rng = np.random.default_rng(200 + KIND[kind])
     q = _random_primitives(kind, 3000, rng)
     eta = 0.05
@@ -126,14 +100,6 @@ def test_gpu_primitives_vs_oracle(api, mode, kind):
     fn = {"vf": oracle.vertex_face_ctcd, "ee": oracle.edge_edge_ctcd, "ve": oracle.vertex_edge_ctcd,
           "vv": oracle.vertex_vertex_ctcd}[kind]
     recs = [fn(q[i], eta) for i in range(len(q))]
-    if mode == "reference":
-        rec = {"hit": np.array([r[0] for r in recs], dtype=np.int32),
-               "t": np.array([r[1] if r[0] else 0.0 for r in recs], dtype=np.float64),
-               "margin": np.array([r[2] for r in recs], dtype=np.float64)}
-        rep = compare_reference(kind, hit.astype(bool), t, rec)
-        say(rep)
-        assert not rep["hard"] and not rep["t_bad"], rep
-        return
     bad = []
     for i, (r, tt, m) in enumerate(recs):
         if (bool(hit[i]) != r and m >= 1e-12) or (r and hit[i] and abs(t[i] - tt) > 1e-9 and m >= 1e-12):
@@ -225,16 +191,15 @@ def test_gpu_candidates_small_meshes(api):
 
 
 @pytest.mark.parametrize("scene", list(SCENES))
-def test_gpu_narrowphase_vs_oracle_and_core(api, core_harness, core_harness_libm, mode, scene):
-    say("case: narrowphase", scene, mode)
+def test_gpu_narrowphase_vs_oracle_and_core(api, core_harness, scene):
+    say("case: narrowphase", scene)
     s = SCENES[scene]()
     h = oracle.Hist.two_entry(s["qstart"], s["qend"])
     vf, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
     r = api.narrowphase(s["qstart"], s["qend"], vf, ee, s["eta"])
     for keys, is_ee, pre in ((vf, False, "vf"), (ee, True, "ee")):
         rec = oracle.narrowphase(h, keys, s["eta"], is_ee)
-        check_vs_oracle(mode, scene + pre, r[pre + "_hit"], r[pre + "_t"], rec, h, keys, s["eta"], is_ee,
-                        libm=libm_core(core_harness_libm, s["qstart"], s["qend"], keys, is_ee, s["eta"]))
+        check_vs_oracle(scene + pre, r[pre + "_hit"], r[pre + "_t"], rec, h, keys, s["eta"], is_ee)
         ch, ct, cs = core_narrowphase(core_harness, s["qstart"], s["qend"], keys, is_ee, s["eta"])
         assert np.array_equal(ch, r[pre + "_hit"])
         assert np.array_equal(ct.view(np.int64), r[pre + "_t"].view(np.int64))
@@ -359,12 +324,16 @@ def test_gpu_work_counts_match_oracle(api):
         assert b[i] == ce[name], name
 
 
-@pytest.mark.parametrize("n", [256])
-def test_gpu_cloth_full_size(api, core_harness_libm, mode, n):
-    """The 256^2 cloth: candidate sets equal the oracle's; flags and times of impact on a sample and on
-    every GPU hit."""
-    say("case: cloth", n, mode)
-    s = scenes.cloth(n)
+def test_gpu_cloth_full_size(api, core_harness):
+    """The 256^2 cloth (65,536 vertices, 130,050 triangles, eta 1e-6), every candidate:
+      * the GPU's candidate sets equal the oracle's;
+      * the per-stencil narrowphase equals the host-built core on all 7.67M stencils (flags, firing
+        sub-test and time-of-impact bits), so no lane's result depends on its neighbours;
+      * the production pipeline (ccd_detect) returns exactly the host core's colliding stencils and
+        time-of-impact bits;
+      * the oracle agrees on every GPU hit and on a random sample of 20,000 stencils per kind."""
+    say("case: cloth 256 full size")
+    s = scenes.cloth(256)
     h = oracle.Hist.two_entry(s["qstart"], s["qend"])
     ovf, oee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
     say("oracle candidates", len(ovf), len(oee))
@@ -372,22 +341,31 @@ def test_gpu_cloth_full_size(api, core_harness_libm, mode, n):
     assert np.array_equal(ovf, gvf) and np.array_equal(oee, gee)
     say("GPU candidates equal")
     r = api.narrowphase(s["qstart"], s["qend"], gvf, gee, s["eta"])
+    d = api.detect(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"])
     rng = np.random.default_rng(1)
     for keys, pre, is_ee in ((gvf, "vf", False), (gee, "ee", True)):
         hit = r[pre + "_hit"]
+        ch, ct, cs = core_narrowphase(core_harness, s["qstart"], s["qend"], keys, is_ee, s["eta"])
+        say(f"cloth {pre}: {len(keys)} stencils, host core hits {int(ch.sum())}, GPU hits {int(hit.sum())}")
+        assert np.array_equal(ch, hit)
+        assert np.array_equal(ct[ch].view(np.int64), r[pre + "_t"][ch].view(np.int64))
+        assert np.array_equal(cs[ch], r[pre + "_sub"][ch].astype(np.int32))
+        pipe = {tuple(k): t for k, t in zip(d[pre].tolist(), d[pre + "_toi"])}
+        host = {tuple(k): t for k, t in zip(keys[ch].tolist(), ct[ch])}
+        assert pipe.keys() == host.keys()
+        assert all(np.float64(pipe[k]).view(np.int64) == np.float64(host[k]).view(np.int64) for k in host)
         idx = np.unique(np.concatenate([rng.choice(len(keys), size=min(20000, len(keys)), replace=False),
                                         np.where(hit)[0]]))
-        say(f"cloth {pre}: comparing {len(idx)} stencils ({int(hit.sum())} GPU hits)")
+        say(f"cloth {pre}: oracle on {len(idx)} stencils")
         rec = oracle.narrowphase(h, keys[idx], s["eta"], is_ee)
-        check_vs_oracle(mode, "cloth" + pre, hit[idx], r[pre + "_t"][idx], rec, h, keys[idx], s["eta"], is_ee,
-                        libm=libm_core(core_harness_libm, s["qstart"], s["qend"], keys[idx], is_ee, s["eta"]))
+        check_vs_oracle("cloth" + pre, hit[idx], r[pre + "_t"][idx], rec, h, keys[idx], s["eta"], is_ee)
 
 
 @pytest.mark.parametrize("scene", ["converging", "cloth64", "cloth128"])
-def test_gpu_detect_pipeline_equals_host_core(api, core_harness, mode, scene):
+def test_gpu_detect_pipeline_equals_host_core(api, core_harness, scene):
     """The production pipeline (ccd_detect: broadphase + staged persistent narrowphase) against the
     host-built core's single-pass cascade on every candidate: the same colliding stencils, TOI bits."""
-    say("case: detect pipeline vs host core", scene, mode)
+    say("case: detect pipeline vs host core", scene)
     s = SCENES[scene]() if scene in SCENES else scenes.cloth(128)
     h = oracle.Hist.two_entry(s["qstart"], s["qend"])
     vf, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree")

@@ -1,6 +1,7 @@
 """GPU separating-plane narrowphase (SeparatingPlaneNarrowPhase.cpp:9-278) against the oracle's recursive
-restatement and the host-built core: statuses (1 collision, 0 none, 2 undecided) equal the oracle's in
-the reference-root mode and the host core's bit for bit in both modes."""
+restatement and the host-built core: statuses (1 collision, 0 none, 2 undecided) equal the host core's
+on every stencil, with pruned and with full leaf cascades, and the oracle's except where an exact-root
+oracle decides a borderline case the device's way."""
 import sys
 
 import numpy as np
@@ -45,40 +46,37 @@ def _case(kind):
 
 
 @pytest.mark.parametrize("kind", ["cloth48_two_entry", "spheres_three_entry", "sheets_long_history"])
-@pytest.mark.parametrize("mode", ["reference", "isolate"])
-def test_gpu_separating_plane_vs_oracle_and_core(api, core_harness, kind, mode):
-    from collisiondetection_amd import _native as N
-
-    if mode == "reference" and not oracle.ref_rpoly_available():
-        pytest.skip("the reference's root finder (oracle/_ref) is not built")
-    say("case: separating plane", kind, mode)
+def test_gpu_separating_plane_vs_oracle_and_core(api, core_harness, kind):
+    say("case: separating plane", kind)
     s, h = _case(kind)
     vf, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
-    ctx = api.default_context()
-    old = ctx.root_mode
-    ctx.set_root_mode(N.ROOTS_REFERENCE if mode == "reference" else N.ROOTS_ISOLATE)
-    core_harness.core_set_ref_roots(1 if mode == "reference" else 0)
-    oracle.use_rootfinder("rpoly" if mode == "reference" else "aberth")
-    try:
-        r = api.separating_plane_narrowphase(h.off, h.ht, h.hp, vf, ee, s["eta"])
-        und = 0
-        for keys, is_ee, got in ((vf, False, r["vf_status"]), (ee, True, r["ee_status"])):
+    r = api.separating_plane_narrowphase(h.off, h.ht, h.hp, vf, ee, s["eta"])
+    und = 0
+    for keys, is_ee, got in ((vf, False, r["vf_status"]), (ee, True, r["ee_status"])):
+        core = core_separating_plane(core_harness, h, keys, is_ee, s["eta"])
+        core_harness.core_set_sp_pruning(0)
+        try:
+            core_full = core_separating_plane(core_harness, h, keys, is_ee, s["eta"])
+        finally:
+            core_harness.core_set_sp_pruning(1)
+        assert np.array_equal(got, core)        # device arithmetic = host arithmetic
+        assert np.array_equal(core, core_full)  # pruning changes no status
+        oracle.use_rootfinder("aberth")
+        try:
             st, margin, depth = oracle.separating_plane(h, keys, s["eta"], is_ee)
-            core = core_separating_plane(core_harness, h, keys, is_ee, s["eta"])
-            say(kind, mode, "ee" if is_ee else "vf", "gpu", np.bincount(got, minlength=3), "oracle",
-                np.bincount(st, minlength=3), "max depth", depth.max())
-            assert np.array_equal(got, core)   # device arithmetic = host arithmetic
-            diff = np.where(got != st)[0]
-            if mode == "reference":
-                assert len(diff) == 0, diff[:5]
-            else:   # isolating roots vs exact roots: only borderline decisions may differ
-                assert all(margin[i] < 1e-12 for i in diff), diff[:5]
-            und += int((got == 2).sum())
-        assert r["undecided"] == und
-    finally:
-        ctx.set_root_mode(old)
-        core_harness.core_set_ref_roots(0)
-        oracle.use_default_rootfinder()
+        finally:
+            oracle.use_default_rootfinder()
+        say(kind, "ee" if is_ee else "vf", "gpu", np.bincount(got, minlength=3), "oracle",
+            np.bincount(st, minlength=3), "max depth", depth.max())
+        diff = np.where(got != st)[0]
+        assert all(margin[i] < 1e-12 for i in diff), diff[:5]   # exact roots: only borderline decisions
+        if oracle.ref_rpoly_available():
+            st_r, margin_r, _ = oracle.separating_plane(h, keys, s["eta"], is_ee)
+            diff_r = np.where(got != st_r)[0]
+            say("vs the reference's root finder:", len(diff_r), "differences, each decided the device's way by exact roots")
+            assert all(margin_r[i] < 1e-12 or st[i] == got[i] for i in diff_r), diff_r[:5]
+        und += int((got == 2).sum())
+    assert r["undecided"] == und
 
 
 def test_gpu_separating_plane_class_mirror(api):
@@ -121,28 +119,15 @@ def test_gpu_separating_plane_empty_and_invalid(api):
 
 
 def test_gpu_separating_plane_full_cloth_vs_host_core(api, core_harness):
-    """All 7.7M candidate stencils of the 256x256 cloth in one launch per kind (reference-root mode),
-    checked against the host build of the same code on a 200,000-stencil random sample: each lane's
-    result must not depend on what the other lanes of its wavefront do."""
-    from collisiondetection_amd import _native as N
-
-    say("case: separating plane, full 256^2 cloth vs host core sample")
+    """All 7.7M candidate stencils of the 256x256 cloth in one launch per kind, checked against the host
+    build of the same code on every stencil: each lane's result must not depend on what the other lanes
+    of its wavefront do."""
+    say("case: separating plane, full 256^2 cloth vs host core")
     s = scenes.cloth(256)
     h = oracle.Hist.two_entry(s["qstart"], s["qend"])
     vf, ee = api.kdop_candidates(s["faces"], s["qstart"], s["qend"], s["eta"])
-    ctx = api.default_context()
-    old = ctx.root_mode
-    ctx.set_root_mode(N.ROOTS_REFERENCE)
-    core_harness.core_set_ref_roots(1)
-    try:
-        r = api.separating_plane_narrowphase(h.off, h.ht, h.hp, vf, ee, s["eta"])
-        rng = np.random.default_rng(4)
-        for keys, got, is_ee in ((vf, r["vf_status"], False), (ee, r["ee_status"], True)):
-            idx = np.sort(rng.choice(len(keys), size=min(len(keys), 100000), replace=False))
-            host = core_separating_plane(core_harness, h, keys[idx], is_ee, s["eta"])
-            say("ee" if is_ee else "vf", "sample statuses", np.bincount(host, minlength=3), "mismatches",
-                int((got[idx] != host).sum()))
-            assert np.array_equal(got[idx], host)
-    finally:
-        ctx.set_root_mode(old)
-        core_harness.core_set_ref_roots(0)
+    r = api.separating_plane_narrowphase(h.off, h.ht, h.hp, vf, ee, s["eta"])
+    for keys, got, is_ee in ((vf, r["vf_status"], False), (ee, r["ee_status"], True)):
+        host = core_separating_plane(core_harness, h, keys, is_ee, s["eta"])
+        say("ee" if is_ee else "vf", "statuses", np.bincount(host, minlength=3), "mismatches", int((got != host).sum()))
+        assert np.array_equal(got, host)
