@@ -9,3 +9,4 @@ g++ -O2 -std=c++17 -Wall -I"$ROOT/include/ccdamd" -o ../_build/cpp_interface_dri
 g++ -O2 -std=c++17 -Wall -I"$ROOT/include/ccdamd" -o ../_build/velocity_filter_driver velocity_filter_driver.cpp \
     -L"$ROOT/collisiondetection_amd" -lccd_amd -Wl,-rpath,"$ROOT/collisiondetection_amd"
 g++ -O2 -std=c++17 -Wall -I"$ROOT/include/ccdamd" -o ../_build/objio_check objio_check.cpp
+g++ -O2 -std=c++17 -Wall -ffp-contract=off -I"$ROOT/include/ccdamd" -o ../_build/history_check history_check.cpp
