@@ -53,6 +53,7 @@ def lib():
             build()
         L = ctypes.CDLL(_LIB_PATH)
         L.orc_set_dot_order.argtypes = [ctypes.c_int]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_set_rootfinder.argtypes = [ctypes.c_void_p]
         L.orc_aberth_roots.argtypes = [_dp, ctypes.c_int, _dp, _dp]
         L.orc_aberth_roots.restype = ctypes.c_int
@@ -319,7 +320,13 @@ def leaf_dops(faces, hist: Hist, eta, aabb=False):
     return mn, mx
 
 
-METHODS = {"brute": 0, "tree_set": 1, "tree": 2}
+METHODS = {"brute": 0, "tree_set": 1, "tree": 2, "tree_parallel": 3}
+
+
+def set_threads(n: int) -> None:
+    """Worker threads for the all-core CPU baseline (the tree_parallel traversal and the narrowphase);
+    1 (the default) is the reference's single thread."""
+    lib().orc_set_threads(int(n))
 
 
 def candidates(faces, hist: Hist, eta, fixed=None, method="brute", aabb=False, want_pairs=False):

@@ -26,6 +26,9 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <cstdint>
 #include <cstring>
 #include <limits>
@@ -1167,8 +1170,54 @@ void dual_traverse(const Tree &T, int a, int b, const int32_t *faces, const uint
     }
 }
 
+// Worker threads of the all-core CPU baseline (orc_set_threads); 1 = the reference's single thread.
+static int g_threads = 1;
+
+// The dual traversal split into independent subtree pairs (the recursion's own branches, expanded
+// breadth-first until there are plenty per thread), each traversed by one thread into its own vector,
+// then merged and made unique: the same set as the single-threaded traversal.
+void dual_traverse_parallel(const Tree &T, int root, const int32_t *faces, const uint8_t *fixed, CandSets &out)
+{
+    std::vector<std::pair<int, int>> tasks{{root, root}}, leaves;
+    const size_t want = (size_t)64 * (size_t)g_threads;
+    while (!tasks.empty() && tasks.size() + leaves.size() < want) {
+        std::vector<std::pair<int, int>> next;
+        for (auto [a, b] : tasks) {
+            const Node &A = T.nodes[a], &B = T.nodes[b];
+            if (!dop_overlap(A.d, B.d, T.na)) continue;
+            if (A.face < 0) {
+                next.push_back({A.left, b});
+                next.push_back({A.right, b});
+            } else if (B.face < 0) {
+                next.push_back({a, B.left});
+                next.push_back({a, B.right});
+            } else {
+                leaves.push_back({a, b});
+            }
+        }
+        tasks.swap(next);
+    }
+    tasks.insert(tasks.end(), leaves.begin(), leaves.end());
+    std::vector<CandSets> part(g_threads);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(g_threads)
+    for (int64_t i = 0; i < (int64_t)tasks.size(); i++) {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        VecSink sink{&part[tid]};
+        dual_traverse(T, tasks[i].first, tasks[i].second, faces, fixed, sink);
+    }
+    for (auto &p : part) {
+        out.vf.insert(out.vf.end(), p.vf.begin(), p.vf.end());
+        out.ee.insert(out.ee.end(), p.ee.begin(), p.ee.end());
+    }
+    sort_unique(out.vf);
+    sort_unique(out.ee);
+}
+
 void candidates_tree(const int32_t *faces, int64_t F, const std::vector<Dop> &dops, int na,
-                     const uint8_t *fixed, CandSets &out, bool use_std_set)
+                     const uint8_t *fixed, CandSets &out, bool use_std_set, bool parallel = false)
 {
     if (F == 0) return;
     Tree T;
@@ -1184,7 +1233,9 @@ void candidates_tree(const int32_t *faces, int64_t F, const std::vector<Dop> &do
         items[f] = (int)f;
     }
     int root = T.build(items);
-    if (use_std_set) {
+    if (parallel) {
+        dual_traverse_parallel(T, root, faces, fixed, out);
+    } else if (use_std_set) {
         std::set<Key> vfs, ees;
         SetSink sink{&vfs, &ees};
         dual_traverse(T, root, root, faces, fixed, sink);
@@ -1906,6 +1957,7 @@ int velocity_filter(const double *qstart, double *qend, int64_t V, const int32_t
 extern "C" {
 
 void orc_set_dot_order(int order) { g_dot_order = order; }
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
 void orc_set_rootfinder(void *fn) { g_rootfinder = (rootfinder_fn)fn; }
 
 int orc_aberth_roots(const double *op, int degree, double *zr, double *zi)
@@ -2042,7 +2094,7 @@ void *orc_candidates(const int32_t *faces, int64_t F, const int64_t *off, const 
     if (method == 0)
         candidates_bruteforce(faces, F, d, na, fixed, r->cs, want_pairs ? &r->pairs : nullptr);
     else
-        candidates_tree(faces, F, d, na, fixed, r->cs, method == 1);
+        candidates_tree(faces, F, d, na, fixed, r->cs, method == 1, method == 3);
     return r;
 }
 int64_t orc_result_count(void *r, int which)
@@ -2071,6 +2123,8 @@ void orc_narrowphase(const int64_t *off, const double *ht, const double *hp, con
 {
     Hist h{off, ht, hp};
     StencilResult *R = (StencilResult *)out_records;
+    int64_t total[ST_COUNT] = {};
+#pragma omp parallel for schedule(dynamic, 64) num_threads(g_threads) reduction(+ : total[:ST_COUNT])
     for (int64_t i = 0; i < n; i++) {
         Ctx ctx;
         StencilResult res;
@@ -2088,9 +2142,10 @@ void orc_narrowphase(const int64_t *off, const double *ht, const double *hp, con
         }
         res.margin = ctx.margin;
         R[i] = res;
-        if (counts)
-            for (int k = 0; k < ST_COUNT; k++) counts[k] += ctx.counts[k];
+        for (int k = 0; k < ST_COUNT; k++) total[k] += ctx.counts[k];
     }
+    if (counts)
+        for (int k = 0; k < ST_COUNT; k++) counts[k] += total[k];
 }
 
 // SeparatingPlaneNarrowPhase::findCollisions (SeparatingPlaneNarrowPhase.cpp:9-24) per stencil: hit
