@@ -140,3 +140,50 @@ def test_batch_graph_after_buffer_regrow():
     rg = b.detect(1e-6, 1e-6, graph=True)
     assert _as_set(rg["vf"], rg["vf_toi"]) == _as_set(r0["vf"], r0["vf_toi"])
     assert _as_set(rg["ee"], rg["ee_toi"]) == _as_set(r0["ee"], r0["ee_toi"])
+
+
+def test_batch_full_workload():
+    """The 512-instance batched workload itself (seed 2; 80-20,480 triangles per mesh, 4.0M triangles):
+    the batched pipeline, its captured-graph replay and the per-instance (plain) detections return the
+    same colliding stencils with the same time-of-impact bits; a sample of instances equals the oracle."""
+    from collisiondetection_amd import api
+
+    print("case: batched workload, 512 instances", flush=True)
+    inst = scenes.batched_instances()
+    b = api.Batch(inst)
+    print("instances", b.ninst, "triangles", b.F, flush=True)
+    r0 = b.detect(1e-6, 1e-6, graph=False)
+    b.detect(1e-6, 1e-6, graph=True)          # synchronous run + capture
+    rg = b.detect(1e-6, 1e-6, graph=True)     # replay
+    ref_vf, ref_ee = _as_set(r0["vf"], r0["vf_toi"]), _as_set(r0["ee"], r0["ee_toi"])
+    assert _as_set(rg["vf"], rg["vf_toi"]) == ref_vf and _as_set(rg["ee"], rg["ee_toi"]) == ref_ee
+    assert (rg["vf_candidates"], rg["ee_candidates"]) == (r0["vf_candidates"], r0["ee_candidates"])
+    print("candidates", r0["vf_candidates"], r0["ee_candidates"], "collisions", len(ref_vf), len(ref_ee), flush=True)
+    per_vf, per_ee, nvf, nee = {}, {}, 0, 0
+    for k, s in enumerate(inst):
+        d = api.detect(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"])
+        off = b.vert_off[k]
+        per_vf.update(_as_set(d["vf"] + off, d["vf_toi"]))
+        per_ee.update(_as_set(d["ee"] + off, d["ee_toi"]))
+        nvf += d["vf_candidates"]
+        nee += d["ee_candidates"]
+    assert (nvf, nee) == (r0["vf_candidates"], r0["ee_candidates"])
+    assert per_vf == ref_vf and per_ee == ref_ee
+    # oracle on a sample of the instances (the smaller half, where the oracle is quick)
+    rng = np.random.default_rng(2)
+    small = [k for k, s in enumerate(inst) if len(s["faces"]) <= 3000]
+    for k in rng.choice(small, size=6, replace=False):
+        s = inst[int(k)]
+        h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+        vf, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
+        off = int(b.vert_off[int(k)])
+        for keys, is_ee, got in ((vf, False, ref_vf), (ee, True, ref_ee)):
+            glob = [tuple(int(v) + off for v in x) for x in keys.tolist()]
+            hit = np.array([g in got for g in glob], dtype=bool)
+            t = np.array([got.get(g, 0.0) for g in glob])
+            rec = oracle.narrowphase(h, keys, s["eta"], is_ee)
+            rep = compare_results(f"inst{k}" + ("ee" if is_ee else "vf"), hit, t, rec,
+                                  arbiter=exact_arbiters(h, keys, s["eta"], is_ee))
+            print(rep["name"], len(keys), "stencils", rep["hits"], "hits", rep["n_ref_root_error"], "ref root errors",
+                  flush=True)
+            assert not rep["hard"] and not rep["t_bad_hard"], rep

@@ -123,3 +123,63 @@ def test_server_overflow_reruns():
     assert (r["vf_candidates"], r["ee_candidates"]) == (d["vf_candidates"], d["ee_candidates"])
     assert {tuple(k): t for k, t in zip(vk.tolist(), vt)} == {tuple(k): t for k, t in zip(d["vf"].tolist(), d["vf_toi"])}
     assert {tuple(k): t for k, t in zip(ek.tolist(), et)} == {tuple(k): t for k, t in zip(d["ee"].tolist(), d["ee_toi"])}
+
+
+def test_server_full_workload_alone_equals_in_mix():
+    """The served workload's 256 resident sessions (seed 4, 1k-500k triangles, 20.6M resident): a
+    sampled trace of requests (each a fresh N(0, 0.002^2) step of the session's end positions) runs
+    in the mix with 16 requests in flight; for a sample of sessions, the same requests run alone on a
+    server holding only that session, and through fresh detections (refit = rebuild): the colliding
+    stencils and time-of-impact bits are the same in all three."""
+    from collisiondetection_amd import api
+
+    print("case: served workload, 256 sessions", flush=True)
+    sess = scenes.served_sessions(256)
+    rng = np.random.default_rng(4)
+    ends = [s["qend"].copy() for s in sess]
+    reqs = []
+    for _ in range(400):
+        k = int(rng.integers(0, len(sess)))
+        ends[k] = ends[k] + rng.normal(0.0, 0.002, size=ends[k].shape)
+        reqs.append((k, ends[k].copy()))
+    srv = api.Server(hbm_budget_bytes=32 << 30, max_inflight=16)
+    srv.keep_hits(True)
+    ids = []
+    for j, s in enumerate(sess):
+        ids.append(srv.open_session(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"]))
+        if j % 64 == 63:
+            print("opened", j + 1, "sessions", flush=True)
+    for rid, (k, q) in enumerate(reqs):
+        srv.submit(ids[k], q, rid)
+    srv.drain()
+    mixed = {}
+    for r in srv.poll(1 << 16):
+        assert r["status"] == 0, r
+        vk, vt, ek, et = srv.fetch_hits(r["request_id"])
+        hits = {("vf",) + tuple(x): t for x, t in zip(vk.tolist(), vt)}
+        hits.update({("ee",) + tuple(x): t for x, t in zip(ek.tolist(), et)})
+        mixed[r["request_id"]] = (r, hits)
+    st = srv.stats()
+    srv.close()
+    print("mixed run", st, flush=True)
+    assert len(mixed) == len(reqs)
+    nf = np.array([len(s["faces"]) for s in sess])
+    used = sorted({k for k, _ in reqs})
+    big = [k for k in used if nf[k] > 100_000]
+    sample = list(rng.choice([k for k in used if nf[k] <= 100_000], size=5, replace=False)) + big[:1]
+    for k in sample:
+        k = int(k)
+        mine = [(rid, q) for rid, (kk, q) in enumerate(reqs) if kk == k]
+        alone, _ = _run([sess[k]], [(0, q) for _, q in mine], max_inflight=1)
+        prev = sess[k]["qend"].copy()
+        for j, (rid, q) in enumerate(mine):
+            ra, ha = alone[j]
+            rm, hm = mixed[rid]
+            assert ha == hm, (k, rid)
+            assert (ra["vf_candidates"], ra["ee_candidates"]) == (rm["vf_candidates"], rm["ee_candidates"])
+            d = api.detect(sess[k]["faces"], prev, q, sess[k]["eta"], sess[k]["eta"])
+            fresh = {("vf",) + tuple(x): t for x, t in zip(d["vf"].tolist(), d["vf_toi"])}
+            fresh.update({("ee",) + tuple(x): t for x, t in zip(d["ee"].tolist(), d["ee_toi"])})
+            assert fresh == hm, (k, rid)
+            prev = q
+        print("session", k, nf[k], "triangles:", len(mine), "requests equal alone, in the mix and fresh", flush=True)
