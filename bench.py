@@ -1,18 +1,22 @@
-"""Benchmark: swept triangles per second of the full swept-primitive CCD path on one MI355X.
+"""Benchmark: swept triangles per second of the swept-primitive CCD path on one MI355X.
 
-Workload (default): the draped cloth self-CCD step -- a 256 x 256 vertex grid (65,536 vertices,
-130,050 triangles) folded as z = 0.15 sin(6x) cos(5y), end = start - 0.02 z + N(0, 0.004^2), eta = 1e-6,
-FP64.  One step = the reference's KDOPBroadPhase (leaf bounds, tree, traversal, stencil set) followed by
-CTCDNarrowPhase on every candidate stencil, with positions already resident in HBM; the colliding
-stencils and their times of impact are the output.
+Headline workload: the batched one -- 512 independent two-mesh instances (icospheres or noisy grids,
+log-uniform 80-20,480 triangles per mesh, 4,018,680 triangles in all), each mesh moved by a random
+rigid motion plus N(0, 0.005^2) per vertex, seed 2, eta = 1e-6, FP64.  One step = the whole batch
+through the reference's KDOPBroadPhase (leaf bounds, tree, traversal, stencil set) and CTCDNarrowPhase
+on every candidate stencil, as one pipeline replayed from a captured HIP graph, positions already
+resident in HBM; the colliding stencils and their times of impact come back to the host.  Beside it,
+"plain": the same instances one ccd_detect call each (the reference runs one mesh per call).
 
-Printed: one JSON line with the metric, the per-stage HIP-event timings, the roofline of the dominant
-kernel and a CPU baseline (the oracle's reference-structured CPU path on a bounded sample), plus
-secondary legs: the isolating root mode, the separating-plane narrowphase on the same candidates, the
-velocity filter (the reference's example input and a falling-sheets contact problem), batched
-instances and the serving loop (each can be switched off with its --no-* flag).
+Printed: one JSON line with the metric, the batched and plain rates, the roofline of the pipeline's
+dominant kernel (HIP-event stage timers on the context's stream), host description and CPU baselines
+(the oracle's reference-structured path on a stratified sample of the instances, 1 thread and all
+cores), plus secondary legs, each switched by --legs: the 256^2 draped cloth (cloth), the 1024^2
+crumpled sheet (sheet), the separating-plane narrowphase on the cloth's candidates (separating_plane),
+the velocity filter (velocity_filter) and the serving loop over 256 resident sessions (served).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -23,43 +27,242 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# FP64 wavefront-instruction issue peak: 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 FP64 instruction.
+FP64_ISSUE_PEAK = 1024 * 2.4e9 / 4
+ETA_BATCH = 1e-6
+ALL_LEGS = ("plain", "cloth", "sheet", "separating_plane", "velocity_filter", "served")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, sample_stencils=60000, seed=0):
-    """The oracle's reference-structured CPU path (top-down k-DOP tree, recursive traversal with std::set
-    insertion, then the per-stencil narrowphase), single-threaded, on a bounded sample: the broadphase
-    in full and the narrowphase on a random sample of its stencils, extrapolated per stencil."""
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0))
+    threads = min(usable, int(os.environ.get("OMP_NUM_THREADS", usable)))
+    return {"cpu_model": model, "usable_cpus": usable, "logical_cpus": os.cpu_count(), "baseline_threads": threads}
+
+
+def load_profile(name, workload):
+    """Per-kernel counter figures from a committed rocprofv3 summary (profiles/<name>.json), if it was
+    made on this same workload."""
+    try:
+        prof = json.load(open(os.path.join(ROOT, "profiles", name)))
+    except (OSError, ValueError):
+        return None
+    return prof if prof.get("workload") == workload else None
+
+
+def roofline(kernel, stage_ms, flops, prof, launches_per_step=1):
+    """FP64 roofline of one narrowphase kernel: algorithmic flops (the survey's coefficient-construction
+    count of the sub-tests the reference executes) over its HIP-event time; plus, from the committed
+    counter profile of the same workload, the FP64 work the kernel issues (SQ_INSTS_VALU_*_F64 wavefront
+    instructions x 64 lanes, FMA counted twice: root finding included, lanes masked off by divergence
+    counted too) and the HBM traffic per launch."""
+    t = stage_ms * 1e-3 / launches_per_step
+    achieved = flops / launches_per_step / t / 1e12
+    roof = {"kernel": kernel, "bound": "fp64", "pipe": "FP64 VALU (no dense contraction, so no MFMA)",
+            "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": None, "algorithmic_flops_per_launch": flops / launches_per_step, "launch_ms": t * 1e3}
+    kp = (prof or {}).get("kernels", {}).get(kernel)
+    if kp:
+        if "hbm_bytes" in kp:
+            roof["traffic"] = kp["hbm_bytes"]
+            roof["traffic_source"] = prof.get("source")
+        if "fp64_wave_instructions" in kp:
+            ex = kp["fp64_issue_flops"] / (kp["avg_ms"] * 1e-3) / 1e12
+            roof["executed"] = {"achieved": ex, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                "frac": ex / FP64_PEAK_TFLOPS,
+                                "issue_frac": kp["fp64_wave_instructions"] / (kp["avg_ms"] * 1e-3) / FP64_ISSUE_PEAK,
+                                "source": prof.get("source")}
+    return roof
+
+
+def work_flops(ctx, qs_ptr, qe_ptr, V, eta):
+    """Algorithmic flops of the last detection's narrowphase on this context (VF, EE)."""
+    from collisiondetection_amd import _native as N
+
+    L = N.lib()
+    pv, pe = ctypes.c_void_p(), ctypes.c_void_p()
+    nv, ne = ctypes.c_int64(), ctypes.c_int64()
+    ctx.check(L.ccd_last_candidates(ctx.ptr, ctypes.byref(pv), ctypes.byref(nv), ctypes.byref(pe), ctypes.byref(ne)))
+    wv = (ctypes.c_int64 * N.NSTAGES)()
+    we = (ctypes.c_int64 * N.NSTAGES)()
+    ctx.check(L.ccd_narrowphase_work(ctx.ptr, qs_ptr, qe_ptr, V, pv, nv.value, pe, ne.value, eta, wv, we))
+    fl = (ctypes.c_int64 * N.NSTAGES)()
+    L.ccd_stage_flops(fl)
+    return (sum(wv[i] * fl[i] for i in range(N.NSTAGES)), sum(we[i] * fl[i] for i in range(N.NSTAGES)),
+            nv.value, ne.value)
+
+
+def stage_profile(ctx, run, steps):
+    """Mean per-stage milliseconds (HIP events on the context's stream) over `steps` calls of run()."""
+    import torch
+
+    from collisiondetection_amd import _native as N
+
+    L = N.lib()
+    L.ccd_set_profiling(ctx.ptr, 1)
+    L.ccd_stage_times(ctx.ptr, None, None, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    ms = (ctypes.c_double * N.NTIMERS)()
+    calls = (ctypes.c_int64 * N.NTIMERS)()
+    L.ccd_stage_times(ctx.ptr, ms, calls, 1)
+    L.ccd_set_profiling(ctx.ptr, 0)
+    return {N.TIMER_NAMES[i]: ms[i] / max(calls[i], 1) for i in range(N.NTIMERS)}, wall
+
+
+def cpu_baseline_batched(inst, threads, seed=0, strata=8, per_stratum=2, stencil_sample=4000):
+    """The oracle's reference-structured CPU path (top-down k-DOP tree, recursive traversal into
+    std::set, then the per-stencil narrowphase with the reference's own root finder) on a stratified
+    sample of the instances: instances sorted by size into `strata` groups, `per_stratum` drawn from
+    each; per drawn instance the broadphase in full and the narrowphase on a random sample of its
+    stencils, extrapolated per stencil; the group's time = its size / drawn x the drawn instances'
+    times.  threads > 1: the same restatement with the traversal's subtree pairs and the stencils spread
+    over OpenMP threads."""
     import numpy as np
 
     import oracle
 
     rf = oracle.use_default_rootfinder()
-    h = oracle.Hist.two_entry(scene["qstart"], scene["qend"])
-    t0 = time.perf_counter()
-    vf, ee = oracle.candidates(scene["faces"], h, scene["eta"], method="tree_set")
-    t_broad = time.perf_counter() - t0
+    oracle.set_threads(threads)
     rng = np.random.default_rng(seed)
-    nv = min(len(vf), sample_stencils * len(vf) // max(1, len(vf) + len(ee)))
-    ne = min(len(ee), sample_stencils - nv)
-    sv = vf[rng.choice(len(vf), size=nv, replace=False)] if nv else vf[:0]
-    se = ee[rng.choice(len(ee), size=ne, replace=False)] if ne else ee[:0]
-    t0 = time.perf_counter()
-    oracle.narrowphase(h, sv, scene["eta"], False)
-    t_vf = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    oracle.narrowphase(h, se, scene["eta"], True)
-    t_ee = time.perf_counter() - t0
-    t_narrow = (t_vf / max(nv, 1)) * len(vf) + (t_ee / max(ne, 1)) * len(ee)
+    sizes = np.array([len(s["faces"]) for s in inst])
+    groups = np.array_split(np.argsort(sizes, kind="stable"), strata)
+    total, spent, drawn = 0.0, 0.0, []
+    try:
+        for g in groups:
+            pick = rng.choice(g, size=min(per_stratum, len(g)), replace=False)
+            t_g = 0.0
+            for k in pick:
+                s = inst[int(k)]
+                h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+                t0 = time.perf_counter()
+                vf, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree_set" if threads == 1 else "tree_parallel")
+                tb = time.perf_counter() - t0
+                tn = 0.0
+                for keys, is_ee in ((vf, False), (ee, True)):
+                    if not len(keys):
+                        continue
+                    m = min(len(keys), stencil_sample)
+                    sub = keys[rng.choice(len(keys), size=m, replace=False)]
+                    t0 = time.perf_counter()
+                    oracle.narrowphase(h, sub, s["eta"], is_ee)
+                    tn += (time.perf_counter() - t0) * len(keys) / m
+                t_g += tb + tn
+                spent += tb
+                drawn.append(int(k))
+            total += t_g * len(g) / len(pick)
+    finally:
+        oracle.set_threads(1)
+    F = int(sizes.sum())
+    return {"value": F / total, "unit": "swept tris/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle (root finder '{rf}', {threads} thread{'s' if threads > 1 else ''}) on {len(drawn)} of "
+                       f"{len(inst)} instances, {per_stratum} drawn from each of {strata} size strata; broadphase in "
+                       f"full, narrowphase on <= {stencil_sample} random stencils per kind and instance, extrapolated: "
+                       f"{total:.1f} s per batch")}
+
+
+def cpu_baseline_scene(scene, threads=1, sample_stencils=60000, seed=0):
+    """The oracle's reference-structured CPU path on one mesh: the broadphase in full and the
+    narrowphase on a random sample of its stencils, extrapolated per stencil."""
+    import numpy as np
+
+    import oracle
+
+    rf = oracle.use_default_rootfinder()
+    oracle.set_threads(threads)
+    try:
+        h = oracle.Hist.two_entry(scene["qstart"], scene["qend"])
+        t0 = time.perf_counter()
+        vf, ee = oracle.candidates(scene["faces"], h, scene["eta"], method="tree_set" if threads == 1 else "tree_parallel")
+        t_broad = time.perf_counter() - t0
+        rng = np.random.default_rng(seed)
+        nv = min(len(vf), sample_stencils * len(vf) // max(1, len(vf) + len(ee)))
+        ne = min(len(ee), sample_stencils - nv)
+        t_narrow = 0.0
+        for keys, m, is_ee in ((vf, nv, False), (ee, ne, True)):
+            if m:
+                sub = keys[rng.choice(len(keys), size=m, replace=False)]
+                t0 = time.perf_counter()
+                oracle.narrowphase(h, sub, scene["eta"], is_ee)
+                t_narrow += (time.perf_counter() - t0) * len(keys) / m
+    finally:
+        oracle.set_threads(1)
     total = t_broad + t_narrow
-    F = len(scene["faces"])
-    return {"value": F / total, "unit": "swept tris/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle with root finder '{rf}'; full broadphase ({t_broad:.2f} s, {len(vf)} VF + {len(ee)} EE stencils) + narrowphase on "
-                       f"{nv} VF and {ne} EE sampled stencils ({t_vf + t_ee:.2f} s), extrapolated to all: "
-                       f"{total:.1f} s per step")}
+    return {"value": len(scene["faces"]) / total, "unit": "swept tris/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle (root finder '{rf}', {threads} thread{'s' if threads > 1 else ''}); full broadphase "
+                       f"({t_broad:.2f} s, {len(vf)} VF + {len(ee)} EE stencils) + narrowphase on {nv} VF and {ne} EE "
+                       f"sampled stencils, extrapolated to all: {total:.1f} s per step")}
+
+
+def bench_detect(scene, name, steps, warmup, log):
+    """One mesh through ccd_detect (broadphase + narrowphase): wall time per step and HIP-event stages."""
+    import torch
+
+    from collisiondetection_amd import _native as N
+    from collisiondetection_amd import api
+
+    L = N.lib()
+    ctx = api.Context(0)
+    ctx.bind_stream()
+    dev = ctx.torch_device
+    F, V = len(scene["faces"]), len(scene["qstart"])
+    faces = api.to_device(scene["faces"], torch.int32, ctx).reshape(-1)
+    qs = api.to_device(scene["qstart"], torch.float64, ctx).reshape(-1)
+    qe = api.to_device(scene["qend"], torch.float64, ctx).reshape(-1)
+    eta = scene["eta"]
+    bufs = {}
+    cnt = [ctypes.c_int64(0) for _ in range(4)]
+
+    def alloc(cap):
+        bufs.update(cap=cap, vk=torch.empty(4 * cap, dtype=torch.int32, device=dev),
+                    vt=torch.empty(cap, dtype=torch.float64, device=dev),
+                    ek=torch.empty(4 * cap, dtype=torch.int32, device=dev),
+                    et=torch.empty(cap, dtype=torch.float64, device=dev))
+
+    alloc(max(1024, F))
+
+    def step():
+        b = bufs
+        rc = L.ccd_detect(ctx.ptr, faces.data_ptr(), F, qs.data_ptr(), qe.data_ptr(), V, eta, eta, None,
+                          b["vk"].data_ptr(), b["vt"].data_ptr(), b["cap"], ctypes.byref(cnt[0]), b["ek"].data_ptr(),
+                          b["et"].data_ptr(), b["cap"], ctypes.byref(cnt[1]), ctypes.byref(cnt[2]),
+                          ctypes.byref(cnt[3]))
+        if rc == N.CCD_E_CAPACITY:   # collision buffers too small (warmup only): grow and repeat
+            alloc(max(cnt[0].value, cnt[1].value) * 5 // 4 + 1024)
+            return step()
+        ctx.check(rc)
+
+    for i in range(warmup):
+        step()
+        log(f"{name}: warmup {i + 1}/{warmup}: {cnt[2].value} VF + {cnt[3].value} EE candidates, "
+            f"{cnt[0].value} + {cnt[1].value} collisions")
+    torch.cuda.synchronize(dev)
+    fv, fe, nv, ne = work_flops(ctx, qs.data_ptr(), qe.data_ptr(), V, eta)
+    stage_ms, wall = stage_profile(ctx, step, steps)
+    log(f"{name}: {wall * 1e3:.2f} ms/step; stages", json.dumps({k: round(v, 3) for k, v in stage_ms.items()}))
+    out = {"triangles": F, "vertices": V, "ms_per_step": wall * 1e3, "swept_tris_per_s": F / wall,
+           "pair_tests_per_s": (nv + ne) / ((stage_ms["narrow_vf"] + stage_ms["narrow_ee"]) * 1e-3),
+           "candidates": {"vf": nv, "ee": ne}, "collisions": {"vf": cnt[0].value, "ee": cnt[1].value},
+           "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
+           "narrowphase_tflops": {"vf": fv / max(stage_ms["narrow_vf"], 1e-9) / 1e9,
+                                  "ee": fe / max(stage_ms["narrow_ee"], 1e-9) / 1e9}}
+    return out, (ctx, qs, qe, stage_ms, fv, fe)
 
 
 def bench_separating_plane(ctx, scene, qs, qe, steps, log, cpu_sample=30000, seed=1):
@@ -176,80 +379,6 @@ def bench_velocity_filter(ctx, log):
     return out
 
 
-def bench_batched(steps, log):
-    """The 512-instance mixed workload: plain (ccd_detect per instance, the same kernels launched per
-    instance) vs batched (one ccd_batch_detect pipeline) vs batched replayed from a captured HIP graph."""
-    import ctypes
-
-    import numpy as np
-    import torch
-
-    from collisiondetection_amd import _native as N
-    from collisiondetection_amd import api, scenes
-
-    inst = scenes.batched_instances()
-    b = api.Batch(inst)
-    ctx = b.ctx
-    L = N.lib()
-    F = b.F
-    log(f"batched workload: {b.ninst} instances, {F} triangles, {b.V} vertices")
-    # per-instance face arrays with local vertex indices (the plain path's input)
-    local_faces = [api.to_device(s["faces"], torch.int32, ctx).reshape(-1) for s in inst]
-    cap = max(1024, 2 * max(len(s["faces"]) for s in inst))
-    dev = ctx.torch_device
-    vk = torch.empty(4 * cap, dtype=torch.int32, device=dev)
-    vt = torch.empty(cap, dtype=torch.float64, device=dev)
-    ek = torch.empty(4 * cap, dtype=torch.int32, device=dev)
-    et = torch.empty(cap, dtype=torch.float64, device=dev)
-    c = [ctypes.c_int64(0) for _ in range(4)]
-    qs0, qe0 = b.qs.data_ptr(), b.qe.data_ptr()
-
-    def plain():
-        hits = 0
-        for k, s in enumerate(inst):
-            off = int(b.vert_off[k]) * 24
-            rc = L.ccd_detect(ctx.ptr, local_faces[k].data_ptr(), len(s["faces"]), qs0 + off, qe0 + off,
-                              len(s["qstart"]), s["eta"], s["eta"], None, vk.data_ptr(), vt.data_ptr(), cap,
-                              ctypes.byref(c[0]), ek.data_ptr(), et.data_ptr(), cap, ctypes.byref(c[1]),
-                              ctypes.byref(c[2]), ctypes.byref(c[3]))
-            ctx.check(rc)
-            hits += c[0].value + c[1].value
-        return hits
-
-    ctx.bind_stream()
-    plain()
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    np_ = max(1, steps // 4)
-    for _ in range(np_):
-        h_plain = plain()
-    torch.cuda.synchronize(dev)
-    t_plain = (time.perf_counter() - t0) / np_
-    r = b.detect(1e-6, 1e-6)
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        r = b.detect(1e-6, 1e-6)
-    torch.cuda.synchronize(dev)
-    t_batch = (time.perf_counter() - t0) / steps
-    h_batch = len(r["vf"]) + len(r["ee"])
-    b.detect(1e-6, 1e-6, graph=True)   # synchronous run + capture
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        rg = b.detect(1e-6, 1e-6, graph=True)
-    torch.cuda.synchronize(dev)
-    t_graph = (time.perf_counter() - t0) / steps
-    h_graph = len(rg["vf"]) + len(rg["ee"])
-    log(f"batched: plain {t_plain * 1e3:.2f} ms, batched {t_batch * 1e3:.2f} ms, graph {t_graph * 1e3:.2f} ms; "
-        f"collisions {h_plain} / {h_batch} / {h_graph}")
-    return {"instances": b.ninst, "triangles": F, "plain_tris_per_s": F / t_plain, "batched_tris_per_s": F / t_batch,
-            "graph_tris_per_s": F / t_graph, "speedup_vs_plain": t_plain / t_graph,
-            "ms": {"plain": t_plain * 1e3, "batched": t_batch * 1e3, "graph": t_graph * 1e3},
-            "collisions_equal": h_plain == h_batch == h_graph, "collisions": h_graph,
-            "candidates": {"vf": rg["vf_candidates"], "ee": rg["ee_candidates"]}}
-
-
 def bench_served(seconds, rates, log, sessions=256, budget_gib=32, max_inflight=16):
     """The served workload: `sessions` device-resident sessions (log-uniform 1k-500k triangles, seed 4);
     for each rate, Poisson arrivals at that many requests/s for `seconds`, each to a uniform session,
@@ -333,172 +462,133 @@ def bench_served(seconds, rates, log, sessions=256, budget_gib=32, max_inflight=
             "client_noise": "one bank of 2^24 N(0, 0.002^2) draws, read at a random offset per request"}
 
 
+def bench_plain(b, inst, steps, log):
+    """The batched workload's instances one ccd_detect call each (per-instance launches and host
+    synchronisation, as the reference runs one mesh per call): wall time per pass over all instances."""
+    import torch
+
+    from collisiondetection_amd import _native as N
+    from collisiondetection_amd import api
+
+    ctx, L = b.ctx, N.lib()
+    local_faces = [api.to_device(s["faces"], torch.int32, ctx).reshape(-1) for s in inst]
+    cap = max(1024, 2 * max(len(s["faces"]) for s in inst))
+    dev = ctx.torch_device
+    vk = torch.empty(4 * cap, dtype=torch.int32, device=dev)
+    vt = torch.empty(cap, dtype=torch.float64, device=dev)
+    ek = torch.empty(4 * cap, dtype=torch.int32, device=dev)
+    et = torch.empty(cap, dtype=torch.float64, device=dev)
+    c = [ctypes.c_int64(0) for _ in range(4)]
+    qs0, qe0 = b.qs.data_ptr(), b.qe.data_ptr()
+
+    def plain():
+        hits = 0
+        for k, s in enumerate(inst):
+            off = int(b.vert_off[k]) * 24
+            rc = L.ccd_detect(ctx.ptr, local_faces[k].data_ptr(), len(s["faces"]), qs0 + off, qe0 + off,
+                              len(s["qstart"]), s["eta"], s["eta"], None, vk.data_ptr(), vt.data_ptr(), cap,
+                              ctypes.byref(c[0]), ek.data_ptr(), et.data_ptr(), cap, ctypes.byref(c[1]),
+                              ctypes.byref(c[2]), ctypes.byref(c[3]))
+            ctx.check(rc)
+            hits += c[0].value + c[1].value
+        return hits
+
+    ctx.bind_stream()
+    plain()
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hits = plain()
+    torch.cuda.synchronize(dev)
+    wall = (time.perf_counter() - t0) / steps
+    log(f"plain: {wall * 1e3:.1f} ms per pass over {len(inst)} instances, {hits} collisions")
+    return wall, hits
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--grid", type=int, default=256, help="cloth grid size (256 = the 130,050-triangle cloth)")
-    ap.add_argument("--workload", choices=["cloth", "sheet"], default="cloth",
-                    help="cloth: the draped cloth (default); sheet: the crumpled sheet (--grid 1024 = 2,093,058 tris)")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--legs", type=str, default=",".join(ALL_LEGS),
+                    help="secondary legs to run, comma separated (%s), or 'none'" % ", ".join(ALL_LEGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--no-separating-plane", action="store_true")
-    ap.add_argument("--no-velocity-filter", action="store_true")
-    ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy 5 times (scale reference for HBM byte counters)")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-rates", type=str, default="160,20",
-                    help="Poisson request rates (req/s), run one after the other on the same sessions; the "
-                         "default pair is near the measured capacity (~175 req/s on the 256 sessions) and light "
-                         "load")
+                    help="Poisson request rates (req/s), run one after the other on the same sessions")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
-
-    import ctypes
+    legs = set() if args.legs == "none" else {x.strip() for x in args.legs.split(",") if x.strip()}
+    unknown = legs - set(ALL_LEGS)
+    if unknown:
+        print(f"unknown legs: {sorted(unknown)}", file=sys.stderr)
+        sys.exit(2)
 
     import numpy as np
     import torch
 
-    from collisiondetection_amd import _native as N
     from collisiondetection_amd import api, scenes
 
     dev = torch.device("cuda", 0)
-    scene = scenes.cloth(args.grid) if args.workload == "cloth" else scenes.crumpled_sheet(args.grid)
-    F, V = len(scene["faces"]), len(scene["qstart"])
-    log(f"workload: {args.workload} {args.grid}^2, {V} vertices, {F} triangles, eta {scene['eta']}")
-    ctx = api.Context(0)
-    ctx.bind_stream()
-    faces = api.to_device(scene["faces"], torch.int32, ctx).reshape(-1)
-    qs = api.to_device(scene["qstart"], torch.float64, ctx).reshape(-1)
-    qe = api.to_device(scene["qend"], torch.float64, ctx).reshape(-1)
-    eta = scene["eta"]
-    bufs = {}
-
-    def alloc(cap):
-        bufs.update(cap=cap, vk=torch.empty(4 * cap, dtype=torch.int32, device=dev),
-                    vt=torch.empty(cap, dtype=torch.float64, device=dev),
-                    ek=torch.empty(4 * cap, dtype=torch.int32, device=dev),
-                    et=torch.empty(cap, dtype=torch.float64, device=dev))
-
-    alloc(4 * F)
-    cnt = [ctypes.c_int64(0) for _ in range(4)]
-    L = N.lib()
-
-    def step():
-        b = bufs
-        rc = L.ccd_detect(ctx.ptr, faces.data_ptr(), F, qs.data_ptr(), qe.data_ptr(), V, eta, eta, None,
-                          b["vk"].data_ptr(), b["vt"].data_ptr(), b["cap"], ctypes.byref(cnt[0]), b["ek"].data_ptr(),
-                          b["et"].data_ptr(), b["cap"], ctypes.byref(cnt[1]), ctypes.byref(cnt[2]),
-                          ctypes.byref(cnt[3]))
-        if rc == N.CCD_E_CAPACITY:   # collision buffers too small (warmup only): grow and repeat
-            alloc(max(cnt[0].value, cnt[1].value) * 5 // 4 + 1024)
-            return step()
-        ctx.check(rc)
-
-    def run_detect():
-        """Warmup, work accounting and the timed region; returns the figures."""
-        for i in range(args.warmup):
-            step()
-            log(f"warmup {i + 1}/{args.warmup}: {cnt[2].value} VF + {cnt[3].value} EE "
-                f"candidates, {cnt[0].value} + {cnt[1].value} collisions")
-        torch.cuda.synchronize(dev)
-
-        # Work accounting for the narrowphase roofline (outside the timed region).
-        pv, pe = ctypes.c_void_p(), ctypes.c_void_p()
-        nv, ne = ctypes.c_int64(), ctypes.c_int64()
-        ctx.check(L.ccd_last_candidates(ctx.ptr, ctypes.byref(pv), ctypes.byref(nv), ctypes.byref(pe),
-                                        ctypes.byref(ne)))
-        wv = (ctypes.c_int64 * N.NSTAGES)()
-        we = (ctypes.c_int64 * N.NSTAGES)()
-        ctx.check(L.ccd_narrowphase_work(ctx.ptr, qs.data_ptr(), qe.data_ptr(), V, pv, nv.value, pe, ne.value, eta,
-                                         wv, we))
-        fl = (ctypes.c_int64 * N.NSTAGES)()
-        L.ccd_stage_flops(fl)
-        flops_vf = sum(wv[i] * fl[i] for i in range(N.NSTAGES))
-        flops_ee = sum(we[i] * fl[i] for i in range(N.NSTAGES))
-
-        # Timed region: K steps, HIP-event stage timers on the context's stream.
-        L.ccd_set_profiling(ctx.ptr, 1)
-        L.ccd_stage_times(ctx.ptr, None, None, 1)
-        torch.cuda.synchronize(dev)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            step()
-        torch.cuda.synchronize(dev)
-        wall = time.perf_counter() - t0
-        ms = (ctypes.c_double * N.NTIMERS)()
-        calls = (ctypes.c_int64 * N.NTIMERS)()
-        L.ccd_stage_times(ctx.ptr, ms, calls, 1)
-        L.ccd_set_profiling(ctx.ptr, 0)
-        stage_ms = {N.TIMER_NAMES[i]: ms[i] / max(calls[i], 1) for i in range(N.NTIMERS)}
-        log("stage ms/step:", json.dumps({k: round(v, 4) for k, v in stage_ms.items()}))
-        return {"wall": wall, "stage_ms": stage_ms, "nv": nv.value, "ne": ne.value, "flops_vf": flops_vf,
-                "flops_ee": flops_ee, "hits": (cnt[0].value, cnt[1].value)}
-
-    res = run_detect()
-    wall, stage_ms, flops_vf, flops_ee = res["wall"], res["stage_ms"], res["flops_vf"], res["flops_ee"]
-    nv, ne = ctypes.c_int64(res["nv"]), ctypes.c_int64(res["ne"])
-    ms_per_step = 1000.0 * wall / args.steps
-    value = F / (wall / args.steps)
-    nstencils = nv.value + ne.value
-
-    out_workload = (f"cloth self-CCD {args.grid}x{args.grid} ({V} V, {F} tris), eta 1e-6, seed 1"
-                    if args.workload == "cloth" else
-                    f"crumpled sheet self-CCD {args.grid}x{args.grid} ({V} V, {F} tris), eta 1e-7, seed 3")
-    # Roofline of the dominant kernel.
-    dom = max((k for k in stage_ms if k != "pipeline"), key=stage_ms.get)
-    if dom in ("narrow_vf", "narrow_ee"):
-        flops = flops_vf if dom == "narrow_vf" else flops_ee
-        achieved = flops / (stage_ms[dom] * 1e-3) / 1e12
-        roof = {"kernel": "k_narrow2_queue<%s>" % ("VF" if dom == "narrow_vf" else "EE"), "bound": "fp64",
-                "pipe": "FP64 VALU (no dense contraction, so no MFMA; peak is MI355X's FP64 figure)",
-                "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                "traffic": None, "algorithmic_flops": flops}
-    else:
-        # Broadphase stages are bandwidth/latency bound: algorithmic bytes per swept triangle =
-        # 12 (face) + 48 V/F (start + end positions) + 16 S/F (unique stencil writes).
-        algo = 12 * F + 48 * V + 16 * nstencils
-        achieved = algo / (stage_ms[dom] * 1e-3) / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": algo}
-    # HBM traffic of the dominant kernel per launch, from the committed rocprofv3 counter passes on this
-    # workload (FETCH_SIZE x 2 + WRITE_SIZE, the factors a plain device copy gives); null elsewhere.
-    try:
-        prof = json.load(open(os.path.join(ROOT, "profiles", "cloth256_traffic.json")))
-        key = roof["kernel"].replace("k_narrow2_queue<VF>", "k_narrow2_queue<true, 256, ccd::NoCount>").replace(
-            "k_narrow2_queue<EE>", "k_narrow2_queue<false, 256, ccd::NoCount>")
-        if prof["workload"] == out_workload and key in prof["kernels"]:
-            kp = prof["kernels"][key]
-            roof["traffic"] = kp["hbm_bytes"]
-            roof["traffic_source"] = "profiles/cloth256_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
-            if "fp64_wave_instructions" in kp and roof.get("unit") == "TFLOP/s":
-                # Executed FP64 wavefront instructions (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 of the same
-                # profile) against the issue peak: 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction.
-                issue_peak = 1024 * 2.4e9 / 4
-                roof["fp64_issue_frac"] = kp["fp64_wave_instructions"] / (kp["avg_ms"] * 1e-3) / issue_peak
-    except (OSError, ValueError, KeyError):
-        pass
+    host = host_info()
+    log("host:", json.dumps(host))
+    inst = scenes.batched_instances()
+    b = api.Batch(inst)
+    ctx = b.ctx
+    F, V = b.F, b.V
+    workload = (f"batched: {b.ninst} two-mesh instances (icospheres / noisy grids, log-uniform 80-20,480 tris per "
+                f"mesh), {F} tris, rigid motion + N(0, 0.005^2), seed 2, eta 1e-6")
+    log(f"workload: {workload}")
+    for i in range(args.warmup):   # the first call runs synchronously and captures the pipeline
+        r = b.detect(ETA_BATCH, ETA_BATCH, graph=True)
+        log(f"warmup {i + 1}/{args.warmup}: {r['vf_candidates']} VF + {r['ee_candidates']} EE candidates, "
+            f"{len(r['vf'])} + {len(r['ee'])} collisions")
+    torch.cuda.synchronize(dev)
+    # Timed region: K graph replays of the whole batch, results back on the host.
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = b.detect(ETA_BATCH, ETA_BATCH, graph=True)
+    torch.cuda.synchronize(dev)
+    wall = (time.perf_counter() - t0) / args.steps
+    value = F / wall
+    hits = len(r["vf"]) + len(r["ee"])
+    log(f"batched (graph replay): {wall * 1e3:.2f} ms/step, {value / 1e6:.3f}M swept tris/s")
+    # The same pipeline launched stage by stage (HIP-event stage timers on the context's stream), and the
+    # algorithmic work of its candidates.
+    fv, fe, nv, ne = work_flops(ctx, b.qs.data_ptr(), b.qe.data_ptr(), V, ETA_BATCH)
+    stage_ms, wall_sync = stage_profile(ctx, lambda: b.detect(ETA_BATCH, ETA_BATCH, graph=False), max(2, args.steps // 3))
+    log(f"batched (stage-timed synchronous run): {wall_sync * 1e3:.2f} ms/step; stages",
+        json.dumps({k: round(v, 3) for k, v in stage_ms.items()}))
+    prof = load_profile("batched512_counters.json", workload)
+    dom = max(("narrow_vf", "narrow_ee"), key=stage_ms.get)
+    kern = "k_narrow2_queue<%s>" % ("VF" if dom == "narrow_vf" else "EE")
+    roof = roofline(kern, stage_ms[dom], fv if dom == "narrow_vf" else fe, prof)
     roof["stage_ms"] = {k: round(v, 5) for k, v in stage_ms.items()}
-    roof["narrowphase_tflops"] = {
-        "vf": flops_vf / max(stage_ms["narrow_vf"], 1e-9) / 1e9, "ee": flops_ee / max(stage_ms["narrow_ee"], 1e-9) / 1e9}
-
+    roof["narrowphase_tflops"] = {"vf": fv / max(stage_ms["narrow_vf"], 1e-9) / 1e9,
+                                  "ee": fe / max(stage_ms["narrow_ee"], 1e-9) / 1e9}
     out = {
         "metric": "swept tris/sec on 1 GPU, batched vs plain, exact sets; p50/p99 CCD query latency",
         "value": value, "unit": "swept tris/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": out_workload,
-                   "global_batch": 1, "parallelism": "single GPU"},
-        "pair_tests_per_s": nstencils / ((stage_ms["narrow_vf"] + stage_ms["narrow_ee"]) * 1e-3),
-        "candidates": {"vf": nv.value, "ee": ne.value}, "collisions": {"vf": res["hits"][0], "ee": res["hits"][1]},
+        "config": {"workload": workload, "instances": b.ninst, "triangles": F, "vertices": V,
+                   "global_batch": b.ninst, "parallelism": "single GPU"},
+        "pair_tests_per_s": (nv + ne) / ((stage_ms["narrow_vf"] + stage_ms["narrow_ee"]) * 1e-3),
+        "candidates": {"vf": nv, "ee": ne}, "collisions": {"vf": len(r["vf"]), "ee": len(r["ee"])},
         "roots": "isolated in [0,1] (derivative bracketing + Newton), in place of the reference's rpoly",
+        "stage_timed_ms_per_step": wall_sync * 1e3,
         "roofline": roof,
+        "host": host,
     }
+    if "plain" in legs:
+        t_plain, h_plain = bench_plain(b, inst, max(1, args.steps // 4), log)
+        out["plain"] = {"ms_per_step": t_plain * 1e3, "swept_tris_per_s": F / t_plain,
+                        "batched_vs_plain": t_plain / wall, "collisions_equal": h_plain == hits}
     if args.copy_probe:
         src = torch.empty(256 << 20, dtype=torch.uint8, device=dev).fill_(1)
         dst = torch.empty_like(src)
@@ -506,17 +596,38 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize(dev)
         out["copy_probe_bytes"] = {"read": 256 << 20, "written": 256 << 20, "dispatches": 5}
-    if not args.no_separating_plane:
-        out["separating_plane"] = bench_separating_plane(ctx, scene, qs, qe, max(3, args.steps // 2), log)
-    if not args.no_velocity_filter:
+    cloth = None
+    if "cloth" in legs or "separating_plane" in legs:
+        cloth = scenes.cloth(256)
+    if "cloth" in legs:
+        leg, _ = bench_detect(cloth, "cloth 256^2", args.steps, args.warmup, log)
+        leg["workload"] = "cloth self-CCD 256x256 (65536 V, 130050 tris), eta 1e-6, seed 1"
+        out["cloth"] = leg
+    if "sheet" in legs:
+        sheet = scenes.crumpled_sheet(1024)
+        leg, _ = bench_detect(sheet, "sheet 1024^2", 1, 1, log)
+        leg["workload"] = "crumpled sheet self-CCD 1024x1024 (1048576 V, 2093058 tris), eta 1e-7, seed 3"
+        out["sheet"] = leg
+        del sheet
+        torch.cuda.empty_cache()
+    if "separating_plane" in legs:
+        qs = api.to_device(cloth["qstart"], torch.float64, ctx).reshape(-1)
+        qe = api.to_device(cloth["qend"], torch.float64, ctx).reshape(-1)
+        out["separating_plane"] = bench_separating_plane(ctx, cloth, qs, qe, max(3, args.steps // 2), log)
+    if "velocity_filter" in legs:
         out["velocity_filter"] = bench_velocity_filter(ctx, log)
-    if not args.no_batched:
-        out["batched"] = bench_batched(max(3, args.steps // 2), log)
-    if not args.no_served:
-        out["served"] = bench_served(args.serve_seconds, [float(r) for r in args.serve_rates.split(",")], log)
+    if "served" in legs:
+        out["served"] = bench_served(args.serve_seconds, [float(x) for x in args.serve_rates.split(",")], log)
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle, 1 thread) ...")
-        out["cpu_baseline"] = cpu_baseline(scene)
+        log("cpu baseline (oracle, 1 thread, stratified sample of the instances) ...")
+        out["cpu_baseline"] = cpu_baseline_batched(inst, 1)
+        log("cpu baseline:", json.dumps(out["cpu_baseline"]))
+        if host["baseline_threads"] > 1:
+            log(f"cpu baseline (oracle, {host['baseline_threads']} threads) ...")
+            out["cpu_baseline_all_cores"] = cpu_baseline_batched(inst, host["baseline_threads"])
+            log("cpu baseline, all cores:", json.dumps(out["cpu_baseline_all_cores"]))
+        if "cloth" in legs:
+            out["cloth"]["cpu_baseline"] = cpu_baseline_scene(cloth, 1)
     print(json.dumps(out), flush=True)
 
 
