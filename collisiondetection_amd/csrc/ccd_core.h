@@ -141,6 +141,13 @@ struct stops_at_solver : std::false_type {};
 template <class C>
 struct stops_at_solver<C, std::void_t<decltype(C::stops_at_solver)>> : std::integral_constant<bool, C::stops_at_solver> {};
 
+// A counter type whose `plain_roots` member is true isolates roots without the Bernstein shortcuts and
+// with plain safeguarded Newton (bisection fallback, plain Horner values): the work-accounting kernel.
+template <class C, class = void>
+struct plain_roots : std::false_type {};
+template <class C>
+struct plain_roots<C, std::void_t<decltype(C::plain_roots)>> : std::integral_constant<bool, C::plain_roots> {};
+
 CCD_FHD double bits_to_double(long long v)
 {
     double d;
@@ -285,7 +292,8 @@ CCD_FHD int quad_roots(double a, double b, double c, double &t0, double &t1)
 // Real-root isolation on [0,1] for degree 1..6 (used for reduced degree >= 3 only).
 // c[0..D] descending, c[0] != 0.  Writes the distinct real roots that lie in [0,1], ascending, and
 // returns their count.  A root exactly at an end (c evaluates to exactly 0 there) is reported as
-// exactly that end.
+// exactly that end.  Polynomials whose Bernstein coefficients show no or one sign change are settled
+// directly; the others recurse on the derivative.
 // ------------------------------------------------------------------------------------------------
 template <int D>
 CCD_FHD double peval(const double *c, double x)
@@ -307,43 +315,207 @@ CCD_FHD void peval_d(const double *c, double x, double &f, double &df)
     }
 }
 
-// The single root of c in [a, b] given opposite nonzero signs fa, fb at the ends.
+// Value, first and second derivative of c at x (Horner with fused multiply-add).
 template <int D>
+CCD_FHD void peval_d2(const double *c, double x, double &f, double &df, double &d2)
+{
+    f = c[0];
+    df = 0.0;
+    d2 = 0.0;
+#pragma unroll
+    for (int i = 1; i <= D; i++) {
+        d2 = dfma(d2, x, df);
+        df = dfma(df, x, f);
+        f = dfma(f, x, c[i]);
+    }
+    d2 *= 2.0;
+}
+
+// Value and derivative of c at x, and m = sum |c_i| |x|^(D-i): the value's rounding error is a small
+// multiple of u m, and below u m its sign is no longer reliable.
+template <int D>
+CCD_FHD void peval_dm(const double *c, double x, double &f, double &df, double &m)
+{
+    f = c[0];
+    df = 0.0;
+    m = dabs(c[0]);
+    const double ax = dabs(x);
+#pragma unroll
+    for (int i = 1; i <= D; i++) {
+        df = dfma(df, x, f);
+        f = dfma(f, x, c[i]);
+        m = dfma(m, ax, dabs(c[i]));
+    }
+}
+
+// c at x by compensated Horner: each product's and sum's rounding error is recovered exactly (the
+// product's with a fused multiply-add, the sum's with Knuth's two-sum) and accumulated in a second
+// Horner pass, so the value is about as accurate as Horner in twice the precision (Graillat, Langlois
+// and Louvet, "Compensated Horner scheme", 2005).
+template <int D>
+CCD_FHD double peval_comp(const double *c, double x)
+{
+    double s = c[0], e = 0.0;
+#pragma unroll
+    for (int i = 1; i <= D; i++) {
+        const double p = s * x;
+        const double pe = dfma(s, x, -p);
+        const double sn = p + c[i];
+        const double t = sn - p;
+        const double se = (p - (sn - t)) + (c[i] - t);
+        e = dfma(e, x, pe + se);
+        s = sn;
+    }
+    return s + e;
+}
+
+// The single root of c in [a, b] given opposite nonzero signs fa, fb at the ends: safeguarded Newton
+// (a step that leaves the bracket becomes a bisection).
+//   * Start: the bracket ends are critical points of c (or 0 and 1), and near a pair of close roots --
+//     which the distance polynomials have, about eta apart -- c is locally quadratic there: Newton from
+//     the far side would close in only linearly.  So the start is the root of the second-order Taylor
+//     model at the end with the smaller |c| (the one nearest it, towards the other end), else the
+//     secant point.
+//   * Once the plain value falls below its rounding level (u sum |c_i| |x|^(D-i)), the value comes from
+//     compensated Horner, so the iteration keeps converging instead of following rounding noise; the
+//     derivative stays plain (it only sets the step).
+//   * Stop when the Newton step is below the resolution of x, the bracket has closed, or c is exactly 0.
+template <int D, bool PLAIN = false>
 CCD_HD double bracketed_root(const double *c, double a, double b, double fa, double fb)
 {
+    if constexpr (PLAIN) {
+        // Newton from the secant point, bisection whenever a step leaves the bracket.
+        double lo = a, hi = b;
+        const bool neg_lo = fa < 0;
+        double x = a + (b - a) * (fa / (fa - fb));
+        if (!(x > lo && x < hi)) x = 0.5 * (lo + hi);
+        for (int it = 0; it < 80; it++) {
+            double f, df;
+            peval_d<D>(c, x, f, df);
+            if (f == 0.0) return x;
+            if ((f < 0) == neg_lo)
+                lo = x;
+            else
+                hi = x;
+            double xn = x - f / df;
+            if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
+            double step = xn - x;
+            if (step < 0) step = -step;
+            x = xn;
+            const double ax = x < 0 ? -x : x;
+            if (step <= 2.3e-16 * ax + 1e-18 || hi - lo <= 2.3e-16 * ax + 1e-18 || !(hi > lo)) break;
+        }
+        return x;
+    }
     double lo = a, hi = b;
-    bool neg_lo = fa < 0;
-    // Start from the secant (regula falsi) point.
+    const bool neg_lo = fa < 0;
     double x = a + (b - a) * (fa / (fa - fb));
+    {
+        const bool at_a = dabs(fa) <= dabs(fb);
+        const double e = at_a ? a : b, dir = at_a ? 1.0 : -1.0;
+        double p, p1, p2;
+        peval_d2<D>(c, e, p, p1, p2);
+        // e + dir u with u > 0 the smallest root of p + (p1 dir) u + (p2 / 2) u^2
+        const double A = 0.5 * p2, B = p1 * dir, C = p;
+        double u = -1.0;
+        if (A != 0.0) {
+            const double disc = B * B - 4.0 * A * C;
+            if (disc >= 0.0) {
+                const double sq = dsqrt(disc);
+                const double q = -0.5 * (B + (B < 0 ? -sq : sq));
+                const double u1 = q / A, u2 = q != 0.0 ? C / q : -1.0;
+                if (u1 > 0.0) u = u1;
+                if (u2 > 0.0 && (u < 0.0 || u2 < u)) u = u2;
+            }
+        } else if (B != 0.0) {
+            u = -C / B;
+        }
+        if (u > 0.0) {
+            const double x0 = e + dir * u;
+            if (x0 > lo && x0 < hi) x = x0;
+        }
+    }
     if (!(x > lo && x < hi)) x = 0.5 * (lo + hi);
-    for (int it = 0; it < 80; it++) {
-        double f, df;
-        peval_d<D>(c, x, f, df);
+    bool compensated = false;
+    for (int it = 0; it < 100; it++) {
+        double f, df, m;
+        peval_dm<D>(c, x, f, df, m);
+        if (!compensated && dabs(f) <= 1.1102230246251565e-16 * m) compensated = true;
+        if (compensated) f = peval_comp<D>(c, x);
         if (f == 0.0) return x;
         if ((f < 0) == neg_lo)
             lo = x;
         else
             hi = x;
-        double xn = x - f / df;
+        const double step = f / df;
+        const double ax = x < 0 ? -x : x;
+        if (dabs(step) <= 2.3e-16 * ax + 1e-18) return x - step;
+        double xn = x - step;
         if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
-        double step = xn - x;
-        if (step < 0) step = -step;
         x = xn;
-        double ax = x < 0 ? -x : x;
-        if (step <= 2.3e-16 * ax + 1e-18 || hi - lo <= 2.3e-16 * ax + 1e-18 || !(hi > lo)) break;
+        const double axn = x < 0 ? -x : x;
+        if (hi - lo <= 2.3e-16 * axn + 1e-18 || !(hi > lo)) break;
     }
     return x;
 }
 
+// Binomial coefficient as a double (exact for the small arguments used here); constant-folded.
+CCD_FHD constexpr double binom(int n, int k)
+{
+    double r = 1.0;
+    for (int i = 1; i <= k; i++) r = r * (double)(n - k + i) / (double)i;
+    return r;
+}
+
+// Sign variations of the Bernstein coefficients of c[0..D] (descending powers) on [0, 1], or -1 when one
+// of them lies within a margin of zero that covers the rounding of the conversion.  p(t) on [0, 1] is a
+// convex combination of its Bernstein coefficients b_k = sum_{i <= k} C(k, i) / C(D, i) a_i (a_i the
+// coefficient of t^i), and the number of roots in (0, 1) is at most the number of sign changes of b and
+// has its parity (Descartes' rule of signs in the Bernstein basis).  So with every b_k clear of zero, no
+// variation means no root in [0, 1] and one variation exactly one root, inside (0, 1).  The margin:
+// each computed b_k is off by at most (k + 2) u sum |a_i| <= 56 u max |a_i| (D <= 6, u = 2^-53), and
+// 1e-13 max |a_i| is 16 times that.  Non-finite coefficients give -1.
 template <int D>
+CCD_FHD int bernstein_variations(const double *c)
+{
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i <= D; i++) mx = smax(mx, dabs(c[i]));
+    const double tol = 1e-13 * mx;
+    int v = 0, last = 0;
+    bool clear = true;
+#pragma unroll
+    for (int k = 0; k <= D; k++) {
+        double b = 0.0;
+#pragma unroll
+        for (int i = 0; i <= k; i++) b += (binom(k, i) / binom(D, i)) * c[D - i];
+        const int sg = b > tol ? 1 : (b < -tol ? -1 : 0);
+        if (sg == 0) clear = false;
+        if (sg != 0 && last != 0 && sg != last) v++;
+        if (sg != 0) last = sg;
+    }
+    return clear ? v : -1;
+}
+
+template <int D, bool PLAIN = false>
 struct RootIsolator {
     CCD_HD static int roots01(const double *c, double (&r)[D])
     {
+        // The common cases first (most polynomials the cascade solves): no root in [0, 1], or exactly one
+        // in (0, 1), bracketed by the ends, where p(0) = c[D] exactly and p(1) has b_D's sign.
+        if constexpr (!PLAIN) {
+            const int v = bernstein_variations<D>(c);
+            if (v == 0) return 0;
+            if (v == 1) {
+                put(r, 0, bracketed_root<D>(c, 0.0, 1.0, peval<D>(c, 0.0), peval<D>(c, 1.0)));
+                return 1;
+            }
+        }
         double d[D];
 #pragma unroll
         for (int i = 0; i < D; i++) d[i] = c[i] * (double)(D - i);
         double cr[D - 1] = {};
-        int nc = RootIsolator<D - 1>::roots01(d, cr);
+        int nc = RootIsolator<D - 1, PLAIN>::roots01(d, cr);
         int nr = 0;
         double a = 0.0;
         double fa = peval<D>(c, 0.0);
@@ -364,7 +536,7 @@ struct RootIsolator {
                             last = b;
                         }
                     } else if (fa != 0.0 && ((fa < 0) != (fb < 0))) {
-                        last = bracketed_root<D>(c, a, b, fa, fb);
+                        last = bracketed_root<D, PLAIN>(c, a, b, fa, fb);
                         put(r, nr++, last);
                     }
                     a = b;
@@ -375,8 +547,8 @@ struct RootIsolator {
         return nr;
     }
 };
-template <>
-struct RootIsolator<1> {
+template <bool PLAIN>
+struct RootIsolator<1, PLAIN> {
     CCD_HD static int roots01(const double *c, double (&r)[1])
     {
         double x = -c[1] / c[0];
@@ -387,8 +559,8 @@ struct RootIsolator<1> {
         return 0;
     }
 };
-template <>
-struct RootIsolator<2> {
+template <bool PLAIN>
+struct RootIsolator<2, PLAIN> {
     CCD_HD static int roots01(const double *c, double (&r)[2])
     {
         double a = c[0], b = c[1], cc = c[2];
@@ -425,25 +597,25 @@ struct RootIsolator<2> {
 };
 
 // Roots of the degree-D polynomial op in [0,1] into time[0..), D <= M.
-template <int D, int M>
+template <int D, int M, bool PLAIN = false>
 CCD_HD int isolate_into(const double *op, double (&time)[M])
 {
     double r[D] = {};
-    int n = RootIsolator<D>::roots01(op, r);
+    int n = RootIsolator<D, PLAIN>::roots01(op, r);
 #pragma unroll
     for (int k = 0; k < D && k < M; k++) time[k] = r[k];
     return n;
 }
 
 // Real roots in [0,1] of op[0..degree] (3 <= degree <= N) by isolation.
-template <int N, int M>
+template <int N, int M, bool PLAIN = false>
 CCD_FHD int isolate_roots(const double *op, int degree, double (&time)[M])
 {
     int roots = 0;
-    if (degree == 3) roots = isolate_into<3>(op, time);
-    if (N >= 4 && degree == 4) roots = isolate_into<(N >= 4 ? 4 : 3)>(op, time);
-    if (N >= 5 && degree == 5) roots = isolate_into<(N >= 5 ? 5 : 3)>(op, time);
-    if (N >= 6 && degree == 6) roots = isolate_into<(N >= 6 ? 6 : 3)>(op, time);
+    if (degree == 3) roots = isolate_into<3, M, PLAIN>(op, time);
+    if (N >= 4 && degree == 4) roots = isolate_into<(N >= 4 ? 4 : 3), M, PLAIN>(op, time);
+    if (N >= 5 && degree == 5) roots = isolate_into<(N >= 5 ? 5 : 3), M, PLAIN>(op, time);
+    if (N >= 6 && degree == 6) roots = isolate_into<(N >= 6 ? 6 : 3), M, PLAIN>(op, time);
     return roots;
 }
 
@@ -489,7 +661,7 @@ CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos, C &&cnt = C())
         if (!could_have_roots<N>(op, reduced, pos)) return;
         cnt(S_ISOLATE);
         if constexpr (stops_at_solver<CT>::value) return;
-        roots = isolate_roots<N>(op, reduced, time);
+        roots = isolate_roots<N, (N > 2 ? N : 2), plain_roots<CT>::value>(op, reduced, time);
         // Pieces between consecutive roots (all in [0,1]) and the ends, as CTCD.cpp:160-176.
         if (roots == 0) {
             check_interval<N>(0.0, 1.0, op, reduced, out, pos);

@@ -185,8 +185,13 @@ __global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue(const double
     }
 }
 
-// Work accounting: the same tests with a per-lane stage counter, reduced per wavefront.
+// Work accounting: the same tests with a per-lane stage counter, reduced per wavefront.  Its root
+// isolation is the plain form (no Bernstein shortcuts, plain Newton): with the product's form this
+// kernel stalled once on the device, on the converging-spheres candidates, for no cause found in the
+// code, its assembly or sanitized host runs of the same path.  Which stages run depends only on the
+// interval structure, which both forms give alike.
 struct LaneCount {
+    static constexpr bool plain_roots = true;
     uint32_t n[ccd::S_COUNT];
     __device__ __forceinline__ void operator()(int s) { n[s]++; }
 };
