@@ -18,6 +18,7 @@ CCD_E_OOM = 3
 CCD_E_HIP = 4
 CCD_E_BUDGET = 5
 CCD_E_NODEVICE = 6
+NOALLOC_MAX_STENCILS = 1 << 20   # CCD_NOALLOC_MAX_STENCILS
 
 PRIM_VERTEX_FACE = 0
 PRIM_EDGE_EDGE = 1
@@ -47,6 +48,8 @@ SIGNATURES = {
     "ccd_device_bytes": ([_V], _I64),
     "ccd_ctcd_primitives": ([_V, _INT, _V, _V, _D, _I64, _V, _V], _INT),
     "ccd_ctcd_narrowphase": ([_V, _V, _V, _I64, _V, _V, _I64, _V, _V, _I64, _D, _V, _V, _V, _V, _V, _V], _INT),
+    "ccd_ctcd_narrowphase_noalloc": ([_V, _V, _V, _I64, _V, _V, _I64, _V, _V, _I64, _D, _V, _V, _V, _V, _V, _V],
+                                     _INT),
     "ccd_separating_plane_narrowphase": ([_V, _V, _V, _V, _I64, _I64, _V, _V, _I64, _V, _V, _I64, _D, _V, _V, _P64],
                                          _INT),
     "ccd_ctcd_narrowphase_history": ([_V, _V, _V, _V, _I64, _I64, _V, _V, _I64, _V, _V, _I64, _D, _V, _V, _V, _V,

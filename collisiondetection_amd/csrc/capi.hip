@@ -1,10 +1,12 @@
 // C ABI of libccd_amd.so (declarations and contracts in include/ccd_amd.h).
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
 #include "ccd_core.h"
+
 #include "ccd_internal.h"
 
 namespace ccdi {
@@ -138,6 +140,42 @@ static int validate_history(ccd_ctx *ctx, const int64_t *off, const double *ht, 
 
 extern "C" {
 
+// ------------------------------------------------------------------------------------------------
+// Device scratch of ccd_ctcd_narrowphase_noalloc: a file-scope device array, one copy per GPU, held
+// until the process ends.  Host code never passes the array's name to a kernel (on the host it is
+// only a stand-in for the symbol): it looks up each device's own address once, at set-up, and passes
+// that.  Layout for up to CCD_NOALLOC_MAX_STENCILS stencils (one stencil kind at a time):
+//   [0, 256)    the invalid-key counter (uint64)
+//   then sig, sig2 (uint64 per stencil), idx, idx2 (int32 per stencil), the radix sort's temporary.
+// ------------------------------------------------------------------------------------------------
+namespace {
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)26 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[MAX_DEVICES] = {};
+
+// The current device's address of `scratch` (the current device must be `device`).
+int scratch_address(int device, unsigned char **out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= MAX_DEVICES) return CCD_E_INVALID;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return CCD_E_HIP;
+        }
+        g_scratch_ptr[device] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return CCD_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+}   // namespace
+
 int ccd_create(ccd_ctx **out, int device)
 {
     if (!out) return CCD_E_INVALID;
@@ -166,6 +204,15 @@ int ccd_create(ccd_ctx **out, int device)
         return CCD_E_OOM;
     }
     ctx->device_bytes = sizeof(int64_t) * NCOUNTERS;
+    // Set-up for the allocation-free entry point: this device's address of its scratch array.
+    unsigned char *scr = nullptr;
+    if (scratch_address(device, &scr) != CCD_OK) {
+        (void)hipFree(ctx->d_counts);
+        (void)hipHostFree(ctx->h_counts);
+        (void)hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+        return CCD_E_HIP;
+    }
     *out = ctx;
     return CCD_OK;
 }
@@ -246,6 +293,59 @@ int ccd_ctcd_narrowphase(ccd_ctx *ctx, const double *qstart, const double *qend,
         prof_collect(ctx);
     }
     return CCD_OK;
+}
+
+int ccd_ctcd_narrowphase_noalloc(ccd_ctx *ctx, const double *qstart, const double *qend, int64_t nverts,
+                                 const int32_t *vf_keys, const double *vf_eta, int64_t nvf, const int32_t *ee_keys,
+                                 const double *ee_eta, int64_t nee, double eta_scalar, uint8_t *vf_hit, double *vf_t,
+                                 uint8_t *vf_sub, uint8_t *ee_hit, double *ee_t, uint8_t *ee_sub)
+{
+    if (!ctx) return CCD_E_INVALID;
+    CCD_REQUIRE(ctx, nverts >= 0 && nvf >= 0 && nee >= 0, "negative size");
+    if (nvf > CCD_NOALLOC_MAX_STENCILS || nee > CCD_NOALLOC_MAX_STENCILS)
+        return set_error(ctx, CCD_E_CAPACITY,
+                         "ccd_ctcd_narrowphase_noalloc takes at most %lld stencils per kind (got %lld vertex-face, "
+                         "%lld edge-edge)",
+                         (long long)CCD_NOALLOC_MAX_STENCILS, (long long)nvf, (long long)nee);
+    CCD_REQUIRE(ctx, (nvf == 0 && nee == 0) || (qstart && qend), "null positions");
+    CCD_REQUIRE(ctx, nvf == 0 || (vf_keys && vf_hit), "null vertex-face keys or outputs");
+    CCD_REQUIRE(ctx, nee == 0 || (ee_keys && ee_hit), "null edge-edge keys or outputs");
+    CCD_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (nvf == 0 && nee == 0) return CCD_OK;
+    unsigned char *base = nullptr;
+    if (scratch_address(ctx->device, &base) != CCD_OK)
+        return set_error(ctx, CCD_E_HIP, "no address for the device scratch array");
+    const size_t n = (size_t)CCD_NOALLOC_MAX_STENCILS;
+    OrderScratch os;
+    size_t o = 256;
+    os.sig = (uint64_t *)(base + o);
+    o = align256(o + 8 * n);
+    os.sig2 = (uint64_t *)(base + o);
+    o = align256(o + 8 * n);
+    os.idx = (int32_t *)(base + o);
+    o = align256(o + 4 * n);
+    os.idx2 = (int32_t *)(base + o);
+    o = align256(o + 4 * n);
+    os.tmp = base + o;
+    o += radix_sort_temp_bytes((int64_t)n);
+    if (o > NOALLOC_SCRATCH_BYTES) return set_error(ctx, CCD_E_INVALID, "scratch layout exceeds the array");
+    // Key validation into the scratch counter (cleared by a fill kernel; the array keeps the last
+    // call's contents), read back through the context's pinned mailbox.
+    int64_t *bad = (int64_t *)base;
+    int rc = fill_u64(ctx, (uint64_t *)bad, 1, 0);
+    if (!rc) rc = launch_validate_keys(ctx, vf_keys, 4 * nvf, nverts, bad);
+    if (!rc) rc = launch_validate_keys(ctx, ee_keys, 4 * nee, nverts, bad);
+    if (rc) return rc;
+    CCD_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_counts + C_BAD, bad, 8, hipMemcpyDeviceToHost, ctx->stream));
+    CCD_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->h_counts[C_BAD] != 0)
+        return set_error(ctx, CCD_E_INVALID, "%lld stencil keys reference vertices outside [0, nverts)",
+                         (long long)ctx->h_counts[C_BAD]);
+    // Both kinds reuse the same ordering buffers: the edge-edge launches follow the vertex-face ones on
+    // the stream.
+    rc = launch_narrow2(ctx, true, qstart, qend, vf_keys, vf_eta, eta_scalar, nvf, vf_hit, vf_t, vf_sub, &os);
+    if (rc) return rc;
+    return launch_narrow2(ctx, false, qstart, qend, ee_keys, ee_eta, eta_scalar, nee, ee_hit, ee_t, ee_sub, &os);
 }
 
 int ccd_ctcd_narrowphase_history(ccd_ctx *ctx, const int64_t *hist_off, const double *hist_t,

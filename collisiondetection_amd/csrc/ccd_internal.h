@@ -96,8 +96,16 @@ void *buffer(ccd_ctx *ctx, const char *name, size_t bytes);
 namespace ccdi {
 int launch_primitives(ccd_ctx *ctx, int kind, const double *q, const double *eta, double eta_s, int64_t n,
                       uint8_t *hit, double *t);
+// Buffers for ordering n stencils by signature before the narrowphase: sig, sig2 (n uint64), idx, idx2
+// (n int32) and tmp (radix_sort_temp_bytes(n) bytes).
+struct OrderScratch {
+    uint64_t *sig = nullptr, *sig2 = nullptr;
+    int32_t *idx = nullptr, *idx2 = nullptr;
+    void *tmp = nullptr;
+};
 int launch_narrow2(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
-                   const double *eta, double eta_s, int64_t n, uint8_t *hit, double *t, uint8_t *sub);
+                   const double *eta, double eta_s, int64_t n, uint8_t *hit, double *t, uint8_t *sub,
+                   const OrderScratch *order_scratch = nullptr);
 // wave: one wavefront per stencil sharing the stitched windows (for long histories), else one lane.
 int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *ht, const double *hp,
                        const int32_t *keys, const double *eta, double eta_s, int64_t n, uint8_t *hit, double *t,

@@ -635,19 +635,29 @@ int launch_primitives(ccd_ctx *ctx, int kind, const double *q, const double *eta
 }
 
 int launch_narrow2(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
-                   const double *eta, double eta_s, int64_t n, uint8_t *hit, double *t, uint8_t *sub)
+                   const double *eta, double eta_s, int64_t n, uint8_t *hit, double *t, uint8_t *sub,
+                   const OrderScratch *order_scratch)
 {
     if (n == 0) return CCD_OK;
     // Run the stencils in the grouping order of the detection pipeline (k_stencil_signature); each
-    // result lands at its stencil's own index, so only the speed depends on the order.
+    // result lands at its stencil's own index, so only the speed depends on the order.  The ordering
+    // buffers are the caller's (order_scratch) or the context's own.
     const int32_t *order = nullptr;
     // (the order holds int32 stencil indices: launches past 2^31 - 1 stencils run in the given order)
     if (n >= 4096 && n <= 0x7fffffffll && ctx->group_stencils) {
-        uint64_t *sig = (uint64_t *)buffer(ctx, "n2o_sig", (size_t)n * 8);
-        uint64_t *sig2 = (uint64_t *)buffer(ctx, "n2o_sig2", (size_t)n * 8);
-        int32_t *idx = (int32_t *)buffer(ctx, "n2o_idx", (size_t)n * 4);
-        int32_t *idx2 = (int32_t *)buffer(ctx, "n2o_idx2", (size_t)n * 4);
-        void *tmp = buffer(ctx, "n2o_tmp", radix_sort_temp_bytes(n));
+        OrderScratch os;
+        if (order_scratch) {
+            os = *order_scratch;
+        } else {
+            os.sig = (uint64_t *)buffer(ctx, "n2o_sig", (size_t)n * 8);
+            os.sig2 = (uint64_t *)buffer(ctx, "n2o_sig2", (size_t)n * 8);
+            os.idx = (int32_t *)buffer(ctx, "n2o_idx", (size_t)n * 4);
+            os.idx2 = (int32_t *)buffer(ctx, "n2o_idx2", (size_t)n * 4);
+            os.tmp = buffer(ctx, "n2o_tmp", radix_sort_temp_bytes(n));
+        }
+        uint64_t *sig = os.sig, *sig2 = os.sig2;
+        int32_t *idx = os.idx, *idx2 = os.idx2;
+        void *tmp = os.tmp;
         if (!sig || !sig2 || !idx || !idx2 || !tmp) return CCD_E_OOM;
         if (vf)
             hipLaunchKernelGGL(k_stencil_signature<true>, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,

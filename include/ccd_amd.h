@@ -103,6 +103,22 @@ int ccd_ctcd_narrowphase(ccd_ctx *ctx, const double *qstart, const double *qend,
                          uint8_t *vf_sub, uint8_t *ee_hit, double *ee_t, uint8_t *ee_sub);
 
 /*
+ * ccd_ctcd_narrowphase_noalloc: ccd_ctcd_narrowphase (same arguments, results and errors) without any
+ * device allocation.  No counterpart in the reference: it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its working memory
+ * (stencil ordering buffers, a validation counter) is one file-scope device array of 26 MiB per GPU,
+ * held until the process ends, whose address ccd_create looks up once per device.  At most
+ * CCD_NOALLOC_MAX_STENCILS stencils per kind and call: larger calls return CCD_E_CAPACITY.  Every call
+ * on one GPU shares that GPU's array, so calls that overlap on one GPU (other threads, other streams)
+ * race; serialise them.
+ */
+#define CCD_NOALLOC_MAX_STENCILS ((int64_t)1 << 20)
+int ccd_ctcd_narrowphase_noalloc(ccd_ctx *ctx, const double *qstart, const double *qend, int64_t nverts,
+                                 const int32_t *vf_keys, const double *vf_eta, int64_t nvf, const int32_t *ee_keys,
+                                 const double *ee_eta, int64_t nee, double eta_scalar, uint8_t *vf_hit, double *vf_t,
+                                 uint8_t *vf_sub, uint8_t *ee_hit, double *ee_t, uint8_t *ee_sub);
+
+/*
  * Narrowphase over multi-entry histories -- the same as ccd_ctcd_narrowphase for histories with any
  * number of entries per vertex (History::addHistory, History.cpp:21-26), stitched per stencil as
  * History::stitchCommonHistory (History.cpp:98-140) and tested window by window.  The history is CSR
