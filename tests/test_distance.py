@@ -114,16 +114,40 @@ def test_gpu_distance_primitives(core_harness, kind):
         else:
             assert np.array_equal(vec[i].view(np.int64), a[0].view(np.int64))
             assert np.array_equal(bary[i].view(np.int64), a[1].view(np.int64))
+            # and the oracle's independent restatement of Distance.h, bit for bit
+            ov, ob = oracle.distance(kind, q[i], eta)
+            assert np.array_equal(vec[i].view(np.int64), np.asarray(ov, np.float64).view(np.int64)), i
+            nb = 3 if kind == 2 else 4
+            assert np.array_equal(bary[i][:nb].view(np.int64), np.asarray(ob, np.float64)[:nb].view(np.int64)), i
+
+
+def _duplicated_vertices():
+    """A cloth whose every 9th vertex has a twin at the same position (its closest pair is 0 apart)."""
+    s = dict(scenes.cloth(24))
+    q = s["qstart"]
+    dup = np.arange(0, len(q), 9)
+    faces = np.concatenate([s["faces"], np.stack([len(q) + np.arange(len(dup)), (dup + 1) % len(q),
+                                                  (dup + 2) % len(q)], axis=1)]).astype(np.int32)
+    s.update(qstart=np.concatenate([q, q[dup]]), faces=faces)
+    return s
+
+
+def _loose_triangles():
+    """Triangles that share no vertex: no face pair of their own vertices qualifies (all-pairs sweep)."""
+    rng = np.random.default_rng(13)
+    q = rng.uniform(-1, 1, size=(3 * 40, 3))
+    return {"qstart": q, "faces": np.arange(3 * 40, dtype=np.int32).reshape(-1, 3)}
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("scene", ["two_spheres", "converging", "cloth64"])
+@pytest.mark.parametrize("scene", ["two_spheres", "converging", "cloth64", "cloth128", "duplicated", "loose"])
 def test_gpu_mesh_self_distance(scene):
     from collisiondetection_amd import api
 
     print("case: mesh self distance", scene, flush=True)
     s = {"two_spheres": scenes.two_spheres, "converging": scenes.converging_spheres,
-         "cloth64": lambda: scenes.cloth(64)}[scene]()
+         "cloth64": lambda: scenes.cloth(64), "cloth128": lambda: scenes.cloth(128),
+         "duplicated": _duplicated_vertices, "loose": _loose_triangles}[scene]()
     for fixed in (None, set(range(0, len(s["qstart"]), 3))):
         d, nvf, nee = oracle.mesh_self_distance(s["qstart"], s["faces"], fixed=fixed)
         g, gvf, gee = api.mesh_self_distance(s["qstart"], s["faces"], fixed=fixed)
