@@ -9,6 +9,7 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libccd_amd.so")
+LOADGEN = os.path.join(PKG, "serve_loadgen")
 OBJDIR = os.path.join(PKG, "_build")
 SOURCES = ["narrowphase.hip", "broadphase.hip", "distance.hip", "devprims.hip", "server.hip", "capi.hip",
            "velocity_filter.hip"]
@@ -40,8 +41,16 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
+    # The serving load generator: a host program over the C ABI (g++, links the library beside it).
+    tmp = LOADGEN + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-pthread", "-o", tmp, os.path.join(PKG, "loadgen", "serve_loadgen.cpp"),
+           "-L" + PKG, "-lccd_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"load generator build failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, LOADGEN)
     if verbose:
-        print(f"built {LIB}", file=sys.stderr)
+        print(f"built {LIB} and {LOADGEN}", file=sys.stderr)
     return LIB
 
 
