@@ -87,7 +87,8 @@ class RequestResult(ctypes.Structure):
     _fields_ = [("request_id", ctypes.c_int64), ("session", ctypes.c_int32), ("status", ctypes.c_int32),
                 ("vf_hits", ctypes.c_int64), ("ee_hits", ctypes.c_int64), ("vf_candidates", ctypes.c_int64),
                 ("ee_candidates", ctypes.c_int64), ("checksum_vf", ctypes.c_uint64), ("checksum_ee", ctypes.c_uint64),
-                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double)]
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("reruns", ctypes.c_int32), ("reserved", ctypes.c_int32), ("face_pairs", ctypes.c_int64)]
 
 
 class VFStats(ctypes.Structure):

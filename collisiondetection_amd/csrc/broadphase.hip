@@ -15,12 +15,13 @@
 //   6. stackless traversal, one lane per leaf i, starting at leaf i's escape link, which visits
 //      exactly the subtrees holding leaves after i: every unordered face pair is found once.  The FP32
 //      test culls; the leaf-leaf decision is the reference's FP64 13-axis test (touching overlaps)
-//      followed by the shared-vertex test (Mesh.cpp:13-20).  Pairs are appended with one atomic per
-//      wavefront (ballot + popcount prefix);
+//      followed by the shared-vertex test (Mesh.cpp:13-20).  Pairs are queued per lane in registers
+//      and written in wavefront-wide flushes (one atomic each);
 //   7. expansion of each face pair into its 6 vertex-face and 9 edge-edge stencils
 //      (KDOPBroadPhase.cpp:150-184) with owner-computes deduplication: a stencil is written only by
-//      the smallest face pair that generates it, which each lane decides from the vertex->face table
-//      and the leaf bounds, so the unique set comes out with no global sort.
+//      the smallest face pair that generates it, which is decided from the face -> related faces table
+//      built from the traversal's pairs (rows sorted; one wavefront per face) and the vertex->face
+//      table, so the unique set comes out with no global sort.
 // Meshes with degenerate faces (a repeated vertex) or duplicate faces break the owner rule's
 // assumptions; for them the raw stencils are emitted and deduplicated by sorting.
 #include <cstring>
@@ -424,6 +425,82 @@ __global__ __launch_bounds__(256) void k_refit(int64_t F, const uint32_t *__rest
     }
 }
 
+// Level-synchronous refit for large trees.  The arrival-counter refit above needs a device-wide fence
+// per node, which writes back and invalidates the L2 of the lane's XCD; with millions of nodes those
+// fences serialise.  Instead each internal node's depth is found once per build by pointer jumping
+// over the parent links (8 rounds: exact up to depth 256, and a Karras tree over 63-bit keys with an
+// index tie-break is at most 97 deep, one level per distinct split bit), and the refit then runs one
+// launch per depth, deepest first: a kernel boundary orders each level's writes before the next
+// level's reads, with no fence inside.  Launches for depths past the tree's maximum return at once.
+constexpr int MAX_TREE_DEPTH = 128;
+constexpr int DEPTH_ROUNDS = 8;
+constexpr int64_t LEVEL_REFIT_MIN_FACES = 1 << 16;
+
+__global__ __launch_bounds__(256) void k_depth_init(int64_t F, const uint32_t *__restrict__ parent,
+                                                    uint32_t *__restrict__ jump, uint32_t *__restrict__ dist,
+                                                    uint32_t *__restrict__ maxd)
+{
+    int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (n == 0) *maxd = 0;
+    if (n >= F - 1) return;
+    jump[n] = n == 0 ? NIL : parent[n];
+    dist[n] = n == 0 ? 0u : 1u;
+}
+
+__global__ __launch_bounds__(256) void k_depth_jump(int64_t F, const uint32_t *__restrict__ jin,
+                                                    const uint32_t *__restrict__ din, uint32_t *__restrict__ jout,
+                                                    uint32_t *__restrict__ dout)
+{
+    int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (n >= F - 1) return;
+    const uint32_t j = jin[n];
+    if (j == NIL) {
+        jout[n] = NIL;
+        dout[n] = din[n];
+    } else {
+        jout[n] = jin[j];
+        dout[n] = din[n] + din[j];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_depth_max(int64_t F, const uint32_t *__restrict__ depth,
+                                                   uint32_t *__restrict__ maxd)
+{
+    uint32_t m = 0;
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < F - 1; n += (int64_t)gridDim.x * blockDim.x)
+        m = depth[n] > m ? depth[n] : m;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        const uint32_t o = __shfl_xor(m, d);
+        m = o > m ? o : m;
+    }
+    if (__lane_id() == 0) atomicMax(maxd, m);
+}
+
+__global__ __launch_bounds__(256) void k_refit_level(int64_t F, uint32_t level, const uint32_t *__restrict__ depth,
+                                                     const uint32_t *__restrict__ maxd,
+                                                     const uint32_t *__restrict__ left,
+                                                     const uint32_t *__restrict__ escape, NodeRec *__restrict__ nodes)
+{
+    if (level > *maxd) return;
+    for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < F - 1; n += (int64_t)gridDim.x * blockDim.x) {
+        if (depth[n] != level) continue;
+        const uint32_t lc = left[n];
+        const uint32_t rc = escape[lc];   // the node after the left child's subtree
+        const NodeRec a = nodes[lc];
+        const NodeRec b = nodes[rc];
+        NodeRec r;
+#pragma unroll
+        for (int k = 0; k < KAX; k++) {
+            r.b[k] = a.b[k] < b.b[k] ? a.b[k] : b.b[k];
+            r.b[KAX + k] = a.b[KAX + k] > b.b[KAX + k] ? a.b[KAX + k] : b.b[KAX + k];
+        }
+        r.child = lc;
+        r.esc = escape[n];
+        nodes[n] = r;
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // 6. Traversal: unordered overlapping non-neighbouring face pairs.
 // ------------------------------------------------------------------------------------------------
@@ -452,6 +529,36 @@ __device__ __forceinline__ bool neighbours(int4 a, int4 b)   // Mesh.cpp:13-20
 // sorted range [inst_off[k], inst_off[k+1])), else null (one instance).  An escape link always leads
 // to a node whose first leaf is its index (internal) or its leaf number: once that reaches inst_end
 // the walk has left the instance's subtree and stops.
+// Pairs are queued per lane (QN face ids in registers) and written when some lane's queue is full or
+// the traversal ends: one atomic per wavefront and flush, positions by a wavefront prefix sum.  A
+// single counter claimed by every wavefront on every iteration that emits serialises the device.
+// The query's FP64 bounds are read from memory only for the leaves that pass the FP32 test, which
+// keeps the lane's registers to its FP32 bounds and its queue.
+constexpr int TRAVERSE_QN = 8;
+__device__ __forceinline__ void traverse_flush(int me, const int (&q)[TRAVERSE_QN], int n, int2 *__restrict__ pairs,
+                                               int64_t cap, unsigned long long *counter)
+{
+    const int lane = __lane_id();
+    int incl = n;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    const int total = __shfl(incl, 63);
+    unsigned long long base = 0;
+    if (lane == 0 && total) base = atomicAdd(counter, (unsigned long long)total);
+    base = __shfl(base, 0);
+    const int64_t pos = (int64_t)base + (incl - n);
+#pragma unroll
+    for (int k = 0; k < TRAVERSE_QN; k++) {
+        if (k < n && pos + k < cap) {
+            const int b = q[k];
+            pairs[pos + k] = me < b ? make_int2(me, b) : make_int2(b, me);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__restrict__ nodes,
                                                   const double *__restrict__ sdop64,
                                                   const int4 *__restrict__ sfaces, const int32_t *__restrict__ face_inst,
@@ -461,28 +568,27 @@ __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__re
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     bool active = i < F;
     float q32[NB];
-    double q64[NB];
+    const double *q64 = sdop64 + NB * (active ? i : 0);
     int4 qf = make_int4(0, 0, 0, 0);
     uint32_t node = NIL;
     int64_t iend = F;
     if (active) {
         const NodeRec &me = nodes[F - 1 + i];
 #pragma unroll
-        for (int k = 0; k < NB; k++) {
-            q32[k] = me.b[k];
-            q64[k] = sdop64[NB * i + k];
-        }
+        for (int k = 0; k < NB; k++) q32[k] = me.b[k];
         qf = sfaces[i];
         node = me.esc;
         if (face_inst) iend = inst_off[face_inst[qf.w] + 1];
+    } else {
+#pragma unroll
+        for (int k = 0; k < NB; k++) q32[k] = 0.0f;
     }
     const uint32_t nint = (uint32_t)(F - 1);
     auto first_leaf = [nint](uint32_t n) -> int64_t { return n < nint ? (int64_t)n : (int64_t)(n - nint); };
     if (node != NIL && first_leaf(node) >= iend) node = NIL;
-    const int lane = __lane_id();
+    int q[TRAVERSE_QN] = {};
+    int nq = 0;
     while (true) {
-        bool emit = false;
-        int other = 0;
         if (node != NIL) {
             const NodeRec &n = nodes[node];
             float b[NB];
@@ -496,8 +602,10 @@ __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__re
                 } else {
                     int4 of = sfaces[child];
                     if (overlap64(q64, sdop64 + NB * (int64_t)child) && !neighbours(qf, of)) {
-                        emit = true;
-                        other = of.w;
+#pragma unroll
+                        for (int k = 0; k < TRAVERSE_QN; k++)
+                            if (k == nq) q[k] = of.w;
+                        nq++;
                     }
                 }
             }
@@ -508,22 +616,12 @@ __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__re
                 if (node != NIL && first_leaf(node) >= iend) node = NIL;
             }
         }
-        uint64_t m = __ballot(emit);
-        if (m) {
-            int leader = __ffsll((unsigned long long)m) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
-            base = __shfl(base, leader);
-            if (emit) {
-                uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
-                int64_t pos = (int64_t)base + __popcll(below);
-                if (pos < cap) {
-                    int a = qf.w, b = other;
-                    pairs[pos] = a < b ? make_int2(a, b) : make_int2(b, a);
-                }
-            }
+        const bool more = __ballot(node != NIL) != 0;
+        if (__ballot(nq == TRAVERSE_QN) || !more) {
+            traverse_flush(qf.w, q, nq, pairs, cap, counter);
+            nq = 0;
         }
-        if (__ballot(node != NIL) == 0) break;
+        if (!more) break;
     }
 }
 
@@ -605,18 +703,13 @@ struct MeshView {
     const int32_t *vf_list;
     const double *dop64;   // by original face id
     const uint8_t *fixed;
-    const int32_t *adj_off = nullptr;   // [F + 1], or null
+    const int32_t *adj_off = nullptr;   // [F + 1] related-face table rows, or null (no table)
     const int32_t *adj = nullptr;       // [2 * pairs]
-    bool use_adj = false;               // set per launch from the device pair count
     const int32_t *opp = nullptr;       // [3F] edge -> opposite face (k_edge_opp), or null
     const int32_t *rank = nullptr;      // [F] face id -> Morton-sorted position
     const double *sdop64 = nullptr;     // FP64 bounds in Morton order
     const int4 *sfaces = nullptr;       // vertex triples and face id in Morton order
 };
-
-// The related-face table is used while the pairs average at most this many per face.
-constexpr int64_t ADJ_MAX_AVG = 40;
-__device__ __forceinline__ bool adj_wanted(int64_t np, int64_t F) { return 2 * np <= ADJ_MAX_AVG * F; }
 
 __global__ __launch_bounds__(256) void k_adj_count(const int2 *__restrict__ pairs,
                                                    const unsigned long long *__restrict__ np_dev, int64_t cap,
@@ -624,7 +717,6 @@ __global__ __launch_bounds__(256) void k_adj_count(const int2 *__restrict__ pair
 {
     int64_t np = (int64_t)*np_dev;
     if (np > cap) np = cap;
-    if (!adj_wanted(np, F)) return;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
         int2 p = pairs[i];
         atomicAdd(&cnt[p.x], 1);
@@ -640,7 +732,6 @@ __global__ __launch_bounds__(256) void k_adj_fill(const int2 *__restrict__ pairs
 {
     int64_t np = (int64_t)*np_dev;
     if (np > cap) np = cap;
-    if (!adj_wanted(np, F)) return;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
         int2 p = pairs[i];
         adj[off[p.x] + atomicSub(&cnt[p.x], 1) - 1] = p.y;
@@ -652,16 +743,32 @@ __device__ __forceinline__ int4 face3(const MeshView &m, int f)
 {
     return make_int4(m.faces[3 * f], m.faces[3 * f + 1], m.faces[3 * f + 2], f);
 }
+// Rows of the related-face table up to this length are sorted (k_row_sort); longer ones stay in fill
+// order and are searched linearly.
+constexpr int ROW_SORT_CAP = 2048;
+
 // Related faces: overlapping swept k-DOPs and no shared vertex, i.e. exactly the face pairs the
-// traversal found.  When the mesh's pairs are few per face, the traversal's output is turned into a
-// face -> related faces table (adj_off / adj, rows unsorted) and `related` scans a row instead of
-// re-reading two 208-byte FP64 k-DOPs; otherwise (use_adj false) it repeats the FP64 test.
+// traversal found.  ROWS: b is in a's row of the related-face table (binary search in the sorted row);
+// otherwise the FP64 test is repeated on the two faces' bounds (when there is no table).
+template <bool ROWS>
 __device__ __forceinline__ bool related(const MeshView &m, int a, int b)
 {
-    if (m.use_adj) {
-        for (int k = m.adj_off[a]; k < m.adj_off[a + 1]; k++)
-            if (m.adj[k] == b) return true;
-        return false;
+    if (ROWS) {
+        const int end = m.adj_off[a + 1];
+        int lo = m.adj_off[a], hi = end;
+        if (hi - lo > ROW_SORT_CAP) {
+            for (int k = lo; k < hi; k++)
+                if (m.adj[k] == b) return true;
+            return false;
+        }
+        while (lo < hi) {   // first entry not below b
+            const int mid = (lo + hi) >> 1;
+            if (m.adj[mid] < b)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        return lo < end && m.adj[lo] == b;
     }
     if (m.rank) {   // the Morton-ordered copies: spatially close faces share cache lines
         int ra = m.rank[a], rb = m.rank[b];
@@ -678,17 +785,19 @@ __device__ __forceinline__ bool is_fixed(const MeshView &m, int v) { return m.fi
 
 // VF stencil (p, face h) generated by pair {a, h} for every face a containing p with a related to h;
 // the owner is the smallest such a.  `mine` is the a of the current pair.
+template <bool ROWS>
 __device__ __forceinline__ bool vf_owner(const MeshView &m, int p, int h, int mine)
 {
     for (int k = m.vf_off[p]; k < m.vf_off[p + 1]; k++) {
         int a = m.vf_list[k];
-        if (a < mine && related(m, a, h)) return false;
+        if (a < mine && related<ROWS>(m, a, h)) return false;
     }
     return true;
 }
 
 // EE stencil (edge (u0,u1), edge (w0,w1)) generated by pairs {a, b}, a containing both u's and b both
 // w's, a related to b; the owner is the smallest (min(a,b), max(a,b)).  The current pair is (f < g).
+template <bool ROWS>
 __device__ __forceinline__ bool ee_owner(const MeshView &m, int u0, int u1, int w0, int w1, int f, int g)
 {
     for (int ka = m.vf_off[u0]; ka < m.vf_off[u0 + 1]; ka++) {
@@ -699,7 +808,7 @@ __device__ __forceinline__ bool ee_owner(const MeshView &m, int u0, int u1, int 
             if (!has_vertex(m, b, w1)) continue;
             int lo = a < b ? a : b, hi = a < b ? b : a;
             if (lo < f || (lo == f && hi < g)) {
-                if (related(m, a, b)) return false;
+                if (related<ROWS>(m, a, b)) return false;
             }
         }
     }
@@ -713,11 +822,12 @@ __device__ __forceinline__ bool precedes(int a, int b, int f, int g)
     int lo = a < b ? a : b, hi = a < b ? b : a;
     return lo < f || (lo == f && hi < g);
 }
+template <bool ROWS>
 __device__ __forceinline__ bool ee_owner_opp(const MeshView &m, int oa, int ob, int f, int g)
 {
-    if (ob >= 0 && precedes(f, ob, f, g) && related(m, f, ob)) return false;
-    if (oa >= 0 && precedes(oa, g, f, g) && related(m, oa, g)) return false;
-    if (oa >= 0 && ob >= 0 && precedes(oa, ob, f, g) && related(m, oa, ob)) return false;
+    if (ob >= 0 && precedes(f, ob, f, g) && related<ROWS>(m, f, ob)) return false;
+    if (oa >= 0 && precedes(oa, g, f, g) && related<ROWS>(m, oa, g)) return false;
+    if (oa >= 0 && ob >= 0 && precedes(oa, ob, f, g) && related<ROWS>(m, oa, ob)) return false;
     return true;
 }
 
@@ -736,21 +846,18 @@ __device__ __forceinline__ int4 ee_key(int a, int b, int c, int d)   // Stencils
     return make_int4(p0, p1, q0, q1);
 }
 
-// raw_dev: the mesh check's count of degenerate or duplicate faces; nonzero switches every lane to raw
-// emission (all 15 stencils of a pair), deduplicated afterwards by sorting.
+// Per-pair expansion, for the two cases the row expansion below does not cover: raw emission and meshes
+// whose related-face table does not fit int32 offsets (rows_path false; the owner tests then repeat the
+// FP64 bounds test).  raw_dev: the mesh check's count of degenerate or duplicate faces; nonzero
+// switches every lane to raw emission (all 15 stencils of a pair), deduplicated afterwards by sorting.
 //
 // Output slots are claimed per block, not per wavefront: each lane first decides which of its 15
 // stencils it emits (two bit masks), the block scans the counts through LDS and one lane claims the
 // block's range with a single atomic per stencil kind.  The counters are single words that every block
 // hits, so per-wavefront claims (15 per wavefront and pass) made their serialisation the bottleneck.
 // The grid-stride loop bound is uniform across a block, so every lane reaches the barriers.
-//
-// ADJ: the instance for the related-face table (true) or for the FP64 re-test (false); both are
-// launched and the one that does not match the device-side choice returns at once, so each carries
-// only its own test.
 constexpr int EXPAND_BLOCK = 256;
-template <bool ADJ>
-__global__ __launch_bounds__(EXPAND_BLOCK) void k_expand(MeshView m, int64_t F, const int2 *__restrict__ pairs,
+__global__ __launch_bounds__(EXPAND_BLOCK) void k_expand(MeshView m, bool rows_path, const int2 *__restrict__ pairs,
                                                          const unsigned long long *__restrict__ npairs_dev,
                                                          int64_t pairs_cap, const unsigned long long *__restrict__ raw_dev,
                                                          int4 *__restrict__ vf_out, int64_t vf_cap,
@@ -761,10 +868,9 @@ __global__ __launch_bounds__(EXPAND_BLOCK) void k_expand(MeshView m, int64_t F, 
     __shared__ unsigned s_off[2][NW];
     __shared__ unsigned long long s_base[2];
     const bool OWNER = *raw_dev == 0;
+    if (OWNER && rows_path) return;   // the row expansion has it
     int64_t np = (int64_t)*npairs_dev;
     if (np > pairs_cap) np = pairs_cap;
-    if ((m.adj != nullptr && adj_wanted(np, F)) != ADJ) return;
-    m.use_adj = ADJ;
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
     for (int64_t base = blockIdx.x * (int64_t)EXPAND_BLOCK; base < np; base += (int64_t)gridDim.x * EXPAND_BLOCK) {
         int64_t i = base + threadIdx.x;
@@ -791,11 +897,11 @@ __global__ __launch_bounds__(EXPAND_BLOCK) void k_expand(MeshView m, int64_t F, 
         for (int s = 0; s < 3; s++) {
             int p = fv[s];
             bool want = valid && !(is_fixed(m, p) && gfix);
-            if (OWNER && want) want = vf_owner(m, p, g, f);
+            if (OWNER && want) want = vf_owner<false>(m, p, g, f);
             if (want) vmask |= 1u << (2 * s);
             int q = gv[s];
             want = valid && !(is_fixed(m, q) && ffix);
-            if (OWNER && want) want = vf_owner(m, q, f, g);
+            if (OWNER && want) want = vf_owner<false>(m, q, f, g);
             if (want) vmask |= 1u << (2 * s + 1);
         }
         // 9 edge-edge stencils (KDOPBroadPhase.cpp:176-183): bit 3s+r for edges (s, r)
@@ -808,9 +914,9 @@ __global__ __launch_bounds__(EXPAND_BLOCK) void k_expand(MeshView m, int64_t F, 
                 bool want = valid && !(is_fixed(m, u0) && is_fixed(m, u1) && is_fixed(m, w0) && is_fixed(m, w1));
                 if (OWNER && want) {
                     if (fo[s] != -2 && go[r] != -2)
-                        want = ee_owner_opp(m, fo[s], go[r], f, g);
+                        want = ee_owner_opp<false>(m, fo[s], go[r], f, g);
                     else
-                        want = ee_owner(m, u0, u1, w0, w1, f, g);
+                        want = ee_owner<false>(m, u0, u1, w0, w1, f, g);
                 }
                 if (want) emask |= 1u << (3 * s + r);
             }
@@ -869,6 +975,186 @@ __global__ __launch_bounds__(EXPAND_BLOCK) void k_expand(MeshView m, int64_t F, 
             }
         }
         __syncthreads();   // s_off / s_base are rewritten by the next pass
+    }
+}
+
+// ---- Row expansion -----------------------------------------------------------------------------
+// The stencils of every face pair, deduplicated by the same owner rule, organised by face: one
+// wavefront per face f walks f's row of the related-face table.  For each related face g it emits the
+// three vertex-face stencils (f's vertices against g) that f owns, and for g > f the nine edge-edge
+// stencils that the pair (f, g) owns.  Every unordered pair is visited from both sides, so each side
+// emits its own vertices' stencils.  The owner tests become membership tests in sorted rows:
+//   - f's own row (related(f, ob));
+//   - the rows of the faces around f's vertices and across f's edges, which every lane of the
+//     wavefront shares (they stay in cache).
+// The per-pair form instead re-reads two 208-byte FP64 bounds per test.  Output slots come from a
+// count pass, an exclusive scan over faces and a write pass that recomputes nothing: the count pass
+// stores each entry's 12-bit emission mask beside the row.  No atomics; the order is deterministic
+// (by face, then by related face).
+//
+// Rows of at most ROW_SORT_CAP entries, sorted ascending in LDS by one wavefront (bitonic network).
+__global__ __launch_bounds__(64) void k_row_sort(const unsigned long long *__restrict__ raw_dev, int64_t F,
+                                                 const int32_t *__restrict__ off, int32_t *__restrict__ adj)
+{
+    __shared__ int32_t s[ROW_SORT_CAP];
+    if (*raw_dev != 0) return;
+    for (int64_t a = blockIdx.x; a < F; a += gridDim.x) {
+        const int lo = off[a], len = off[a + 1] - lo;
+        if (len < 2 || len > ROW_SORT_CAP) continue;   // uniform across the block
+        int n2 = 2;
+        while (n2 < len) n2 <<= 1;
+        for (int i = threadIdx.x; i < n2; i += 64) s[i] = i < len ? adj[lo + i] : INT32_MAX;
+        __syncthreads();
+        for (int k = 2; k <= n2; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = threadIdx.x; i < n2; i += 64) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const int x = s[i], y = s[l];
+                        if ((x > y) == ((i & k) == 0)) {
+                            s[i] = y;
+                            s[l] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int i = threadIdx.x; i < len; i += 64) adj[lo + i] = s[i];
+        __syncthreads();   // s is rewritten for the next row
+    }
+}
+
+// Emission mask of entry (f, g): bit s (0..2) for vertex-face (fv[s], g), bit 3 + 3s + r for edge-edge
+// (edge s of f, edge r of g), the latter only for g > f.  fv and fo are f's (wave-uniform).
+__device__ __forceinline__ unsigned row_mask(const MeshView &m, int f, const int fv[3], const int fo[3], int g, int4 G0)
+{
+    const int gv[3] = {G0.x, G0.y, G0.z};
+    const bool gfix = is_fixed(m, gv[0]) && is_fixed(m, gv[1]) && is_fixed(m, gv[2]);
+    unsigned msk = 0;
+    // KDOPBroadPhase.cpp:157-175 (this side's three)
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+        if (is_fixed(m, fv[s]) && gfix) continue;
+        if (vf_owner<true>(m, fv[s], g, f)) msk |= 1u << s;
+    }
+    if (g < f) return msk;
+    int go[3] = {-2, -2, -2};
+    if (m.opp) {
+#pragma unroll
+        for (int r = 0; r < 3; r++) go[r] = m.opp[3 * (int64_t)g + r];
+    }
+    // KDOPBroadPhase.cpp:176-183
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+        const int u0 = fv[s], u1 = fv[(s + 1) % 3];
+        const bool ufix = is_fixed(m, u0) && is_fixed(m, u1);
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            const int w0 = gv[r], w1 = gv[(r + 1) % 3];
+            if (ufix && is_fixed(m, w0) && is_fixed(m, w1)) continue;
+            const bool own = (fo[s] != -2 && go[r] != -2) ? ee_owner_opp<true>(m, fo[s], go[r], f, g)
+                                                          : ee_owner<true>(m, u0, u1, w0, w1, f, g);
+            if (own) msk |= 1u << (3 + 3 * s + r);
+        }
+    }
+    return msk;
+}
+
+__device__ __forceinline__ int64_t wave_sum(int64_t v)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
+    return v;
+}
+
+// WRITE false: masks and per-face counts (cnt_vf / cnt_ee).  WRITE true: the keys, at the exclusive
+// offsets the scan left in cnt_vf / cnt_ee.  Both return at once for raw emission (the count pass then
+// writes zero counts, so the scan totals are zero and the per-pair kernel's atomics start from there).
+constexpr int ROWS_BLOCK = 256;
+template <bool WRITE>
+__global__ __launch_bounds__(ROWS_BLOCK) void k_expand_rows(MeshView m, int64_t F,
+                                                            const unsigned long long *__restrict__ raw_dev,
+                                                            uint16_t *__restrict__ mask, int64_t *__restrict__ cnt_vf,
+                                                            int64_t *__restrict__ cnt_ee, int4 *__restrict__ vf_out,
+                                                            int64_t vf_cap, int4 *__restrict__ ee_out, int64_t ee_cap)
+{
+    const bool raw = *raw_dev != 0;
+    if (raw && WRITE) return;
+    const int lane = __lane_id();
+    const int64_t nwaves = (int64_t)gridDim.x * (ROWS_BLOCK / 64);
+    for (int64_t fw = blockIdx.x * (int64_t)(ROWS_BLOCK / 64) + (threadIdx.x >> 6); fw < F; fw += nwaves) {
+        const int f = __builtin_amdgcn_readfirstlane((int)fw);
+        if (raw) {
+            if (lane == 0) cnt_vf[f] = cnt_ee[f] = 0;
+            continue;
+        }
+        const int lo = m.adj_off[f], hi = m.adj_off[f + 1];
+        const int4 F0 = face3(m, f);
+        const int fv[3] = {F0.x, F0.y, F0.z};
+        if (!WRITE) {
+            int fo[3] = {-2, -2, -2};
+            if (m.opp) {
+#pragma unroll
+                for (int s = 0; s < 3; s++) fo[s] = m.opp[3 * (int64_t)f + s];
+            }
+            int64_t nv = 0, ne = 0;
+            for (int base = lo; base < hi; base += 64) {
+                const int k = base + lane;
+                if (k < hi) {
+                    const int g = m.adj[k];
+                    const unsigned msk = row_mask(m, f, fv, fo, g, face3(m, g));
+                    mask[k] = (uint16_t)msk;
+                    nv += __popc(msk & 7u);
+                    ne += __popc(msk >> 3);
+                }
+            }
+            nv = wave_sum(nv);
+            ne = wave_sum(ne);
+            if (lane == 0) {
+                cnt_vf[f] = nv;
+                cnt_ee[f] = ne;
+            }
+        } else {
+            int64_t pos_v = cnt_vf[f], pos_e = cnt_ee[f];
+            for (int base = lo; base < hi; base += 64) {
+                const int k = base + lane;
+                const bool valid = k < hi;
+                const unsigned msk = valid ? mask[k] : 0u;
+                const int4 G0 = valid && msk ? face3(m, m.adj[k]) : make_int4(0, 0, 0, 0);
+                const unsigned nv = __popc(msk & 7u), ne = __popc(msk >> 3);
+                unsigned pv = nv, pe = ne;   // inclusive prefix over the wavefront
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const unsigned tv = __shfl_up(pv, d), te = __shfl_up(pe, d);
+                    if (lane >= d) {
+                        pv += tv;
+                        pe += te;
+                    }
+                }
+                int64_t v = pos_v + (pv - nv), e = pos_e + (pe - ne);
+                const int gv[3] = {G0.x, G0.y, G0.z};
+#pragma unroll
+                for (int s = 0; s < 3; s++) {
+                    if (msk & (1u << s)) {
+                        if (v < vf_cap) vf_out[v] = vf_key(fv[s], gv[0], gv[1], gv[2]);
+                        v++;
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < 3; s++) {
+#pragma unroll
+                    for (int r = 0; r < 3; r++) {
+                        if (msk & (1u << (3 + 3 * s + r))) {
+                            if (e < ee_cap) ee_out[e] = ee_key(fv[s], fv[(s + 1) % 3], gv[r], gv[(r + 1) % 3]);
+                            e++;
+                        }
+                    }
+                }
+                pos_v += __shfl(pv, 63);
+                pos_e += __shfl(pe, 63);
+            }
+        }
     }
 }
 
@@ -981,10 +1267,15 @@ struct BPBufs {
     void *sort_tmp;
     int32_t *adj_cnt = nullptr, *adj_off = nullptr, *adj = nullptr;   // related-face table (null if too large)
     void *adj_tmp = nullptr;
+    uint16_t *adj_mask = nullptr;                  // per table entry: the row expansion's emission mask
+    int64_t *exp_vf = nullptr, *exp_ee = nullptr;   // per face: stencil counts, then output offsets
     int2 *pairs;
     int4 *vfk, *eek;
     int32_t *opp;    // edge -> opposite face, 3F (kept with the vertex -> face table for refits)
     int32_t *rank;   // face id -> Morton-sorted position
+    // level-synchronous refit (large trees): depth of every internal node, its pointer-jumping
+    // scratch, and the maximum depth
+    uint32_t *tdepth = nullptr, *tdepth2 = nullptr, *tjump = nullptr, *tjump2 = nullptr, *tmaxd = nullptr;
 };
 constexpr int NPARTS = 512;
 
@@ -1013,12 +1304,24 @@ static int bp_buffers(ccd_ctx *ctx, int64_t F, int64_t V, bool batched, BPBufs &
     b.eek = (int4 *)buffer(ctx, "ee_keys", (size_t)ctx->cap_ee * 16);
     b.opp = (int32_t *)buffer(ctx, "edge_opp", (size_t)(3 * F + 1) * 4);
     b.rank = (int32_t *)buffer(ctx, "face_rank", (size_t)F * 4);
+    if (F >= LEVEL_REFIT_MIN_FACES) {
+        b.tdepth = (uint32_t *)buffer(ctx, "tree_depth", (size_t)F * 4);
+        b.tdepth2 = (uint32_t *)buffer(ctx, "tree_depth2", (size_t)F * 4);
+        b.tjump = (uint32_t *)buffer(ctx, "tree_jump", (size_t)F * 4);
+        b.tjump2 = (uint32_t *)buffer(ctx, "tree_jump2", (size_t)F * 4);
+        b.tmaxd = (uint32_t *)buffer(ctx, "tree_maxd", 256);
+        if (!b.tdepth || !b.tdepth2 || !b.tjump || !b.tjump2 || !b.tmaxd) return CCD_E_OOM;
+    }
     if (2 * ctx->cap_pairs < ((int64_t)1 << 31) - 1) {   // int32 row offsets
         b.adj_cnt = (int32_t *)buffer(ctx, "adj_cnt", (size_t)(F + 1) * 4);
         b.adj_off = (int32_t *)buffer(ctx, "adj_off", (size_t)(F + 1) * 4);
         b.adj = (int32_t *)buffer(ctx, "adj", (size_t)(2 * ctx->cap_pairs + 1) * 4);
         b.adj_tmp = buffer(ctx, "adj_scan_tmp", scan_temp_bytes(F + 1));
-        if (!b.adj_cnt || !b.adj_off || !b.adj || !b.adj_tmp) return CCD_E_OOM;
+        b.adj_mask = (uint16_t *)buffer(ctx, "adj_mask", (size_t)(2 * ctx->cap_pairs + 1) * 2);
+        b.exp_vf = (int64_t *)buffer(ctx, "exp_vf", (size_t)(F + 1) * 8);
+        b.exp_ee = (int64_t *)buffer(ctx, "exp_ee", (size_t)(F + 1) * 8);
+        if (!b.adj_cnt || !b.adj_off || !b.adj || !b.adj_tmp || !b.adj_mask || !b.exp_vf || !b.exp_ee)
+            return CCD_E_OOM;
     }
     if (!b.dop64 || !b.centre || !b.partial || !b.keys || !b.keys2 || !b.vals || !b.order ||
         (batched && !b.face_inst) || !b.left || !b.rf || !b.rl || !b.parent || !b.escape || !b.arrivals ||
@@ -1027,8 +1330,7 @@ static int bp_buffers(ccd_ctx *ctx, int64_t F, int64_t V, bool batched, BPBufs &
     return CCD_OK;
 }
 
-// Related-face table from the traversal's pairs (device-side count; skipped inside the kernels when
-// the pairs average more than ADJ_MAX_AVG per face).
+// Related-face table from the traversal's pairs (device-side count), rows in fill order.
 static int build_related_table(ccd_ctx *ctx, const BPBufs &b, int64_t F, MeshView &mv)
 {
     if (!b.adj) return CCD_OK;
@@ -1049,16 +1351,67 @@ static int build_related_table(ccd_ctx *ctx, const BPBufs &b, int64_t F, MeshVie
     return CCD_OK;
 }
 
-static int launch_expand(ccd_ctx *ctx, const MeshView &mv, int64_t F, const BPBufs &b, unsigned grid)
+// Interior bounds of the tree from its leaf records: the arrival-counter refit for small trees, the
+// level-synchronous one for large trees (new_topology: the tree was just built, so its depths are
+// computed first; a refit of the same tree keeps them).
+static int refit_tree(ccd_ctx *ctx, const BPBufs &b, int64_t F, bool new_topology)
 {
-    unsigned long long *c = (unsigned long long *)ctx->d_counts;
-    if (mv.adj) {
-        hipLaunchKernelGGL(k_expand<true>, dim3(grid), dim3(EXPAND_BLOCK), 0, ctx->stream, mv, F, b.pairs, c + C_PAIRS,
-                           ctx->cap_pairs, c + C_BAD, b.vfk, ctx->cap_vf, c + C_VF, b.eek, ctx->cap_ee, c + C_EE);
+    hipStream_t st = ctx->stream;
+    if (F < LEVEL_REFIT_MIN_FACES) {
+        int rc = fill_u32(ctx, b.arrivals, F, 0);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_refit, dim3(blocks_for(F, 256)), dim3(256), 0, st, F, b.parent, b.left, b.escape,
+                           b.nodes, b.arrivals);
+        CCD_LAUNCH_CHECK(ctx);
+        return CCD_OK;
+    }
+    const unsigned g = blocks_for(F - 1, 256);
+    if (new_topology) {
+        hipLaunchKernelGGL(k_depth_init, dim3(g), dim3(256), 0, st, F, b.parent, b.tjump, b.tdepth, b.tmaxd);
+        CCD_LAUNCH_CHECK(ctx);
+        for (int r = 0; r < DEPTH_ROUNDS; r++) {   // an even count: the result ends in (tjump, tdepth)
+            const bool even = (r & 1) == 0;
+            hipLaunchKernelGGL(k_depth_jump, dim3(g), dim3(256), 0, st, F, even ? b.tjump : b.tjump2,
+                               even ? b.tdepth : b.tdepth2, even ? b.tjump2 : b.tjump, even ? b.tdepth2 : b.tdepth);
+            CCD_LAUNCH_CHECK(ctx);
+        }
+        hipLaunchKernelGGL(k_depth_max, dim3(std::min<unsigned>(g, 1024)), dim3(256), 0, st, F, b.tdepth, b.tmaxd);
         CCD_LAUNCH_CHECK(ctx);
     }
-    hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(EXPAND_BLOCK), 0, ctx->stream, mv, F, b.pairs, c + C_PAIRS,
-                       ctx->cap_pairs, c + C_BAD, b.vfk, ctx->cap_vf, c + C_VF, b.eek, ctx->cap_ee, c + C_EE);
+    const unsigned gl = std::min<unsigned>(g, 2048);
+    for (int level = MAX_TREE_DEPTH - 1; level >= 0; level--) {
+        hipLaunchKernelGGL(k_refit_level, dim3(gl), dim3(256), 0, st, F, (uint32_t)level, b.tdepth, b.tmaxd, b.left,
+                           b.escape, b.nodes);
+        CCD_LAUNCH_CHECK(ctx);
+    }
+    return CCD_OK;
+}
+
+// Expansion: with a related-face table, sort its rows and run the row expansion (count, scan, write;
+// the scans leave the unique stencil counts in the device counters); the per-pair kernel follows for
+// raw emission, or does everything when there is no table.
+static int launch_expand(ccd_ctx *ctx, const MeshView &mv, int64_t F, const BPBufs &b, unsigned grid)
+{
+    hipStream_t st = ctx->stream;
+    unsigned long long *c = (unsigned long long *)ctx->d_counts;
+    const bool rows = mv.adj != nullptr;
+    if (rows) {
+        const unsigned gs = (unsigned)std::max<int64_t>(1, std::min<int64_t>(F, 65536));
+        hipLaunchKernelGGL(k_row_sort, dim3(gs), dim3(64), 0, st, c + C_BAD, F, mv.adj_off, b.adj);
+        CCD_LAUNCH_CHECK(ctx);
+        const unsigned gw = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(F, ROWS_BLOCK / 64), 65536));
+        hipLaunchKernelGGL(k_expand_rows<false>, dim3(gw), dim3(ROWS_BLOCK), 0, st, mv, F, c + C_BAD, b.adj_mask,
+                           b.exp_vf, b.exp_ee, (int4 *)nullptr, (int64_t)0, (int4 *)nullptr, (int64_t)0);
+        CCD_LAUNCH_CHECK(ctx);
+        int rc = exclusive_scan_i64(ctx, b.exp_vf, b.exp_vf, F, b.adj_tmp, (int64_t *)(c + C_VF));
+        if (!rc) rc = exclusive_scan_i64(ctx, b.exp_ee, b.exp_ee, F, b.adj_tmp, (int64_t *)(c + C_EE));
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_expand_rows<true>, dim3(gw), dim3(ROWS_BLOCK), 0, st, mv, F, c + C_BAD, b.adj_mask,
+                           b.exp_vf, b.exp_ee, b.vfk, ctx->cap_vf, b.eek, ctx->cap_ee);
+        CCD_LAUNCH_CHECK(ctx);
+    }
+    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(EXPAND_BLOCK), 0, st, mv, rows, b.pairs, c + C_PAIRS, ctx->cap_pairs,
+                       c + C_BAD, b.vfk, ctx->cap_vf, c + C_VF, b.eek, ctx->cap_ee, c + C_EE);
     CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
 }
@@ -1134,11 +1487,8 @@ int broadphase_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const doubl
     hipLaunchKernelGGL(k_leaf_records, dim3(blocks_for(F, 256)), dim3(256), 0, st, order, b.dop64, faces, F,
                        b.escape, b.nodes, b.sdop64, b.sfaces, b.rank);
     CCD_LAUNCH_CHECK(ctx);
-    rc = fill_u32(ctx, b.arrivals, F, 0);
+    rc = refit_tree(ctx, b, F, true);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_refit, dim3(blocks_for(F, 256)), dim3(256), 0, st, F, b.parent, b.left, b.escape, b.nodes,
-                       b.arrivals);
-    CCD_LAUNCH_CHECK(ctx);
     prof_end(ctx, CCD_T_BUILD);
 
     prof_begin(ctx, CCD_T_TABLES);
@@ -1201,11 +1551,8 @@ int broadphase_refit_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const
     hipLaunchKernelGGL(k_leaf_records, dim3(blocks_for(F, 256)), dim3(256), 0, st, ctx->tree_order, b.dop64, faces,
                        F, b.escape, b.nodes, b.sdop64, b.sfaces, b.rank);
     CCD_LAUNCH_CHECK(ctx);
-    rc = fill_u32(ctx, b.arrivals, F, 0);
+    rc = refit_tree(ctx, b, F, false);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_refit, dim3(blocks_for(F, 256)), dim3(256), 0, st, F, b.parent, b.left, b.escape, b.nodes,
-                       b.arrivals);
-    CCD_LAUNCH_CHECK(ctx);
     prof_end(ctx, CCD_T_BUILD);
     const int32_t *voff = nullptr, *vlist = nullptr;
     vertex_face_table(ctx, &voff, &vlist);

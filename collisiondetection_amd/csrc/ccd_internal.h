@@ -177,6 +177,7 @@ int fill_u32(ccd_ctx *ctx, uint32_t *p, int64_t n, uint32_t v);
 int fill_u64(ccd_ctx *ctx, uint64_t *p, int64_t n, uint64_t v);
 size_t scan_temp_bytes(int64_t n);
 int exclusive_scan_i32(ccd_ctx *ctx, const int32_t *in, int32_t *out, int64_t n, void *temp, int64_t *total);
+int exclusive_scan_i64(ccd_ctx *ctx, const int64_t *in, int64_t *out, int64_t n, void *temp, int64_t *total);
 size_t radix_sort_temp_bytes(int64_t n);
 // Sorts by the low `bits` bits of the keys; the result is in (keys_alt, vals_alt) when result_in_alt.
 int radix_sort_pairs(ccd_ctx *ctx, uint64_t *keys, int32_t *vals, uint64_t *keys_alt, int32_t *vals_alt, int64_t n,

@@ -213,6 +213,11 @@ int exclusive_scan_i32(ccd_ctx *ctx, const int32_t *in, int32_t *out, int64_t n,
     return scan_impl<int32_t>(ctx, in, out, n, temp, total);
 }
 
+int exclusive_scan_i64(ccd_ctx *ctx, const int64_t *in, int64_t *out, int64_t n, void *temp, int64_t *total)
+{
+    return scan_impl<int64_t>(ctx, in, out, n, temp, total);
+}
+
 size_t radix_sort_temp_bytes(int64_t n)
 {
     int64_t nt = blocks_for(n, RS_TILE);

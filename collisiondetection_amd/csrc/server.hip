@@ -15,6 +15,7 @@
 // submit / start / done timestamps for latency statistics.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -117,6 +118,7 @@ struct ccd_request_rec {
     int session;
     double t_submit;
     std::vector<double> qend;   // new end positions (host copy)
+    int reruns = 0;             // workspace overflows so far
 };
 
 struct Session {
@@ -130,6 +132,7 @@ struct Session {
     bool busy = false;
     // workspace sizing from the last completed request
     int64_t last_pairs = 0, last_vf = 0, last_ee = 0, last_hits = 0;
+    bool moved = false;   // a request has completed: the last counts come from a moving mesh
 };
 
 struct InFlight {
@@ -160,16 +163,25 @@ struct ccd_server {
     bool stop = false;
     std::string err;
     int64_t submitted = 0, completed = 0, retries = 0, mem_waits = 0, max_request_bytes = 0;
+    // One pinned counter record and one completion event per in-flight slot, made at creation:
+    // allocating or freeing pinned memory inside the request loop can wait for the whole device.
+    int64_t *rec_block = nullptr;
+    std::vector<int64_t *> rec_free;
+    std::vector<hipEvent_t> ev_free;
 };
 
 namespace {
 
 using namespace ccdi;
 
+int64_t align256(int64_t x) { return (x + 255) & ~255ll; }
+
+// A request's transient device memory: face pairs, the related-face table and its emission masks
+// (two entries per pair), the stencil keys and the hit lists.
 int64_t workspace_bytes(int64_t cp, int64_t cv, int64_t ce, int64_t ch)
 {
-    return ((cp * 8 + 255) & ~255ll) + ((cv * 16 + 255) & ~255ll) + ((ce * 16 + 255) & ~255ll) +
-           2 * (((ch * 16 + 255) & ~255ll) + ((ch * 8 + 255) & ~255ll)) + 256;
+    return align256(cp * 8) + align256((2 * cp + 1) * 4) + align256((2 * cp + 1) * 2) + align256(cv * 16) +
+           align256(ce * 16) + 2 * (align256(ch * 16) + align256(ch * 8)) + 256;
 }
 
 // Point the session context's transient buffers at a slice of the pool.
@@ -181,12 +193,14 @@ void bind_workspace(ccd_server *S, Session &s, InFlight &f)
         b.ptr = p;
         b.bytes = (size_t)bytes;
         b.external = true;
-        p += (bytes + 255) & ~255ll;
+        p += align256(bytes);
     };
     put("pairs", f.cap_pairs * 8);
+    put("adj", (2 * f.cap_pairs + 1) * 4);
+    put("adj_mask", (2 * f.cap_pairs + 1) * 2);
     put("vf_keys", f.cap_vf * 16);
     put("ee_keys", f.cap_ee * 16);
-    const int64_t kb = (f.cap_hits * 16 + 255) & ~255ll, tb = (f.cap_hits * 8 + 255) & ~255ll;
+    const int64_t kb = align256(f.cap_hits * 16), tb = align256(f.cap_hits * 8);
     f.hit_vk = (int32_t *)p;
     f.hit_vt = (double *)(p + kb);
     p += kb + tb;
@@ -253,6 +267,7 @@ void worker_loop(ccd_server *S)
             if (overflow) {
                 // Rerun with workspace sized from these counts; positions have not advanced.
                 S->retries++;
+                f.req.reruns++;
                 S->pending.push_front(f.req);
             } else {
                 if (S->keep_hits) {
@@ -280,7 +295,8 @@ void worker_loop(ccd_server *S)
                     S->kept[f.req.id] = std::move(v);
                 }
                 s.cur = 1 - s.cur;   // the new end becomes the next request's start
-                ccd_request_result r;
+                s.moved = true;
+                ccd_request_result r{};
                 r.request_id = f.req.id;
                 r.session = f.req.session;
                 r.status = CCD_OK;
@@ -290,6 +306,8 @@ void worker_loop(ccd_server *S)
                 r.ee_candidates = h[C_EE];
                 r.checksum_vf = (uint64_t)h[NCOUNTERS];
                 r.checksum_ee = (uint64_t)h[NCOUNTERS + 1];
+                r.reruns = f.req.reruns;
+                r.face_pairs = h[C_PAIRS];
                 r.t_submit = f.req.t_submit;
                 r.t_start = f.t_start;
                 r.t_done = now_s();
@@ -298,8 +316,8 @@ void worker_loop(ccd_server *S)
             }
             s.busy = false;
             S->pool.release(f.pool_off, f.pool_bytes);
-            (void)hipEventDestroy(f.done);
-            (void)hipHostFree(f.h_rec);
+            S->ev_free.push_back(f.done);
+            S->rec_free.push_back(f.h_rec);
             S->inflight.erase(S->inflight.begin() + k);
             progressed = true;
         }
@@ -311,12 +329,17 @@ void worker_loop(ccd_server *S)
                 continue;
             }
             InFlight f;
-            // Margins over the session's last counts: consecutive requests of a moving mesh vary by
-            // tens of percent, and every overflow costs a rerun of the whole request.
-            f.cap_pairs = s.last_pairs + s.last_pairs / 2 + 4096;
-            f.cap_vf = s.last_vf + s.last_vf / 2 + 4096;
-            f.cap_ee = s.last_ee + s.last_ee / 2 + 4096;
-            f.cap_hits = 2 * s.last_hits + 4096;
+            // Workspace from the session's last counts with headroom: twice the last counts (four times
+            // for a session's first request, whose last counts come from its opening build, where the
+            // mesh may not have moved at all) plus a share of the face count.  Counts of a session that
+            // keeps moving grow by factors from one request to the next (a drifting mesh starts to touch
+            // itself), and every overflow costs a rerun of the whole request; the headroom costs only
+            // pool memory, about 60 bytes per face.
+            const int64_t mul = s.moved ? 2 : 4;
+            f.cap_pairs = mul * s.last_pairs + s.F / 4 + 4096;
+            f.cap_vf = mul * s.last_vf + s.F + 4096;
+            f.cap_ee = mul * s.last_ee + 2 * s.F + 4096;
+            f.cap_hits = mul * s.last_hits + s.F / 16 + 4096;
             f.pool_bytes = workspace_bytes(f.cap_pairs, f.cap_vf, f.cap_ee, f.cap_hits);
             if (f.pool_bytes > S->pool.size) {   // can never be admitted: fail it rather than wait forever
                 ccd_request_result r{};
@@ -340,8 +363,10 @@ void worker_loop(ccd_server *S)
             it = S->pending.erase(it);
             s.busy = true;
             f.t_start = now_s();
-            (void)hipHostMalloc((void **)&f.h_rec, sizeof(int64_t) * (NCOUNTERS + 2), hipHostMallocDefault);
-            (void)hipEventCreateWithFlags(&f.done, hipEventDisableTiming);
+            f.h_rec = S->rec_free.back();
+            S->rec_free.pop_back();
+            f.done = S->ev_free.back();
+            S->ev_free.pop_back();
             bind_workspace(S, s, f);
             int rc = launch_request(S, s, f);
             if (rc) {
@@ -357,8 +382,8 @@ void worker_loop(ccd_server *S)
                 s.busy = false;
                 (void)hipStreamSynchronize(s.ctx->stream);
                 S->pool.release(f.pool_off, f.pool_bytes);
-                (void)hipEventDestroy(f.done);
-                (void)hipHostFree(f.h_rec);
+                S->ev_free.push_back(f.done);
+                S->rec_free.push_back(f.h_rec);
                 continue;
             }
             S->inflight.push_back(std::move(f));
@@ -406,6 +431,27 @@ int ccd_server_create(int device, int64_t hbm_budget_bytes, int max_inflight, cc
     S->pool.base = (char *)p;
     S->pool.size = hbm_budget_bytes;
     S->pool.free_[0] = hbm_budget_bytes;
+    const size_t stride = NCOUNTERS + 2;
+    if (hipHostMalloc((void **)&S->rec_block, sizeof(int64_t) * stride * (size_t)max_inflight, hipHostMallocDefault) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(S->pool.base);
+        delete S;
+        return CCD_E_OOM;
+    }
+    for (int k = 0; k < max_inflight; k++) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            for (hipEvent_t x : S->ev_free) (void)hipEventDestroy(x);
+            (void)hipHostFree(S->rec_block);
+            (void)hipFree(S->pool.base);
+            delete S;
+            return CCD_E_HIP;
+        }
+        S->ev_free.push_back(e);
+        S->rec_free.push_back(S->rec_block + stride * (size_t)k);
+    }
     S->worker = std::thread(worker_loop, S);
     *out = S;
     return CCD_OK;
@@ -456,7 +502,7 @@ int ccd_server_open_session(ccd_server *S, const int32_t *faces, int64_t nfaces,
     s.last_ee = nec;
     s.last_hits = std::max(nvh, neh);
     // The transient buffers come from the pool per request: release the context's own copies.
-    for (const char *name : {"pairs", "vf_keys", "ee_keys"}) {
+    for (const char *name : {"pairs", "adj", "adj_mask", "vf_keys", "ee_keys"}) {
         ccd_ctx::Buf &b = ctx->bufs[name];
         if (b.ptr && !b.external) {
             (void)hipFree(b.ptr);
@@ -564,12 +610,14 @@ int ccd_server_destroy(ccd_server *S)
     if (S->worker.joinable()) S->worker.join();
     (void)hipSetDevice(S->device);
     for (Session &s : S->sessions) {
-        for (const char *name : {"pairs", "vf_keys", "ee_keys"}) s.ctx->bufs[name] = ccd_ctx::Buf();
+        for (const char *name : {"pairs", "adj", "adj_mask", "vf_keys", "ee_keys"}) s.ctx->bufs[name] = ccd_ctx::Buf();
         (void)hipFree(s.faces);
         (void)hipFree(s.pos[0]);
         (void)hipFree(s.pos[1]);
         ccd_destroy(s.ctx);
     }
+    for (hipEvent_t e : S->ev_free) (void)hipEventDestroy(e);
+    (void)hipHostFree(S->rec_block);
     (void)hipFree(S->pool.base);
     delete S;
     return CCD_OK;

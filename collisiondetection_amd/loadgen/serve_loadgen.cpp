@@ -267,11 +267,19 @@ int main(int argc, char **argv)
     ccd_server_stats(srv, stats);
     std::vector<double> lat, lat_small;
     double t_first = 1e300, t_last = -1e300, tris = 0;
-    int64_t failed = 0;
+    int64_t failed = 0, rerun_requests = 0, shown = 0;
     for (const auto &r : res) {
         if (r.status != CCD_OK) {
             failed++;
             continue;
+        }
+        if (r.reruns > 0) {
+            rerun_requests++;
+            if (shown++ < 8)
+                std::fprintf(stderr, "loadgen: request %lld (session %d, %lld triangles) ran %d extra time(s): %lld face "
+                             "pairs, %lld + %lld stencils\n", (long long)r.request_id, r.session,
+                             (long long)sess[r.session].F, r.reruns, (long long)r.face_pairs,
+                             (long long)r.vf_candidates, (long long)r.ee_candidates);
         }
         const double l = (r.t_done - sched[r.request_id]) * 1e3;
         lat.push_back(l);
@@ -286,7 +294,7 @@ int main(int argc, char **argv)
                 "\"completed\": %zu, \"failed\": %lld, \"overloaded\": %s, \"offered_wall_s\": %.3f, "
                 "\"drain_wall_s\": %.3f, \"offered_rate_achieved\": %.3f, \"requests_per_s\": %.3f, "
                 "\"swept_tris_per_s\": %.1f, \"latency_ms\": {\"p50\": %.4f, \"p99\": %.4f, \"p50_le%lldk\": %.4f, "
-                "\"p99_le%lldk\": %.4f, \"max\": %.4f}, \"reruns\": %lld, \"mem_waits\": %lld, \"client_max_lag_ms\": %.3f, "
+                "\"p99_le%lldk\": %.4f, \"max\": %.4f}, \"reruns\": %lld, \"rerun_requests\": %lld, \"mem_waits\": %lld, \"client_max_lag_ms\": %.3f, "
                 "\"sessions\": %zu, \"resident_triangles\": %lld, \"inflight\": %d, \"budget_gib\": %.1f, "
                 "\"client\": \"C++ (%d generator threads), fresh N(0, %g^2) draws per request\"}\n",
                 rate, seconds, (long long)N, (long long)offered, lat.size(), (long long)failed,
@@ -295,7 +303,7 @@ int main(int argc, char **argv)
                 percentile(lat, 50), percentile(lat, 99), (long long)(small_tris / 1000), percentile(lat_small, 50),
                 (long long)(small_tris / 1000), percentile(lat_small, 99),
                 lat.empty() ? NAN : *std::max_element(lat.begin(), lat.end()), (long long)(stats[6] - reruns0),
-                (long long)(stats[8] - waits0), lag_max * 1e3, sess.size(), (long long)resident_tris, inflight,
+                (long long)rerun_requests, (long long)(stats[8] - waits0), lag_max * 1e3, sess.size(), (long long)resident_tris, inflight,
                 budget_gib, threads, sigma);
     ccd_server_destroy(srv);
     return 0;

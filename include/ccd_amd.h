@@ -330,6 +330,9 @@ typedef struct {
     int64_t vf_hits, ee_hits, vf_candidates, ee_candidates;
     uint64_t checksum_vf, checksum_ee; /* order-independent hashes of (key, TOI bits) of the collisions */
     double t_submit, t_start, t_done;   /* seconds, steady clock */
+    int32_t reruns;                     /* times the request was rerun after outgrowing its workspace */
+    int32_t reserved;
+    int64_t face_pairs;                 /* overlapping face pairs the broadphase found */
 } ccd_request_result;
 
 int ccd_server_create(int device, int64_t hbm_budget_bytes, int max_inflight, ccd_server **out);

@@ -99,27 +99,32 @@ def test_server_small_pool_admission():
 
 
 def test_server_overflow_reruns():
-    """A request whose stencils outgrow the workspace sized from the session's last counts (a static
-    first build, then a large motion): the narrowphase reads only the stored stencils, the request is
-    rerun with a larger workspace, and its result equals a fresh detection."""
+    """A request whose stencils outgrow the workspace sized from the session's last counts (a nearly
+    static request, then a large motion): the narrowphase reads only the stored stencils, the request
+    is rerun with a larger workspace, and its result equals a fresh detection."""
     from collisiondetection_amd import api
 
     print("case: server workspace overflow and rerun", flush=True)
-    s = scenes.cloth(48, drop=0.0, noise=0.0)   # qend == qstart: the first build sees a static sheet
+    s = scenes.cloth(48, drop=0.0, noise=0.0)   # qend == qstart: a static sheet
     rng = np.random.default_rng(2)
-    q = s["qstart"] + np.array([0.0, 0.0, -0.05]) + rng.normal(0.0, 0.02, size=s["qstart"].shape)
+    q1 = s["qstart"] + rng.normal(0.0, 1e-5, size=s["qstart"].shape)   # sizes the next workspace small
+    q = q1 + np.array([0.0, 0.0, -0.05]) + rng.normal(0.0, 0.02, size=s["qstart"].shape)
     srv = api.Server(hbm_budget_bytes=1 << 30, max_inflight=2)
     srv.keep_hits(True)
     sid = srv.open_session(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"])
-    srv.submit(sid, q, 0)
+    srv.submit(sid, q1, 0)
+    srv.drain()
+    srv.poll()
+    reruns0 = srv.stats()["reruns"]
+    srv.submit(sid, q, 1)
     srv.drain()
     (r,) = srv.poll()
     st = srv.stats()
-    vk, vt, ek, et = srv.fetch_hits(0)
+    vk, vt, ek, et = srv.fetch_hits(1)
     srv.close()
     print("stats", st, "result", r, flush=True)
-    assert r["status"] == 0 and st["reruns"] >= 1
-    d = api.detect(s["faces"], s["qend"], q, s["eta"], s["eta"])
+    assert r["status"] == 0 and st["reruns"] > reruns0
+    d = api.detect(s["faces"], q1, q, s["eta"], s["eta"])
     assert (r["vf_candidates"], r["ee_candidates"]) == (d["vf_candidates"], d["ee_candidates"])
     assert {tuple(k): t for k, t in zip(vk.tolist(), vt)} == {tuple(k): t for k, t in zip(d["vf"].tolist(), d["vf_toi"])}
     assert {tuple(k): t for k, t in zip(ek.tolist(), et)} == {tuple(k): t for k, t in zip(d["ee"].tolist(), d["ee_toi"])}
