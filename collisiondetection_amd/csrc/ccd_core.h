@@ -161,17 +161,6 @@ CCD_FHD long long double_to_bits(double d)
     return v;
 }
 
-// ------------------------------------------------------------------------------------------------
-// Interval lists (CTCD.h:7-26).  CAP is the most pieces a degree-D polynomial can produce.
-// ------------------------------------------------------------------------------------------------
-template <int CAP>
-// Every slot starts defined: the lists are read through unrolled loops whose guards the compiler may
-// evaluate together with the read, and a read of an uninitialised slot would make that undefined.
-struct Ivls {
-    double l[CAP] = {}, u[CAP] = {};
-    int n = 0;
-};
-
 // Static-index access.  On the GPU an array indexed by a runtime value lives in scratch memory (per-lane
 // private memory, cached but far slower than registers).  Every list here is small, so reads and writes
 // at a runtime index go through these unrolled selects and the arrays stay in registers.
@@ -194,30 +183,60 @@ CCD_FHD void put(double (&a)[N], int i, double v)
         if (k == i) a[k] = v;
 }
 
-// Append without the TimeInterval constructor (a copy of an existing interval).
-template <int CAP>
-CCD_FHD void append_ti(Ivls<CAP> &L, double tl, double tu)
-{
-    if (L.n < CAP) {
-        put(L.l, L.n, tl);
-        put(L.u, L.n, tu);
-        L.n++;
-    }
-}
+// checkInterval's clamp of an interval end to [0, 1] (CTCD.cpp:62-65), in its operation order.
+CCD_FHD double clamp_cut(double t) { return smin(1.0, smax(0.0, t)); }
 
-// TimeInterval(tl, tu): order, then clamp (CTCD.h:9-14).
+// ------------------------------------------------------------------------------------------------
+// Interval lists (CTCD.h:7-26).  Every list findIntervals builds is a subset of the pieces of one
+// partition of [0, 1], in ascending order: the partition's cut points are the polynomial's roots, each
+// clamped to [0, 1] as checkInterval clamps it, and the list holds the pieces that pass the midpoint
+// sign test.  So a list is kept as its nb cut points c[0..nb) and a bit mask of the pieces it holds:
+// piece p runs from cut p-1 (0 for p = 0) to cut p (1 for p = nb), and its TimeInterval is those two
+// ends ordered and clamped as the constructor does (CTCD.h:9-14).  The i-th interval of the reference's
+// list is the i-th piece in `keep`.  CAP pieces need CAP - 1 cut points, so a list takes about half the
+// registers of (lower, upper) pairs.
+// ------------------------------------------------------------------------------------------------
 template <int CAP>
-CCD_FHD void push_ti(Ivls<CAP> &L, double tl, double tu)
-{
-    if (tl > tu) {
-        double s = tl;
-        tl = tu;
-        tu = s;
+// Every slot starts defined: the cut points are read through unrolled selects whose guards the
+// compiler may evaluate together with the read.
+struct Ivls {
+    double c[CAP > 1 ? CAP - 1 : 1] = {};
+    int nb = 0;
+    unsigned keep = 0;
+
+    CCD_FHD bool empty() const { return keep == 0; }
+    // The piece's ends as checkInterval sees them (already clamped, in the order given).
+    CCD_FHD double cut_lo(int p) const { return p == 0 ? 0.0 : sel(c, p - 1); }
+    CCD_FHD double cut_hi(int p) const { return p == nb ? 1.0 : sel(c, p); }
+    // TimeInterval(cut_lo, cut_hi): ordered, then clamped (CTCD.h:9-14).
+    CCD_FHD void get(int p, double &l, double &u) const
+    {
+        l = cut_lo(p);
+        u = cut_hi(p);
+        if (l > u) {
+            double s = l;
+            l = u;
+            u = s;
+        }
+        l = smax(l, 0.0);
+        u = smin(u, 1.0);
     }
-    tl = smax(tl, 0.0);
-    tu = smin(tu, 1.0);
-    append_ti(L, tl, tu);
-}
+    CCD_FHD double lo(int p) const
+    {
+        double l, u;
+        get(p, l, u);
+        return l;
+    }
+    CCD_FHD double hi(int p) const
+    {
+        double l, u;
+        get(p, l, u);
+        return u;
+    }
+};
+
+// Lowest set bit of a nonzero mask.
+CCD_FHD int low_bit(unsigned m) { return __builtin_ctz(m); }
 
 // Horner in the reference's order (CTCD.cpp:68-73), no fused multiply-add.
 template <int D>
@@ -234,17 +253,31 @@ CCD_FHD double horner(const double *op, int degree, double x)
     return f;
 }
 
-// checkInterval (CTCD.cpp:59-79)
+// checkInterval's sign test (CTCD.cpp:59-79) on ends that are already clamped.
+template <int D>
+CCD_FHD bool midpoint_passes(double t1, double t2, const double *op, int degree, bool pos)
+{
+    double tmid = (t2 + t1) / 2;
+    double f = horner<D>(op, degree, tmid);
+    return (pos && f >= 0) || (!pos && f <= 0);
+}
+
+// checkInterval (CTCD.cpp:59-79) for piece p of `out`: keeps it if its midpoint passes.
 template <int D, int CAP>
-CCD_FHD void check_interval(double t1, double t2, const double *op, int degree, Ivls<CAP> &out, bool pos)
+CCD_FHD void check_piece(int p, const double *op, int degree, Ivls<CAP> &out, bool pos)
+{
+    if (midpoint_passes<D>(out.cut_lo(p), out.cut_hi(p), op, degree, pos)) out.keep |= 1u << p;
+}
+
+// checkInterval for an interval that is not kept, only tested (vertexVertexCTCD's pieces).
+template <int D>
+CCD_FHD bool check_interval_passes(double t1, double t2, const double *op, int degree, bool pos)
 {
     t1 = smax(0.0, t1);
     t2 = smax(0.0, t2);
     t1 = smin(1.0, t1);
     t2 = smin(1.0, t2);
-    double tmid = (t2 + t1) / 2;
-    double f = horner<D>(op, degree, tmid);
-    if ((pos && f >= 0) || (!pos && f <= 0)) push_ti(out, t1, t2);
+    return midpoint_passes<D>(t1, t2, op, degree, pos);
 }
 
 // couldHaveRoots (CTCD.cpp:81-94) on op[0..degree], degree <= D.
@@ -625,8 +658,10 @@ CCD_FHD int isolate_roots(const double *op, int degree, double (&time)[M])
 template <int N, int CAP, class C = NoCount>
 CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos, C &&cnt = C())
 {
+    static_assert(CAP >= (N > 2 ? N : 2) + 1, "an interval list holds every piece of its partition");
     using CT = std::decay_t<C>;
-    out.n = 0;
+    out.nb = 0;
+    out.keep = 0;
     double maxval = 0;
 #pragma unroll
     for (int i = 0; i <= N; i++) maxval = smax(maxval, dabs(op[i]));
@@ -655,27 +690,26 @@ CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos, C &&cnt = C())
         }
     }
 
-    double time[N > 2 ? N : 2] = {};
+    constexpr int NT = N > 2 ? N : 2;
+    double time[NT] = {};
     int roots = 0;
+    // The partition's cut points: the first `roots` of time[], clamped.
+    auto set_cuts = [&](int n) {
+        out.nb = n;
+#pragma unroll
+        for (int k = 0; k < NT; k++)
+            if (k < n) out.c[k] = clamp_cut(time[k]);
+    };
     if (reduced > 2) {
         if (!could_have_roots<N>(op, reduced, pos)) return;
         cnt(S_ISOLATE);
         if constexpr (stops_at_solver<CT>::value) return;
-        roots = isolate_roots<N, (N > 2 ? N : 2), plain_roots<CT>::value>(op, reduced, time);
-        // Pieces between consecutive roots (all in [0,1]) and the ends, as CTCD.cpp:160-176.
-        if (roots == 0) {
-            check_interval<N>(0.0, 1.0, op, reduced, out, pos);
-            return;
-        }
-        check_interval<N>(0.0, time[0], op, reduced, out, pos);
+        roots = isolate_roots<N, NT, plain_roots<CT>::value>(op, reduced, time);
+        // Pieces between consecutive roots (all in [0,1], ascending) and the ends, as CTCD.cpp:160-176.
+        set_cuts(roots);
 #pragma unroll
-        for (int i = 0; i + 1 < (N > 2 ? N : 2); i++)
-            if (i + 1 < roots) check_interval<N>(time[i], time[i + 1], op, reduced, out, pos);
-        double last = time[0];
-#pragma unroll
-        for (int i = 1; i < (N > 2 ? N : 2); i++)
-            if (i < roots) last = time[i];
-        check_interval<N>(last, 1.0, op, reduced, out, pos);
+        for (int p = 0; p <= NT; p++)
+            if (p <= roots) check_piece<N>(p, op, reduced, out, pos);
         return;
     } else if (reduced == 2) {
         roots = quad_roots(op[0], op[1], op[2], time[0], time[1]);
@@ -690,19 +724,20 @@ CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos, C &&cnt = C())
         time[0] = -op[1] / op[0];
         roots = 1;
     } else {
-        if ((!pos && op[0] <= 0) || (pos && op[0] >= 0)) push_ti(out, 0.0, 1.0);
+        if ((!pos && op[0] <= 0) || (pos && op[0] >= 0)) out.keep = 1u;   // the piece [0, 1]
         return;
     }
+    set_cuts(roots);
     if (roots > 0) {
-        if (time[0] >= 0) check_interval<N>(0, time[0], op, reduced, out, pos);
+        if (time[0] >= 0) check_piece<N>(0, op, reduced, out, pos);
         if (roots == 2) {
             if (!((time[0] < 0 && time[1] < 0) || (time[0] > 1.0 && time[1] > 1.0)))
-                check_interval<N>(time[0], time[1], op, reduced, out, pos);
+                check_piece<N>(1, op, reduced, out, pos);
         }
         double last = roots == 2 ? time[1] : time[0];
-        if (last <= 1.0) check_interval<N>(last, 1.0, op, reduced, out, pos);
+        if (last <= 1.0) check_piece<N>(roots, op, reduced, out, pos);
     } else {
-        check_interval<N>(0.0, 1.0, op, reduced, out, pos);
+        check_piece<N>(0, op, reduced, out, pos);
     }
 }
 
@@ -794,13 +829,18 @@ CCD_FHD bool pairwise_overlap(const double (&L)[N], const double (&U)[N])
             if (L[x] > U[y] || L[y] > U[x]) ov = false;
     return ov;
 }
+// Lists are walked piece by piece over their kept pieces (set bits, ascending: the list order).
 template <int A, int B>
 CCD_HD bool any_overlap(const Ivls<A> &a, unsigned mask, const Ivls<B> &b)
 {
-    for (int i = 0; i < a.n; i++) {
-        if (!((mask >> i) & 1u)) continue;
-        for (int j = 0; j < b.n; j++) {
-            double L[2] = {sel(a.l, i), sel(b.l, j)}, U[2] = {sel(a.u, i), sel(b.u, j)};
+    for (unsigned mi = a.keep & mask; mi; mi &= mi - 1) {
+        double al, au;
+        a.get(low_bit(mi), al, au);
+        for (unsigned mj = b.keep; mj; mj &= mj - 1) {
+            double L[2], U[2];
+            L[0] = al;
+            U[0] = au;
+            b.get(low_bit(mj), L[1], U[1]);
             if (pairwise_overlap(L, U)) return true;
         }
     }
@@ -809,28 +849,35 @@ CCD_HD bool any_overlap(const Ivls<A> &a, unsigned mask, const Ivls<B> &b)
 template <int A, int B, int C>
 CCD_HD bool any_overlap(const Ivls<A> &a, unsigned mask, const Ivls<B> &b, const Ivls<C> &c)
 {
-    for (int i = 0; i < a.n; i++) {
-        if (!((mask >> i) & 1u)) continue;
-        for (int j = 0; j < b.n; j++)
-            for (int k = 0; k < c.n; k++) {
-                double L[3] = {sel(a.l, i), sel(b.l, j), sel(c.l, k)}, U[3] = {sel(a.u, i), sel(b.u, j), sel(c.u, k)};
+    for (unsigned mi = a.keep & mask; mi; mi &= mi - 1) {
+        double L[3], U[3];
+        a.get(low_bit(mi), L[0], U[0]);
+        for (unsigned mj = b.keep; mj; mj &= mj - 1) {
+            b.get(low_bit(mj), L[1], U[1]);
+            for (unsigned mk = c.keep; mk; mk &= mk - 1) {
+                c.get(low_bit(mk), L[2], U[2]);
                 if (pairwise_overlap(L, U)) return true;
             }
+        }
     }
     return false;
 }
 template <int A, int B, int C, int D>
 CCD_HD bool any_overlap(const Ivls<A> &a, unsigned mask, const Ivls<B> &b, const Ivls<C> &c, const Ivls<D> &d)
 {
-    for (int i = 0; i < a.n; i++) {
-        if (!((mask >> i) & 1u)) continue;
-        for (int j = 0; j < b.n; j++)
-            for (int k = 0; k < c.n; k++)
-                for (int m = 0; m < d.n; m++) {
-                    double L[4] = {sel(a.l, i), sel(b.l, j), sel(c.l, k), sel(d.l, m)};
-                    double U[4] = {sel(a.u, i), sel(b.u, j), sel(c.u, k), sel(d.u, m)};
+    for (unsigned mi = a.keep & mask; mi; mi &= mi - 1) {
+        double L[4], U[4];
+        a.get(low_bit(mi), L[0], U[0]);
+        for (unsigned mj = b.keep; mj; mj &= mj - 1) {
+            b.get(low_bit(mj), L[1], U[1]);
+            for (unsigned mk = c.keep; mk; mk &= mk - 1) {
+                c.get(low_bit(mk), L[2], U[2]);
+                for (unsigned mm = d.keep; mm; mm &= mm - 1) {
+                    d.get(low_bit(mm), L[3], U[3]);
                     if (pairwise_overlap(L, U)) return true;
                 }
+            }
+        }
     }
     return false;
 }
@@ -851,23 +898,23 @@ CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3
 
     // Parallel-edge classification of each raw interval at its midpoint (CTCD.cpp:294-308).  The
     // reference copies the raw intervals into a coplanar and a parallel list; here they stay in `raw`
-    // and bit i of `par` marks interval i parallel.  Both lists keep the raw order, which is all the
+    // and bit p of `par` marks piece p parallel.  Both lists keep the raw order, which is all the
     // combination step below depends on, and two fewer 7-interval lists stay live across the four
     // barycentric solves.
     unsigned par = 0;
     int ncop = 0;
-#pragma unroll
-    for (int i = 0; i < 7; i++) {
-        if (i < raw.n) {
-            cnt(S_EE_PAR);
-            double midt = (raw.u[i] + raw.l[i]) / 2;
-            V3 a = (q0s - p0s) + midt * (vq0 - vp0);
-            V3 b = (q1s - p1s) + midt * (vq1 - vp1);
-            if (norm(cross(a, b)) < 1e-8)
-                par |= 1u << i;
-            else
-                ncop++;
-        }
+    for (unsigned m = raw.keep; m; m &= m - 1) {
+        const int p = low_bit(m);
+        cnt(S_EE_PAR);
+        double rl, ru;
+        raw.get(p, rl, ru);
+        double midt = (ru + rl) / 2;
+        V3 a = (q0s - p0s) + midt * (vq0 - vp0);
+        V3 b = (q1s - p1s) + midt * (vq1 - vp1);
+        if (norm(cross(a, b)) < 1e-8)
+            par |= 1u << p;
+        else
+            ncop++;
     }
     if (ncop == 0) return false;
 
@@ -875,34 +922,37 @@ CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3
     V3 x10 = p1s - q1s, v10 = vp1 - vq1;
     cnt(S_EE_BARY);
     barycentric_poly(x10, p0s - q0s, q0s - q1s, v10, vp0 - vq0, vq0 - vq1, a0, cnt);
-    if (a0.n == 0) return false;
+    if (a0.empty()) return false;
     if (prunes<C>::value && !any_overlap(raw, ~par, a0)) return false;
     cnt(S_EE_BARY);
     barycentric_poly(x10, q0s - p0s, p0s - q1s, v10, vq0 - vp0, vp0 - vq1, a1, cnt);
-    if (a1.n == 0) return false;
+    if (a1.empty()) return false;
     if (prunes<C>::value && !any_overlap(raw, ~par, a0, a1)) return false;
     x10 = p0s - q0s;
     v10 = vp0 - vq0;
     cnt(S_EE_BARY);
     barycentric_poly(x10, p1s - q1s, q1s - q0s, v10, vp1 - vq1, vq1 - vq0, b0, cnt);
-    if (b0.n == 0) return false;
+    if (b0.empty()) return false;
     if (prunes<C>::value && !any_overlap(raw, ~par, a0, a1, b0)) return false;
     cnt(S_EE_BARY);
     barycentric_poly(x10, q1s - p1s, p1s - q0s, v10, vq1 - vp1, vp1 - vq0, b1, cnt);
-    if (b1.n == 0) return false;
+    if (b1.empty()) return false;
 
     // All combinations (CTCD.cpp:352-396): a combination counts if its five intervals overlap
     // pairwise and their intersection touches no parallel interval; t is the smallest lower end.
     bool col = false;
     double mint = 1.0;
-    for (int i = 0; i < raw.n; i++) {
-        if ((par >> i) & 1u) continue;   // the coplanar intervals, in order
-        for (int j = 0; j < a0.n; j++)
-            for (int k = 0; k < a1.n; k++)
-                for (int l = 0; l < b0.n; l++)
-                    for (int m = 0; m < b1.n; m++) {
-                        double L[5] = {sel(raw.l, i), sel(a0.l, j), sel(a1.l, k), sel(b0.l, l), sel(b1.l, m)};
-                        double U[5] = {sel(raw.u, i), sel(a0.u, j), sel(a1.u, k), sel(b0.u, l), sel(b1.u, m)};
+    for (unsigned mi = raw.keep & ~par; mi; mi &= mi - 1) {   // the coplanar intervals, in order
+        double L[5], U[5];
+        raw.get(low_bit(mi), L[0], U[0]);
+        for (unsigned mj = a0.keep; mj; mj &= mj - 1) {
+            a0.get(low_bit(mj), L[1], U[1]);
+            for (unsigned mk = a1.keep; mk; mk &= mk - 1) {
+                a1.get(low_bit(mk), L[2], U[2]);
+                for (unsigned ml = b0.keep; ml; ml &= ml - 1) {
+                    b0.get(low_bit(ml), L[3], U[3]);
+                    for (unsigned mm = b1.keep; mm; mm &= mm - 1) {
+                        b1.get(low_bit(mm), L[4], U[4]);
                         bool ov = true;
 #pragma unroll
                         for (int x = 0; x < 5; x++)
@@ -917,14 +967,19 @@ CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3
                             iu = smin(U[x], iu);
                         }
                         bool skip = false;
-#pragma unroll
-                        for (int p = 0; p < 7; p++)
-                            if (((par >> p) & 1u) && !(il > raw.u[p] || raw.l[p] > iu)) skip = true;
+                        for (unsigned mp = par; mp; mp &= mp - 1) {
+                            double pl, pu;
+                            raw.get(low_bit(mp), pl, pu);
+                            if (!(il > pu || pl > iu)) skip = true;
+                        }
                         if (!skip) {
                             mint = smin(mint, il);
                             col = true;
                         }
                     }
+                }
+            }
+        }
     }
     if (col) t = mint;
     return col;
@@ -944,28 +999,31 @@ CCD_HD bool vertex_face(V3 q0s, V3 q1s, V3 q2s, V3 q3s, V3 q0e, V3 q1e, V3 q2e, 
     Ivls<4> e1, e2, e3;
     cnt(S_VF_PLANE);
     plane_poly(q0s - q1s, cross(q3s - q1s, q2s - q1s), q3s - q1s, v0 - v1, cross(v3 - v1, v2 - v1), v3 - v1, e1, cnt);
-    if (e1.n == 0) return false;
+    if (e1.empty()) return false;
     cnt(S_VF_PLANE);
     plane_poly(q0s - q2s, cross(q1s - q2s, q3s - q2s), q1s - q2s, v0 - v2, cross(v1 - v2, v3 - v2), v1 - v2, e2, cnt);
-    if (e2.n == 0) return false;
+    if (e2.empty()) return false;
     if (prunes<C>::value && !any_overlap(e1, ~0u, e2)) return false;
     cnt(S_VF_PLANE);
     plane_poly(q0s - q3s, cross(q2s - q3s, q1s - q3s), q2s - q3s, v0 - v3, cross(v2 - v3, v1 - v3), v2 - v3, e3, cnt);
-    if (e3.n == 0) return false;
+    if (e3.empty()) return false;
     if (prunes<C>::value && !any_overlap(e1, ~0u, e2, e3)) return false;
     Ivls<7> cop;
     cnt(S_VF_DIST);
     distance_poly(q0s - q1s, q2s - q1s, q3s - q1s, v0 - v1, v2 - v1, v3 - v1, minD, cop, cnt);
-    if (cop.n == 0) return false;
+    if (cop.empty()) return false;
 
     bool col = false;
     double mint = 1.0;
-    for (int i = 0; i < cop.n; i++)
-        for (int j = 0; j < e1.n; j++)
-            for (int k = 0; k < e2.n; k++)
-                for (int l = 0; l < e3.n; l++) {
-                    double L[4] = {sel(cop.l, i), sel(e1.l, j), sel(e2.l, k), sel(e3.l, l)};
-                    double U[4] = {sel(cop.u, i), sel(e1.u, j), sel(e2.u, k), sel(e3.u, l)};
+    for (unsigned mi = cop.keep; mi; mi &= mi - 1) {
+        double L[4], U[4];
+        cop.get(low_bit(mi), L[0], U[0]);
+        for (unsigned mj = e1.keep; mj; mj &= mj - 1) {
+            e1.get(low_bit(mj), L[1], U[1]);
+            for (unsigned mk = e2.keep; mk; mk &= mk - 1) {
+                e2.get(low_bit(mk), L[2], U[2]);
+                for (unsigned ml = e3.keep; ml; ml &= ml - 1) {
+                    e3.get(low_bit(ml), L[3], U[3]);
                     bool ov = true;
 #pragma unroll
                     for (int x = 0; x < 4; x++)
@@ -979,6 +1037,9 @@ CCD_HD bool vertex_face(V3 q0s, V3 q1s, V3 q2s, V3 q3s, V3 q0e, V3 q1e, V3 q2e, 
                     mint = smin(il, mint);
                     col = true;
                 }
+            }
+        }
+    }
     if (col) t = mint;
     return col;
 }
@@ -1004,13 +1065,13 @@ CCD_HD bool ve_sides(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, Ivls<3> &e1
     op[2] = dot(ab, ac);
     cnt(S_VE_QUAD);
     find_intervals<2>(op, e1, true, cnt);
-    if (e1.n == 0) return false;
+    if (e1.empty()) return false;
     op[0] = dot(vab, vcb);
     op[1] = dot(cb, vab) + dot(ab, vcb);
     op[2] = dot(ab, cb);
     cnt(S_VE_QUAD);
     find_intervals<2>(op, e2, true, cnt);
-    return e2.n != 0;
+    return !e2.empty();
 }
 
 // Which vertex-edge sub-tests of a stencil's window would reach their quartic (both side quadratics
@@ -1115,15 +1176,17 @@ CCD_HD bool vertex_edge(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double e
     op[1] = B * F + C * E - 2 * H * I;
     op[0] = C * F - I * I;
     find_intervals<4>(op, colin, false, cnt);
-    if (colin.n == 0) return false;
+    if (colin.empty()) return false;
 
     bool col = false;
     double mint = 1.0;
-    for (int i = 0; i < colin.n; i++)
-        for (int j = 0; j < e1.n; j++)
-            for (int k = 0; k < e2.n; k++) {
-                double L[3] = {sel(colin.l, i), sel(e1.l, j), sel(e2.l, k)};
-                double U[3] = {sel(colin.u, i), sel(e1.u, j), sel(e2.u, k)};
+    for (unsigned mi = colin.keep; mi; mi &= mi - 1) {
+        double L[3], U[3];
+        colin.get(low_bit(mi), L[0], U[0]);
+        for (unsigned mj = e1.keep; mj; mj &= mj - 1) {
+            e1.get(low_bit(mj), L[1], U[1]);
+            for (unsigned mk = e2.keep; mk; mk &= mk - 1) {
+                e2.get(low_bit(mk), L[2], U[2]);
                 bool ov = true;
 #pragma unroll
                 for (int x = 0; x < 3; x++)
@@ -1137,6 +1200,8 @@ CCD_HD bool vertex_edge(V3 q0s, V3 q1s, V3 q2s, V3 q0e, V3 q1e, V3 q2e, double e
                 mint = smin(il, mint);
                 col = true;
             }
+        }
+    }
     if (col) t = mint;
     return col;
 }
@@ -1168,40 +1233,32 @@ CCD_HD bool vertex_vertex(V3 q1s, V3 q2s, V3 q1e, V3 q2e, double eta, double &t,
         return false;
     }
     double op[3] = {a, b, c};
-    Ivls<1> iv;
-    iv.n = 0;
     if (roots == 2) {
-        check_interval<2>(0, t1, op, 2, iv, false);
-        if (iv.n) {
+        if (check_interval_passes<2>(0, t1, op, 2, false)) {
             t = 0;
             return true;
         }
-        check_interval<2>(t1, t2, op, 2, iv, false);
-        if (iv.n) {
+        if (check_interval_passes<2>(t1, t2, op, 2, false)) {
             t = t1;
             return true;
         }
-        check_interval<2>(t2, 1.0, op, 2, iv, false);
-        if (iv.n) {
+        if (check_interval_passes<2>(t2, 1.0, op, 2, false)) {
             t = t2;
             return true;
         }
         return false;
     } else if (roots == 1) {
-        check_interval<2>(0, t1, op, 2, iv, false);
-        if (iv.n) {
+        if (check_interval_passes<2>(0, t1, op, 2, false)) {
             t = 0;
             return true;
         }
-        check_interval<2>(t1, 1.0, op, 2, iv, false);
-        if (iv.n) {
+        if (check_interval_passes<2>(t1, 1.0, op, 2, false)) {
             t = t1;
             return true;
         }
         return false;
     }
-    check_interval<2>(0, 1.0, op, 2, iv, false);
-    if (iv.n) {
+    if (check_interval_passes<2>(0, 1.0, op, 2, false)) {
         t = 0;
         return true;
     }

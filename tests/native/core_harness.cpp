@@ -61,17 +61,14 @@ extern "C" int core_find_intervals(const double *op, int n, int pos, double *out
     double c[7];
     for (int i = 0; i <= n; i++) c[i] = op[i];
     ccd::Ivls<8> L;
-    L.n = 0;
     if (n == 2) ccd::find_intervals<2>(c, L, pos != 0);
     else if (n == 3) ccd::find_intervals<3>(c, L, pos != 0);
     else if (n == 4) ccd::find_intervals<4>(c, L, pos != 0);
     else if (n == 6) ccd::find_intervals<6>(c, L, pos != 0);
     else return -1;
-    for (int i = 0; i < L.n; i++) {
-        out[2 * i] = L.l[i];
-        out[2 * i + 1] = L.u[i];
-    }
-    return L.n;
+    int k = 0;
+    for (unsigned m = L.keep; m; m &= m - 1, k++) L.get(ccd::low_bit(m), out[2 * k], out[2 * k + 1]);
+    return k;
 }
 
 // The core's real roots of c[0..n] (n = 3..6) in [0,1].
