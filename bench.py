@@ -473,6 +473,8 @@ def main():
     ap.add_argument("--legs", type=str, default=",".join(ALL_LEGS),
                     help="secondary legs to run, comma separated (%s), or 'none'" % ", ".join(ALL_LEGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tune-cache", type=str, default=os.path.join(ROOT, "collisiondetection_amd", "queue_shapes.json"),
+                    help="JSON cache of tuned narrowphase shapes per GPU and workload shape")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy 5 times (scale reference for HBM byte counters)")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
@@ -506,6 +508,10 @@ def main():
     workload = (f"batched: {b.ninst} two-mesh instances (icospheres / noisy grids, log-uniform 80-20,480 tris per "
                 f"mesh), {F} tris, rigid motion + N(0, 0.005^2), seed 2, eta 1e-6")
     log(f"workload: {workload}")
+    # Queue shape and stencil grouping per kind from the committed tuning cache for this workload shape
+    # (tuned now, and stored, when the cache has no entry for this GPU and workload).
+    shape, how = b.tune(ETA_BATCH, ETA_BATCH, cache=args.tune_cache)
+    log(f"narrowphase shape ({how}): {shape}")
     for i in range(args.warmup):   # the first call runs synchronously and captures the pipeline
         r = b.detect(ETA_BATCH, ETA_BATCH, graph=True)
         log(f"warmup {i + 1}/{args.warmup}: {r['vf_candidates']} VF + {r['ee_candidates']} EE candidates, "
@@ -544,6 +550,8 @@ def main():
         "candidates": {"vf": nv, "ee": ne}, "collisions": {"vf": len(r["vf"]), "ee": len(r["ee"])},
         "roots": "isolated in [0,1] (derivative bracketing + Newton), in place of the reference's rpoly",
         "stage_timed_ms_per_step": wall_sync * 1e3,
+        "narrowphase_shape": {"source": how, "vf": shape[0:3] + [shape[6]], "ee": shape[3:6] + [shape[7]],
+                              "fields": ["block", "tile", "waves_per_cu", "grouping"]},
         "roofline": roof,
         "host": host,
     }

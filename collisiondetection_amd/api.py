@@ -607,6 +607,39 @@ class Batch:
     def instance_of_vertex(self, v):
         return np.searchsorted(self.vert_off, v, side="right") - 1
 
+    def tune(self, outer_eta, eta, cache=None):
+        """Tune the persistent narrowphase (queue shape and stencil grouping per kind) on this batch's
+        candidates and install the result on the batch's context.  With `cache` (a JSON file path) the
+        shape stored there for this workload shape (GPU architecture and compute units, faces, vertices,
+        instances) is installed
+        instead, and a newly tuned shape is stored.  Returns (shape, 'cache' or 'tuned')."""
+        import json
+        import os
+
+        torch = _torch()
+        props = torch.cuda.get_device_properties(self.ctx.torch_device)
+        arch = (getattr(props, "gcnArchName", "") or props.name).split(":")[0]
+        key = f"{arch}|cus={props.multi_processor_count}|faces={self.F}|verts={self.V}|instances={self.ninst}"
+        table = {}
+        if cache and os.path.exists(cache):
+            with open(cache) as fh:
+                table = json.load(fh)
+        if key in table:
+            set_queue_shape(table[key], self.ctx)
+            return list(table[key]), "cache"
+        self.detect(outer_eta, eta, graph=False)   # the candidates the tuner times
+        self.ctx.bind_stream()
+        shape = (ctypes.c_int * 8)()
+        self.ctx.check(N.lib().ccd_autotune(self.ctx.ptr, _ptr(self.qs), _ptr(self.qe), float(eta), shape))
+        shape = list(shape)
+        if cache:
+            table[key] = shape
+            tmp = cache + ".tmp"
+            with open(tmp, "w") as fh:
+                json.dump(table, fh, indent=1, sort_keys=True)
+            os.replace(tmp, cache)
+        return shape, "tuned"
+
     def detect(self, outer_eta, eta, graph=False):
         """One ccd_batch_detect call -> dict(vf, vf_toi, ee, ee_toi, vf_candidates, ee_candidates) with global
         vertex indices (use instance_of_vertex to split)."""
@@ -632,15 +665,23 @@ class Batch:
 
 
 def autotune(qstart, qend, eta, ctx: Context | None = None):
-    """Tune the persistent narrowphase on the last detection's candidates; returns the 6-int shape."""
+    """Tune the persistent narrowphase on the last detection's candidates; returns the 8-int shape
+    (block, tile, waves/CU per kind, then the grouping flag per kind)."""
     torch = _torch()
     ctx = ctx or default_context()
     ctx.bind_stream()
     qs = to_device(qstart, torch.float64, ctx).reshape(-1)
     qe = to_device(qend, torch.float64, ctx).reshape(-1)
-    shape = (ctypes.c_int * 6)()
+    shape = (ctypes.c_int * 8)()
     ctx.check(N.lib().ccd_autotune(ctx.ptr, _ptr(qs), _ptr(qe), float(eta), shape))
     return list(shape)
+
+
+def set_queue_shape(shape, ctx: Context | None = None):
+    """Install an 8-int shape from autotune (e.g. a cached one)."""
+    ctx = ctx or default_context()
+    arr = (ctypes.c_int * 8)(*[int(x) for x in shape])
+    ctx.check(N.lib().ccd_set_queue_shape(ctx.ptr, arr))
 
 
 # ------------------------------------------------------------------------------------------------

@@ -519,9 +519,10 @@ static int narrow_launches(ccd_ctx *ctx, const DetectArgs &a, int64_t nvf_max, i
     const int32_t *eek = (const int32_t *)ctx->bufs["ee_keys"].ptr;
     // The stage timers include the grouping.  Without exact host counts (a captured pipeline) the
     // signature pass reads the device counts and sorts the unused capacity last.
-    const bool group = ctx->group_stencils;
+    const bool group_vf = ctx->group_stencils && ctx->vf_shape.group;
+    const bool group_ee = ctx->group_stencils && ctx->ee_shape.group;
     prof_begin(ctx, CCD_T_NARROW_VF);
-    if (group) {
+    if (group_vf) {
         rc = group_by_signature(ctx, true, a.qs, a.qe, vfk, nvf_max, a.eta, exact ? nullptr : ctx->d_counts + C_VF, &vfk);
         if (rc) return rc;
     }
@@ -530,7 +531,7 @@ static int narrow_launches(ccd_ctx *ctx, const DetectArgs &a, int64_t nvf_max, i
     if (rc) return rc;
     prof_end(ctx, CCD_T_NARROW_VF);
     prof_begin(ctx, CCD_T_NARROW_EE);
-    if (group) {
+    if (group_ee) {
         rc = group_by_signature(ctx, false, a.qs, a.qe, eek, nee_max, a.eta, exact ? nullptr : ctx->d_counts + C_EE, &eek);
         if (rc) return rc;
     }
@@ -609,7 +610,7 @@ static int detect_graph(ccd_ctx *ctx, const DetectArgs &a)
     memcpy(saved, ctx->h_counts, sizeof(saved));
     // Capture with the capacities the synchronous run settled on; buffers used inside the capture are
     // sized for them first (nothing can be allocated while capturing).
-    if (ctx->group_stencils) {
+    if (ctx->group_stencils && (ctx->vf_shape.group || ctx->ee_shape.group)) {
         rc = reserve_grouping(ctx, ctx->cap_vf, ctx->cap_ee);
         if (rc) return rc;
     }
@@ -714,8 +715,9 @@ int ccd_set_queue_shape(ccd_ctx *ctx, const int *shape)
         s[k].block = shape[3 * k];
         s[k].tile = shape[3 * k + 1];
         s[k].waves_per_cu = shape[3 * k + 2];
+        s[k].group = shape[6 + k];
         if ((s[k].block != 64 && s[k].block != 128 && s[k].block != 256) || s[k].tile < 64 || s[k].tile % 64 ||
-            s[k].waves_per_cu < 1 || s[k].waves_per_cu > 32)
+            s[k].waves_per_cu < 1 || s[k].waves_per_cu > 32 || (s[k].group != 0 && s[k].group != 1))
             return set_error(ctx, CCD_E_INVALID, "invalid queue shape");
     }
     ctx->vf_shape = s[0];
@@ -745,10 +747,41 @@ int ccd_autotune(ccd_ctx *ctx, const double *qstart, const double *qend, double 
     hipEvent_t e0, e1;
     CCD_HIP_CHECK(ctx, hipEventCreate(&e0));
     CCD_HIP_CHECK(ctx, hipEventCreate(&e1));
+    // Milliseconds of `launch` on the context's stream: the second of two runs (the first warms caches).
+    auto timed = [&](auto &&launch, float &ms) -> int {
+        for (int rep = 0; rep < 2; rep++) {
+            CCD_HIP_CHECK(ctx, hipEventRecord(e0, st));
+            int rc = launch();
+            if (rc) return rc;
+            CCD_HIP_CHECK(ctx, hipEventRecord(e1, st));
+            CCD_HIP_CHECK(ctx, hipEventSynchronize(e1));
+            CCD_HIP_CHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
+        }
+        return CCD_OK;
+    };
+    auto queue = [&](int kind, const int32_t *k, const QueueShape &s) {
+        return [&, kind, k, s]() -> int {
+            int rc = fill_u64(ctx, (uint64_t *)work, 1, 0);
+            if (!rc) rc = fill_u64(ctx, (uint64_t *)cnt, 1, 0);
+            if (!rc)
+                rc = launch_narrow2_queue(ctx, kind == 0, qstart, qend, k, ndev, nmax[kind], eta, sink, sink_t, 0, cnt,
+                                          work, s);
+            return rc;
+        };
+    };
     QueueShape best[2];
     for (int kind = 0; kind < 2; kind++) {
         if (nmax[kind] == 0) continue;
         CCD_HIP_CHECK(ctx, hipMemcpyAsync(ndev, &nmax[kind], 8, hipMemcpyHostToDevice, st));
+        // The grouping as the pipeline runs it (signature pass, sort, gather), timed; the queue shapes
+        // on the grouped order; then the best shape on the stored order.  Grouping stays on only when
+        // it pays for itself.
+        const int32_t *grouped = keys[kind];
+        float t_group = 0.f;
+        int rc = timed(
+            [&]() { return group_by_signature(ctx, kind == 0, qstart, qend, keys[kind], nmax[kind], eta, nullptr, &grouped); },
+            t_group);
+        if (rc) return rc;
         float best_ms = 1e30f;
         for (int b : blocks)
             for (int t : tiles)
@@ -757,26 +790,18 @@ int ccd_autotune(ccd_ctx *ctx, const double *qstart, const double *qend, double 
                     s.block = b;
                     s.tile = t;
                     s.waves_per_cu = w;
-                    float total = 0.f;
-                    for (int rep = 0; rep < 2; rep++) {   // first launch warms caches
-                        int rc = fill_u64(ctx, (uint64_t *)work, 1, 0);
-                        if (!rc) rc = fill_u64(ctx, (uint64_t *)cnt, 1, 0);
-                        if (rc) return rc;
-                        CCD_HIP_CHECK(ctx, hipEventRecord(e0, st));
-                        rc = launch_narrow2_queue(ctx, kind == 0, qstart, qend, keys[kind], ndev, nmax[kind], eta, sink,
-                                                  sink_t, 0, cnt, work, s);
-                        if (rc) return rc;
-                        CCD_HIP_CHECK(ctx, hipEventRecord(e1, st));
-                        CCD_HIP_CHECK(ctx, hipEventSynchronize(e1));
-                        float ms = 0.f;
-                        CCD_HIP_CHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
-                        if (rep == 1) total = ms;
-                    }
-                    if (total < best_ms) {
-                        best_ms = total;
+                    float ms = 0.f;
+                    rc = timed(queue(kind, grouped, s), ms);
+                    if (rc) return rc;
+                    if (ms < best_ms) {
+                        best_ms = ms;
                         best[kind] = s;
                     }
                 }
+        float t_plain = 0.f;
+        rc = timed(queue(kind, keys[kind], best[kind]), t_plain);
+        if (rc) return rc;
+        best[kind].group = (grouped != keys[kind] && t_group + best_ms <= t_plain) ? 1 : 0;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -788,6 +813,7 @@ int ccd_autotune(ccd_ctx *ctx, const double *qstart, const double *qend, double 
             shape[3 * k] = best[k].block;
             shape[3 * k + 1] = best[k].tile;
             shape[3 * k + 2] = best[k].waves_per_cu;
+            shape[6 + k] = best[k].group;
         }
     }
     return CCD_OK;

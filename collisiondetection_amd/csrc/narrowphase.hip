@@ -644,7 +644,7 @@ int launch_narrow2(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, co
     // buffers are the caller's (order_scratch) or the context's own.
     const int32_t *order = nullptr;
     // (the order holds int32 stencil indices: launches past 2^31 - 1 stencils run in the given order)
-    if (n >= 4096 && n <= 0x7fffffffll && ctx->group_stencils) {
+    if (n >= 4096 && n <= 0x7fffffffll && ctx->group_stencils && (vf ? ctx->vf_shape : ctx->ee_shape).group) {
         OrderScratch os;
         if (order_scratch) {
             os = *order_scratch;

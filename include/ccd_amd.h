@@ -294,9 +294,11 @@ int ccd_batch_detect(ccd_ctx *ctx, int64_t ninst, const int64_t *inst_face_off, 
 /*
  * Tuning of the persistent narrowphase: times the work-queue kernel on the candidates of the last
  * detection on this context over block sizes {64, 128, 256}, tiles {64, 128, 256, 512, 1024} and
- * {4, 8} resident wavefronts per CU, keeps the fastest shape per stencil kind and returns it in
- * shape[0..5] = {vf block, vf tile, vf waves/CU, ee block, ee tile, ee waves/CU} (host, may be NULL).
- * ccd_set_queue_shape installs a shape (e.g. one cached from an earlier tuning run).
+ * {4, 8} resident wavefronts per CU, on the stencils grouped by sub-test signature, and then whether
+ * that grouping (its own passes included) beats the stored order; keeps the fastest per stencil kind
+ * and returns it in shape[0..7] = {vf block, vf tile, vf waves/CU, ee block, ee tile, ee waves/CU,
+ * vf grouping, ee grouping} (host, may be NULL; grouping 1 or 0).  ccd_set_queue_shape installs such
+ * a shape (e.g. one cached from an earlier tuning run).
  */
 int ccd_autotune(ccd_ctx *ctx, const double *qstart, const double *qend, double eta, int *shape);
 int ccd_set_queue_shape(ccd_ctx *ctx, const int *shape);

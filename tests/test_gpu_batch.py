@@ -105,10 +105,18 @@ def test_autotune_keeps_results():
     s = scenes.cloth(96, drop=0.05)
     d0 = api.detect(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"])
     shape = api.autotune(s["qstart"], s["qend"], s["eta"])
-    assert len(shape) == 6 and shape[0] in (64, 128, 256) and shape[3] in (64, 128, 256)
+    assert len(shape) == 8 and shape[0] in (64, 128, 256) and shape[3] in (64, 128, 256)
+    assert shape[6] in (0, 1) and shape[7] in (0, 1)
     d1 = api.detect(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"])
     assert _as_set(d0["vf"], d0["vf_toi"]) == _as_set(d1["vf"], d1["vf_toi"])
     assert _as_set(d0["ee"], d0["ee_toi"]) == _as_set(d1["ee"], d1["ee_toi"])
+    # the grouping switched either way leaves every result as it was
+    for g in (0, 1):
+        api.set_queue_shape(shape[:6] + [g, 1 - g])
+        d2 = api.detect(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"])
+        assert _as_set(d0["vf"], d0["vf_toi"]) == _as_set(d2["vf"], d2["vf_toi"]), g
+        assert _as_set(d0["ee"], d0["ee_toi"]) == _as_set(d2["ee"], d2["ee_toi"]), g
+    api.set_queue_shape(shape)
 
 
 def test_batch_graph_after_buffer_regrow():
