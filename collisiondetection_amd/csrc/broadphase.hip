@@ -1025,42 +1025,6 @@ __global__ __launch_bounds__(64) void k_row_sort(const unsigned long long *__res
     }
 }
 
-// Emission mask of entry (f, g): bit s (0..2) for vertex-face (fv[s], g), bit 3 + 3s + r for edge-edge
-// (edge s of f, edge r of g), the latter only for g > f.  fv and fo are f's (wave-uniform).
-__device__ __forceinline__ unsigned row_mask(const MeshView &m, int f, const int fv[3], const int fo[3], int g, int4 G0)
-{
-    const int gv[3] = {G0.x, G0.y, G0.z};
-    const bool gfix = is_fixed(m, gv[0]) && is_fixed(m, gv[1]) && is_fixed(m, gv[2]);
-    unsigned msk = 0;
-    // KDOPBroadPhase.cpp:157-175 (this side's three)
-#pragma unroll
-    for (int s = 0; s < 3; s++) {
-        if (is_fixed(m, fv[s]) && gfix) continue;
-        if (vf_owner<true>(m, fv[s], g, f)) msk |= 1u << s;
-    }
-    if (g < f) return msk;
-    int go[3] = {-2, -2, -2};
-    if (m.opp) {
-#pragma unroll
-        for (int r = 0; r < 3; r++) go[r] = m.opp[3 * (int64_t)g + r];
-    }
-    // KDOPBroadPhase.cpp:176-183
-#pragma unroll
-    for (int s = 0; s < 3; s++) {
-        const int u0 = fv[s], u1 = fv[(s + 1) % 3];
-        const bool ufix = is_fixed(m, u0) && is_fixed(m, u1);
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-            const int w0 = gv[r], w1 = gv[(r + 1) % 3];
-            if (ufix && is_fixed(m, w0) && is_fixed(m, w1)) continue;
-            const bool own = (fo[s] != -2 && go[r] != -2) ? ee_owner_opp<true>(m, fo[s], go[r], f, g)
-                                                          : ee_owner<true>(m, u0, u1, w0, w1, f, g);
-            if (own) msk |= 1u << (3 + 3 * s + r);
-        }
-    }
-    return msk;
-}
-
 __device__ __forceinline__ int64_t wave_sum(int64_t v)
 {
 #pragma unroll
@@ -1068,55 +1032,27 @@ __device__ __forceinline__ int64_t wave_sum(int64_t v)
     return v;
 }
 
-// WRITE false: masks and per-face counts (cnt_vf / cnt_ee).  WRITE true: the keys, at the exclusive
-// offsets the scan left in cnt_vf / cnt_ee.  Both return at once for raw emission (the count pass then
-// writes zero counts, so the scan totals are zero and the per-pair kernel's atomics start from there).
+// Write pass of the row expansion: the keys of each entry's emission mask, at the exclusive offsets the
+// scan left in cnt_vf / cnt_ee.  Returns at once for raw emission.
 constexpr int ROWS_BLOCK = 256;
-template <bool WRITE>
-__global__ __launch_bounds__(ROWS_BLOCK) void k_expand_rows(MeshView m, int64_t F,
-                                                            const unsigned long long *__restrict__ raw_dev,
-                                                            uint16_t *__restrict__ mask, int64_t *__restrict__ cnt_vf,
-                                                            int64_t *__restrict__ cnt_ee, int4 *__restrict__ vf_out,
-                                                            int64_t vf_cap, int4 *__restrict__ ee_out, int64_t ee_cap)
+__global__ __launch_bounds__(ROWS_BLOCK) void k_expand_rows_write(MeshView m, int64_t F,
+                                                                  const unsigned long long *__restrict__ raw_dev,
+                                                                  const uint16_t *__restrict__ mask,
+                                                                  const int64_t *__restrict__ cnt_vf,
+                                                                  const int64_t *__restrict__ cnt_ee,
+                                                                  int4 *__restrict__ vf_out, int64_t vf_cap,
+                                                                  int4 *__restrict__ ee_out, int64_t ee_cap)
 {
-    const bool raw = *raw_dev != 0;
-    if (raw && WRITE) return;
+    if (*raw_dev != 0) return;
     const int lane = __lane_id();
     const int64_t nwaves = (int64_t)gridDim.x * (ROWS_BLOCK / 64);
     for (int64_t fw = blockIdx.x * (int64_t)(ROWS_BLOCK / 64) + (threadIdx.x >> 6); fw < F; fw += nwaves) {
         const int f = __builtin_amdgcn_readfirstlane((int)fw);
-        if (raw) {
-            if (lane == 0) cnt_vf[f] = cnt_ee[f] = 0;
-            continue;
-        }
         const int lo = m.adj_off[f], hi = m.adj_off[f + 1];
         const int4 F0 = face3(m, f);
         const int fv[3] = {F0.x, F0.y, F0.z};
-        if (!WRITE) {
-            int fo[3] = {-2, -2, -2};
-            if (m.opp) {
-#pragma unroll
-                for (int s = 0; s < 3; s++) fo[s] = m.opp[3 * (int64_t)f + s];
-            }
-            int64_t nv = 0, ne = 0;
-            for (int base = lo; base < hi; base += 64) {
-                const int k = base + lane;
-                if (k < hi) {
-                    const int g = m.adj[k];
-                    const unsigned msk = row_mask(m, f, fv, fo, g, face3(m, g));
-                    mask[k] = (uint16_t)msk;
-                    nv += __popc(msk & 7u);
-                    ne += __popc(msk >> 3);
-                }
-            }
-            nv = wave_sum(nv);
-            ne = wave_sum(ne);
-            if (lane == 0) {
-                cnt_vf[f] = nv;
-                cnt_ee[f] = ne;
-            }
-        } else {
-            int64_t pos_v = cnt_vf[f], pos_e = cnt_ee[f];
+        int64_t pos_v = cnt_vf[f], pos_e = cnt_ee[f];
+        {
             for (int base = lo; base < hi; base += 64) {
                 const int k = base + lane;
                 const bool valid = k < hi;
@@ -1154,6 +1090,197 @@ __global__ __launch_bounds__(ROWS_BLOCK) void k_expand_rows(MeshView m, int64_t 
                 pos_v += __shfl(pv, 63);
                 pos_e += __shfl(pe, 63);
             }
+        }
+    }
+}
+
+// Count pass of the row expansion with the rows its owner tests read cached in LDS: f's own row and
+// those of the faces around f's vertices that precede f.  Every lane of the wavefront searches the same
+// few rows, so they are loaded once, coalesced, and searched in LDS instead of with dependent global
+// loads.  A directory of cached rows sits one entry per lane (face, LDS offset, length); lookups of a
+// face run in converged code with a ballot.  The tests that read other rows (the pair across both
+// edges when f's neighbour follows f; the generic rule for non-manifold edges) and rows that do not
+// fit use the table in global memory.
+constexpr int ROWS_LDS_PER_WAVE = 1536;   // int32 entries per wavefront
+
+struct RowDir {
+    const int32_t *lds;
+    int face = -1;   // this lane's directory entry (-1: none)
+    int off = 0, len = -1;   // its row in lds (len < 0: not cached)
+};
+
+// Directory lookup of a face (wave-uniform `a`, all lanes active): the row's LDS offset and length,
+// or false when the face is not cached.
+__device__ __forceinline__ bool dir_find(const RowDir &d, int a, int &off, int &len)
+{
+    const uint64_t hit = __ballot(d.face == a);
+    if (!hit) return false;
+    const int j = __ffsll((unsigned long long)hit) - 1;
+    off = __shfl(d.off, j);
+    len = __shfl(d.len, j);
+    return len >= 0;
+}
+
+// b in a sorted LDS row.
+__device__ __forceinline__ bool lds_row_has(const int32_t *lds, int off, int len, int b)
+{
+    int lo = off, hi = off + len;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (lds[mid] < b)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo < off + len && lds[lo] == b;
+}
+
+// related(a, b) with a's row from the directory when it is cached (cached: the lookup's result).
+__device__ __forceinline__ bool related_dir(const MeshView &m, const int32_t *lds, bool cached, int off, int len, int a,
+                                           int b)
+{
+    return cached ? lds_row_has(lds, off, len, b) : related<true>(m, a, b);
+}
+
+__global__ __launch_bounds__(ROWS_BLOCK) void k_expand_rows_count(MeshView m, int64_t F,
+                                                                  const unsigned long long *__restrict__ raw_dev,
+                                                                  uint16_t *__restrict__ mask,
+                                                                  int64_t *__restrict__ cnt_vf,
+                                                                  int64_t *__restrict__ cnt_ee)
+{
+    __shared__ int32_t s_rows[ROWS_BLOCK / 64][ROWS_LDS_PER_WAVE];
+    const bool raw = *raw_dev != 0;
+    const int lane = __lane_id();
+    int32_t *lds = s_rows[threadIdx.x >> 6];
+    const int64_t nwaves = (int64_t)gridDim.x * (ROWS_BLOCK / 64);
+    for (int64_t fw = blockIdx.x * (int64_t)(ROWS_BLOCK / 64) + (threadIdx.x >> 6); fw < F; fw += nwaves) {
+        const int f = __builtin_amdgcn_readfirstlane((int)fw);
+        if (raw) {
+            if (lane == 0) cnt_vf[f] = cnt_ee[f] = 0;
+            continue;
+        }
+        const int lo = m.adj_off[f], hi = m.adj_off[f + 1];
+        const int4 F0 = face3(m, f);
+        const int fv[3] = {F0.x, F0.y, F0.z};
+        int fo[3] = {-2, -2, -2};
+        if (m.opp) {
+#pragma unroll
+            for (int s = 0; s < 3; s++) fo[s] = m.opp[3 * (int64_t)f + s];
+        }
+        // Directory: f, then the faces around f's vertices that precede f, each once.
+        RowDir d;
+        d.lds = lds;
+        int nd = 0;
+        auto add = [&](int a) {
+            if (nd < 64 && __ballot(d.face == a) == 0) {
+                if (lane == nd) d.face = a;
+                nd++;
+            }
+        };
+        add(f);
+#pragma unroll
+        for (int s = 0; s < 3; s++)
+            for (int k = m.vf_off[fv[s]]; k < m.vf_off[fv[s] + 1]; k++) {
+                const int a = m.vf_list[k];
+                if (a < f) add(a);
+            }
+        int rlen = 0;
+        if (lane < nd) rlen = m.adj_off[d.face + 1] - m.adj_off[d.face];
+        int incl = rlen;
+#pragma unroll
+        for (int dd = 1; dd < 64; dd <<= 1) {
+            const int t = __shfl_up(incl, dd);
+            if (lane >= dd) incl += t;
+        }
+        if (lane < nd) {
+            d.off = incl - rlen;
+            d.len = (incl <= ROWS_LDS_PER_WAVE && rlen <= ROW_SORT_CAP) ? rlen : -1;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();   // the previous face's reads of lds are done
+        for (int j = 0; j < nd; j++) {
+            const int a = __shfl(d.face, j), off = __shfl(d.off, j), len = __shfl(d.len, j);
+            if (len < 0) continue;
+            const int src = m.adj_off[a];
+            for (int i = lane; i < len; i += 64) lds[off + i] = m.adj[src + i];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        int f_off = 0, f_len = 0;
+        const bool f_cached = dir_find(d, f, f_off, f_len);
+        int64_t nv = 0, ne = 0;
+        for (int base = lo; base < hi; base += 64) {
+            const int k = base + lane;
+            const bool valid = k < hi;
+            const int g = valid ? m.adj[k] : 0;
+            const int4 G0 = valid ? face3(m, g) : make_int4(0, 0, 0, 0);
+            const int gv[3] = {G0.x, G0.y, G0.z};
+            const bool gfix = valid && is_fixed(m, gv[0]) && is_fixed(m, gv[1]) && is_fixed(m, gv[2]);
+            // KDOPBroadPhase.cpp:157-175 (this side's three): owned unless a face around fv[s] that
+            // precedes f is related to g.
+            unsigned vmask = 0;
+#pragma unroll
+            for (int s = 0; s < 3; s++) {
+                bool own = valid && !(is_fixed(m, fv[s]) && gfix);
+                for (int kk = m.vf_off[fv[s]]; kk < m.vf_off[fv[s] + 1]; kk++) {
+                    const int a = m.vf_list[kk];
+                    if (a >= f) continue;
+                    int off = 0, len = 0;
+                    const bool cached = dir_find(d, a, off, len);
+                    if (own && related_dir(m, lds, cached, off, len, a, g)) own = false;
+                }
+                if (own) vmask |= 1u << s;
+            }
+            // KDOPBroadPhase.cpp:176-183, for g > f
+            unsigned emask = 0;
+            const bool ee = valid && g > f;
+            int go[3] = {-2, -2, -2};
+            if (ee && m.opp) {
+#pragma unroll
+                for (int r = 0; r < 3; r++) go[r] = m.opp[3 * (int64_t)g + r];
+            }
+#pragma unroll
+            for (int s = 0; s < 3; s++) {
+                const int u0 = fv[s], u1 = fv[(s + 1) % 3];
+                const bool ufix = is_fixed(m, u0) && is_fixed(m, u1);
+                const int oa = fo[s];
+                int oa_off = 0, oa_len = 0;
+                const bool oa_cached = oa >= 0 && oa < f && dir_find(d, oa, oa_off, oa_len);
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    const int w0 = gv[r], w1 = gv[(r + 1) % 3];
+                    if (!ee || (ufix && is_fixed(m, w0) && is_fixed(m, w1))) continue;
+                    const int ob = go[r];
+                    bool own;
+                    if (oa != -2 && ob != -2) {
+                        // ee_owner_opp with the cached rows: the other generating pairs (f, ob),
+                        // (oa, g) and (oa, ob)
+                        own = true;
+                        if (ob >= 0 && precedes(f, ob, f, g) && related_dir(m, lds, f_cached, f_off, f_len, f, ob))
+                            own = false;
+                        if (own && oa >= 0 && precedes(oa, g, f, g) &&
+                            related_dir(m, lds, oa_cached, oa_off, oa_len, oa, g))
+                            own = false;
+                        if (own && oa >= 0 && ob >= 0 && precedes(oa, ob, f, g) &&
+                            related_dir(m, lds, oa_cached, oa_off, oa_len, oa, ob))
+                            own = false;
+                    } else {
+                        own = ee_owner<true>(m, u0, u1, w0, w1, f, g);
+                    }
+                    if (own) emask |= 1u << (3 * s + r);
+                }
+            }
+            const unsigned msk = vmask | (emask << 3);
+            if (valid) mask[k] = (uint16_t)msk;
+            nv += __popc(vmask);
+            ne += __popc(emask);
+        }
+        nv = wave_sum(nv);
+        ne = wave_sum(ne);
+        if (lane == 0) {
+            cnt_vf[f] = nv;
+            cnt_ee[f] = ne;
         }
     }
 }
@@ -1400,13 +1527,13 @@ static int launch_expand(ccd_ctx *ctx, const MeshView &mv, int64_t F, const BPBu
         hipLaunchKernelGGL(k_row_sort, dim3(gs), dim3(64), 0, st, c + C_BAD, F, mv.adj_off, b.adj);
         CCD_LAUNCH_CHECK(ctx);
         const unsigned gw = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(F, ROWS_BLOCK / 64), 65536));
-        hipLaunchKernelGGL(k_expand_rows<false>, dim3(gw), dim3(ROWS_BLOCK), 0, st, mv, F, c + C_BAD, b.adj_mask,
-                           b.exp_vf, b.exp_ee, (int4 *)nullptr, (int64_t)0, (int4 *)nullptr, (int64_t)0);
+        hipLaunchKernelGGL(k_expand_rows_count, dim3(gw), dim3(ROWS_BLOCK), 0, st, mv, F, c + C_BAD, b.adj_mask,
+                           b.exp_vf, b.exp_ee);
         CCD_LAUNCH_CHECK(ctx);
         int rc = exclusive_scan_i64(ctx, b.exp_vf, b.exp_vf, F, b.adj_tmp, (int64_t *)(c + C_VF));
         if (!rc) rc = exclusive_scan_i64(ctx, b.exp_ee, b.exp_ee, F, b.adj_tmp, (int64_t *)(c + C_EE));
         if (rc) return rc;
-        hipLaunchKernelGGL(k_expand_rows<true>, dim3(gw), dim3(ROWS_BLOCK), 0, st, mv, F, c + C_BAD, b.adj_mask,
+        hipLaunchKernelGGL(k_expand_rows_write, dim3(gw), dim3(ROWS_BLOCK), 0, st, mv, F, c + C_BAD, b.adj_mask,
                            b.exp_vf, b.exp_ee, b.vfk, ctx->cap_vf, b.eek, ctx->cap_ee);
         CCD_LAUNCH_CHECK(ctx);
     }
