@@ -11,8 +11,8 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libccd_amd.so")
 LOADGEN = os.path.join(PKG, "serve_loadgen")
 OBJDIR = os.path.join(PKG, "_build")
-SOURCES = ["narrowphase.hip", "broadphase.hip", "distance.hip", "devprims.hip", "server.hip", "capi.hip",
-           "velocity_filter.hip"]
+SOURCES = ["narrowphase.hip", "workcount.hip", "broadphase.hip", "distance.hip", "devprims.hip", "server.hip",
+           "capi.hip", "velocity_filter.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off: the reference's arithmetic has no fused multiply-add; every a*b+c must round twice.

@@ -30,7 +30,14 @@
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
+// The device cascade is inlined into its kernels: its interval lists and polynomials then stay in
+// registers instead of the stack frames of out-of-line calls (per-lane scratch).  A file that defines
+// CCD_OUTLINE_CASCADE leaves the inlining to the compiler.
+#if defined(CCD_OUTLINE_CASCADE)
 #define CCD_HD __host__ __device__ inline
+#else
+#define CCD_HD __host__ __device__ __forceinline__
+#endif
 #define CCD_FHD __host__ __device__ __forceinline__
 #define CCD_HD_NI __host__ __device__   // with static __attribute__((noinline)): one out-of-line copy per file
 #else

@@ -185,51 +185,6 @@ __global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue(const double
     }
 }
 
-// Work accounting: the same tests with a per-lane stage counter, reduced per wavefront.  Its root
-// isolation is the plain form (no Bernstein shortcuts, plain Newton): with the product's form this
-// kernel stalled once on the device, on the converging-spheres candidates, for no cause found in the
-// code, its assembly or sanitized host runs of the same path.  Which stages run depends only on the
-// interval structure, which both forms give alike.
-struct LaneCount {
-    static constexpr bool plain_roots = true;
-    uint32_t n[ccd::S_COUNT];
-    __device__ __forceinline__ void operator()(int s) { n[s]++; }
-};
-
-template <bool VF, class LC>
-__global__ __launch_bounds__(256) CCD_TWO_WAVES void k_narrow2_count(const double *__restrict__ qs, const double *__restrict__ qe,
-                                                       const int4 *__restrict__ keys, int64_t n, double eta,
-                                                       unsigned long long *counts)
-{
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    LC c;
-#pragma unroll
-    for (int k = 0; k < ccd::S_COUNT; k++) c.n[k] = 0;
-    if (i < n) {
-        int4 k = keys[i];
-        int v[4] = {k.x, k.y, k.z, k.w};
-        V3 s[4], e[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            V3 a = load3(qs, v[j]);
-            V3 b = load3(qe, v[j]);
-            s[j] = ccd::mk(ccd::stitch0(a.x, b.x), ccd::stitch0(a.y, b.y), ccd::stitch0(a.z, b.z));
-            e[j] = b;
-        }
-        double t = 0.0;
-        if (VF)
-            ccd::check_vf_window(s, e, eta, t, c);
-        else
-            ccd::check_ee_window(s, e, eta, t, c);
-    }
-#pragma unroll
-    for (int k = 0; k < ccd::S_COUNT; k++) {
-        uint32_t v = c.n[k];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (__lane_id() == 0 && v) atomicAdd(&counts[k], (unsigned long long)v);
-    }
-}
-
 // Multi-entry histories: stitch on the fly (History.cpp:98-140) and test window by window.
 template <bool VF, class M>
 __global__ __launch_bounds__(256) CCD_TWO_WAVES void k_narrow_hist(const int64_t *__restrict__ off, const double *__restrict__ ht,
@@ -696,22 +651,6 @@ int launch_narrow2_compact(ccd_ctx *ctx, bool vf, const double *qs, const double
         CCD_PRODUCT_LAUNCH(hipLaunchKernelGGL((k_narrow2_compact<false, M>), dim3(blocks_for(n_max, 256)),
                                                   dim3(256), 0, ctx->stream, qs, qe, (const int4 *)keys, n_dev, n_max,
                                                   eta, (int4 *)out_keys, out_toi, cap, (unsigned long long *)counter));
-    CCD_LAUNCH_CHECK(ctx);
-    return CCD_OK;
-}
-
-int launch_narrow2_count(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
-                         int64_t n, double eta, unsigned long long *counts)
-{
-    if (n == 0) return CCD_OK;
-#define CCD_COUNT_LAUNCH(V, LCT)                                                                                    \
-    hipLaunchKernelGGL((k_narrow2_count<V, LCT>), dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream, qs, qe,     \
-                       (const int4 *)keys, n, eta, counts)
-    if (vf)
-        CCD_COUNT_LAUNCH(true, LaneCount);
-    else
-        CCD_COUNT_LAUNCH(false, LaneCount);
-#undef CCD_COUNT_LAUNCH
     CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
 }
