@@ -185,6 +185,168 @@ __global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue(const double
     }
 }
 
+// Edge-edge queue kernel with the window's vertex-edge sub-tests shared within a wavefront.  After the
+// edge-edge test, a stencil that it did not decide runs four vertex-edge tests, (s0; s2 s3),
+// (s1; s2 s3), (s2; s0 s1), (s3; s0 s1) (CTCDNarrowPhase.cpp:104-115), then four vertex-vertex tests.
+// A vertex-edge test depends only on its three vertices in call order (every stencil of a two-entry
+// history has the same window), and neighbouring stencils share them: a wavefront's 64 consecutive
+// stencils come from a few neighbouring face pairs, where each vertex meets the same edge in several
+// stencils.  So the wavefront hashes its up to 256 tests in LDS by their vertex triples, runs each
+// distinct one once (spread over the lanes), and every stencil then takes the first hit in its own
+// order, then its vertex-vertex tests: the same sub-test, time and result as the sequential cascade.
+constexpr int VE_TASKS = 256;   // 4 per lane
+constexpr int VE_TAB = 512;     // hash slots (more than the tasks: probing always ends)
+struct VeShare {
+    int kv[VE_TASKS], ka[VE_TASKS], kb[VE_TASKS];   // task: vertex, edge ends (call order); kv -1: none
+    int tab[VE_TAB];                                // hash table: a task index, -1 empty
+    int rep[VE_TASKS];                              // each task's representative (the first with its key)
+    int uniq[VE_TASKS];                             // representatives, compacted
+    double t[VE_TASKS];                             // representative's time of impact
+    unsigned char hit[VE_TASKS];                    // representative's result
+};
+
+// Orders a wavefront's LDS accesses (its own region; LDS instructions of one wavefront execute in
+// order, so this only stops the compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void window_pos(const double *__restrict__ qs, const double *__restrict__ qe, int v, V3 &s,
+                                           V3 &e)
+{
+    const V3 a = load3(qs, v), b = load3(qe, v);
+    s = ccd::mk(ccd::stitch0(a.x, b.x), ccd::stitch0(a.y, b.y), ccd::stitch0(a.z, b.z));
+    e = b;
+}
+
+template <int BS, class M>
+__global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue_ee_shared(
+    const double *__restrict__ qs, const double *__restrict__ qe, const int4 *__restrict__ keys,
+    const int64_t *__restrict__ n_dev, int64_t n_max, double eta, int4 *__restrict__ out_keys,
+    double *__restrict__ out_toi, int64_t cap, unsigned long long *counter, unsigned long long *work, int tile)
+{
+    __shared__ VeShare s_share[BS / 64];
+    VeShare &L = s_share[threadIdx.x >> 6];
+    const int64_t n = *n_dev < n_max ? *n_dev : n_max;
+    const int lane = __lane_id();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    while (true) {
+        unsigned long long start = 0;
+        if (lane == 0) start = atomicAdd(work, (unsigned long long)tile);
+        start = __shfl(start, 0);
+        if ((int64_t)start >= n) break;
+        int64_t end = (int64_t)start + tile < n ? (int64_t)start + tile : n;
+        for (int64_t j0 = (int64_t)start; j0 < end; j0 += 64) {
+            const int64_t i = j0 + lane;
+            const bool valid = i < end;
+            int4 k = make_int4(0, 0, 0, 0);
+            V3 s[4], e[4];
+            double t = 0.0;
+            int sub = -1;
+            if (valid) {
+                k = keys[i];
+                const int v[4] = {k.x, k.y, k.z, k.w};
+#pragma unroll
+                for (int j = 0; j < 4; j++) window_pos(qs, qe, v[j], s[j], e[j]);
+                if (ccd::edge_edge(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t, M())) sub = 0;
+            }
+            const bool need = valid && sub < 0;
+            // The stencil's four vertex-edge tests as tasks r * 64 + lane.
+            const int tv[4] = {k.x, k.y, k.z, k.w};
+            for (int q = lane; q < VE_TAB; q += 64) L.tab[q] = -1;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int a = r < 2 ? 2 : 0;   // the edge: (s2, s3) for r = 0, 1; (s0, s1) for r = 2, 3
+                const int j = r * 64 + lane;
+                L.kv[j] = need ? tv[r] : -1;
+                L.ka[j] = tv[a];
+                L.kb[j] = tv[a + 1];
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int j = r * 64 + lane;
+                int rep = -1;
+                if (need) {
+                    const int a = r < 2 ? 2 : 0;
+                    const int kv = tv[r], ka = tv[a], kb = tv[a + 1];
+                    unsigned h = ((unsigned)kv * 0x9E3779B1u) ^ ((unsigned)ka * 0x85EBCA6Bu) ^ ((unsigned)kb * 0xC2B2AE35u);
+                    h = (h ^ (h >> 15)) & (VE_TAB - 1);
+                    while (true) {
+                        const int c = atomicCAS(&L.tab[h], -1, j);
+                        if (c == -1) {
+                            rep = j;
+                            break;
+                        }
+                        if (L.kv[c] == kv && L.ka[c] == ka && L.kb[c] == kb) {
+                            rep = c;
+                            break;
+                        }
+                        h = (h + 1) & (VE_TAB - 1);
+                    }
+                }
+                L.rep[j] = rep;
+            }
+            wave_lds_sync();
+            int nu = 0;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int j = r * 64 + lane;
+                const bool own = need && L.rep[j] == j;
+                const uint64_t m = __ballot(own);
+                if (own) L.uniq[nu + __popcll(m & lt)] = j;
+                nu += __popcll(m);
+            }
+            wave_lds_sync();
+            for (int u = lane; u < nu; u += 64) {
+                const int j = L.uniq[u];
+                V3 ps, pe, as, ae, bs, be;
+                window_pos(qs, qe, L.kv[j], ps, pe);
+                window_pos(qs, qe, L.ka[j], as, ae);
+                window_pos(qs, qe, L.kb[j], bs, be);
+                double tt = 0.0;
+                const bool h = ccd::vertex_edge(ps, as, bs, pe, ae, be, eta, tt, M());
+                L.hit[j] = h ? 1 : 0;
+                L.t[j] = tt;
+            }
+            wave_lds_sync();
+            if (need) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int rr = L.rep[r * 64 + lane];
+                    if (sub < 0 && L.hit[rr]) {
+                        sub = 1 + r;
+                        t = L.t[rr];
+                    }
+                }
+                if (sub < 0 && ccd::vertex_vertex(s[0], s[2], e[0], e[2], eta, t, M())) sub = 5;
+                if (sub < 0 && ccd::vertex_vertex(s[0], s[3], e[0], e[3], eta, t, M())) sub = 6;
+                if (sub < 0 && ccd::vertex_vertex(s[1], s[2], e[1], e[2], eta, t, M())) sub = 7;
+                if (sub < 0 && ccd::vertex_vertex(s[1], s[3], e[1], e[3], eta, t, M())) sub = 8;
+            }
+            wave_lds_sync();   // the next chunk rewrites L
+            const bool hit = sub >= 0;
+            const uint64_t mask = __ballot(hit);
+            if (mask) {
+                const int leader = __ffsll((unsigned long long)mask) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+                base = __shfl(base, leader);
+                if (hit) {
+                    const int64_t pos = (int64_t)base + __popcll(mask & lt);
+                    if (pos < cap) {
+                        out_keys[pos] = k;
+                        if (out_toi) out_toi[pos] = t;
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Multi-entry histories: stitch on the fly (History.cpp:98-140) and test window by window.
 template <bool VF, class M>
 __global__ __launch_bounds__(256) CCD_TWO_WAVES void k_narrow_hist(const int64_t *__restrict__ off, const double *__restrict__ ht,
@@ -660,10 +822,16 @@ static void launch_queue_bs(ccd_ctx *ctx, int bs, unsigned grid, const double *q
                             const int32_t *keys, const int64_t *n_dev, int64_t n_max, double eta, int32_t *out_keys,
                             double *out_toi, int64_t cap, int64_t *counter, unsigned long long *work, int tile)
 {
+    // Edge-edge stencils share their vertex-edge tests within a wavefront (k_narrow2_queue_ee_shared).
 #define CCD_QUEUE_LAUNCH(B)                                                                                         \
-    hipLaunchKernelGGL((k_narrow2_queue<VF, B, M>), dim3(grid), dim3(B), 0, ctx->stream, qs, qe,                 \
-                       (const int4 *)keys, n_dev, n_max, eta, (int4 *)out_keys, out_toi, cap,                      \
-                       (unsigned long long *)counter, work, tile)
+    if (VF)                                                                                                         \
+        hipLaunchKernelGGL((k_narrow2_queue<true, B, M>), dim3(grid), dim3(B), 0, ctx->stream, qs, qe,           \
+                           (const int4 *)keys, n_dev, n_max, eta, (int4 *)out_keys, out_toi, cap,                  \
+                           (unsigned long long *)counter, work, tile);                                            \
+    else                                                                                                            \
+        hipLaunchKernelGGL((k_narrow2_queue_ee_shared<B, M>), dim3(grid), dim3(B), 0, ctx->stream, qs, qe,       \
+                           (const int4 *)keys, n_dev, n_max, eta, (int4 *)out_keys, out_toi, cap,                  \
+                           (unsigned long long *)counter, work, tile)
     if (bs == 64)
         CCD_QUEUE_LAUNCH(64);
     else if (bs == 128)
