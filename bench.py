@@ -478,7 +478,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy 5 times (scale reference for HBM byte counters)")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
-    ap.add_argument("--serve-rates", type=str, default="2000,400",
+    ap.add_argument("--serve-rates", type=str, default="2000,250",
                     help="Poisson request rates (req/s), each on freshly opened sessions: the configured 2,000 "
                          "(above one GPU's capacity for these sessions: the run stops at overload) and a "
                          "sustainable one")

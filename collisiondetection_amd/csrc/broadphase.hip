@@ -58,33 +58,6 @@ Axes make_axes()
     return ax;
 }
 
-// FP32 neighbours of a float (f finite or +-inf, not NaN).
-__device__ __forceinline__ float next_down(float f)
-{
-    if (f == 0.0f) return __int_as_float(0x80000001);
-    int b = __float_as_int(f);
-    return __int_as_float(f > 0 ? b - 1 : b + 1);
-}
-__device__ __forceinline__ float next_up(float f)
-{
-    if (f == 0.0f) return __int_as_float(0x00000001);
-    int b = __float_as_int(f);
-    return __int_as_float(f > 0 ? b + 1 : b - 1);
-}
-// Outward rounding of a double bound to float: the float interval always contains the double one.
-__device__ __forceinline__ float f32_down(double d)
-{
-    float f = (float)d;
-    if ((double)f > d) f = next_down(f);
-    return f;
-}
-__device__ __forceinline__ float f32_up(double d)
-{
-    float f = (float)d;
-    if ((double)f < d) f = next_up(f);
-    return f;
-}
-
 inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // ------------------------------------------------------------------------------------------------
@@ -357,19 +330,115 @@ __global__ __launch_bounds__(256) void k_escape(const uint32_t *__restrict__ rng
     escape[n] = e;
 }
 
-// Node record used by the traversal: 26 FP32 bounds, then the first child (internal) or the sorted
-// leaf index (leaf), then the escape link.  112 bytes = 7 x 16.
+// Node record used by the traversal: 26 bounds quantised to 16 bits, then the first child (internal)
+// or the sorted leaf index (leaf), then the escape link: 60 bytes, one 64-byte line (112 with FP32
+// bounds; the traversal is bound by fetching node records, so their size is its cost).
+//
+// Quantisation.  Per axis, a bound v maps to x = (v - origin) * inv, with origin the smallest finite
+// lower bound of all leaves and inv = 65535 / (largest finite upper bound - origin); lower bounds take
+// floor(x), upper bounds ceil(x), both clamped to [0, 65535], NaN to the full range.  Correctly rounded
+// subtraction and multiplication are monotone, and so are floor, ceil and the clamps, so overlapping
+// boxes keep overlapping (a.hi >= b.lo gives q(a.hi) >= q(b.lo)): the quantised test only ever keeps
+// more, and the FP64 leaf test decides.  Values outside the range (a refit whose mesh moved) clamp,
+// which stays conservative.
 struct __align__(16) NodeRec {
-    float b[NB];
+    uint16_t b[NB];
     uint32_t child;
     uint32_t esc;
 };
+constexpr double QMAX = 65535.0;
+__device__ __forceinline__ uint16_t quant_lo(double v, double o, double inv)
+{
+    const double x = (v - o) * inv;
+    if (!(x >= 0.0)) return 0;   // below the range, -inf, NaN: the smallest
+    if (x >= QMAX) return (uint16_t)QMAX;
+    return (uint16_t)floor(x);
+}
+__device__ __forceinline__ uint16_t quant_hi(double v, double o, double inv)
+{
+    const double x = (v - o) * inv;
+    if (x != x) return (uint16_t)QMAX;   // NaN: the largest
+    if (x <= 0.0) return 0;
+    if (x >= QMAX) return (uint16_t)QMAX;
+    return (uint16_t)ceil(x);
+}
+
+// Quantisation parameters: per-block partial min of the finite lower and max of the finite upper
+// bounds per axis (wavefront shuffles, then LDS), then one block folds the partials into origin[13]
+// and inv[13] (qparam[0..12], qparam[13..25]).
+__global__ __launch_bounds__(256) void k_qrange_partial(const double *__restrict__ dop64, int64_t F,
+                                                        double *__restrict__ partial)
+{
+    __shared__ double s[4][NB];
+    double lo[KAX], hi[KAX];
+#pragma unroll
+    for (int k = 0; k < KAX; k++) {
+        lo[k] = __builtin_inf();
+        hi[k] = -__builtin_inf();
+    }
+    for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < F; f += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < KAX; k++) {
+            const double a = dop64[NB * f + k], b = dop64[NB * f + KAX + k];
+            if (__builtin_isfinite(a) && a < lo[k]) lo[k] = a;
+            if (__builtin_isfinite(b) && b > hi[k]) hi[k] = b;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KAX; k++) {
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) {
+            const double a = __shfl_xor(lo[k], d), b = __shfl_xor(hi[k], d);
+            lo[k] = a < lo[k] ? a : lo[k];
+            hi[k] = b > hi[k] ? b : hi[k];
+        }
+    }
+    const int w = threadIdx.x >> 6;
+    if (__lane_id() == 0) {
+#pragma unroll
+        for (int k = 0; k < KAX; k++) {
+            s[w][k] = lo[k];
+            s[w][KAX + k] = hi[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < NB) {
+        const int c = threadIdx.x;
+        double v = s[0][c];
+        for (int ww = 1; ww < 4; ww++) v = c < KAX ? (s[ww][c] < v ? s[ww][c] : v) : (s[ww][c] > v ? s[ww][c] : v);
+        partial[NB * blockIdx.x + c] = v;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_qrange_final(const double *__restrict__ partial, int nparts,
+                                                     double *__restrict__ qparam)
+{
+    const int k = blockIdx.x;   // one wavefront per axis
+    double lo = __builtin_inf(), hi = -__builtin_inf();
+    for (int p = threadIdx.x; p < nparts; p += 64) {
+        const double a = partial[NB * p + k], b = partial[NB * p + KAX + k];
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        const double a = __shfl_xor(lo, d), b = __shfl_xor(hi, d);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    if (threadIdx.x != 0) return;
+    const double ext = hi - lo;
+    const bool ok = __builtin_isfinite(lo) && __builtin_isfinite(hi) && ext > 0.0 && __builtin_isfinite(QMAX / ext);
+    qparam[k] = __builtin_isfinite(lo) ? lo : 0.0;
+    qparam[KAX + k] = ok ? QMAX / ext : 0.0;
+}
 
 // Leaf records in sorted order, and the FP64 bounds and vertex triples in sorted order.
 __global__ __launch_bounds__(256) void k_leaf_records(const int32_t *__restrict__ order,
                                                       const double *__restrict__ dop64,
                                                       const int32_t *__restrict__ faces, int64_t F,
                                                       const uint32_t *__restrict__ escape,
+                                                      const double *__restrict__ qparam,
                                                       NodeRec *__restrict__ nodes, double *__restrict__ sdop64,
                                                       int4 *__restrict__ sfaces, int32_t *__restrict__ rank)
 {
@@ -382,8 +451,8 @@ __global__ __launch_bounds__(256) void k_leaf_records(const int32_t *__restrict_
 #pragma unroll
     for (int k = 0; k < KAX; k++) {
         double lo = src[k], hi = src[KAX + k];
-        r.b[k] = f32_down(lo);
-        r.b[KAX + k] = f32_up(hi);
+        r.b[k] = quant_lo(lo, qparam[k], qparam[KAX + k]);
+        r.b[KAX + k] = quant_hi(hi, qparam[k], qparam[KAX + k]);
         sdop64[NB * j + k] = lo;
         sdop64[NB * j + KAX + k] = hi;
     }
@@ -504,7 +573,7 @@ __global__ __launch_bounds__(256) void k_refit_level(int64_t F, uint32_t level, 
 // ------------------------------------------------------------------------------------------------
 // 6. Traversal: unordered overlapping non-neighbouring face pairs.
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ bool overlap32(const float *q, const float *b)
+__device__ __forceinline__ bool overlap16(const uint16_t *q, const uint16_t *b)
 {
     bool ok = true;
 #pragma unroll
@@ -567,7 +636,7 @@ __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__re
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     bool active = i < F;
-    float q32[NB];
+    uint16_t q16[NB];
     const double *q64 = sdop64 + NB * (active ? i : 0);
     int4 qf = make_int4(0, 0, 0, 0);
     uint32_t node = NIL;
@@ -575,13 +644,13 @@ __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__re
     if (active) {
         const NodeRec &me = nodes[F - 1 + i];
 #pragma unroll
-        for (int k = 0; k < NB; k++) q32[k] = me.b[k];
+        for (int k = 0; k < NB; k++) q16[k] = me.b[k];
         qf = sfaces[i];
         node = me.esc;
         if (face_inst) iend = inst_off[face_inst[qf.w] + 1];
     } else {
 #pragma unroll
-        for (int k = 0; k < NB; k++) q32[k] = 0.0f;
+        for (int k = 0; k < NB; k++) q16[k] = 0;
     }
     const uint32_t nint = (uint32_t)(F - 1);
     auto first_leaf = [nint](uint32_t n) -> int64_t { return n < nint ? (int64_t)n : (int64_t)(n - nint); };
@@ -591,12 +660,12 @@ __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__re
     while (true) {
         if (node != NIL) {
             const NodeRec &n = nodes[node];
-            float b[NB];
+            uint16_t b[NB];
 #pragma unroll
             for (int k = 0; k < NB; k++) b[k] = n.b[k];
             uint32_t child = n.child, esc = n.esc;
             bool descend = false;
-            if (overlap32(q32, b)) {
+            if (overlap16(q16, b)) {
                 if (node < nint) {
                     descend = true;
                 } else {
@@ -1184,8 +1253,20 @@ __global__ __launch_bounds__(ROWS_BLOCK) void k_expand_rows_count(MeshView m, in
                 const int a = m.vf_list[k];
                 if (a < f) add(a);
             }
-        int rlen = 0;
-        if (lane < nd) rlen = m.adj_off[d.face + 1] - m.adj_off[d.face];
+        // Per directory entry: its row in the table, and which of f's vertices it holds (bit s: fv[s]).
+        // The directory holds every face around f's vertices that precedes f unless it overflowed.
+        const bool complete = nd < 64;
+        int rlen = 0, src = 0;
+        unsigned vbits = 0;
+        if (lane < nd) {
+            src = m.adj_off[d.face];
+            rlen = m.adj_off[d.face + 1] - src;
+            if (lane > 0) {
+#pragma unroll
+                for (int s = 0; s < 3; s++)
+                    if (has_vertex(m, d.face, fv[s])) vbits |= 1u << s;
+            }
+        }
         int incl = rlen;
 #pragma unroll
         for (int dd = 1; dd < 64; dd <<= 1) {
@@ -1199,11 +1280,13 @@ __global__ __launch_bounds__(ROWS_BLOCK) void k_expand_rows_count(MeshView m, in
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();   // the previous face's reads of lds are done
         for (int j = 0; j < nd; j++) {
-            const int a = __shfl(d.face, j), off = __shfl(d.off, j), len = __shfl(d.len, j);
+            const int off = __shfl(d.off, j), len = __shfl(d.len, j), from = __shfl(src, j);
             if (len < 0) continue;
-            const int src = m.adj_off[a];
-            for (int i = lane; i < len; i += 64) lds[off + i] = m.adj[src + i];
+            for (int i = lane; i < len; i += 64) lds[off + i] = m.adj[from + i];
         }
+        unsigned long long star[3];   // directory entries around fv[s] (f excluded)
+#pragma unroll
+        for (int s = 0; s < 3; s++) star[s] = __ballot((vbits >> s) & 1u);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1223,12 +1306,20 @@ __global__ __launch_bounds__(ROWS_BLOCK) void k_expand_rows_count(MeshView m, in
 #pragma unroll
             for (int s = 0; s < 3; s++) {
                 bool own = valid && !(is_fixed(m, fv[s]) && gfix);
-                for (int kk = m.vf_off[fv[s]]; kk < m.vf_off[fv[s] + 1]; kk++) {
-                    const int a = m.vf_list[kk];
-                    if (a >= f) continue;
-                    int off = 0, len = 0;
-                    const bool cached = dir_find(d, a, off, len);
-                    if (own && related_dir(m, lds, cached, off, len, a, g)) own = false;
+                if (complete) {
+                    for (unsigned long long st = star[s]; st; st &= st - 1) {
+                        const int j = __ffsll(st) - 1;
+                        const int a = __shfl(d.face, j), off = __shfl(d.off, j), len = __shfl(d.len, j);
+                        if (own && related_dir(m, lds, len >= 0, off, len, a, g)) own = false;
+                    }
+                } else {
+                    for (int kk = m.vf_off[fv[s]]; kk < m.vf_off[fv[s] + 1]; kk++) {
+                        const int a = m.vf_list[kk];
+                        if (a >= f) continue;
+                        int off = 0, len = 0;
+                        const bool cached = dir_find(d, a, off, len);
+                        if (own && related_dir(m, lds, cached, off, len, a, g)) own = false;
+                    }
                 }
                 if (own) vmask |= 1u << s;
             }
@@ -1386,6 +1477,7 @@ void vertex_face_table(ccd_ctx *ctx, const int32_t **off, const int32_t **list)
 // Device buffers of one broadphase, sized for F faces and V vertices and the capacities in caps.
 struct BPBufs {
     double *dop64, *centre, *partial, *sdop64;
+    double *qpartial, *qparam;   // node quantisation: per-block partial ranges, origin[13] + inv[13]
     uint64_t *keys, *keys2;
     int32_t *vals, *order, *face_inst;
     uint32_t *left, *rf, *rl, *parent, *escape, *arrivals;
@@ -1405,12 +1497,15 @@ struct BPBufs {
     uint32_t *tdepth = nullptr, *tdepth2 = nullptr, *tjump = nullptr, *tjump2 = nullptr, *tmaxd = nullptr;
 };
 constexpr int NPARTS = 512;
+constexpr int QPARTS = 4096;   // blocks of the quantisation range pass (enough to fill the GPU)
 
 static int bp_buffers(ccd_ctx *ctx, int64_t F, int64_t V, bool batched, BPBufs &b)
 {
     b.dop64 = (double *)buffer(ctx, "dop64", (size_t)F * NB * 8);
     b.centre = (double *)buffer(ctx, "centre", (size_t)F * 3 * 8);
     b.partial = (double *)buffer(ctx, "bpartial", (size_t)NPARTS * 6 * 8 + 64);
+    b.qpartial = (double *)buffer(ctx, "qpartial", (size_t)QPARTS * NB * 8);
+    b.qparam = (double *)buffer(ctx, "qparam", (size_t)NB * 8);
     b.keys = (uint64_t *)buffer(ctx, "mkeys", (size_t)F * 8);
     b.keys2 = (uint64_t *)buffer(ctx, "mkeys2", (size_t)F * 8);
     b.vals = (int32_t *)buffer(ctx, "mvals", (size_t)F * 4);
@@ -1450,7 +1545,7 @@ static int bp_buffers(ccd_ctx *ctx, int64_t F, int64_t V, bool batched, BPBufs &
         if (!b.adj_cnt || !b.adj_off || !b.adj || !b.adj_tmp || !b.adj_mask || !b.exp_vf || !b.exp_ee)
             return CCD_E_OOM;
     }
-    if (!b.dop64 || !b.centre || !b.partial || !b.keys || !b.keys2 || !b.vals || !b.order ||
+    if (!b.dop64 || !b.centre || !b.partial || !b.qpartial || !b.qparam || !b.keys || !b.keys2 || !b.vals || !b.order ||
         (batched && !b.face_inst) || !b.left || !b.rf || !b.rl || !b.parent || !b.escape || !b.arrivals ||
         !b.nodes || !b.sdop64 || !b.sfaces || !b.sort_tmp || !b.pairs || !b.vfk || !b.eek || !b.opp || !b.rank)
         return CCD_E_OOM;
@@ -1475,6 +1570,17 @@ static int build_related_table(ccd_ctx *ctx, const BPBufs &b, int64_t F, MeshVie
     CCD_LAUNCH_CHECK(ctx);
     mv.adj_off = b.adj_off;
     mv.adj = b.adj;
+    return CCD_OK;
+}
+
+// The node quantisation's range from the leaves' FP64 bounds (before the leaf records are written).
+static int launch_qrange(ccd_ctx *ctx, const BPBufs &b, int64_t F)
+{
+    const int nb = (int)std::min<int64_t>(QPARTS, blocks_for(F, 256));
+    hipLaunchKernelGGL(k_qrange_partial, dim3(nb), dim3(256), 0, ctx->stream, b.dop64, F, b.qpartial);
+    CCD_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_qrange_final, dim3(KAX), dim3(64), 0, ctx->stream, b.qpartial, nb, b.qparam);
+    CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
 }
 
@@ -1611,8 +1717,10 @@ int broadphase_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const doubl
     CCD_LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_escape, dim3(blocks_for(2 * F - 1, 256)), dim3(256), 0, st, b.rf, b.rl, F, b.escape);
     CCD_LAUNCH_CHECK(ctx);
+    rc = launch_qrange(ctx, b, F);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_leaf_records, dim3(blocks_for(F, 256)), dim3(256), 0, st, order, b.dop64, faces, F,
-                       b.escape, b.nodes, b.sdop64, b.sfaces, b.rank);
+                       b.escape, b.qparam, b.nodes, b.sdop64, b.sfaces, b.rank);
     CCD_LAUNCH_CHECK(ctx);
     rc = refit_tree(ctx, b, F, true);
     if (rc) return rc;
@@ -1675,8 +1783,10 @@ int broadphase_refit_launch(ccd_ctx *ctx, const int32_t *faces, int64_t F, const
     CCD_LAUNCH_CHECK(ctx);
     prof_end(ctx, CCD_T_BOUNDS);
     prof_begin(ctx, CCD_T_BUILD);
+    rc = launch_qrange(ctx, b, F);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_leaf_records, dim3(blocks_for(F, 256)), dim3(256), 0, st, ctx->tree_order, b.dop64, faces,
-                       F, b.escape, b.nodes, b.sdop64, b.sfaces, b.rank);
+                       F, b.escape, b.qparam, b.nodes, b.sdop64, b.sfaces, b.rank);
     CCD_LAUNCH_CHECK(ctx);
     rc = refit_tree(ctx, b, F, false);
     if (rc) return rc;
