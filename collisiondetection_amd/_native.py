@@ -70,6 +70,7 @@ SIGNATURES = {
     "ccd_autotune": ([_V, _V, _V, _D, ctypes.POINTER(_INT)], _INT),
     "ccd_set_queue_shape": ([_V, ctypes.POINTER(_INT)], _INT),
     "ccd_set_profiling": ([_V, _INT], _INT),
+    "ccd_set_edge_edge_split": ([_V, _INT], _INT),
     "ccd_stage_times": ([_V, ctypes.POINTER(_D), _P64, _INT], _INT),
     "ccd_narrowphase_work": ([_V, _V, _V, _I64, _V, _I64, _V, _I64, _D, _P64, _P64], _INT),
     "ccd_stage_flops": ([_P64], _INT),

@@ -684,6 +684,12 @@ def set_queue_shape(shape, ctx: Context | None = None):
     ctx.check(N.lib().ccd_set_queue_shape(ctx.ptr, arr))
 
 
+def set_edge_edge_split(on: bool, ctx: Context | None = None):
+    """Two-pass (True, the default) or one-pass edge-edge narrowphase; the results are the same."""
+    ctx = ctx or default_context()
+    ctx.check(N.lib().ccd_set_edge_edge_split(ctx.ptr, 1 if on else 0))
+
+
 # ------------------------------------------------------------------------------------------------
 # Serving
 # ------------------------------------------------------------------------------------------------

@@ -510,11 +510,13 @@ struct DetectArgs {
 // be grouped by their vertex-edge signature first; the captured path passes capacities instead.
 static int narrow_launches(ccd_ctx *ctx, const DetectArgs &a, int64_t nvf_max, int64_t nee_max, bool exact)
 {
+    // work[0..1]: the two queues; work[2..3]: the edge-edge deferral list's counter and second queue.
     unsigned long long *work = (unsigned long long *)buffer(ctx, "queue_work", 48);
     if (!work) return CCD_E_OOM;
     int rc = fill_u64(ctx, (uint64_t *)(ctx->d_counts + C_VF_HITS), 2, 0);
-    if (!rc) rc = fill_u64(ctx, (uint64_t *)work, 2, 0);
+    if (!rc) rc = fill_u64(ctx, (uint64_t *)work, 4, 0);
     if (rc) return rc;
+    const EeDeferral def = ee_deferral(ctx, nee_max, work + 2);
     const int32_t *vfk = (const int32_t *)ctx->bufs["vf_keys"].ptr;
     const int32_t *eek = (const int32_t *)ctx->bufs["ee_keys"].ptr;
     // The stage timers include the grouping.  Without exact host counts (a captured pipeline) the
@@ -536,7 +538,7 @@ static int narrow_launches(ccd_ctx *ctx, const DetectArgs &a, int64_t nvf_max, i
         if (rc) return rc;
     }
     rc = launch_narrow2_queue(ctx, false, a.qs, a.qe, eek, ctx->d_counts + C_EE, nee_max, a.eta, a.ee_hits, a.ee_toi,
-                              a.ee_cap, ctx->d_counts + C_EE_HITS, work + 1, ctx->ee_shape);
+                              a.ee_cap, ctx->d_counts + C_EE_HITS, work + 1, ctx->ee_shape, &def);
     if (rc) return rc;
     prof_end(ctx, CCD_T_NARROW_EE);
     return CCD_OK;
@@ -614,6 +616,10 @@ static int detect_graph(ccd_ctx *ctx, const DetectArgs &a)
         rc = reserve_grouping(ctx, ctx->cap_vf, ctx->cap_ee);
         if (rc) return rc;
     }
+    if (!buffer(ctx, "queue_work", 48)) return CCD_E_OOM;
+    if (ctx->ee_split && ctx->cap_ee <= 0x7fffffffll &&
+        (!buffer(ctx, "ee_def_idx", (size_t)ctx->cap_ee * 4) || !buffer(ctx, "ee_def_sub", (size_t)ctx->cap_ee)))
+        return CCD_E_OOM;
     int64_t caps2[3] = {ctx->cap_pairs, ctx->cap_vf, ctx->cap_ee};
     memcpy(key + sizeof(DetectArgs), caps2, sizeof(caps2));
     ctx->capturing = true;
@@ -726,6 +732,14 @@ int ccd_set_queue_shape(ccd_ctx *ctx, const int *shape)
     return CCD_OK;
 }
 
+int ccd_set_edge_edge_split(ccd_ctx *ctx, int on)
+{
+    if (!ctx) return CCD_E_INVALID;
+    ctx->ee_split = on != 0;
+    graph_drop(ctx);   // captured launches carry the old form
+    return CCD_OK;
+}
+
 int ccd_autotune(ccd_ctx *ctx, const double *qstart, const double *qend, double eta, int *shape)
 {
     if (!ctx) return CCD_E_INVALID;
@@ -740,7 +754,7 @@ int ccd_autotune(ccd_ctx *ctx, const double *qstart, const double *qend, double 
     const int32_t *keys[2] = {ctx->last_vf, ctx->last_ee};
     int32_t *sink = (int32_t *)buffer(ctx, "tune_sink", 16 * 4096 + 64);
     double *sink_t = (double *)buffer(ctx, "tune_sink_t", 8 * 4096 + 64);
-    unsigned long long *work = (unsigned long long *)buffer(ctx, "tune_work", 16);
+    unsigned long long *work = (unsigned long long *)buffer(ctx, "tune_work", 32);   // queue; [2..3] deferral
     int64_t *cnt = (int64_t *)buffer(ctx, "tune_cnt", 16);
     int64_t *ndev = (int64_t *)buffer(ctx, "tune_n", 16);
     if (!sink || !sink_t || !work || !cnt || !ndev) return CCD_E_OOM;
@@ -759,13 +773,15 @@ int ccd_autotune(ccd_ctx *ctx, const double *qstart, const double *qend, double 
         }
         return CCD_OK;
     };
+    // Edge-edge shapes are timed as the pipeline runs them (two passes unless switched off).
+    const EeDeferral tdef = ee_deferral(ctx, nmax[1], work + 2);
     auto queue = [&](int kind, const int32_t *k, const QueueShape &s) {
         return [&, kind, k, s]() -> int {
-            int rc = fill_u64(ctx, (uint64_t *)work, 1, 0);
+            int rc = fill_u64(ctx, (uint64_t *)work, 4, 0);
             if (!rc) rc = fill_u64(ctx, (uint64_t *)cnt, 1, 0);
             if (!rc)
                 rc = launch_narrow2_queue(ctx, kind == 0, qstart, qend, k, ndev, nmax[kind], eta, sink, sink_t, 0, cnt,
-                                          work, s);
+                                          work, s, kind == 1 ? &tdef : nullptr);
             return rc;
         };
     };

@@ -22,6 +22,7 @@ struct ccd_ctx {
     int device = 0;
     int num_cus = 256;
     bool group_stencils = true;   // group stencils by vertex-edge signature before the narrowphase
+    bool ee_split = true;         // edge-edge narrowphase in two passes (deferred sextic recursions)
     QueueShape vf_shape, ee_shape;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
@@ -116,10 +117,25 @@ int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *
 int launch_narrow2_compact(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
                            const int64_t *n_dev, int64_t n_max, double eta, int32_t *out_keys, double *out_toi,
                            int64_t cap, int64_t *counter);
-// Persistent work-queue narrowphase; *work must be zero before the launch.
+// The deferred-stencil list of the two-pass edge-edge narrowphase: positions (idx) and first-pass
+// vertex-edge / vertex-vertex results (sub) of up to cap stencils, the list's device counter and the
+// second pass's queue counter (both zero before the launch).
+struct EeDeferral {
+    uint32_t *idx = nullptr;
+    uint8_t *sub = nullptr;
+    unsigned long long *count = nullptr;
+    unsigned long long *work = nullptr;
+    int64_t cap = 0;
+};
+// The context's deferral list for up to n stencils (grow-only buffers "ee_def_idx", "ee_def_sub"; counters
+// at count_and_work[0..1]), or an empty one when the context runs edge-edge in one pass.
+EeDeferral ee_deferral(ccd_ctx *ctx, int64_t n, unsigned long long *count_and_work);
+// Persistent work-queue narrowphase; *work must be zero before the launch.  Edge-edge runs in two
+// passes when def holds a list for n_max stencils (n_max < 2^31), else in one.
 int launch_narrow2_queue(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
                          const int64_t *n_dev, int64_t n_max, double eta, int32_t *out_keys, double *out_toi,
-                         int64_t cap, int64_t *counter, unsigned long long *work, const QueueShape &shape);
+                         int64_t cap, int64_t *counter, unsigned long long *work, const QueueShape &shape,
+                         const EeDeferral *def = nullptr);
 // Work accounting: per-stage execution counts (ccd::Stage order) summed over the stencils into counts.
 int launch_narrow2_count(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
                          int64_t n, double eta, unsigned long long *counts);

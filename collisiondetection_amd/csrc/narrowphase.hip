@@ -222,11 +222,19 @@ __device__ __forceinline__ void window_pos(const double *__restrict__ qs, const 
     e = b;
 }
 
+//
+// First pass of the two-pass form (def_idx not null): a stencil whose coplanar-distance sextic needs the
+// derivative recursion (ccd::DeferSextic) leaves its edge-edge test undecided and is appended to the
+// deferred list, position i and the sub-test of its first vertex-edge / vertex-vertex hit (255: none),
+// instead of being emitted; k_ee_deferred finishes it.  About a quarter of the stencils of a moving mesh
+// need that recursion and take most of the time; among the easy ones a wavefront's lanes then stay
+// on the same short path.
 template <int BS, class M>
 __global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue_ee_shared(
     const double *__restrict__ qs, const double *__restrict__ qe, const int4 *__restrict__ keys,
     const int64_t *__restrict__ n_dev, int64_t n_max, double eta, int4 *__restrict__ out_keys,
-    double *__restrict__ out_toi, int64_t cap, unsigned long long *counter, unsigned long long *work, int tile)
+    double *__restrict__ out_toi, int64_t cap, unsigned long long *counter, unsigned long long *work, int tile,
+    uint32_t *__restrict__ def_idx, uint8_t *__restrict__ def_sub, unsigned long long *def_count, int64_t def_cap)
 {
     __shared__ VeShare s_share[BS / 64];
     VeShare &L = s_share[threadIdx.x >> 6];
@@ -246,13 +254,16 @@ __global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue_ee_shared(
             V3 s[4], e[4];
             double t = 0.0;
             int sub = -1;
+            ccd::DeferSextic ec;
+            ec.enabled = def_idx != nullptr;
             if (valid) {
                 k = keys[i];
                 const int v[4] = {k.x, k.y, k.z, k.w};
 #pragma unroll
                 for (int j = 0; j < 4; j++) window_pos(qs, qe, v[j], s[j], e[j]);
-                if (ccd::edge_edge(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t, M())) sub = 0;
+                if (ccd::edge_edge(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t, ec)) sub = 0;
             }
+            const bool deferred = ec.deferred;
             const bool need = valid && sub < 0;
             // The stencil's four vertex-edge tests as tasks r * 64 + lane.
             const int tv[4] = {k.x, k.y, k.z, k.w};
@@ -328,7 +339,90 @@ __global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue_ee_shared(
                 if (sub < 0 && ccd::vertex_vertex(s[1], s[3], e[1], e[3], eta, t, M())) sub = 8;
             }
             wave_lds_sync();   // the next chunk rewrites L
-            const bool hit = sub >= 0;
+            const bool hit = sub >= 0 && !deferred;
+            const uint64_t mask = __ballot(hit);
+            if (mask) {
+                const int leader = __ffsll((unsigned long long)mask) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+                base = __shfl(base, leader);
+                if (hit) {
+                    const int64_t pos = (int64_t)base + __popcll(mask & lt);
+                    if (pos < cap) {
+                        out_keys[pos] = k;
+                        if (out_toi) out_toi[pos] = t;
+                    }
+                }
+            }
+            const uint64_t dmask = __ballot(deferred);
+            if (dmask) {
+                const int leader = __ffsll((unsigned long long)dmask) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(def_count, (unsigned long long)__popcll(dmask));
+                base = __shfl(base, leader);
+                if (deferred) {
+                    // The host sizes the list for every stored stencil, so the guard never drops one.
+                    const int64_t pos = (int64_t)base + __popcll(dmask & lt);
+                    if (pos < def_cap) {
+                        def_idx[pos] = (uint32_t)i;
+                        def_sub[pos] = sub < 0 ? (uint8_t)255 : (uint8_t)sub;
+                    }
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ V3 pick4(const V3 (&a)[4], int j)
+{
+    return j == 0 ? a[0] : (j == 1 ? a[1] : (j == 2 ? a[2] : a[3]));
+}
+
+// Second pass of the two-pass edge-edge narrowphase: the deferred stencils' edge-edge tests in full
+// (NoCount, the recursion included), on a persistent queue over the list.  A stencil whose edge-edge
+// test hits takes its time (sub-test 0, first in CTCDNarrowPhase.cpp:91-132); otherwise the first
+// pass's vertex-edge / vertex-vertex result stands, and its time comes from running that one sub-test
+// again on the same positions (the same call as in the first pass, so the same value).
+template <int BS>
+__global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_ee_deferred(
+    const double *__restrict__ qs, const double *__restrict__ qe, const int4 *__restrict__ keys,
+    const uint32_t *__restrict__ def_idx, const uint8_t *__restrict__ def_sub,
+    const unsigned long long *__restrict__ def_count, int64_t def_cap, double eta, int4 *__restrict__ out_keys,
+    double *__restrict__ out_toi, int64_t cap, unsigned long long *counter, unsigned long long *work, int tile)
+{
+    const int64_t n = (int64_t)*def_count < def_cap ? (int64_t)*def_count : def_cap;
+    const int lane = __lane_id();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    while (true) {
+        unsigned long long start = 0;
+        if (lane == 0) start = atomicAdd(work, (unsigned long long)tile);
+        start = __shfl(start, 0);
+        if ((int64_t)start >= n) break;
+        const int64_t end = (int64_t)start + tile < n ? (int64_t)start + tile : n;
+        for (int64_t j0 = (int64_t)start; j0 < end; j0 += 64) {
+            const int64_t i = j0 + lane;
+            bool hit = false;
+            double t = 0.0;
+            int4 k = make_int4(0, 0, 0, 0);
+            if (i < end) {
+                k = keys[def_idx[i]];
+                const int v[4] = {k.x, k.y, k.z, k.w};
+                V3 s[4], e[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) window_pos(qs, qe, v[j], s[j], e[j]);
+                hit = ccd::edge_edge(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t, ccd::NoCount());
+                const int sub = def_sub[i];
+                if (!hit && sub >= 1 && sub <= 4) {
+                    // vertex-edge (s[r]; s[a], s[a + 1]), r = sub - 1, a = 2 for r < 2 else 0
+                    const int r = sub - 1, a = r < 2 ? 2 : 0;
+                    hit = ccd::vertex_edge(pick4(s, r), pick4(s, a), pick4(s, a + 1), pick4(e, r), pick4(e, a),
+                                           pick4(e, a + 1), eta, t, ccd::NoCount());
+                } else if (!hit && sub >= 5 && sub <= 8) {
+                    // vertex-vertex (0, 2), (0, 3), (1, 2), (1, 3)
+                    const int x = sub < 7 ? 0 : 1, y = (sub == 5 || sub == 7) ? 2 : 3;
+                    hit = ccd::vertex_vertex(pick4(s, x), pick4(s, y), pick4(e, x), pick4(e, y), eta, t, ccd::NoCount());
+                }
+            }
             const uint64_t mask = __ballot(hit);
             if (mask) {
                 const int leader = __ffsll((unsigned long long)mask) - 1;
@@ -820,7 +914,8 @@ int launch_narrow2_compact(ccd_ctx *ctx, bool vf, const double *qs, const double
 template <bool VF, class M>
 static void launch_queue_bs(ccd_ctx *ctx, int bs, unsigned grid, const double *qs, const double *qe,
                             const int32_t *keys, const int64_t *n_dev, int64_t n_max, double eta, int32_t *out_keys,
-                            double *out_toi, int64_t cap, int64_t *counter, unsigned long long *work, int tile)
+                            double *out_toi, int64_t cap, int64_t *counter, unsigned long long *work, int tile,
+                            const EeDeferral *def)
 {
     // Edge-edge stencils share their vertex-edge tests within a wavefront (k_narrow2_queue_ee_shared).
 #define CCD_QUEUE_LAUNCH(B)                                                                                         \
@@ -831,7 +926,8 @@ static void launch_queue_bs(ccd_ctx *ctx, int bs, unsigned grid, const double *q
     else                                                                                                            \
         hipLaunchKernelGGL((k_narrow2_queue_ee_shared<B, M>), dim3(grid), dim3(B), 0, ctx->stream, qs, qe,       \
                            (const int4 *)keys, n_dev, n_max, eta, (int4 *)out_keys, out_toi, cap,                  \
-                           (unsigned long long *)counter, work, tile)
+                           (unsigned long long *)counter, work, tile, def ? def->idx : nullptr,                    \
+                           def ? def->sub : nullptr, def ? def->count : nullptr, def ? def->cap : 0)
     if (bs == 64)
         CCD_QUEUE_LAUNCH(64);
     else if (bs == 128)
@@ -843,7 +939,8 @@ static void launch_queue_bs(ccd_ctx *ctx, int bs, unsigned grid, const double *q
 
 int launch_narrow2_queue(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,
                          const int64_t *n_dev, int64_t n_max, double eta, int32_t *out_keys, double *out_toi,
-                         int64_t cap, int64_t *counter, unsigned long long *work, const QueueShape &shape)
+                         int64_t cap, int64_t *counter, unsigned long long *work, const QueueShape &shape,
+                         const EeDeferral *def)
 {
     if (n_max == 0) return CCD_OK;
     int bs = shape.block == 64 || shape.block == 128 ? shape.block : 256;
@@ -851,13 +948,25 @@ int launch_narrow2_queue(ccd_ctx *ctx, bool vf, const double *qs, const double *
     int64_t need = (n_max + shape.tile - 1) / shape.tile;   // no more waves than tiles
     waves = std::max<int64_t>(1, std::min(waves, need));
     unsigned grid = (unsigned)((waves * 64 + bs - 1) / bs);
+    // Two passes only with a list that holds every stored stencil (int32 positions).
+    const bool split = !vf && def && def->idx && def->sub && def->count && def->work && n_max <= def->cap &&
+                       n_max <= 0x7fffffffll;
     if (vf)
         CCD_PRODUCT_LAUNCH((launch_queue_bs<true, M>(ctx, bs, grid, qs, qe, keys, n_dev, n_max, eta, out_keys,
-                                                         out_toi, cap, counter, work, shape.tile)));
+                                                         out_toi, cap, counter, work, shape.tile, nullptr)));
     else
         CCD_PRODUCT_LAUNCH((launch_queue_bs<false, M>(ctx, bs, grid, qs, qe, keys, n_dev, n_max, eta, out_keys,
-                                                          out_toi, cap, counter, work, shape.tile)));
+                                                          out_toi, cap, counter, work, shape.tile,
+                                                          split ? def : nullptr)));
     CCD_LAUNCH_CHECK(ctx);
+    if (split) {
+        // The deferred stencils cost about ten times the others: small tiles keep the tail short.
+        const int64_t w2 = (int64_t)ctx->num_cus * shape.waves_per_cu;
+        hipLaunchKernelGGL(k_ee_deferred<64>, dim3((unsigned)std::max<int64_t>(1, w2)), dim3(64), 0, ctx->stream, qs,
+                           qe, (const int4 *)keys, def->idx, def->sub, def->count, def->cap, eta, (int4 *)out_keys,
+                           out_toi, cap, (unsigned long long *)counter, def->work, 64);
+        CCD_LAUNCH_CHECK(ctx);
+    }
     return CCD_OK;
 }
 
@@ -889,6 +998,19 @@ int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *
                                                   sub, win));
     CCD_LAUNCH_CHECK(ctx);
     return CCD_OK;
+}
+
+EeDeferral ee_deferral(ccd_ctx *ctx, int64_t n, unsigned long long *count_and_work)
+{
+    EeDeferral d;
+    if (!ctx->ee_split || n <= 0 || n > 0x7fffffffll || !count_and_work) return d;
+    d.idx = (uint32_t *)buffer(ctx, "ee_def_idx", (size_t)n * 4);
+    d.sub = (uint8_t *)buffer(ctx, "ee_def_sub", (size_t)n);
+    if (!d.idx || !d.sub) return EeDeferral();   // one pass instead (the same results)
+    d.count = count_and_work;
+    d.work = count_and_work + 1;
+    d.cap = n;
+    return d;
 }
 
 // Sizes the grouping buffers for up to nvf / nee stencils (before a capture, which cannot grow them).

@@ -181,7 +181,7 @@ int64_t align256(int64_t x) { return (x + 255) & ~255ll; }
 int64_t workspace_bytes(int64_t cp, int64_t cv, int64_t ce, int64_t ch)
 {
     return align256(cp * 8) + align256((2 * cp + 1) * 4) + align256((2 * cp + 1) * 2) + align256(cv * 16) +
-           align256(ce * 16) + 2 * (align256(ch * 16) + align256(ch * 8)) + 256;
+           align256(ce * 16) + align256(ce * 4) + align256(ce) + 2 * (align256(ch * 16) + align256(ch * 8)) + 256;
 }
 
 // Point the session context's transient buffers at a slice of the pool.
@@ -200,6 +200,8 @@ void bind_workspace(ccd_server *S, Session &s, InFlight &f)
     put("adj_mask", (2 * f.cap_pairs + 1) * 2);
     put("vf_keys", f.cap_vf * 16);
     put("ee_keys", f.cap_ee * 16);
+    put("ee_def_idx", f.cap_ee * 4);   // the two-pass edge-edge narrowphase's deferred list
+    put("ee_def_sub", f.cap_ee);
     const int64_t kb = align256(f.cap_hits * 16), tb = align256(f.cap_hits * 8);
     f.hit_vk = (int32_t *)p;
     f.hit_vt = (double *)(p + kb);
@@ -220,18 +222,20 @@ int launch_request(ccd_server *S, Session &s, InFlight &f)
     CCD_HIP_CHECK(ctx, hipMemcpyAsync(end, f.req.qend.data(), (size_t)s.V * 24, hipMemcpyHostToDevice, st));
     int rc = broadphase_refit_launch(ctx, s.faces, s.F, start, end, s.V, s.outer_eta, nullptr, nullptr, 1);
     if (rc) return rc;
-    unsigned long long *work = (unsigned long long *)buffer(ctx, "queue_work", 32);
+    // work[0..1]: the two queues; [2..3]: the hit checksums; [4..5]: the edge-edge deferral list.
+    unsigned long long *work = (unsigned long long *)buffer(ctx, "queue_work", 48);
     if (!work) return CCD_E_OOM;
     rc = fill_u64(ctx, (uint64_t *)(ctx->d_counts + C_VF_HITS), 2, 0);
-    if (!rc) rc = fill_u64(ctx, (uint64_t *)work, 4, 0);
+    if (!rc) rc = fill_u64(ctx, (uint64_t *)work, 6, 0);
     if (rc) return rc;
+    const EeDeferral def = ee_deferral(ctx, f.cap_ee, work + 4);
     rc = launch_narrow2_queue(ctx, true, start, end, (const int32_t *)ctx->bufs["vf_keys"].ptr, ctx->d_counts + C_VF,
                               f.cap_vf, s.eta, f.hit_vk, f.hit_vt, f.cap_hits, ctx->d_counts + C_VF_HITS, work,
                               ctx->vf_shape);
     if (rc) return rc;
     rc = launch_narrow2_queue(ctx, false, start, end, (const int32_t *)ctx->bufs["ee_keys"].ptr, ctx->d_counts + C_EE,
                               f.cap_ee, s.eta, f.hit_ek, f.hit_et, f.cap_hits, ctx->d_counts + C_EE_HITS, work + 1,
-                              ctx->ee_shape);
+                              ctx->ee_shape, &def);
     if (rc) return rc;
     unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(2 * f.cap_hits, 256), 1024));
     hipLaunchKernelGGL(k_hit_checksum, dim3(g), dim3(256), 0, st, (const int4 *)f.hit_vk, f.hit_vt,
@@ -502,7 +506,7 @@ int ccd_server_open_session(ccd_server *S, const int32_t *faces, int64_t nfaces,
     s.last_ee = nec;
     s.last_hits = std::max(nvh, neh);
     // The transient buffers come from the pool per request: release the context's own copies.
-    for (const char *name : {"pairs", "adj", "adj_mask", "vf_keys", "ee_keys"}) {
+    for (const char *name : {"pairs", "adj", "adj_mask", "vf_keys", "ee_keys", "ee_def_idx", "ee_def_sub"}) {
         ccd_ctx::Buf &b = ctx->bufs[name];
         if (b.ptr && !b.external) {
             (void)hipFree(b.ptr);
@@ -610,7 +614,7 @@ int ccd_server_destroy(ccd_server *S)
     if (S->worker.joinable()) S->worker.join();
     (void)hipSetDevice(S->device);
     for (Session &s : S->sessions) {
-        for (const char *name : {"pairs", "adj", "adj_mask", "vf_keys", "ee_keys"}) s.ctx->bufs[name] = ccd_ctx::Buf();
+        for (const char *name : {"pairs", "adj", "adj_mask", "vf_keys", "ee_keys", "ee_def_idx", "ee_def_sub"}) s.ctx->bufs[name] = ccd_ctx::Buf();
         (void)hipFree(s.faces);
         (void)hipFree(s.pos[0]);
         (void)hipFree(s.pos[1]);

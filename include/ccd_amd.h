@@ -304,6 +304,15 @@ int ccd_autotune(ccd_ctx *ctx, const double *qstart, const double *qend, double 
 int ccd_set_queue_shape(ccd_ctx *ctx, const int *shape);
 
 /*
+ * ccd_set_edge_edge_split (no counterpart in the reference; a scheduling choice, same results): on (the
+ * default), the persistent edge-edge narrowphase of ccd_detect / ccd_batch_detect / the server runs in
+ * two passes: stencils whose coplanar-distance sextic needs the derivative recursion of the root
+ * isolation are set aside by the first pass (4 + 1 bytes per stored stencil of device memory) and
+ * finished by a second pass over that list; off, one pass.
+ */
+int ccd_set_edge_edge_split(ccd_ctx *ctx, int on);
+
+/*
  * Serving (no counterpart in the reference): device-resident sessions and a continuous-batching loop.
  *
  * ccd_server_create: a server on `device` with an HBM pool of hbm_budget_bytes for the transient

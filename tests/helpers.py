@@ -21,6 +21,23 @@ def core_narrowphase(H, qs, qe, keys, is_ee, eta):
     return hit.astype(bool), t, sub
 
 
+def core_ee_two_pass(H, qs, qe, keys, eta):
+    """The host-built core's two-pass edge-edge schedule: (hit, t, sub, deferred) per stencil."""
+    keys = np.ascontiguousarray(keys, np.int32).reshape(-1, 4)
+    n = len(keys)
+    hit = np.zeros(n, np.int32)
+    t = np.zeros(n)
+    sub = np.zeros(n, np.int32)
+    dfr = np.zeros(n, np.int32)
+    qs = np.ascontiguousarray(qs, dtype=np.float64)
+    qe = np.ascontiguousarray(qe, dtype=np.float64)
+    H.core_ee_two_pass.restype = ctypes.c_int64
+    H.core_ee_two_pass(qs.ctypes.data_as(dp), qe.ctypes.data_as(dp), keys.ctypes.data_as(ip), ctypes.c_int64(n),
+                       ctypes.c_double(eta), hit.ctypes.data_as(ip), t.ctypes.data_as(dp), sub.ctypes.data_as(ip),
+                       dfr.ctypes.data_as(ip))
+    return hit.astype(bool), t, sub, dfr.astype(bool)
+
+
 def core_separating_plane(H, hist, keys, is_ee, eta):
     """The host-built core's separating-plane narrowphase: statuses 1 / 0 / 2 (undecided)."""
     keys = np.ascontiguousarray(keys, np.int32).reshape(-1, 4)

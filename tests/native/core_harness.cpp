@@ -55,6 +55,61 @@ extern "C" void core_narrowphase(const double *qs, const double *qe, const int32
     }
 }
 
+// The two-pass edge-edge schedule of the device (k_narrow2_queue_ee_shared with a deferred list, then
+// k_ee_deferred), stencil by stencil on the host: first pass with ccd::DeferSextic, the vertex-edge and
+// vertex-vertex tests in the window's order for every stencil the edge-edge test did not decide; a
+// deferred stencil keeps that sub-test number, then gets the full edge-edge test and, if that misses,
+// the recorded sub-test run again for its time.  Writes hit / t / sub as core_narrowphase does and
+// deferred[i] = 1 for the stencils the first pass set aside; returns how many.
+extern "C" int64_t core_ee_two_pass(const double *qs, const double *qe, const int32_t *keys, int64_t n, double eta,
+                                    int32_t *hit, double *t, int32_t *sub, int32_t *deferred)
+{
+    int64_t nd = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : nd)
+    for (int64_t i = 0; i < n; i++) {
+        V3 s[4], e[4];
+        for (int j = 0; j < 4; j++) {
+            const double *a = qs + 3 * (int64_t)keys[4 * i + j];
+            const double *b = qe + 3 * (int64_t)keys[4 * i + j];
+            e[j] = ld(b);
+            s[j] = ccd::mk(ccd::stitch0(a[0], b[0]), ccd::stitch0(a[1], b[1]), ccd::stitch0(a[2], b[2]));
+        }
+        double tt = 0.0;
+        int r = -1;
+        ccd::DeferSextic ec;
+        if (ccd::edge_edge(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, tt, ec)) r = 0;
+        if (r < 0) {
+            const int ve[4][3] = {{0, 2, 3}, {1, 2, 3}, {2, 0, 1}, {3, 0, 1}};
+            for (int k = 0; k < 4 && r < 0; k++)
+                if (ccd::vertex_edge(s[ve[k][0]], s[ve[k][1]], s[ve[k][2]], e[ve[k][0]], e[ve[k][1]], e[ve[k][2]], eta, tt))
+                    r = 1 + k;
+            const int vv[4][2] = {{0, 2}, {0, 3}, {1, 2}, {1, 3}};
+            for (int k = 0; k < 4 && r < 0; k++)
+                if (ccd::vertex_vertex(s[vv[k][0]], s[vv[k][1]], e[vv[k][0]], e[vv[k][1]], eta, tt)) r = 5 + k;
+        }
+        deferred[i] = ec.deferred ? 1 : 0;
+        if (ec.deferred) {
+            nd++;
+            const int first = r;
+            tt = 0.0;
+            r = -1;
+            if (ccd::edge_edge(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, tt)) {
+                r = 0;
+            } else if (first >= 1 && first <= 4) {
+                const int k = first - 1, a = k < 2 ? 2 : 0;
+                if (ccd::vertex_edge(s[k], s[a], s[a + 1], e[k], e[a], e[a + 1], eta, tt)) r = first;
+            } else if (first >= 5) {
+                const int x = first < 7 ? 0 : 1, y = (first == 5 || first == 7) ? 2 : 3;
+                if (ccd::vertex_vertex(s[x], s[y], e[x], e[y], eta, tt)) r = first;
+            }
+        }
+        hit[i] = r >= 0;
+        t[i] = r >= 0 ? tt : 0.0;
+        sub[i] = r;
+    }
+    return nd;
+}
+
 // findIntervals on a copy of op (degree 2, 3, 4 or 6); returns the count, pieces in out[2k], out[2k+1].
 extern "C" int core_find_intervals(const double *op, int n, int pos, double *out)
 {

@@ -7,8 +7,8 @@ import pytest
 
 import oracle
 from collisiondetection_amd import scenes
-from tests.helpers import (compare_results, core_find_intervals, core_narrowphase, core_primitive, core_roots01, dp,
-                           exact_arbiters)
+from tests.helpers import (compare_results, core_ee_two_pass, core_find_intervals, core_narrowphase, core_primitive,
+                           core_roots01, dp, exact_arbiters)
 from tests.test_oracle import ETA, TESTCTCD, crossing_cases, expected_testctcd
 
 KIND = {"vf": 0, "ee": 1, "ve": 2, "vv": 3}
@@ -113,6 +113,26 @@ def test_core_narrowphase_vs_oracle(core_harness, scene):
         assert not rep["t_bad_hard"], rep
         both = hit & rec["hit"].astype(bool)
         assert np.array_equal(sub[both], rec["subtest"][both])
+
+
+@pytest.mark.parametrize("scene", ["converging", "batched"])
+def test_core_ee_two_pass_equals_one_pass(core_harness, scene):
+    """The two-pass edge-edge schedule (stencils whose distance sextic needs the derivative recursion set
+    aside, their vertex-edge / vertex-vertex results kept, the edge-edge test finished later) gives the
+    one-pass cascade's flags, sub-tests and time bits on every stencil; both scenes defer some."""
+    if scene == "converging":
+        s = scenes.converging_spheres()
+    else:
+        s = sorted(scenes.batched_instances(count=24, seed=2), key=lambda x: len(x["faces"]))[12]
+    h = oracle.Hist.two_entry(s["qstart"], s["qend"])
+    _, ee = oracle.candidates(s["faces"], h, s["eta"], method="tree")
+    hit1, t1, sub1 = core_narrowphase(core_harness, s["qstart"], s["qend"], ee, True, s["eta"])
+    hit2, t2, sub2, dfr = core_ee_two_pass(core_harness, s["qstart"], s["qend"], ee, s["eta"])
+    print(scene, len(ee), "stencils,", int(dfr.sum()), "deferred,", int(hit1.sum()), "hits")
+    assert dfr.sum() > 0
+    assert np.array_equal(hit1, hit2)
+    assert np.array_equal(sub1, sub2)
+    assert np.array_equal(t1.view(np.int64), t2.view(np.int64))
 
 
 # ------------------------------------------------------------------------------------------------

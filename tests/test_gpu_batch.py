@@ -119,6 +119,39 @@ def test_autotune_keeps_results():
     api.set_queue_shape(shape)
 
 
+def test_edge_edge_split_keeps_results():
+    """The two-pass edge-edge narrowphase (the default) and the one-pass form give the same colliding
+    stencils and time-of-impact bits: in a batch (synchronous run and graph replays) and for one mesh."""
+    from collisiondetection_amd import api
+
+    print("case: edge-edge split on / off", flush=True)
+    inst = scenes.batched_instances(count=16, seed=5, tri_lo=400, tri_hi=4000)
+    b = api.Batch(inst)
+    out = {}
+    try:
+        for on in (True, False):
+            api.set_edge_edge_split(on, ctx=b.ctx)
+            rs = [b.detect(1e-6, 1e-6, graph=g) for g in (False, True, True)]
+            for r in rs[1:]:
+                assert _as_set(r["ee"], r["ee_toi"]) == _as_set(rs[0]["ee"], rs[0]["ee_toi"]), on
+                assert _as_set(r["vf"], r["vf_toi"]) == _as_set(rs[0]["vf"], rs[0]["vf_toi"]), on
+            out[on] = rs[0]
+        print("ee candidates", out[True]["ee_candidates"], "hits", len(out[True]["ee"]), flush=True)
+        assert _as_set(out[True]["ee"], out[True]["ee_toi"]) == _as_set(out[False]["ee"], out[False]["ee_toi"])
+        assert _as_set(out[True]["vf"], out[True]["vf_toi"]) == _as_set(out[False]["vf"], out[False]["vf_toi"])
+        assert len(out[True]["ee"]) > 0
+        s = scenes.converging_spheres()
+        d = {}
+        for on in (True, False):
+            api.set_edge_edge_split(on)
+            d[on] = api.detect(s["faces"], s["qstart"], s["qend"], s["eta"], s["eta"])
+        assert _as_set(d[True]["ee"], d[True]["ee_toi"]) == _as_set(d[False]["ee"], d[False]["ee_toi"])
+        assert len(d[True]["ee"]) > 0
+    finally:
+        api.set_edge_edge_split(True, ctx=b.ctx)
+        api.set_edge_edge_split(True)
+
+
 def test_batch_graph_after_buffer_regrow():
     """A captured pipeline holds raw buffer addresses.  Detections of other meshes on the same context
     that grow buffers sized by vertex or face count (without changing the stencil capacities) must make
