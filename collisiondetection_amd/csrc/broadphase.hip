@@ -628,69 +628,89 @@ __device__ __forceinline__ void traverse_flush(int me, const int (&q)[TRAVERSE_Q
     }
 }
 
+// Packet traversal: one wavefront walks the tree once for its 64 consecutive sorted leaves.  The walk
+// starts at the escape link of the wavefront's first leaf i0 (the subtrees holding the leaves after
+// i0, which hold every leaf any lane pairs with) and follows child links into a subtree when some
+// lane's query overlaps it and the subtree holds a leaf in that lane's range (after its own leaf,
+// before its instance's end), escape links otherwise, until it passes the last instance end of the
+// wavefront.  The node index is the same in every lane, so a node record is one scalar load for the
+// whole wavefront, where 64 lanes walking their own paths fetched up to 64 different records per
+// step.  A lane tests the leaves the walk reaches that overlap its query in the quantised bounds and
+// lie in its range; every pair in a lane's range whose FP64 boxes overlap is reached (the quantised
+// bounds of its leaf and of that leaf's ancestors contain the FP64 ones), so each lane finds exactly
+// the pairs of a walk of its own.
 __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__restrict__ nodes,
                                                   const double *__restrict__ sdop64,
                                                   const int4 *__restrict__ sfaces, const int32_t *__restrict__ face_inst,
                                                   const int64_t *__restrict__ inst_off, int2 *__restrict__ pairs,
                                                   int64_t cap, unsigned long long *counter)
 {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    bool active = i < F;
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63);   // the wavefront's first leaf
+    if (i0 >= F || F < 2) return;   // the whole wavefront (lane 0 included) is past the leaves
+    const bool active = i < F;
     uint16_t q16[NB];
     const double *q64 = sdop64 + NB * (active ? i : 0);
     int4 qf = make_int4(0, 0, 0, 0);
-    uint32_t node = NIL;
-    int64_t iend = F;
+    int64_t iend = 0;   // inactive lanes accept no leaf
     if (active) {
         const NodeRec &me = nodes[F - 1 + i];
 #pragma unroll
         for (int k = 0; k < NB; k++) q16[k] = me.b[k];
         qf = sfaces[i];
-        node = me.esc;
-        if (face_inst) iend = inst_off[face_inst[qf.w] + 1];
+        iend = face_inst ? inst_off[face_inst[qf.w] + 1] : F;
     } else {
 #pragma unroll
         for (int k = 0; k < NB; k++) q16[k] = 0;
     }
+    // The last instance end of the wavefront's lanes.
+    int64_t jmax = iend;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        const int64_t o = __shfl_xor(jmax, d);
+        jmax = o > jmax ? o : jmax;
+    }
     const uint32_t nint = (uint32_t)(F - 1);
-    auto first_leaf = [nint](uint32_t n) -> int64_t { return n < nint ? (int64_t)n : (int64_t)(n - nint); };
-    if (node != NIL && first_leaf(node) >= iend) node = NIL;
+    // The walk: node (uniform) and the first leaf of its subtree.
+    uint32_t node = nodes[F - 1 + i0].esc;
+    int64_t fl = i0 + 1;
+    if (fl >= jmax) node = NIL;
     int q[TRAVERSE_QN] = {};
     int nq = 0;
-    while (true) {
-        if (node != NIL) {
-            const NodeRec &n = nodes[node];
-            uint16_t b[NB];
+    while (node != NIL) {
+        node = __builtin_amdgcn_readfirstlane(node);
+        const NodeRec &n = nodes[node];
+        const uint32_t child = n.child, esc = n.esc;
+        // The subtree's leaves are [fl, r1): r1 is the first leaf of the node after it.
+        const int64_t r1 = esc == NIL ? F : (esc < nint ? (int64_t)esc : (int64_t)(esc - nint));
+        uint16_t b[NB];
 #pragma unroll
-            for (int k = 0; k < NB; k++) b[k] = n.b[k];
-            uint32_t child = n.child, esc = n.esc;
-            bool descend = false;
-            if (overlap16(q16, b)) {
-                if (node < nint) {
-                    descend = true;
-                } else {
-                    int4 of = sfaces[child];
-                    if (overlap64(q64, sdop64 + NB * (int64_t)child) && !neighbours(qf, of)) {
+        for (int k = 0; k < NB; k++) b[k] = n.b[k];
+        const bool lane_ok = r1 - 1 > i && fl < iend && overlap16(q16, b);
+        bool descend = false;
+        if (node < nint) {
+            descend = __ballot(lane_ok) != 0;
+        } else if (lane_ok) {
+            // leaf: child is its sorted index (== fl)
+            const int4 of = sfaces[child];
+            if (overlap64(q64, sdop64 + NB * (int64_t)child) && !neighbours(qf, of)) {
 #pragma unroll
-                        for (int k = 0; k < TRAVERSE_QN; k++)
-                            if (k == nq) q[k] = of.w;
-                        nq++;
-                    }
-                }
-            }
-            if (descend) {
-                node = child;
-            } else {
-                node = esc;
-                if (node != NIL && first_leaf(node) >= iend) node = NIL;
+                for (int k = 0; k < TRAVERSE_QN; k++)
+                    if (k == nq) q[k] = of.w;
+                nq++;
             }
         }
-        const bool more = __ballot(node != NIL) != 0;
-        if (__ballot(nq == TRAVERSE_QN) || !more) {
+        if (descend) {
+            node = child;   // the left child: its subtree starts at the same leaf
+        } else {
+            node = esc;
+            fl = r1;
+            if (fl >= jmax) node = NIL;
+        }
+        if (__ballot(nq == TRAVERSE_QN) || node == NIL) {
             traverse_flush(qf.w, q, nq, pairs, cap, counter);
             nq = 0;
         }
-        if (!more) break;
     }
 }
 
