@@ -639,6 +639,43 @@ __device__ __forceinline__ void traverse_flush(int me, const int (&q)[TRAVERSE_Q
 // lie in its range; every pair in a lane's range whose FP64 boxes overlap is reached (the quantised
 // bounds of its leaf and of that leaf's ancestors contain the FP64 ones), so each lane finds exactly
 // the pairs of a walk of its own.
+// The FP64 leaf test issues all its loads together and evaluates every comparison (an early-exit
+// test waits for one load after another: per leaf, up to 26 memory latencies in a row).  Loading the
+// child and escape records of each node ahead of the test was measured slower (the extra registers
+// halve the resident wavefronts).
+struct NodeWords {
+    uint4 w[4];   // a NodeRec as four 16-byte words: bounds in w[0..3] (26 halves), child w[3].y, escape w[3].z
+};
+static_assert(sizeof(NodeRec) == 64, "node records are 64 bytes");
+__device__ __forceinline__ void load_node(const uint4 *__restrict__ recs, uint32_t n, NodeWords &r)
+{
+    const uint4 *p = recs + 4 * (int64_t)n;
+#pragma unroll
+    for (int k = 0; k < 4; k++) r.w[k] = p[k];
+}
+// Bound k (0 <= k < 26) of a record: 16-bit half k % 2 of 32-bit word k / 2.
+__device__ __forceinline__ uint32_t node_bound(const NodeWords &r, int k)
+{
+    const int wd = k >> 1;
+    const uint4 v = r.w[wd >> 2];
+    const uint32_t x = (wd & 3) == 0 ? v.x : ((wd & 3) == 1 ? v.y : ((wd & 3) == 2 ? v.z : v.w));
+    return (k & 1) ? (x >> 16) : (x & 0xffffu);
+}
+// overlap64 with every comparison evaluated (no early exit: the loads issue together).
+__device__ __forceinline__ bool overlap64_all(const double *__restrict__ a, const double *__restrict__ b)
+{
+    double av[NB], bv[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        av[k] = a[k];
+        bv[k] = b[k];
+    }
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < KAX; k++) ok = ok & !((av[KAX + k] < bv[k]) | (bv[KAX + k] < av[k]));
+    return ok;
+}
+
 __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__restrict__ nodes,
                                                   const double *__restrict__ sdop64,
                                                   const int4 *__restrict__ sfaces, const int32_t *__restrict__ face_inst,
@@ -648,20 +685,21 @@ __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__re
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t i0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63);   // the wavefront's first leaf
     if (i0 >= F || F < 2) return;   // the whole wavefront (lane 0 included) is past the leaves
+    const uint4 *__restrict__ recs = reinterpret_cast<const uint4 *>(nodes);
     const bool active = i < F;
-    uint16_t q16[NB];
+    uint32_t q16[NB];
     const double *q64 = sdop64 + NB * (active ? i : 0);
     int4 qf = make_int4(0, 0, 0, 0);
     int64_t iend = 0;   // inactive lanes accept no leaf
-    if (active) {
-        const NodeRec &me = nodes[F - 1 + i];
+    {
+        NodeWords me;
+        load_node(recs, (uint32_t)(F - 1 + (active ? i : i0)), me);
 #pragma unroll
-        for (int k = 0; k < NB; k++) q16[k] = me.b[k];
+        for (int k = 0; k < NB; k++) q16[k] = node_bound(me, k);
+    }
+    if (active) {
         qf = sfaces[i];
         iend = face_inst ? inst_off[face_inst[qf.w] + 1] : F;
-    } else {
-#pragma unroll
-        for (int k = 0; k < NB; k++) q16[k] = 0;
     }
     // The last instance end of the wavefront's lanes.
     int64_t jmax = iend;
@@ -671,29 +709,33 @@ __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__re
         jmax = o > jmax ? o : jmax;
     }
     const uint32_t nint = (uint32_t)(F - 1);
-    // The walk: node (uniform) and the first leaf of its subtree.
+    // The walk: node (uniform), its record, and the first leaf of its subtree.
     uint32_t node = nodes[F - 1 + i0].esc;
     int64_t fl = i0 + 1;
     if (fl >= jmax) node = NIL;
+    node = __builtin_amdgcn_readfirstlane(node);
+    NodeWords cur;
+    if (node != NIL) load_node(recs, node, cur);
     int q[TRAVERSE_QN] = {};
     int nq = 0;
     while (node != NIL) {
-        node = __builtin_amdgcn_readfirstlane(node);
-        const NodeRec &n = nodes[node];
-        const uint32_t child = n.child, esc = n.esc;
+        const uint32_t child = __builtin_amdgcn_readfirstlane(cur.w[3].y);
+        const uint32_t esc = __builtin_amdgcn_readfirstlane(cur.w[3].z);
+        const bool internal = node < nint;
         // The subtree's leaves are [fl, r1): r1 is the first leaf of the node after it.
         const int64_t r1 = esc == NIL ? F : (esc < nint ? (int64_t)esc : (int64_t)(esc - nint));
-        uint16_t b[NB];
+        bool ov = true;
 #pragma unroll
-        for (int k = 0; k < NB; k++) b[k] = n.b[k];
-        const bool lane_ok = r1 - 1 > i && fl < iend && overlap16(q16, b);
+        for (int k = 0; k < KAX; k++)
+            ov = ov & !((q16[KAX + k] < node_bound(cur, k)) | (node_bound(cur, KAX + k) < q16[k]));
+        const bool lane_ok = (r1 - 1 > i) & (fl < iend) & ov;
         bool descend = false;
-        if (node < nint) {
+        if (internal) {
             descend = __ballot(lane_ok) != 0;
         } else if (lane_ok) {
             // leaf: child is its sorted index (== fl)
             const int4 of = sfaces[child];
-            if (overlap64(q64, sdop64 + NB * (int64_t)child) && !neighbours(qf, of)) {
+            if (overlap64_all(q64, sdop64 + NB * (int64_t)child) && !neighbours(qf, of)) {
 #pragma unroll
                 for (int k = 0; k < TRAVERSE_QN; k++)
                     if (k == nq) q[k] = of.w;
@@ -702,10 +744,12 @@ __global__ __launch_bounds__(256) void k_traverse(int64_t F, const NodeRec *__re
         }
         if (descend) {
             node = child;   // the left child: its subtree starts at the same leaf
+            load_node(recs, node, cur);
         } else {
             node = esc;
             fl = r1;
             if (fl >= jmax) node = NIL;
+            if (node != NIL) load_node(recs, node, cur);
         }
         if (__ballot(nq == TRAVERSE_QN) || node == NIL) {
             traverse_flush(qf.w, q, nq, pairs, cap, counter);
