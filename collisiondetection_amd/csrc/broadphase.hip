@@ -1311,12 +1311,27 @@ __global__ __launch_bounds__(ROWS_BLOCK) void k_expand_rows_count(MeshView m, in
             }
         };
         add(f);
+        // The three stars are loaded lane-parallel (64 entries per load, all three in flight together)
+        // and walked in registers, in the same order as one entry at a time.
+        int sb[3], se[3], sa[3];
 #pragma unroll
-        for (int s = 0; s < 3; s++)
-            for (int k = m.vf_off[fv[s]]; k < m.vf_off[fv[s] + 1]; k++) {
-                const int a = m.vf_list[k];
-                if (a < f) add(a);
+        for (int s = 0; s < 3; s++) {
+            sb[s] = m.vf_off[fv[s]];
+            se[s] = m.vf_off[fv[s] + 1];
+        }
+#pragma unroll
+        for (int s = 0; s < 3; s++) sa[s] = sb[s] + lane < se[s] ? m.vf_list[sb[s] + lane] : 0x7fffffff;
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+            for (int base = sb[s]; base < se[s]; base += 64) {
+                const int a = base == sb[s] ? sa[s] : (base + lane < se[s] ? m.vf_list[base + lane] : 0x7fffffff);
+                const int cnt = se[s] - base < 64 ? se[s] - base : 64;
+                for (int j = 0; j < cnt; j++) {
+                    const int aj = __shfl(a, j);
+                    if (aj < f) add(aj);
+                }
             }
+        }
         // Per directory entry: its row in the table, and which of f's vertices it holds (bit s: fv[s]).
         // The directory holds every face around f's vertices that precedes f unless it overflowed.
         const bool complete = nd < 64;
@@ -1343,10 +1358,27 @@ __global__ __launch_bounds__(ROWS_BLOCK) void k_expand_rows_count(MeshView m, in
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();   // the previous face's reads of lds are done
-        for (int j = 0; j < nd; j++) {
-            const int off = __shfl(d.off, j), len = __shfl(d.len, j), from = __shfl(src, j);
-            if (len < 0) continue;
-            for (int i = lane; i < len; i += 64) lds[off + i] = m.adj[from + i];
+        // Row copies four entries at a time: their first 64 elements load together.
+        for (int j0 = 0; j0 < nd; j0 += 4) {
+            int val[4], dst[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int j = (j0 + u) & 63;
+                const int off = __shfl(d.off, j), len = __shfl(d.len, j), from = __shfl(src, j);
+                const bool ok = j0 + u < nd && lane < len;
+                val[u] = ok ? m.adj[from + lane] : 0;
+                dst[u] = ok ? off + lane : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (dst[u] >= 0) lds[dst[u]] = val[u];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int j = (j0 + u) & 63;
+                const int off = __shfl(d.off, j), len = __shfl(d.len, j), from = __shfl(src, j);
+                if (j0 + u < nd)
+                    for (int i = 64 + lane; i < len; i += 64) lds[off + i] = m.adj[from + i];
+            }
         }
         unsigned long long star[3];   // directory entries around fv[s] (f excluded)
 #pragma unroll
