@@ -185,25 +185,37 @@ CCD_FHD long long double_to_bits(double d)
 }
 
 // Static-index access.  On the GPU an array indexed by a runtime value lives in scratch memory (per-lane
-// private memory, cached but far slower than registers).  Every list here is small, so reads and writes
-// at a runtime index go through these unrolled selects and the arrays stay in registers.
+// private memory, cached but far slower than registers, and written back to memory).  Every list here
+// is small, so reads and writes at a runtime index go through selects and the arrays stay in registers.
+// The selects are spelled out by template recursion, so every element access has a constant index
+// from the start (a loop over the elements still has a variable index when arrays are first promoted
+// to registers), and a write assigns every element (a conditional store per element would be merged
+// back into one store at a runtime address).
+template <int K, int N>
+CCD_FHD double sel_from(const double (&a)[N], int i)
+{
+    if constexpr (K == N - 1)
+        return a[K];
+    else
+        return i == K ? a[K] : sel_from<K + 1>(a, i);
+}
 template <int N>
 CCD_FHD double sel(const double (&a)[N], int i)
 {
     CCD_INDEX_CHECK(i, N);
-    double v = a[0];
-#pragma unroll
-    for (int k = 1; k < N; k++)
-        if (k == i) v = a[k];
-    return v;
+    return sel_from<0>(a, i);
+}
+template <int K, int N>
+CCD_FHD void put_from(double (&a)[N], int i, double v)
+{
+    a[K] = i == K ? v : a[K];
+    if constexpr (K + 1 < N) put_from<K + 1>(a, i, v);
 }
 template <int N>
 CCD_FHD void put(double (&a)[N], int i, double v)
 {
     CCD_INDEX_CHECK(i, N);
-#pragma unroll
-    for (int k = 0; k < N; k++)
-        if (k == i) a[k] = v;
+    put_from<0>(a, i, v);
 }
 
 // checkInterval's clamp of an interval end to [0, 1] (CTCD.cpp:62-65), in its operation order.

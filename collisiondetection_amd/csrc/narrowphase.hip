@@ -373,11 +373,6 @@ __global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue_ee_shared(
     }
 }
 
-__device__ __forceinline__ V3 pick4(const V3 (&a)[4], int j)
-{
-    return j == 0 ? a[0] : (j == 1 ? a[1] : (j == 2 ? a[2] : a[3]));
-}
-
 // Second pass of the two-pass edge-edge narrowphase: the deferred stencils' edge-edge tests in full
 // (NoCount, the recursion included), on a persistent queue over the list.  A stencil whose edge-edge
 // test hits takes its time (sub-test 0, first in CTCDNarrowPhase.cpp:91-132); otherwise the first
@@ -412,15 +407,24 @@ __global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_ee_deferred(
                 for (int j = 0; j < 4; j++) window_pos(qs, qe, v[j], s[j], e[j]);
                 hit = ccd::edge_edge(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t, ccd::NoCount());
                 const int sub = def_sub[i];
+                // The sub-test's vertices are read again by id: selecting them from s[] / e[] at a runtime
+                // index would keep those arrays in scratch memory for every stencil.
+                auto vid = [&](int j) { return j == 0 ? k.x : (j == 1 ? k.y : (j == 2 ? k.z : k.w)); };
                 if (!hit && sub >= 1 && sub <= 4) {
-                    // vertex-edge (s[r]; s[a], s[a + 1]), r = sub - 1, a = 2 for r < 2 else 0
+                    // vertex-edge (v[r]; v[a], v[a + 1]), r = sub - 1, a = 2 for r < 2 else 0
                     const int r = sub - 1, a = r < 2 ? 2 : 0;
-                    hit = ccd::vertex_edge(pick4(s, r), pick4(s, a), pick4(s, a + 1), pick4(e, r), pick4(e, a),
-                                           pick4(e, a + 1), eta, t, ccd::NoCount());
+                    V3 ps, pe, as, ae, bs, be;
+                    window_pos(qs, qe, vid(r), ps, pe);
+                    window_pos(qs, qe, vid(a), as, ae);
+                    window_pos(qs, qe, vid(a + 1), bs, be);
+                    hit = ccd::vertex_edge(ps, as, bs, pe, ae, be, eta, t, ccd::NoCount());
                 } else if (!hit && sub >= 5 && sub <= 8) {
                     // vertex-vertex (0, 2), (0, 3), (1, 2), (1, 3)
                     const int x = sub < 7 ? 0 : 1, y = (sub == 5 || sub == 7) ? 2 : 3;
-                    hit = ccd::vertex_vertex(pick4(s, x), pick4(s, y), pick4(e, x), pick4(e, y), eta, t, ccd::NoCount());
+                    V3 xs, xe, ys, ye;
+                    window_pos(qs, qe, vid(x), xs, xe);
+                    window_pos(qs, qe, vid(y), ys, ye);
+                    hit = ccd::vertex_vertex(xs, ys, xe, ye, eta, t, ccd::NoCount());
                 }
             }
             const uint64_t mask = __ballot(hit);
