@@ -617,9 +617,10 @@ static int detect_graph(ccd_ctx *ctx, const DetectArgs &a)
         if (rc) return rc;
     }
     if (!buffer(ctx, "queue_work", 48)) return CCD_E_OOM;
-    if (ctx->ee_split && ctx->cap_ee <= 0x7fffffffll &&
-        (!buffer(ctx, "ee_def_idx", (size_t)ctx->cap_ee * 4) || !buffer(ctx, "ee_def_sub", (size_t)ctx->cap_ee)))
-        return CCD_E_OOM;
+    if (ctx->ee_split && ctx->cap_ee <= 0x7fffffffll) {
+        const int64_t cd = ee_deferral_capacity(ctx, ctx->cap_ee);
+        if (!buffer(ctx, "ee_def_idx", (size_t)cd * 4) || !buffer(ctx, "ee_def_sub", (size_t)cd)) return CCD_E_OOM;
+    }
     int64_t caps2[3] = {ctx->cap_pairs, ctx->cap_vf, ctx->cap_ee};
     memcpy(key + sizeof(DetectArgs), caps2, sizeof(caps2));
     ctx->capturing = true;

@@ -138,24 +138,25 @@ struct NoCount {
 struct FullCascade {
     CCD_FHD void operator()(int) const {}
 };
-// The product's counter type for the first pass of the two-pass edge-edge narrowphase: NoCount's
-// cascade, except that a coplanar-distance sextic of full degree whose real roots need the derivative
-// recursion (its Bernstein coefficients change sign twice or more, or one lies within rounding of zero)
-// is not solved: `deferred` is set, find_intervals returns with no intervals, and edge_edge returns
-// false at once.  The caller then runs the whole edge-edge test again with NoCount later, on a stencil
-// list that holds only such deferred stencils, so that the expensive recursion runs on full wavefronts.
-// With `enabled` false it runs NoCount's cascade exactly, so one kernel serves the single-pass launch
-// and the first pass.
-struct DeferSextic {
-    bool enabled = true;
+// The product's counter type for the first pass of the two-pass edge-edge narrowphase: it screens the
+// coplanar-distance sextic and solves nothing.  find_intervals on the sextic decides only the case
+// "full degree, and no root in [0, 1] by the Bernstein test" (couldHaveRoots false, or no sign change
+// of the Bernstein coefficients with every one clear of zero), where the list is the one piece [0, 1]
+// or nothing, as the isolator gives it; any other sextic sets `deferred`.  edge_edge then returns
+// false at once, and also sets `deferred` when the sextic leaves a coplanar interval.  So a stencil
+// the screen does not defer has no edge-edge collision, exactly as the full test finds; the deferred
+// ones get the full test later, on a list that holds only them.  The first pass then carries no root
+// solver, no parallel check and no barycentric quartics for the edge-edge test (fewer registers: more
+// wavefronts per SIMD), and the stencils whose sextics need the solver run together.
+struct ScreenSextic {
     bool deferred = false;
     CCD_FHD void operator()(int) const {}
 };
 template <class C>
 struct prunes : std::integral_constant<bool, std::is_same<std::decay_t<C>, NoCount>::value ||
-                                                 std::is_same<std::decay_t<C>, DeferSextic>::value> {};
+                                                 std::is_same<std::decay_t<C>, ScreenSextic>::value> {};
 template <class C>
-struct defers_sextic : std::integral_constant<bool, std::is_same<std::decay_t<C>, DeferSextic>::value> {};
+struct screens_sextic : std::integral_constant<bool, std::is_same<std::decay_t<C>, ScreenSextic>::value> {};
 // A counter type whose `stops_at_solver` member is true makes find_intervals return, with no intervals,
 // where it would call the root solver.  SolverProbe records whether that point was reached; the
 // stencil-ordering pre-pass uses it to see which polynomials need the solver.
@@ -725,6 +726,12 @@ CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos, C &&cnt = C())
         }
     }
 
+    if constexpr (screens_sextic<CT>::value && N == 6) {
+        if (reduced <= 2) {   // a degree-reduced sextic: not screened
+            cnt.deferred = true;
+            return;
+        }
+    }
     constexpr int NT = N > 2 ? N : 2;
     double time[NT] = {};
     int roots = 0;
@@ -737,15 +744,15 @@ CCD_HD void find_intervals(double *op, Ivls<CAP> &out, bool pos, C &&cnt = C())
     };
     if (reduced > 2) {
         if (!could_have_roots<N>(op, reduced, pos)) return;
-        if constexpr (defers_sextic<CT>::value && N == 6) {
-            // The same test RootIsolator<6> starts with (its result is reused there after inlining).
-            if (cnt.enabled && reduced == 6) {
-                const int v = bernstein_variations<6>(op);
-                if (v != 0 && v != 1) {
-                    cnt.deferred = true;
-                    return;
-                }
+        if constexpr (screens_sextic<CT>::value && N == 6) {
+            // RootIsolator<6> starts with the same test and returns no root when it finds no sign change.
+            if (reduced != 6 || bernstein_variations<6>(op) != 0) {
+                cnt.deferred = true;
+                return;
             }
+            set_cuts(0);
+            check_piece<N>(0, op, reduced, out, pos);
+            return;
         }
         cnt(S_ISOLATE);
         if constexpr (stops_at_solver<CT>::value) return;
@@ -940,8 +947,9 @@ CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3
     V3 vq1 = q1e - q1s;
     Ivls<7> raw;
     distance_poly(p0s - p1s, p0s - q0s, p1s - q1s, vp0 - vp1, vp0 - vq0, vp1 - vq1, minD, raw, cnt);
-    if constexpr (defers_sextic<C>::value) {
-        if (cnt.deferred) return false;
+    if constexpr (screens_sextic<C>::value) {
+        if (raw.keep != 0) cnt.deferred = true;   // coplanar intervals: the full test decides
+        return false;
     }
 
     // Parallel-edge classification of each raw interval at its midpoint (CTCD.cpp:294-308).  The

@@ -130,6 +130,8 @@ struct EeDeferral {
 // The context's deferral list for up to n stencils (grow-only buffers "ee_def_idx", "ee_def_sub"; counters
 // at count_and_work[0..1]), or an empty one when the context runs edge-edge in one pass.
 EeDeferral ee_deferral(ccd_ctx *ctx, int64_t n, unsigned long long *count_and_work);
+// The list slots the two-pass narrowphase needs for n stored stencils (the list is filled in blocks).
+int64_t ee_deferral_capacity(const ccd_ctx *ctx, int64_t n);
 // Persistent work-queue narrowphase; *work must be zero before the launch.  Edge-edge runs in two
 // passes when def holds a list for n_max stencils (n_max < 2^31), else in one.
 int launch_narrow2_queue(ccd_ctx *ctx, bool vf, const double *qs, const double *qe, const int32_t *keys,

@@ -194,6 +194,13 @@ __global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue(const double
 // stencils.  So the wavefront hashes its up to 256 tests in LDS by their vertex triples, runs each
 // distinct one once (spread over the lanes), and every stencil then takes the first hit in its own
 // order, then its vertex-vertex tests: the same sub-test, time and result as the sequential cascade.
+// The deferred list is filled in blocks of DEF_BLOCK slots, each claimed by one wavefront with one
+// atomic: an atomic per 64-stencil chunk on the list's single counter serialised the whole pass (a
+// third of its time).  A wavefront marks the slots of a block it leaves unused DEF_HOLE, and the second
+// pass skips them.  ccdi::ee_deferral_capacity bounds the slots this takes.
+constexpr int DEF_BLOCK = 512;
+constexpr uint32_t DEF_HOLE = 0xffffffffu;
+
 constexpr int VE_TASKS = 256;   // 4 per lane
 constexpr int VE_TAB = 512;     // hash slots (more than the tasks: probing always ends)
 struct VeShare {
@@ -223,13 +230,13 @@ __device__ __forceinline__ void window_pos(const double *__restrict__ qs, const 
 }
 
 //
-// First pass of the two-pass form (def_idx not null): a stencil whose coplanar-distance sextic needs the
-// derivative recursion (ccd::DeferSextic) leaves its edge-edge test undecided and is appended to the
-// deferred list, position i and the sub-test of its first vertex-edge / vertex-vertex hit (255: none),
-// instead of being emitted; k_ee_deferred finishes it.  About a quarter of the stencils of a moving mesh
-// need that recursion and take most of the time; among the easy ones a wavefront's lanes then stay
-// on the same short path.
-template <int BS, class M>
+// SCREEN: the first pass of the two-pass form.  The edge-edge test only screens its distance sextic
+// (ccd::ScreenSextic): a stencil the screen cannot clear is appended to the deferred list, position i
+// and the sub-test of its first vertex-edge / vertex-vertex hit (255: none), instead of being emitted;
+// k_ee_deferred finishes it.  About a quarter of the stencils of a moving mesh are deferred, the ones
+// whose sextics need the root solver, and they take most of the time; the first pass carries no solver
+// for them (about half the registers, twice the resident wavefronts).
+template <int BS, class M, bool SCREEN>
 __global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue_ee_shared(
     const double *__restrict__ qs, const double *__restrict__ qe, const int4 *__restrict__ keys,
     const int64_t *__restrict__ n_dev, int64_t n_max, double eta, int4 *__restrict__ out_keys,
@@ -241,6 +248,9 @@ __global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue_ee_shared(
     const int64_t n = *n_dev < n_max ? *n_dev : n_max;
     const int lane = __lane_id();
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    // This wavefront's block of the deferred list (SCREEN): its first slot, and the slots used.
+    int64_t blk_base = -1;
+    int blk_used = DEF_BLOCK;
     while (true) {
         unsigned long long start = 0;
         if (lane == 0) start = atomicAdd(work, (unsigned long long)tile);
@@ -254,16 +264,18 @@ __global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue_ee_shared(
             V3 s[4], e[4];
             double t = 0.0;
             int sub = -1;
-            ccd::DeferSextic ec;
-            ec.enabled = def_idx != nullptr;
+            ccd::ScreenSextic ec;   // used when SCREEN
             if (valid) {
                 k = keys[i];
                 const int v[4] = {k.x, k.y, k.z, k.w};
 #pragma unroll
                 for (int j = 0; j < 4; j++) window_pos(qs, qe, v[j], s[j], e[j]);
-                if (ccd::edge_edge(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t, ec)) sub = 0;
+                if constexpr (SCREEN)
+                    ccd::edge_edge(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t, ec);
+                else if (ccd::edge_edge(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, t, M()))
+                    sub = 0;
             }
-            const bool deferred = ec.deferred;
+            const bool deferred = SCREEN && ec.deferred;
             const bool need = valid && sub < 0;
             // The stencil's four vertex-edge tests as tasks r * 64 + lane.
             const int tv[4] = {k.x, k.y, k.z, k.w};
@@ -354,23 +366,35 @@ __global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue_ee_shared(
                     }
                 }
             }
-            const uint64_t dmask = __ballot(deferred);
+            const uint64_t dmask = SCREEN ? __ballot(deferred) : 0ull;
             if (dmask) {
-                const int leader = __ffsll((unsigned long long)dmask) - 1;
-                unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(def_count, (unsigned long long)__popcll(dmask));
-                base = __shfl(base, leader);
+                const int nd = __popcll(dmask);
+                if (blk_used + nd > DEF_BLOCK) {
+                    // The rest of the current block becomes holes; a new block is claimed.
+                    if (blk_base >= 0)
+                        for (int q = blk_used + lane; q < DEF_BLOCK; q += 64)
+                            if (blk_base + q < def_cap) def_idx[blk_base + q] = DEF_HOLE;
+                    unsigned long long b = 0;
+                    if (lane == 0) b = atomicAdd(def_count, (unsigned long long)DEF_BLOCK);
+                    blk_base = (int64_t)__shfl(b, 0);
+                    blk_used = 0;
+                }
                 if (deferred) {
-                    // The host sizes the list for every stored stencil, so the guard never drops one.
-                    const int64_t pos = (int64_t)base + __popcll(dmask & lt);
+                    // The host sizes the list for every stored stencil and the holes, so the guard never
+                    // drops one.
+                    const int64_t pos = blk_base + blk_used + __popcll(dmask & lt);
                     if (pos < def_cap) {
                         def_idx[pos] = (uint32_t)i;
                         def_sub[pos] = sub < 0 ? (uint8_t)255 : (uint8_t)sub;
                     }
                 }
+                blk_used += nd;
             }
         }
     }
+    if (SCREEN && blk_base >= 0)
+        for (int q = blk_used + lane; q < DEF_BLOCK; q += 64)
+            if (blk_base + q < def_cap) def_idx[blk_base + q] = DEF_HOLE;
 }
 
 // Second pass of the two-pass edge-edge narrowphase: the deferred stencils' edge-edge tests in full
@@ -399,8 +423,9 @@ __global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_ee_deferred(
             bool hit = false;
             double t = 0.0;
             int4 k = make_int4(0, 0, 0, 0);
-            if (i < end) {
-                k = keys[def_idx[i]];
+            const uint32_t di = i < end ? def_idx[i] : DEF_HOLE;
+            if (di != DEF_HOLE) {
+                k = keys[di];
                 const int v[4] = {k.x, k.y, k.z, k.w};
                 V3 s[4], e[4];
 #pragma unroll
@@ -927,11 +952,14 @@ static void launch_queue_bs(ccd_ctx *ctx, int bs, unsigned grid, const double *q
         hipLaunchKernelGGL((k_narrow2_queue<true, B, M>), dim3(grid), dim3(B), 0, ctx->stream, qs, qe,           \
                            (const int4 *)keys, n_dev, n_max, eta, (int4 *)out_keys, out_toi, cap,                  \
                            (unsigned long long *)counter, work, tile);                                            \
-    else                                                                                                            \
-        hipLaunchKernelGGL((k_narrow2_queue_ee_shared<B, M>), dim3(grid), dim3(B), 0, ctx->stream, qs, qe,       \
+    else if (def)                                                                                                   \
+        hipLaunchKernelGGL((k_narrow2_queue_ee_shared<B, M, true>), dim3(grid), dim3(B), 0, ctx->stream, qs, qe, \
                            (const int4 *)keys, n_dev, n_max, eta, (int4 *)out_keys, out_toi, cap,                  \
-                           (unsigned long long *)counter, work, tile, def ? def->idx : nullptr,                    \
-                           def ? def->sub : nullptr, def ? def->count : nullptr, def ? def->cap : 0)
+                           (unsigned long long *)counter, work, tile, def->idx, def->sub, def->count, def->cap);   \
+    else                                                                                                            \
+        hipLaunchKernelGGL((k_narrow2_queue_ee_shared<B, M, false>), dim3(grid), dim3(B), 0, ctx->stream, qs, qe,\
+                           (const int4 *)keys, n_dev, n_max, eta, (int4 *)out_keys, out_toi, cap,                  \
+                           (unsigned long long *)counter, work, tile, nullptr, nullptr, nullptr, 0)
     if (bs == 64)
         CCD_QUEUE_LAUNCH(64);
     else if (bs == 128)
@@ -953,8 +981,8 @@ int launch_narrow2_queue(ccd_ctx *ctx, bool vf, const double *qs, const double *
     waves = std::max<int64_t>(1, std::min(waves, need));
     unsigned grid = (unsigned)((waves * 64 + bs - 1) / bs);
     // Two passes only with a list that holds every stored stencil (int32 positions).
-    const bool split = !vf && def && def->idx && def->sub && def->count && def->work && n_max <= def->cap &&
-                       n_max <= 0x7fffffffll;
+    const bool split = !vf && def && def->idx && def->sub && def->count && def->work &&
+                       ee_deferral_capacity(ctx, n_max) <= def->cap && n_max <= 0x7fffffffll;
     if (vf)
         CCD_PRODUCT_LAUNCH((launch_queue_bs<true, M>(ctx, bs, grid, qs, qe, keys, n_dev, n_max, eta, out_keys,
                                                          out_toi, cap, counter, work, shape.tile, nullptr)));
@@ -964,11 +992,12 @@ int launch_narrow2_queue(ccd_ctx *ctx, bool vf, const double *qs, const double *
                                                           split ? def : nullptr)));
     CCD_LAUNCH_CHECK(ctx);
     if (split) {
-        // The deferred stencils cost about ten times the others: small tiles keep the tail short.
+        // The deferred stencils cost about ten times the others: small tiles keep the tail short, and
+        // 256 keeps the claims on the queue counter few.
         const int64_t w2 = (int64_t)ctx->num_cus * shape.waves_per_cu;
         hipLaunchKernelGGL(k_ee_deferred<64>, dim3((unsigned)std::max<int64_t>(1, w2)), dim3(64), 0, ctx->stream, qs,
                            qe, (const int4 *)keys, def->idx, def->sub, def->count, def->cap, eta, (int4 *)out_keys,
-                           out_toi, cap, (unsigned long long *)counter, def->work, 64);
+                           out_toi, cap, (unsigned long long *)counter, def->work, 256);
         CCD_LAUNCH_CHECK(ctx);
     }
     return CCD_OK;
@@ -1004,16 +1033,26 @@ int launch_narrow_hist(ccd_ctx *ctx, bool vf, const int64_t *off, const double *
     return CCD_OK;
 }
 
+int64_t ee_deferral_capacity(const ccd_ctx *ctx, int64_t n)
+{
+    // Every deferred stencil, the holes a wavefront leaves when a chunk does not fit its block (at most
+    // 63 per claimed block, and a block holds at least DEF_BLOCK - 63 entries when it is left, so fewer
+    // than n / 4 in all), and the unused end of each wavefront's last block (at most 32 wavefronts per
+    // compute unit).
+    return n + n / 4 + (int64_t)(ctx->num_cus * 32 + 1) * DEF_BLOCK;
+}
+
 EeDeferral ee_deferral(ccd_ctx *ctx, int64_t n, unsigned long long *count_and_work)
 {
     EeDeferral d;
     if (!ctx->ee_split || n <= 0 || n > 0x7fffffffll || !count_and_work) return d;
-    d.idx = (uint32_t *)buffer(ctx, "ee_def_idx", (size_t)n * 4);
-    d.sub = (uint8_t *)buffer(ctx, "ee_def_sub", (size_t)n);
+    const int64_t cap = ee_deferral_capacity(ctx, n);
+    d.idx = (uint32_t *)buffer(ctx, "ee_def_idx", (size_t)cap * 4);
+    d.sub = (uint8_t *)buffer(ctx, "ee_def_sub", (size_t)cap);
     if (!d.idx || !d.sub) return EeDeferral();   // one pass instead (the same results)
     d.count = count_and_work;
     d.work = count_and_work + 1;
-    d.cap = n;
+    d.cap = cap;
     return d;
 }
 

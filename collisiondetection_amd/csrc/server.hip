@@ -177,11 +177,11 @@ using namespace ccdi;
 int64_t align256(int64_t x) { return (x + 255) & ~255ll; }
 
 // A request's transient device memory: face pairs, the related-face table and its emission masks
-// (two entries per pair), the stencil keys and the hit lists.
-int64_t workspace_bytes(int64_t cp, int64_t cv, int64_t ce, int64_t ch)
+// (two entries per pair), the stencil keys, the edge-edge deferred list (cd slots) and the hit lists.
+int64_t workspace_bytes(int64_t cp, int64_t cv, int64_t ce, int64_t cd, int64_t ch)
 {
     return align256(cp * 8) + align256((2 * cp + 1) * 4) + align256((2 * cp + 1) * 2) + align256(cv * 16) +
-           align256(ce * 16) + align256(ce * 4) + align256(ce) + 2 * (align256(ch * 16) + align256(ch * 8)) + 256;
+           align256(ce * 16) + align256(cd * 4) + align256(cd) + 2 * (align256(ch * 16) + align256(ch * 8)) + 256;
 }
 
 // Point the session context's transient buffers at a slice of the pool.
@@ -200,8 +200,9 @@ void bind_workspace(ccd_server *S, Session &s, InFlight &f)
     put("adj_mask", (2 * f.cap_pairs + 1) * 2);
     put("vf_keys", f.cap_vf * 16);
     put("ee_keys", f.cap_ee * 16);
-    put("ee_def_idx", f.cap_ee * 4);   // the two-pass edge-edge narrowphase's deferred list
-    put("ee_def_sub", f.cap_ee);
+    const int64_t cd = ee_deferral_capacity(s.ctx, f.cap_ee);   // the edge-edge deferred list
+    put("ee_def_idx", cd * 4);
+    put("ee_def_sub", cd);
     const int64_t kb = align256(f.cap_hits * 16), tb = align256(f.cap_hits * 8);
     f.hit_vk = (int32_t *)p;
     f.hit_vt = (double *)(p + kb);
@@ -344,7 +345,8 @@ void worker_loop(ccd_server *S)
             f.cap_vf = mul * s.last_vf + s.F + 4096;
             f.cap_ee = mul * s.last_ee + 2 * s.F + 4096;
             f.cap_hits = mul * s.last_hits + s.F / 16 + 4096;
-            f.pool_bytes = workspace_bytes(f.cap_pairs, f.cap_vf, f.cap_ee, f.cap_hits);
+            f.pool_bytes = workspace_bytes(f.cap_pairs, f.cap_vf, f.cap_ee, ee_deferral_capacity(s.ctx, f.cap_ee),
+                                           f.cap_hits);
             if (f.pool_bytes > S->pool.size) {   // can never be admitted: fail it rather than wait forever
                 ccd_request_result r{};
                 r.request_id = it->id;

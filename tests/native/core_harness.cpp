@@ -56,8 +56,8 @@ extern "C" void core_narrowphase(const double *qs, const double *qe, const int32
 }
 
 // The two-pass edge-edge schedule of the device (k_narrow2_queue_ee_shared with a deferred list, then
-// k_ee_deferred), stencil by stencil on the host: first pass with ccd::DeferSextic, the vertex-edge and
-// vertex-vertex tests in the window's order for every stencil the edge-edge test did not decide; a
+// k_ee_deferred), stencil by stencil on the host: first pass with ccd::ScreenSextic, the vertex-edge and
+// vertex-vertex tests in the window's order for every stencil the edge-edge screen did not clear; a
 // deferred stencil keeps that sub-test number, then gets the full edge-edge test and, if that misses,
 // the recorded sub-test run again for its time.  Writes hit / t / sub as core_narrowphase does and
 // deferred[i] = 1 for the stencils the first pass set aside; returns how many.
@@ -76,8 +76,8 @@ extern "C" int64_t core_ee_two_pass(const double *qs, const double *qe, const in
         }
         double tt = 0.0;
         int r = -1;
-        ccd::DeferSextic ec;
-        if (ccd::edge_edge(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, tt, ec)) r = 0;
+        ccd::ScreenSextic ec;   // the screen never reports a hit
+        ccd::edge_edge(s[0], s[1], s[2], s[3], e[0], e[1], e[2], e[3], eta, tt, ec);
         if (r < 0) {
             const int ve[4][3] = {{0, 2, 3}, {1, 2, 3}, {2, 0, 1}, {3, 0, 1}};
             for (int k = 0; k < 4 && r < 0; k++)
