@@ -750,7 +750,7 @@ int ccd_autotune(ccd_ctx *ctx, const double *qstart, const double *qend, double 
     hipStream_t st = ctx->stream;
     const int blocks[3] = {64, 128, 256};
     const int tiles[7] = {64, 128, 256, 512, 1024, 2048, 4096};
-    const int waves[2] = {4, 8};
+    const int waves[4] = {4, 8, 12, 16};
     int64_t nmax[2] = {ctx->last_nvf, ctx->last_nee};
     const int32_t *keys[2] = {ctx->last_vf, ctx->last_ee};
     int32_t *sink = (int32_t *)buffer(ctx, "tune_sink", 16 * 4096 + 64);

@@ -994,7 +994,8 @@ int launch_narrow2_queue(ccd_ctx *ctx, bool vf, const double *qs, const double *
     if (split) {
         // The deferred stencils cost about ten times the others: small tiles keep the tail short, and
         // 256 keeps the claims on the queue counter few.
-        const int64_t w2 = (int64_t)ctx->num_cus * shape.waves_per_cu;
+        // Two wavefronts per SIMD fit this kernel's registers.
+        const int64_t w2 = (int64_t)ctx->num_cus * 8;
         hipLaunchKernelGGL(k_ee_deferred<64>, dim3((unsigned)std::max<int64_t>(1, w2)), dim3(64), 0, ctx->stream, qs,
                            qe, (const int4 *)keys, def->idx, def->sub, def->count, def->cap, eta, (int4 *)out_keys,
                            out_toi, cap, (unsigned long long *)counter, def->work, 256);

@@ -294,8 +294,8 @@ int ccd_batch_detect(ccd_ctx *ctx, int64_t ninst, const int64_t *inst_face_off, 
 /*
  * Tuning of the persistent narrowphase: times the work-queue kernel on the candidates of the last
  * detection on this context over block sizes {64, 128, 256}, tiles {64, 128, ..., 4096} and
- * {4, 8} resident wavefronts per CU, on the stencils grouped by sub-test signature, and then whether
- * that grouping (its own passes included) beats the stored order; keeps the fastest per stencil kind
+ * {4, 8, 12, 16} resident wavefronts per CU, on the stencils grouped by sub-test signature, and then
+ * whether that grouping (its own passes included) beats the stored order; keeps the fastest per stencil kind
  * and returns it in shape[0..7] = {vf block, vf tile, vf waves/CU, ee block, ee tile, ee waves/CU,
  * vf grouping, ee grouping} (host, may be NULL; grouping 1 or 0).  ccd_set_queue_shape installs such
  * a shape (e.g. one cached from an earlier tuning run).
