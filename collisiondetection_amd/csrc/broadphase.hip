@@ -1134,6 +1134,24 @@ __global__ __launch_bounds__(64) void k_row_sort(const unsigned long long *__res
     for (int64_t a = blockIdx.x; a < F; a += gridDim.x) {
         const int lo = off[a], len = off[a + 1] - lo;
         if (len < 2 || len > ROW_SORT_CAP) continue;   // uniform across the block
+        if (len <= 64) {
+            // One element per lane, the same bitonic network in registers (no LDS round trips or
+            // barriers): lane i and its partner i ^ j keep the smaller value in the lower lane of an
+            // ascending run, the larger in a descending one.
+            const int lane = threadIdx.x;
+            int x = lane < len ? adj[lo + lane] : INT32_MAX;
+#pragma unroll
+            for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    const int y = __shfl_xor(x, j);
+                    const bool lower = (lane & j) == 0, up = (lane & k) == 0;
+                    x = (lower == up) ? (x < y ? x : y) : (x < y ? y : x);
+                }
+            }
+            if (lane < len) adj[lo + lane] = x;
+            continue;
+        }
         int n2 = 2;
         while (n2 < len) n2 <<= 1;
         for (int i = threadIdx.x; i < n2; i += 64) s[i] = i < len ? adj[lo + i] : INT32_MAX;
