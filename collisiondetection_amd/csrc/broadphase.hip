@@ -1350,15 +1350,21 @@ __global__ __launch_bounds__(ROWS_BLOCK) void k_expand_rows_count(MeshView m, in
                 }
             }
         }
-        // Per directory entry: its row in the table, and which of f's vertices it holds (bit s: fv[s]).
         // The directory holds every face around f's vertices that precedes f unless it overflowed.
         const bool complete = nd < 64;
+        // Then the faces across f's edges that follow f: the edge-edge owner tests search their rows too
+        // (after the completeness test above, which concerns the faces that precede f).
+#pragma unroll
+        for (int s = 0; s < 3; s++)
+            if (fo[s] > f) add(fo[s]);
+        // Per directory entry: its row in the table, and which of f's vertices it holds (bit s: fv[s];
+        // only for the faces that precede f, the ones the vertex-face owner tests walk).
         int rlen = 0, src = 0;
         unsigned vbits = 0;
         if (lane < nd) {
             src = m.adj_off[d.face];
             rlen = m.adj_off[d.face + 1] - src;
-            if (lane > 0) {
+            if (lane > 0 && d.face < f) {
 #pragma unroll
                 for (int s = 0; s < 3; s++)
                     if (has_vertex(m, d.face, fv[s])) vbits |= 1u << s;
@@ -1445,13 +1451,29 @@ __global__ __launch_bounds__(ROWS_BLOCK) void k_expand_rows_count(MeshView m, in
 #pragma unroll
                 for (int r = 0; r < 3; r++) go[r] = m.opp[3 * (int64_t)g + r];
             }
+            // ee_owner_opp's first two tests depend on one edge each: (f, ob) on g's edge r, (oa, g) on f's
+            // edge s.  Each is evaluated once here, not once per edge pair.
+            bool t_fob[3], t_oag[3];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const int ob = go[r];
+                t_fob[r] = ee && ob >= 0 && precedes(f, ob, f, g) &&
+                           related_dir(m, lds, f_cached, f_off, f_len, f, ob);
+            }
+            int oa_off[3] = {0, 0, 0}, oa_len[3] = {0, 0, 0};
+            bool oa_cached[3];
+#pragma unroll
+            for (int s = 0; s < 3; s++) {
+                const int oa = fo[s];
+                oa_cached[s] = oa >= 0 && dir_find(d, oa, oa_off[s], oa_len[s]);
+                t_oag[s] = ee && oa >= 0 && precedes(oa, g, f, g) &&
+                           related_dir(m, lds, oa_cached[s], oa_off[s], oa_len[s], oa, g);
+            }
 #pragma unroll
             for (int s = 0; s < 3; s++) {
                 const int u0 = fv[s], u1 = fv[(s + 1) % 3];
                 const bool ufix = is_fixed(m, u0) && is_fixed(m, u1);
                 const int oa = fo[s];
-                int oa_off = 0, oa_len = 0;
-                const bool oa_cached = oa >= 0 && oa < f && dir_find(d, oa, oa_off, oa_len);
 #pragma unroll
                 for (int r = 0; r < 3; r++) {
                     const int w0 = gv[r], w1 = gv[(r + 1) % 3];
@@ -1461,14 +1483,9 @@ __global__ __launch_bounds__(ROWS_BLOCK) void k_expand_rows_count(MeshView m, in
                     if (oa != -2 && ob != -2) {
                         // ee_owner_opp with the cached rows: the other generating pairs (f, ob),
                         // (oa, g) and (oa, ob)
-                        own = true;
-                        if (ob >= 0 && precedes(f, ob, f, g) && related_dir(m, lds, f_cached, f_off, f_len, f, ob))
-                            own = false;
-                        if (own && oa >= 0 && precedes(oa, g, f, g) &&
-                            related_dir(m, lds, oa_cached, oa_off, oa_len, oa, g))
-                            own = false;
+                        own = !t_fob[r] && !t_oag[s];
                         if (own && oa >= 0 && ob >= 0 && precedes(oa, ob, f, g) &&
-                            related_dir(m, lds, oa_cached, oa_off, oa_len, oa, ob))
+                            related_dir(m, lds, oa_cached[s], oa_off[s], oa_len[s], oa, ob))
                             own = false;
                     } else {
                         own = ee_owner<true>(m, u0, u1, w0, w1, f, g);
