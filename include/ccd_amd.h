@@ -306,9 +306,10 @@ int ccd_set_queue_shape(ccd_ctx *ctx, const int *shape);
 /*
  * ccd_set_edge_edge_split (no counterpart in the reference; a scheduling choice, same results): on (the
  * default), the persistent edge-edge narrowphase of ccd_detect / ccd_batch_detect / the server runs in
- * two passes: stencils whose coplanar-distance sextic needs the derivative recursion of the root
- * isolation are set aside by the first pass (4 + 1 bytes per stored stencil of device memory) and
- * finished by a second pass over that list; off, one pass.
+ * two passes: the first screens each stencil's coplanar-distance sextic and sets aside the stencils
+ * whose sextic needs the root solver, the second finishes them.  The list of set-aside stencils takes
+ * 4 + 1 bytes per slot, about 1.25 slots per stored edge-edge stencil plus 512 per resident wavefront.
+ * Off, one pass.
  */
 int ccd_set_edge_edge_split(ccd_ctx *ctx, int on);
 
