@@ -442,19 +442,28 @@ __global__ __launch_bounds__(256) void k_leaf_records(const int32_t *__restrict_
                                                       NodeRec *__restrict__ nodes, double *__restrict__ sdop64,
                                                       int4 *__restrict__ sfaces, int32_t *__restrict__ rank)
 {
-    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (j >= F) return;
-    int64_t f = order[j];
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int lane = __lane_id();
+    const int64_t j0 = j - lane;   // the wavefront's first leaf
+    if (j0 >= F) return;           // the whole wavefront is past the leaves
+    const bool active = j < F;
+    const int64_t f = active ? order[j] : 0;
+    // The wavefront's 64 rows of sorted FP64 bounds are one contiguous block: written element by element
+    // across the lanes (each store instruction covers consecutive doubles), not row by row per lane.
+    const int nrow = F - j0 < 64 ? (int)(F - j0) : 64;
+    for (int e = lane; e < 64 * NB; e += 64) {
+        const int rr = e / NB, c = e - rr * NB;
+        const int64_t fr = __shfl(f, rr);
+        if (rr < nrow) sdop64[NB * j0 + e] = dop64[NB * fr + c];
+    }
+    if (!active) return;
     rank[f] = (int32_t)j;
     const double *src = dop64 + NB * f;
     NodeRec r;
 #pragma unroll
     for (int k = 0; k < KAX; k++) {
-        double lo = src[k], hi = src[KAX + k];
-        r.b[k] = quant_lo(lo, qparam[k], qparam[KAX + k]);
-        r.b[KAX + k] = quant_hi(hi, qparam[k], qparam[KAX + k]);
-        sdop64[NB * j + k] = lo;
-        sdop64[NB * j + KAX + k] = hi;
+        r.b[k] = quant_lo(src[k], qparam[k], qparam[KAX + k]);
+        r.b[KAX + k] = quant_hi(src[KAX + k], qparam[k], qparam[KAX + k]);
     }
     r.child = (uint32_t)j;
     r.esc = escape[F - 1 + j];
