@@ -870,30 +870,22 @@ CCD_HD void distance_poly(V3 x10, V3 x20, V3 x30, V3 v10, V3 v20, V3 v30, double
 // list can produce a combination, so the result is false whatever the later root solves return; the
 // product kernels return false there without solving them (the reference solves them and then finds no
 // combination).  `mask` selects the first list's members (the coplanar intervals of edge-edge).
-template <int N>
-CCD_FHD bool pairwise_overlap(const double (&L)[N], const double (&U)[N])
-{
-    bool ov = true;
-#pragma unroll
-    for (int x = 0; x < N; x++)
-#pragma unroll
-        for (int y = x + 1; y < N; y++)
-            if (L[x] > U[y] || L[y] > U[x]) ov = false;
-    return ov;
-}
 // Lists are walked piece by piece over their kept pieces (set bits, ascending: the list order).
+// Intervals on a line overlap pairwise exactly when the largest lower end is at most the smallest upper
+// end (closed intervals; TimeInterval::overlap tests pairs).  A list's ends are clamped cut points, 0 or
+// 1: finite, in [+0, 1], never NaN or -0 (clamp_cut maps both to +0), so the running maximum and
+// minimum below decide exactly what the pairwise test decides, and a branch whose running interval is
+// already empty is left early.
 template <int A, int B>
 CCD_HD bool any_overlap(const Ivls<A> &a, unsigned mask, const Ivls<B> &b)
 {
     for (unsigned mi = a.keep & mask; mi; mi &= mi - 1) {
-        double al, au;
-        a.get(low_bit(mi), al, au);
+        double l0, u0;
+        a.get(low_bit(mi), l0, u0);
         for (unsigned mj = b.keep; mj; mj &= mj - 1) {
-            double L[2], U[2];
-            L[0] = al;
-            U[0] = au;
-            b.get(low_bit(mj), L[1], U[1]);
-            if (pairwise_overlap(L, U)) return true;
+            double l1, u1;
+            b.get(low_bit(mj), l1, u1);
+            if (smax(l0, l1) <= smin(u0, u1)) return true;
         }
     }
     return false;
@@ -902,13 +894,18 @@ template <int A, int B, int C>
 CCD_HD bool any_overlap(const Ivls<A> &a, unsigned mask, const Ivls<B> &b, const Ivls<C> &c)
 {
     for (unsigned mi = a.keep & mask; mi; mi &= mi - 1) {
-        double L[3], U[3];
-        a.get(low_bit(mi), L[0], U[0]);
+        double l0, u0;
+        a.get(low_bit(mi), l0, u0);
         for (unsigned mj = b.keep; mj; mj &= mj - 1) {
-            b.get(low_bit(mj), L[1], U[1]);
+            double l1, u1;
+            b.get(low_bit(mj), l1, u1);
+            l1 = smax(l0, l1);
+            u1 = smin(u0, u1);
+            if (l1 > u1) continue;
             for (unsigned mk = c.keep; mk; mk &= mk - 1) {
-                c.get(low_bit(mk), L[2], U[2]);
-                if (pairwise_overlap(L, U)) return true;
+                double l2, u2;
+                c.get(low_bit(mk), l2, u2);
+                if (smax(l1, l2) <= smin(u1, u2)) return true;
             }
         }
     }
@@ -918,15 +915,24 @@ template <int A, int B, int C, int D>
 CCD_HD bool any_overlap(const Ivls<A> &a, unsigned mask, const Ivls<B> &b, const Ivls<C> &c, const Ivls<D> &d)
 {
     for (unsigned mi = a.keep & mask; mi; mi &= mi - 1) {
-        double L[4], U[4];
-        a.get(low_bit(mi), L[0], U[0]);
+        double l0, u0;
+        a.get(low_bit(mi), l0, u0);
         for (unsigned mj = b.keep; mj; mj &= mj - 1) {
-            b.get(low_bit(mj), L[1], U[1]);
+            double l1, u1;
+            b.get(low_bit(mj), l1, u1);
+            l1 = smax(l0, l1);
+            u1 = smin(u0, u1);
+            if (l1 > u1) continue;
             for (unsigned mk = c.keep; mk; mk &= mk - 1) {
-                c.get(low_bit(mk), L[2], U[2]);
+                double l2, u2;
+                c.get(low_bit(mk), l2, u2);
+                l2 = smax(l1, l2);
+                u2 = smin(u1, u2);
+                if (l2 > u2) continue;
                 for (unsigned mm = d.keep; mm; mm &= mm - 1) {
-                    d.get(low_bit(mm), L[3], U[3]);
-                    if (pairwise_overlap(L, U)) return true;
+                    double l3, u3;
+                    d.get(low_bit(mm), l3, u3);
+                    if (smax(l2, l3) <= smin(u2, u3)) return true;
                 }
             }
         }
@@ -995,33 +1001,37 @@ CCD_HD bool edge_edge(V3 q0s, V3 p0s, V3 q1s, V3 p1s, V3 q0e, V3 p0e, V3 q1e, V3
     if (b1.empty()) return false;
 
     // All combinations (CTCD.cpp:352-396): a combination counts if its five intervals overlap
-    // pairwise and their intersection touches no parallel interval; t is the smallest lower end.
+    // pairwise and their intersection touches no parallel interval; t is the smallest lower end.  The
+    // intersection [il, iu] is the running maximum of the lower ends and minimum of the upper ends
+    // (exact: see any_overlap), and a combination is valid exactly when il <= iu.
     bool col = false;
     double mint = 1.0;
     for (unsigned mi = raw.keep & ~par; mi; mi &= mi - 1) {   // the coplanar intervals, in order
-        double L[5], U[5];
-        raw.get(low_bit(mi), L[0], U[0]);
+        double l0, u0;
+        raw.get(low_bit(mi), l0, u0);
         for (unsigned mj = a0.keep; mj; mj &= mj - 1) {
-            a0.get(low_bit(mj), L[1], U[1]);
+            double l1, u1;
+            a0.get(low_bit(mj), l1, u1);
+            l1 = smax(l0, l1);
+            u1 = smin(u0, u1);
+            if (l1 > u1) continue;
             for (unsigned mk = a1.keep; mk; mk &= mk - 1) {
-                a1.get(low_bit(mk), L[2], U[2]);
+                double l2, u2;
+                a1.get(low_bit(mk), l2, u2);
+                l2 = smax(l1, l2);
+                u2 = smin(u1, u2);
+                if (l2 > u2) continue;
                 for (unsigned ml = b0.keep; ml; ml &= ml - 1) {
-                    b0.get(low_bit(ml), L[3], U[3]);
+                    double l3, u3;
+                    b0.get(low_bit(ml), l3, u3);
+                    l3 = smax(l2, l3);
+                    u3 = smin(u2, u3);
+                    if (l3 > u3) continue;
                     for (unsigned mm = b1.keep; mm; mm &= mm - 1) {
-                        b1.get(low_bit(mm), L[4], U[4]);
-                        bool ov = true;
-#pragma unroll
-                        for (int x = 0; x < 5; x++)
-#pragma unroll
-                            for (int y = x + 1; y < 5; y++)
-                                if (L[x] > U[y] || L[y] > U[x]) ov = false;
-                        if (!ov) continue;
-                        double il = 0.0, iu = 1.0;
-#pragma unroll
-                        for (int x = 0; x < 5; x++) {
-                            il = smax(L[x], il);
-                            iu = smin(U[x], iu);
-                        }
+                        double l4, u4;
+                        b1.get(low_bit(mm), l4, u4);
+                        const double il = smax(l3, l4), iu = smin(u3, u4);
+                        if (il > iu) continue;
                         bool skip = false;
                         for (unsigned mp = par; mp; mp &= mp - 1) {
                             double pl, pu;
