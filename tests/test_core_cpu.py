@@ -117,9 +117,10 @@ def test_core_narrowphase_vs_oracle(core_harness, scene):
 
 @pytest.mark.parametrize("scene", ["converging", "batched"])
 def test_core_ee_two_pass_equals_one_pass(core_harness, scene):
-    """The two-pass edge-edge schedule (stencils whose distance sextic needs the derivative recursion set
-    aside, their vertex-edge / vertex-vertex results kept, the edge-edge test finished later) gives the
-    one-pass cascade's flags, sub-tests and time bits on every stencil; both scenes defer some."""
+    """The two-pass edge-edge schedule (stencils whose distance sextic the first pass's screen cannot
+    clear set aside, their vertex-edge / vertex-vertex results kept, the edge-edge test finished later)
+    gives the one-pass cascade's flags, sub-tests and time bits on every stencil; both scenes defer
+    some."""
     if scene == "converging":
         s = scenes.converging_spheres()
     else:
