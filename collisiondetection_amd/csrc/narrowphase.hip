@@ -402,8 +402,11 @@ __global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue_ee_shared(
 // test hits takes its time (sub-test 0, first in CTCDNarrowPhase.cpp:91-132); otherwise the first
 // pass's vertex-edge / vertex-vertex result stands, and its time comes from running that one sub-test
 // again on the same positions (the same call as in the first pass, so the same value).
+// Three wavefronts per SIMD (168 registers, a few spills): its root solves are chains of dependent FP64
+// operations, and a third wavefront hides more of their latency than the spills cost (42.6 -> 36.4 ms
+// on the batch; four wavefronts spill 636 bytes per lane and take 63.9 ms).
 template <int BS>
-__global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_ee_deferred(
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(3))) void k_ee_deferred(
     const double *__restrict__ qs, const double *__restrict__ qe, const int4 *__restrict__ keys,
     const uint32_t *__restrict__ def_idx, const uint8_t *__restrict__ def_sub,
     const unsigned long long *__restrict__ def_count, int64_t def_cap, double eta, int4 *__restrict__ out_keys,
@@ -994,8 +997,8 @@ int launch_narrow2_queue(ccd_ctx *ctx, bool vf, const double *qs, const double *
     if (split) {
         // The deferred stencils cost about ten times the others: small tiles keep the tail short, and
         // 256 keeps the claims on the queue counter few.
-        // Two wavefronts per SIMD fit this kernel's registers.
-        const int64_t w2 = (int64_t)ctx->num_cus * 8;
+        // Three wavefronts per SIMD, what the kernel's registers are set for.
+        const int64_t w2 = (int64_t)ctx->num_cus * 12;
         hipLaunchKernelGGL(k_ee_deferred<64>, dim3((unsigned)std::max<int64_t>(1, w2)), dim3(64), 0, ctx->stream, qs,
                            qe, (const int4 *)keys, def->idx, def->sub, def->count, def->cap, eta, (int4 *)out_keys,
                            out_toi, cap, (unsigned long long *)counter, def->work, 256);
