@@ -14,7 +14,7 @@
 struct QueueShape {
     int block = 256;       // threads per block: 64, 128 or 256
     int tile = 64;         // stencils a wavefront takes per atomic
-    int waves_per_cu = 8;  // resident wavefronts per compute unit in the persistent grid
+    int waves_per_cu = 12; // resident wavefronts per compute unit in the persistent grid (three per SIMD)
     int group = 1;         // order the stencils by their sub-test signature first (1) or not (0)
 };
 
