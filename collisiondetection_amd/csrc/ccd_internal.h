@@ -23,7 +23,8 @@ struct ccd_ctx {
     int num_cus = 256;
     bool group_stencils = true;   // group stencils by vertex-edge signature before the narrowphase
     bool ee_split = true;         // edge-edge narrowphase in two passes (deferred sextic recursions)
-    QueueShape vf_shape, ee_shape;
+    QueueShape vf_shape;
+    QueueShape ee_shape{256, 64, 16, 1};   // the screening pass holds four wavefronts per SIMD
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     std::string err;

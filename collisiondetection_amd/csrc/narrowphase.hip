@@ -238,8 +238,12 @@ __device__ __forceinline__ void window_pos(const double *__restrict__ qs, const 
 // k_ee_deferred finishes it.  About a quarter of the stencils of a moving mesh are deferred, the ones
 // whose sextics need the root solver, and they take most of the time; the first pass carries no solver
 // for them (about half the registers, twice the resident wavefronts).
+//
+// The screening pass runs at four wavefronts per SIMD (128 registers, a few spills) and pays with a
+// persistent grid of 16 wavefronts per CU (the 512-mesh batch's edge-edge stage: 71.0 -> 65.3 ms);
+// the single-pass form stays at two.
 template <int BS, class M, bool SCREEN>
-__global__ __launch_bounds__(BS) CCD_TWO_WAVES void k_narrow2_queue_ee_shared(
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(SCREEN ? 4 : 2))) void k_narrow2_queue_ee_shared(
     const double *__restrict__ qs, const double *__restrict__ qe, const int4 *__restrict__ keys,
     const int64_t *__restrict__ n_dev, int64_t n_max, double eta, int4 *__restrict__ out_keys,
     double *__restrict__ out_toi, int64_t cap, unsigned long long *counter, unsigned long long *work, int tile,
